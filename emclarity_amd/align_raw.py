@@ -1,0 +1,143 @@
+"""Per-particle alignment of BH_alignRaw3d_v2 on the GPU (the plain drop-in).
+
+``align_particle`` follows alignment/BH_alignRaw3d_v2.m for one particle and
+one reference: initial CCC (:1222-1232, :1355-1369), coarse grid (:1119-1419),
+best-of with the initial pose (:1423-1445), local refinement (:1448-1644),
+final translational pass (:1679-1765) and the 15 % CCC-drop revert
+(:1785-1793). The orientation loops run inside emc_align_grid; this module
+only does the host bookkeeping the MATLAB code does between them. The
+azimuthal randomiser (:1142, drawn after rng('shuffle')) is fixed at zero so
+results are reproducible.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .device import DeviceArray, as_device
+from .fourier import FFTPlan
+
+
+@dataclass
+class AlignResult:
+    row: np.ndarray          # [iRef, pIdx, phi, theta, psi-phi, CCC, 1, dx, dy, dz] (:1764-1765)
+    ccc_initial: float
+    coarse_ccc: np.ndarray
+    coarse_shift: np.ndarray
+    best_coarse: int
+    refine_ccc: np.ndarray | None
+    reverted: bool
+
+
+class ParticleAligner:
+    """Device-resident inputs of one particle and reference
+    (the per-particle state of BH_alignRaw3d_v2.m:1084-1110)."""
+
+    def __init__(self, particle, ref, wedge, bandpass, peak_mask, vol_mask=None, ref_wgt_rot=None,
+                 ref_wgt=None, angles=None, shift_val=(0.0, 0.0, 0.0), peak_com=(3, 3, 3), ctx=None):
+        self.particle = as_device(particle, np.float32, ctx)
+        self.ctx = self.particle.ctx
+        dims = self.particle.dims
+        if not (dims[0] == dims[1] == dims[2]):
+            raise ValueError("cubic boxes only")
+        self.n = dims[0]
+        dev = lambda a: None if a is None else as_device(a, np.float32, self.ctx)
+        self.ref = dev(ref)
+        self.wedge = dev(wedge)
+        self.bandpass = dev(bandpass)
+        self.peak_mask = dev(peak_mask)
+        self.vol_mask = dev(vol_mask)
+        self.ref_wgt_rot = dev(ref_wgt_rot)
+        self.ref_wgt = dev(ref_wgt)
+        self.angles = np.eye(3) if angles is None else np.asarray(angles, dtype=np.float64).reshape(3, 3)
+        self.shift_val = np.asarray(shift_val, dtype=np.float64)
+        self.peak_com = [int(c) for c in peak_com]
+        # ref_FT1 = conj(BH_bandLimitCenterNormalize(ref.*volMask, 1, ...)) (:607-608)
+        self.plan = FFTPlan(dims, 1, self.ctx)
+        self.ref_ft1 = DeviceArray((self.n, self.n, self.n // 2 + 1), np.complex64, self.ctx)
+        _lib.call("emc_ref_ft1", self.ctx.handle, self.plan.handle, self.ref.vp,
+                  self.vol_mask.vp if self.vol_mask is not None else None, _lib.i3(dims),
+                  _lib.i3((0, 0, 0)), _lib.i3((0, 0, 0)), self.ref_ft1.vp)
+
+    def grid(self, angles_pts, est0, mode=3):
+        """emc_align_grid over (phi, theta, psi) rows -> (ccc[n], estPeak[n,3])."""
+        pts = np.ascontiguousarray(np.asarray(angles_pts, dtype=np.float64).reshape(-1, 3))
+        n = pts.shape[0]
+        ccc = np.zeros(n)
+        est = np.zeros((n, 3))
+        dp = C.POINTER(C.c_double)
+        vp = lambda a: a.vp if a is not None else None
+        _lib.call("emc_align_grid", self.ctx.handle, self.n, self.particle.vp, self.ref.vp,
+                  vp(self.ref_wgt_rot), vp(self.ref_wgt), self.ref_ft1.vp, vp(self.vol_mask),
+                  self.peak_mask.vp, self.bandpass.vp, self.wedge.vp,
+                  _lib.d_arr(self.angles.reshape(-1, order="F")), _lib.d_arr(est0),
+                  _lib.i3(self.peak_com), pts.ctypes.data_as(dp), n, int(mode),
+                  ccc.ctypes.data_as(dp), est.ctypes.data_as(dp))
+        return ccc, est
+
+
+def refine_list(best_phi, best_theta, best_psi, phi_inc, theta_inc, psi_inc):
+    """Local grid around the coarse best (BH_alignRaw3d_v2.m:1457-1499); rows
+    (phi, theta, psi) with the matrix built from [phi, theta, psi - phi]."""
+    r_phi_inc = phi_inc / 4.0
+    r_the_inc = theta_inc / 2.0
+    r_psi_inc = psi_inc / 2.0
+    psi_step = math.floor(math.sqrt(r_the_inc)) if r_the_inc else 1
+    theta_step, phi_step = 1, 2
+
+    def colon(a, s, b):
+        if s == 0:
+            return np.array([a])
+        n = int(math.floor((b - a) / s + 1e-10)) + 1
+        return a + s * np.arange(max(n, 0))
+
+    if r_psi_inc == 0:
+        in_plane = colon(best_psi - psi_step * r_the_inc / 2, r_the_inc / 2, best_psi + psi_step * r_the_inc / 2)
+    else:
+        in_plane = colon(best_psi - psi_step * r_psi_inc, r_psi_inc, best_psi + psi_step * r_psi_inc)
+    polar = colon(best_theta - theta_step * r_the_inc, r_the_inc, best_theta + theta_step * r_the_inc)
+    azim = colon(best_phi - phi_step * r_phi_inc, r_phi_inc, best_phi + phi_step * r_phi_inc)
+    rows = [(p, t, s) for p in azim for t in polar for s in in_plane]
+    return np.array(rows, dtype=np.float64)
+
+
+def align_particle(al: ParticleAligner, grid_pts, phi_inc_of=None, theta_inc=0.0, psi_inc=0.0,
+                   refine=True, i_ref=1, p_idx=1) -> AlignResult:
+    """One particle through the alignRaw stages (see module docstring)."""
+    sv = al.shift_val
+    ccc0, _ = al.grid([[0.0, 0.0, 0.0]], sv, mode=2)          # initial CCC
+    ccc_initial = float(ccc0[0])
+    ccc, est = al.grid(grid_pts, sv, mode=3)                  # coarse
+    b = int(np.flatnonzero(ccc == ccc.max())[0])              # sortrows(-6), stable
+    best = dict(phi=grid_pts[b][0], theta=grid_pts[b][1], psi=grid_pts[b][2], ccc=ccc[b], est=est[b])
+    if ccc_initial > best["ccc"]:                             # :1433-1435
+        best = dict(phi=0.0, theta=0.0, psi=0.0, ccc=ccc_initial, est=sv.copy())
+    refine_ccc = None
+    row_angles = (best["phi"], best["theta"], best["psi"] - best["phi"])
+    est_best = best["est"]
+    ccc_best = best["ccc"]
+    if refine:
+        phi_inc = phi_inc_of(best["theta"]) if callable(phi_inc_of) else (phi_inc_of or 0.0)
+        rl = refine_list(best["phi"], best["theta"], best["psi"], phi_inc, theta_inc, psi_inc)
+        rc, re = al.grid(rl, best["est"], mode=3)
+        refine_ccc = rc
+        k = int(np.flatnonzero(rc == rc.max())[0])
+        row_angles = (rl[k][0], rl[k][1], rl[k][2] - rl[k][0])
+        est_best = re[k]
+        ccc_best = rc[k]
+    # final translational pass at the best pose (:1679-1765): stage 1 only,
+    # matrix from [phi, theta, psi'] with psi' already psi - phi
+    fin = np.array([[row_angles[0], row_angles[1], row_angles[2] + row_angles[0]]])
+    _, fest = al.grid(fin, est_best, mode=1)
+    shift = fest[0] - sv                                       # :1764-1765
+    row = np.array([i_ref, p_idx, *row_angles, ccc_best, 1.0, *shift])
+    reverted = False
+    d = row[5] - ccc_initial
+    if d < 0 and abs(d) > 0.15 * ccc_initial:                  # :1785-1793
+        row = np.array([i_ref, p_idx, 0.0, 0.0, 0.0, ccc_initial, 1.0, *sv])
+        reverted = True
+    return AlignResult(row, ccc_initial, ccc, est, b, refine_ccc, reverted)

@@ -1,0 +1,275 @@
+// The plain per-particle grid search of BH_alignRaw3d_v2 on the GPU
+// (reference: alignment/BH_alignRaw3d_v2.m:1119-1419 coarse loop, :1501-1639
+// refinement body, :1679-1765 final translational pass).
+//
+// Every orientation runs the reference's two-stage body with the work kept
+// on the device and no host synchronisation until the whole list is done:
+//   stage 1 (translation): rotate reference, reference weight and peak mask
+//     by RotMat' (forward); R2C; X = P1 conj(F_ref) w_sym / N; C2R; masked
+//     fftshifted argmax + two COM passes -> estPeak (device);
+//   stage 2 (score): rotate the particle (inverse, shift estPeak read from
+//     device memory) times volMask, and its SF3D wedge; R2C; bandpass;
+//     rotational CC with FP64 sums -> ccc (device).
+// The reference works on full complex grids (fftn/ifftn). Both stages only
+// need real(ifftn(.)) and real sums, so the half spectrum is used with the
+// weight product symmetrised over k and -k, which reproduces the full-grid
+// result exactly (the products of the two Hermitian spectra are Hermitian;
+// the weights are not, so w(k) and w(-k) both enter).
+// Normalisation factors of BH_bandLimitCenterNormalize scale the map and both
+// CC factors by positive constants; argmax, COM and the CC ratio do not depend
+// on them, so they are not computed in this loop (DC removal and the bandpass
+// are applied).
+#include <cmath>
+#include <cstring>
+
+#include "ops.hpp"
+
+namespace emc {
+
+void define_matrix(const double ang_deg[3], int convention, int direction, double out_colmajor[9]);
+
+// X(k) = P1(k) conj(F(k)) ws(k) / N over the half grid; F(0) treated as 0
+// (BH_bandLimitCenterNormalize.m:51). ws = (wa(k)wb(k) + wa(-k)wb(-k))/2
+// with wa = ifftshift(rotated ref weight), wb = ifftshift(SF3D wedge).
+__global__ void __launch_bounds__(256) xcorr_half_kernel(const float2* __restrict__ P1,
+                                                         const float2* __restrict__ F,
+                                                         const float* __restrict__ wa_c,
+                                                         const float* __restrict__ wb_c,
+                                                         float2* __restrict__ X, int nx, int ny, int nz,
+                                                         float inv_n) {
+  const int cx = nx / 2 + 1;
+  const int64_t nh = (int64_t)cx * ny * nz;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nh;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int x = (int)(i % cx);
+    const int64_t yz = i / cx;
+    const int y = (int)(yz % ny), z = (int)(yz / ny);
+    const int64_t cp = ((int64_t)shifted_index(z, nz) * ny + shifted_index(y, ny)) * nx +
+                       shifted_index(x, nx);
+    const int mx = mirror_index(x, nx), my = mirror_index(y, ny), mz = mirror_index(z, nz);
+    const int64_t cm = ((int64_t)shifted_index(mz, nz) * ny + shifted_index(my, ny)) * nx +
+                       shifted_index(mx, nx);
+    const float wa0 = wa_c ? wa_c[cp] : 1.0f, wa1 = wa_c ? wa_c[cm] : 1.0f;
+    const float wb0 = wb_c ? wb_c[cp] : 1.0f, wb1 = wb_c ? wb_c[cm] : 1.0f;
+    const float ws = 0.5f * (wa0 * wb0 + wa1 * wb1) * inv_n;
+    float2 p = P1[i], f = F[i];
+    if (i == 0) f = make_float2(0.0f, 0.0f);
+    // P1 * conj(F)
+    X[i] = make_float2((p.x * f.x + p.y * f.y) * ws, (p.y * f.x - p.x * f.y) * ws);
+  }
+}
+
+// Rotational CC sums (BH_multi_xcf_Rotational.m:73-78) over the half grid:
+//   P = F2 * bp (DC = 0), R = ref_ft1, Wp = ifftshift(rotated wedge),
+//   Wr = ref weight (FFT order, null = 1). Writes per-block partials of
+//   (num, sum|R Wp|^2, sum|P Wr|^2).
+__global__ void __launch_bounds__(kRedThreads) rot_sums_half_kernel(
+    const float2* __restrict__ F2, const float* __restrict__ bp, const float2* __restrict__ R,
+    const float* __restrict__ wp_c, const float* __restrict__ wr, int nx, int ny, int nz,
+    double* __restrict__ part) {
+  const int cx = nx / 2 + 1;
+  const int64_t nh = (int64_t)cx * ny * nz;
+  // NaN first element -> weight 1 (:34-40); the first element of
+  // ifftshift(wedge) is the centred volume at the shifted origin.
+  const int64_t c0 = ((int64_t)(nz / 2) * ny + ny / 2) * nx + nx / 2;
+  const bool use_wp = wp_c && !(wp_c[c0] != wp_c[c0]);
+  const bool use_wr = wr && !(wr[0] != wr[0]);
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nh;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (i == 0) continue;  // P(0) = 0
+    const int x = (int)(i % cx);
+    const int64_t yz = i / cx;
+    const int y = (int)(yz % ny), z = (int)(yz / ny);
+    const bool self = (x == 0) || (nx % 2 == 0 && x == nx / 2);
+    const int mx = mirror_index(x, nx), my = mirror_index(y, ny), mz = mirror_index(z, nz);
+    const int64_t fp = ((int64_t)z * ny + y) * nx + x;      // full-grid FFT order
+    const int64_t fm = ((int64_t)mz * ny + my) * nx + mx;
+    const int64_t cp = ((int64_t)shifted_index(z, nz) * ny + shifted_index(y, ny)) * nx +
+                       shifted_index(x, nx);
+    const int64_t cm = ((int64_t)shifted_index(mz, nz) * ny + shifted_index(my, ny)) * nx +
+                       shifted_index(mx, nx);
+    const double b = bp ? (double)bp[fp] : 1.0;
+    const float2 f = F2[i];
+    const double px = f.x * b, py = f.y * b;
+    const float2 r = R[i];
+    const double re = px * r.x - py * r.y;
+    const double r2 = (double)r.x * r.x + (double)r.y * r.y;
+    const double p2 = px * px + py * py;
+    const double wp0 = use_wp ? (double)wp_c[cp] : 1.0, wp1 = use_wp ? (double)wp_c[cm] : 1.0;
+    const double wr0 = use_wr ? (double)wr[fp] : 1.0, wr1 = use_wr ? (double)wr[fm] : 1.0;
+    if (self) {
+      v[0] += re * wr0 * wp0;
+      v[1] += r2 * wp0 * wp0;
+      v[2] += p2 * wr0 * wr0;
+    } else {
+      v[0] += re * (wr0 * wp0 + wr1 * wp1);
+      v[1] += r2 * (wp0 * wp0 + wp1 * wp1);
+      v[2] += p2 * (wr0 * wr0 + wr1 * wr1);
+    }
+  }
+  block_sum<3>(v);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 3; ++k) part[3 * blockIdx.x + k] = v[k];
+}
+
+__global__ void __launch_bounds__(kRedThreads) rot_cc_fold_kernel(const double* __restrict__ part,
+                                                                  int nb, double* __restrict__ cc) {
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+    for (int k = 0; k < 3; ++k) v[k] += part[3 * b + k];
+  block_sum<3>(v);
+  if (threadIdx.x == 0) cc[0] = v[0] / sqrt(v[1] * v[2]);
+}
+
+__global__ void copy_peak_kernel(const PeakResult* __restrict__ r, double* __restrict__ est) {
+  if (threadIdx.x < 3) est[threadIdx.x] = r->peak[threadIdx.x];
+}
+
+struct AlignBuffers {
+  float *rref, *rwdg, *rmask, *map, *rpart, *rwedge;
+  float2 *fref, *P1, *X, *F2;
+  PeakResult* res;
+  double *ccc, *est, *est0, *part;
+};
+
+}  // namespace emc
+
+using namespace emc;
+
+extern "C" {
+
+emc_status emc_align_grid(emc_ctx* ctx, int n, const float* d_particle, const float* d_ref,
+                          const float* d_ref_wgt_rot, const float* d_ref_wgt, const void* d_ref_ft1,
+                          const float* d_vol_mask, const float* d_peak_mask, const float* d_bandpass,
+                          const float* d_wedge, const double angles_colmajor[9],
+                          const double est0[3], const int peak_com[3], const double* angles_pts,
+                          int n_orient, int mode, double* ccc_out, double* shift_out) {
+  return guard([&] {
+    use_device(ctx);
+    EMC_REQUIRE(n > 0 && n <= 1024, "bad box size %d", n);
+    EMC_REQUIRE(d_particle && d_ref && d_ref_ft1 && d_bandpass && d_wedge && angles_colmajor &&
+                    est0 && peak_com && angles_pts && ccc_out && shift_out,
+                "null pointer");
+    EMC_REQUIRE(n_orient > 0, "n_orient must be > 0");
+    EMC_REQUIRE(mode >= 1 && mode <= 3, "mode must be 1 (translation), 2 (score) or 3 (both)");
+    const int dims[3] = {n, n, n};
+    const int64_t nv = (int64_t)n * n * n;
+    const int64_t nh = (int64_t)(n / 2 + 1) * n * n;
+    emc_fft_plan* plan = cached_plan(ctx, n, n, n, 1);
+
+    AlignBuffers B;
+    char* base = (char*)scratch(ctx, "align_buf",
+                                sizeof(float) * nv * 6 + sizeof(float2) * nh * 4 + 256 * 8);
+    B.rref = (float*)base;
+    B.rwdg = B.rref + nv;
+    B.rmask = B.rwdg + nv;
+    B.map = B.rmask + nv;
+    B.rpart = B.map + nv;
+    B.rwedge = B.rpart + nv;
+    B.fref = (float2*)(B.rwedge + nv);
+    B.P1 = B.fref + nh;
+    B.X = B.P1 + nh;
+    B.F2 = B.X + nh;
+    B.res = (PeakResult*)scratch(ctx, "align_res", sizeof(PeakResult) * n_orient);
+    B.ccc = (double*)scratch(ctx, "align_ccc", sizeof(double) * n_orient);
+    B.est = (double*)scratch(ctx, "align_est", sizeof(double) * 3 * n_orient);
+    B.est0 = (double*)scratch(ctx, "align_est0", sizeof(double) * 3);
+    B.part = (double*)scratch(ctx, "align_part", sizeof(double) * 3 * kRedBlocks);
+    double* h_stage = (double*)pinned(ctx, sizeof(double) * 4 * (size_t)n_orient + 64);
+    std::memcpy(h_stage, est0, sizeof(double) * 3);
+    EMC_HIP(hipMemcpyAsync(B.est0, h_stage, sizeof(double) * 3, hipMemcpyHostToDevice, ctx->stream));
+    EMC_HIP(hipStreamSynchronize(ctx->stream));
+
+    const bool stage1 = mode & 1, stage2 = mode & 2;
+    const float zero3[3] = {0.0f, 0.0f, 0.0f};
+    const float est0f[3] = {(float)est0[0], (float)est0[1], (float)est0[2]};
+    if (stage1) {
+      // rotPart_FT of stage 1 is orientation independent (:1316-1318):
+      // R2C(particle) with DC = 0 and the bandpass, computed once.
+      fft_r2c(plan, d_particle, B.P1);
+      double* d_scale = B.part;  // unused scale
+      blcn_half(ctx, B.P1, dims, 1, d_bandpass, d_scale, false);
+    }
+    const int g1 = 1024;
+    for (int o = 0; o < n_orient; ++o) {
+      const double* a = angles_pts + 3 * o;
+      const double ang[3] = {a[0], a[1], a[2] - a[0]};  // [phi, theta, psi - phi] (:1175)
+      double Rl[9], R[9];
+      define_matrix(ang, 0, 1, Rl);  // 'Bah', 'inv'
+      // RotMat = reshape(angles,3,3) * RotMat (:1176), column-major
+      for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) {
+          double s = 0.0;
+          for (int k = 0; k < 3; ++k) s += angles_colmajor[k * 3 + r] * Rl[c * 3 + k];
+          R[c * 3 + r] = s;
+        }
+      float rows[9], rowsT[9];
+      colmajor_to_rows(R, rows);
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) rowsT[r * 3 + c] = rows[c * 3 + r];  // RotMat'
+      const double* d_shift2 = B.est0;
+      if (stage1) {
+        XformArgs a1 = xform_args(dims, rowsT, 1, est0f, 1);
+        launch_xform(ctx, d_ref, B.rref, a1);  // :1287-1290
+        XformArgs a0 = xform_args(dims, rowsT, 1, zero3, 1);
+        const float* wa = nullptr;
+        if (d_ref_wgt_rot) {
+          launch_xform(ctx, d_ref_wgt_rot, B.rwdg, a0);  // :1294-1297
+          wa = B.rwdg;
+        }
+        launch_xform(ctx, d_peak_mask, B.rmask, a0);  // :1300-1303
+        fft_r2c(plan, B.rref, B.fref);
+        hipLaunchKernelGGL(xcorr_half_kernel, dim3(g1), dim3(256), 0, ctx->stream, (const float2*)B.P1,
+                           (const float2*)B.fref, wa, d_wedge, B.X, n, n, n,
+                           (float)(1.0 / (double)nv));
+        EMC_HIP(hipGetLastError());
+        fft_c2r(plan, B.X, B.map);
+        // estPeak = peakCoord + estPeakCoord (:1341-1344)
+        peak_search(ctx, B.map, 1, dims, B.rmask, peak_com, B.res + o, B.est0);
+        hipLaunchKernelGGL(copy_peak_kernel, dim3(1), dim3(64), 0, ctx->stream,
+                           (const PeakResult*)(B.res + o), B.est + 3 * o);
+        d_shift2 = B.est + 3 * o;
+      }
+      if (stage2) {
+        XformArgs a2 = xform_args(dims, rows, 1, zero3, 0);
+        a2.d_shift = d_shift2;
+        a2.d_mul = d_vol_mask;
+        launch_xform(ctx, d_particle, B.rpart, a2);  // :1213-1216, x volMask (:1375)
+        XformArgs a3 = xform_args(dims, rows, 1, zero3, 0);
+        launch_xform(ctx, d_wedge, B.rwedge, a3);  // :1251-1254
+        fft_r2c(plan, B.rpart, B.F2);
+        hipLaunchKernelGGL(rot_sums_half_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, ctx->stream,
+                           (const float2*)B.F2, d_bandpass, (const float2*)d_ref_ft1,
+                           (const float*)B.rwedge, d_ref_wgt, n, n, n, B.part);
+        hipLaunchKernelGGL(rot_cc_fold_kernel, dim3(1), dim3(kRedThreads), 0, ctx->stream,
+                           (const double*)B.part, kRedBlocks, B.ccc + o);
+        EMC_HIP(hipGetLastError());
+      }
+    }
+    // one synchronisation for the whole list
+    double* h_ccc = h_stage;
+    double* h_est = h_stage + n_orient;
+    PeakResult* h_res = nullptr;
+    std::vector<PeakResult> res;
+    if (stage2)
+      EMC_HIP(hipMemcpyAsync(h_ccc, B.ccc, sizeof(double) * n_orient, hipMemcpyDeviceToHost,
+                             ctx->stream));
+    if (stage1) {
+      res.resize(n_orient);
+      EMC_HIP(hipMemcpyAsync(res.data(), B.res, sizeof(PeakResult) * n_orient,
+                             hipMemcpyDeviceToHost, ctx->stream));
+      h_res = res.data();
+    }
+    EMC_HIP(hipStreamSynchronize(ctx->stream));
+    (void)h_est;
+    for (int o = 0; o < n_orient; ++o) {
+      if (stage1 && h_res[o].status != 0)
+        fail(EMC_ERR_RANGE, "orientation %d: failed to box out COM calc", o);
+      ccc_out[o] = stage2 ? h_ccc[o] : 0.0;
+      for (int k = 0; k < 3; ++k) shift_out[3 * o + k] = stage1 ? h_res[o].peak[k] : est0[k];
+    }
+  });
+}
+
+}  // extern "C"

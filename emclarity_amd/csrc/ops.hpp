@@ -1,0 +1,79 @@
+// Cross-file launch helpers of libemc_hip.so (host-side declarations and the
+// argument structs the kernels take by value).
+#pragma once
+
+#include "emc_internal.hpp"
+
+namespace emc {
+
+// ---------------------------------------------------------------- rotate
+
+struct XformArgs {
+  int nx, ny, nz;     // input grid
+  int ox, oy, oz;     // output grid
+  int wx, wy, wz;     // rotated window inside the output
+  int px, py, pz;     // window offset in the output
+  float pad_value;    // value outside the window
+  int nsym;
+  float inv_nsym;
+  int fwd;
+  float t[3];
+  const double* d_shift;  // optional device shift added to t
+  const float* d_mul;     // optional window-sized multiplier applied to the result
+  float m[kMaxSym][9];    // row-major 3x3 per symmetry mate
+};
+
+XformArgs xform_args(const int in_dims[3], const float* Rrow, int nsym, const float t[3], int fwd);
+void launch_xform(emc_ctx* ctx, const float* in, float* out, const XformArgs& a);
+// d_mats: batch row-major 3x3 (FP32); d_shifts: batch x 3 (FP32)
+void launch_xform_batched(emc_ctx* ctx, const float* in, float* out, const float* d_mats,
+                          const float* d_shifts, int batch, const XformArgs& a);
+void colmajor_to_rows(const double* cm, float* rows);
+
+// ---------------------------------------------------------------- padding
+
+struct PadArgs {
+  int nx, ny, nz;   // input dims
+  int tx, ty, tz;   // trim at the low side
+  int wx, wy, wz;   // trimmed image dims
+  int lx, ly, lz;   // pad at the low side
+  int ox, oy, oz;   // output dims
+  float fill;
+  int taper;
+};
+
+PadArgs pad_args(const int dims[3], const int low[3], const int top[3], float fill, int taper);
+void launch_pad(emc_ctx* ctx, const float* in, float* out, const PadArgs& a, const double* d_fill);
+
+// ---------------------------------------------------------------- spectra
+
+void bandpass_impl(emc_ctx* ctx, const int dims[3], int half, double thr, double high_cut,
+                   double low_cut, double pixel, float* d_out);
+// In place on `batch` half spectra of real dims `dims`: F(0)=0, F *= bp
+// (full grid, may be null) and d_scale[b] = numel / sqrt(full-grid sum |F|^2);
+// if apply_scale the spectra are also multiplied by d_scale[b].
+void blcn_half(emc_ctx* ctx, float2* F, const int dims[3], int batch, const float* bp,
+               double* d_scale, bool apply_scale);
+
+// ---------------------------------------------------------------- FFT
+
+void fft_plan_dims(const emc_fft_plan* p, int dims[3]);
+
+// ---------------------------------------------------------------- peaks
+
+// Result of a translational peak search, in device memory.
+struct PeakResult {
+  double peak[3];   // shift (x,y,z) in pixels (reference return value)
+  double max_val;   // map maximum after masking
+  int status;       // 0 ok, 1 COM box outside the map
+  int pad_;
+};
+
+// Peak search of BH_multi_xcf_Translational on a real map stored in FFT order
+// (the map the reference fftshifts), read with `stride` floats between
+// voxels (2 = real parts of complex data). mask: centred FP32 volume or null.
+// d_add: optional device shift (3 doubles) added to the returned peak.
+void peak_search(emc_ctx* ctx, const float* map, int stride, const int dims[3], const float* mask,
+                 const int com[3], PeakResult* d_res, const double* d_add);
+
+}  // namespace emc

@@ -1,0 +1,231 @@
+// Template search over one tomogram chunk: the loop body of
+// BH_templateSearch3d_2 (reference: alignment/BH_templateSearch3d_2.m:632-641
+// swapQuadrants, :713-764 rotate/pad/correlate/normalise, :851-887 max/argmax
+// update).
+//
+// Per orientation: rotate the template and write it straight into the
+// zero-padded chunk buffer (no separate pad pass), R2C, multiply by the
+// pre-centred tomogram spectrum, C2R, std over the valid area (FP64 sums),
+// then a fused per-voxel max/argmax update of the score and angle maps.
+// Orientations are processed in blocks: the rotate, FFTs and multiply are
+// batched over the block; the updates are applied in orientation order, so
+// the strict '<' tie rule of the reference is preserved.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "ops.hpp"
+
+namespace emc {
+
+void define_matrix(const double ang_deg[3], int convention, int direction, double out_colmajor[9]);
+
+// tomoFou = swapQuadrants .* R2C(chunk) (:632-641, :713-714). swapQuadrants on
+// the half grid: exp(-2 pi i sum_k c_k (n_k/2 + 1)) with the centred
+// coordinate c_k = (j_k - n_k/2)/n_k of the array position j_k; the phase is
+// reduced exactly in integers before the sincos.
+__global__ void swapq_kernel(float2* __restrict__ F, int nx, int ny, int nz) {
+  const int cx = nx / 2 + 1;
+  const int64_t nh = (int64_t)cx * ny * nz;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nh;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int x = (int)(i % cx);
+    const int64_t yz = i / cx;
+    const int y = (int)(yz % ny), z = (int)(yz / ny);
+    auto frac = [](int j, int n) {
+      int64_t a = ((int64_t)(j - n / 2) * (n / 2 + 1)) % n;
+      if (a < 0) a += n;
+      return (double)a / (double)n;
+    };
+    double f = frac(x, nx) + frac(y, ny) + frac(z, nz);
+    f -= floor(f);
+    double s, c;
+    sincospi(-2.0 * f, &s, &c);
+    const float2 v = F[i];
+    F[i] = make_float2((float)(v.x * c - v.y * s), (float)(v.x * s + v.y * c));
+  }
+}
+
+// X_b = T0 * conj(T_b) for a block of spectra (no scaling: mexFFT does not
+// scale, :759-761)
+__global__ void conj_mul_batch_kernel(const float2* __restrict__ T0, float2* __restrict__ T,
+                                      int64_t nh) {
+  float2* Tb = T + (int64_t)blockIdx.y * nh;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nh;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float2 a = T0[i], b = Tb[i];
+    Tb[i] = make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+  }
+}
+
+// per-map sums over the valid area (partials[b][block][2])
+__global__ void __launch_bounds__(kRedThreads) valid_sums_kernel(const float* __restrict__ ccf,
+                                                                 int nx, int ny, int nz, int vx0,
+                                                                 int vy0, int vz0, int vx, int vy,
+                                                                 int vz, double* __restrict__ part) {
+  const int64_t nv = (int64_t)vx * vy * vz;
+  const float* m = ccf + (int64_t)blockIdx.y * nx * ny * nz;
+  double v[2] = {0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int x = (int)(i % vx);
+    const int64_t yz = i / vx;
+    const int y = (int)(yz % vy), z = (int)(yz / vy);
+    const double a = m[((int64_t)(z + vz0) * ny + (y + vy0)) * nx + (x + vx0)];
+    v[0] += a;
+    v[1] += a * a;
+  }
+  block_sum<2>(v);
+  if (threadIdx.x == 0) {
+    double* p = part + 2 * ((int64_t)blockIdx.y * gridDim.x + blockIdx.x);
+    p[0] = v[0];
+    p[1] = v[1];
+  }
+}
+
+// std with N-1 normalisation (MATLAB std, :764) per map -> inv_std[b]
+__global__ void __launch_bounds__(kRedThreads) std_fold_kernel(const double* __restrict__ part, int nb,
+                                                               double nvalid,
+                                                               float* __restrict__ stdv) {
+  double v[2] = {0.0, 0.0};
+  const double* p = part + (int64_t)2 * nb * blockIdx.x;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    v[0] += p[2 * b];
+    v[1] += p[2 * b + 1];
+  }
+  block_sum<2>(v);
+  if (threadIdx.x == 0) {
+    const double mean = v[0] / nvalid;
+    double var = (v[1] - nvalid * mean * mean) / (nvalid - 1.0);
+    if (var < 0.0) var = 0.0;
+    stdv[blockIdx.x] = (float)sqrt(var);
+  }
+}
+
+// score/angle update over the valid area for a block of maps, in order
+__global__ void __launch_bounds__(256) mip_update_kernel(const float* __restrict__ ccf, int nb_maps,
+                                                         int nx, int ny, int nz, int vx0, int vy0,
+                                                         int vz0, int vx, int vy, int vz,
+                                                         const float* __restrict__ stdv,
+                                                         float* __restrict__ peak,
+                                                         int32_t* __restrict__ angle, int first_id,
+                                                         int init_first) {
+  const int64_t nv = (int64_t)vx * vy * vz;
+  const int64_t nc = (int64_t)nx * ny * nz;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int x = (int)(i % vx);
+    const int64_t yz = i / vx;
+    const int y = (int)(yz % vy), z = (int)(yz / vy);
+    const int64_t src = ((int64_t)(z + vz0) * ny + (y + vy0)) * nx + (x + vx0);
+    float pk = peak[i];
+    int32_t an = angle[i];
+    for (int b = 0; b < nb_maps; ++b) {
+      const float v = ccf[b * nc + src] / stdv[b];
+      if (b == 0 && init_first) {  // magTmp = ccfmap; angTmp = 1 (:802-823)
+        pk = v;
+        an = 1;
+      } else if (pk < v) {  // strict < (:851-854)
+        pk = v;
+        an = first_id + b;
+      }
+    }
+    peak[i] = pk;
+    angle[i] = an;
+  }
+}
+
+}  // namespace emc
+
+using namespace emc;
+
+extern "C" {
+
+emc_status emc_template_search_chunk(emc_ctx* ctx, const float* d_chunk, const int cd[3],
+                                     const float* d_template, const int td[3], const int pad_low[3],
+                                     const int valid_low[3], const int vd[3],
+                                     const double* angles_pts, int n_orient, int first_angle_id,
+                                     int init_first, float* d_peak, int32_t* d_angle) {
+  return guard([&] {
+    use_device(ctx);
+    EMC_REQUIRE(d_chunk && cd && d_template && td && pad_low && valid_low && vd && angles_pts &&
+                    d_peak && d_angle,
+                "null pointer");
+    EMC_REQUIRE(n_orient > 0, "n_orient must be > 0");
+    for (int k = 0; k < 3; ++k) {
+      EMC_REQUIRE(pad_low[k] >= 0 && pad_low[k] + td[k] <= cd[k], "template does not fit the chunk");
+      EMC_REQUIRE(valid_low[k] >= 0 && vd[k] > 0 && valid_low[k] + vd[k] <= cd[k],
+                  "valid area outside the chunk");
+    }
+    const int64_t nc = (int64_t)cd[0] * cd[1] * cd[2];
+    const int64_t nh = (int64_t)(cd[0] / 2 + 1) * cd[1] * cd[2];
+    const int64_t nvalid = (int64_t)vd[0] * vd[1] * vd[2];
+    EMC_REQUIRE(nvalid > 1, "valid area must hold more than one voxel");
+    // orientation block: bounded by ~1.5 GiB of batch buffers
+    int blk = (int)std::max<int64_t>(1, std::min<int64_t>(16, ((int64_t)3 << 29) / (nc * 4 + nh * 8)));
+    blk = std::min(blk, n_orient);
+    emc_fft_plan* plan1 = cached_plan(ctx, cd[0], cd[1], cd[2], 1);
+    emc_fft_plan* planb = cached_plan(ctx, cd[0], cd[1], cd[2], blk);
+
+    float2* tomo = (float2*)scratch(ctx, "ts_tomo", sizeof(float2) * nh);
+    float* tpad = (float*)scratch(ctx, "ts_tpad", sizeof(float) * nc * blk);
+    float2* T = (float2*)scratch(ctx, "ts_T", sizeof(float2) * nh * blk);
+    float* mats = (float*)scratch(ctx, "ts_mats", sizeof(float) * 12 * blk);
+    double* part = (double*)scratch(ctx, "ts_part", sizeof(double) * 2 * 256 * blk);
+    float* stdv = (float*)scratch(ctx, "ts_std", sizeof(float) * blk);
+
+    // tomoFou = swapQuadrants .* R2C(chunk); copy first: the FFT input is not
+    // guaranteed to be preserved
+    float* tmp = tpad;
+    EMC_HIP(hipMemcpyAsync(tmp, d_chunk, sizeof(float) * nc, hipMemcpyDeviceToDevice, ctx->stream));
+    fft_r2c(plan1, tmp, tomo);
+    hipLaunchKernelGGL(swapq_kernel, dim3(2048), dim3(256), 0, ctx->stream, tomo, cd[0], cd[1], cd[2]);
+    EMC_HIP(hipGetLastError());
+
+    float* h_par = (float*)pinned(ctx, sizeof(float) * 12 * blk);
+    const float one3[3] = {1.0f, 1.0f, 1.0f};
+    for (int o0 = 0; o0 < n_orient; o0 += blk) {
+      const int nb = std::min(blk, n_orient - o0);
+      // the pinned staging buffer is rewritten below: wait for the previous block
+      EMC_HIP(hipStreamSynchronize(ctx->stream));
+      for (int b = 0; b < nb; ++b) {
+        const double* a = angles_pts + 3 * (o0 + b);
+        const double ang[3] = {a[0], a[1], a[2] - a[0]};  // [phi, theta, psi-phi] (:744-747)
+        double R[9];
+        define_matrix(ang, 0, 0, R);  // 'Bah', 'forward'
+        colmajor_to_rows(R, h_par + 9 * b);
+        for (int k = 0; k < 3; ++k) h_par[9 * blk + 3 * b + k] = one3[k];  // shift [1,1,1]
+      }
+      EMC_HIP(hipMemcpyAsync(mats, h_par, sizeof(float) * 12 * blk, hipMemcpyHostToDevice,
+                             ctx->stream));
+      // rotate into the zero-padded chunk (:744-754)
+      XformArgs xa = xform_args(td, nullptr, 1, one3, 1);
+      xa.ox = cd[0];
+      xa.oy = cd[1];
+      xa.oz = cd[2];
+      xa.px = pad_low[0];
+      xa.py = pad_low[1];
+      xa.pz = pad_low[2];
+      xa.pad_value = 0.0f;
+      launch_xform_batched(ctx, d_template, tpad, mats, mats + 9 * blk, nb, xa);
+      emc_fft_plan* p = nb == blk ? planb : cached_plan(ctx, cd[0], cd[1], cd[2], nb);
+      fft_r2c(p, tpad, T);  // tempFou (:756)
+      hipLaunchKernelGGL(conj_mul_batch_kernel, dim3(1024, nb), dim3(256), 0, ctx->stream,
+                         (const float2*)tomo, T, nh);
+      EMC_HIP(hipGetLastError());
+      fft_c2r(p, T, tpad);  // ccfmap (:759-761), C2R reuses the rotate buffer
+      hipLaunchKernelGGL(valid_sums_kernel, dim3(256, nb), dim3(kRedThreads), 0, ctx->stream,
+                         (const float*)tpad, cd[0], cd[1], cd[2], valid_low[0], valid_low[1],
+                         valid_low[2], vd[0], vd[1], vd[2], part);
+      hipLaunchKernelGGL(std_fold_kernel, dim3(nb), dim3(kRedThreads), 0, ctx->stream,
+                         (const double*)part, 256, (double)nvalid, stdv);
+      hipLaunchKernelGGL(mip_update_kernel, dim3(2048), dim3(256), 0, ctx->stream, (const float*)tpad,
+                         nb, cd[0], cd[1], cd[2], valid_low[0], valid_low[1], valid_low[2], vd[0],
+                         vd[1], vd[2], (const float*)stdv, d_peak, d_angle, first_angle_id + o0,
+                         (init_first && o0 == 0) ? 1 : 0);
+      EMC_HIP(hipGetLastError());
+    }
+  });
+}
+
+}  // extern "C"

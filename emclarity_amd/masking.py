@@ -1,0 +1,109 @@
+"""Weights and spectral helpers: BH_bandpass3d, BH_padZeros3d,
+BH_bandLimitCenterNormalize, mexSF3D, BH_weightMaskMex, mexCTF."""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+from .device import DeviceArray, as_device, default_context
+from .fourier import FFTPlan
+
+
+def BH_bandpass3d(SIZE, HIGH_THRESH, HIGH_CUT, LOW_CUT, METHOD="GPU", PIXEL_SIZE=1.0, half=False,
+                  ctx=None) -> DeviceArray:
+    """masking/BH_bandpass3d.m (numeric pixel size): full grid, FFT order,
+    shape (nz, ny, nx) (or the first nx//2+1 x-planes with half=True)."""
+    ctx = ctx or default_context()
+    nx, ny, nz = (list(SIZE) + [1])[:3]
+    cx = nx // 2 + 1 if half else nx
+    out = DeviceArray((nz, ny, cx), np.float32, ctx)
+    _lib.call("emc_bandpass3d", ctx.handle, _lib.i3((nx, ny, nz)), int(bool(half)),
+              float(HIGH_THRESH), float(HIGH_CUT), float(LOW_CUT), float(PIXEL_SIZE), out.vp)
+    return out
+
+
+def BH_padZeros3d(IMAGE, PADLOW, PADTOP, METHOD="GPU", PRECISION="single", extrapVal=0.0,
+                  ctx=None) -> DeviceArray:
+    """masking/BH_padZeros3d.m: pads (x,y,z) may be negative (trim);
+    PRECISION 'singleTaper' applies the 7-voxel cosine edge."""
+    if PRECISION not in ("single", "singleTaper"):
+        raise ValueError(f"PRECISION must be single or singleTaper, not {PRECISION}")
+    v = as_device(IMAGE, np.float32, ctx)
+    lo = [int(x) for x in PADLOW]
+    hi = [int(x) for x in PADTOP]
+    w = [v.dims[k] - max(-lo[k], 0) - max(-hi[k], 0) for k in range(3)]
+    o = [w[k] + max(lo[k], 0) + max(hi[k], 0) for k in range(3)]
+    out = DeviceArray((o[2], o[1], o[0]), np.float32, v.ctx)
+    _lib.call("emc_pad_zeros3d", v.ctx.handle, v.vp, _lib.i3(v.dims), _lib.i3(lo), _lib.i3(hi),
+              float(extrapVal), int(PRECISION == "singleTaper"), out.vp)
+    return out
+
+
+def BH_bandLimitCenterNormalize(IMG, BANDPASS, MASK, PAD, PRECISION="single",
+                                plan: FFTPlan | None = None, ctx=None) -> DeviceArray:
+    """masking/BH_bandLimitCenterNormalize.m on the half spectrum.
+    BANDPASS: DeviceArray / array (full grid FFT order of the padded size),
+    or None / '' for none. PAD: [[lowx,lowy,lowz],[topx,topy,topz]]."""
+    v = as_device(IMG, np.float32, ctx)
+    pad = np.asarray(PAD, dtype=int).reshape(2, 3)
+    pdims = tuple(int(v.dims[k] + pad[0, k] + pad[1, k]) for k in range(3))
+    plan = plan or FFTPlan(pdims, 1, v.ctx)
+    bp = None
+    if BANDPASS is not None and not isinstance(BANDPASS, str):
+        if np.isscalar(BANDPASS):
+            bp = DeviceArray.from_numpy(np.full(pdims[::-1], BANDPASS, np.float32), ctx=v.ctx)
+        else:
+            bp = as_device(BANDPASS, np.float32, v.ctx)
+    out = DeviceArray((pdims[2], pdims[1], pdims[0] // 2 + 1), np.complex64, v.ctx)
+    _lib.call("emc_band_limit_center_normalize", v.ctx.handle, plan.handle, v.vp, _lib.i3(v.dims),
+              _lib.i3(pad[0]), _lib.i3(pad[1]), bp.vp if bp is not None else None, out.vp)
+    return out
+
+
+def mexSF3D(doHalfGrid, doSqCTF, wantedSize, pixelSize, waveLength, CS, defocus1, defocus2,
+            defocusAst, AmpContrast, nTilts, tiltAngle, exposure, occupancy, launch=1,
+            ctx=None) -> DeviceArray:
+    """SF3D = mexSF3D(...) (mexFiles/mexSF3D.cu:85-303); per-tilt arrays."""
+    ctx = ctx or default_context()
+    nx, ny, nz = (int(s) for s in wantedSize)
+    n = int(nTilts)
+    arrs = [np.asarray(a, np.float32).ravel() for a in (pixelSize, waveLength, CS, defocus1,
+                                                        defocus2, defocusAst, AmpContrast,
+                                                        tiltAngle, exposure, occupancy)]
+    for a in arrs:
+        if a.size < n:
+            raise ValueError("per-tilt arrays shorter than nTilts")
+    out = DeviceArray((nz, ny, nx), np.float32, ctx)
+    _lib.call("emc_sf3d", ctx.handle, int(bool(doHalfGrid)), int(bool(doSqCTF)),
+              _lib.i3((nx, ny, nz)), n, *[_lib.f_arr(a[:n]) for a in arrs], out.vp)
+    return out
+
+
+def BH_weightMaskMex(SIZE, SAMPLING, TLT, xyzSubTomo, reconGeometry, ctx=None) -> DeviceArray:
+    """masking/BH_weightMaskMex.m: TLT (nTilts x 22) -> SF3D - (min - 1e-4).
+    reconGeometry = [[size x,y,z],[shift x,y,z]]."""
+    ctx = ctx or default_context()
+    nx, ny, nz = (int(s) for s in SIZE)
+    T = np.ascontiguousarray(np.asarray(TLT, dtype=np.float64))
+    if T.ndim != 2 or T.shape[1] < 22:
+        raise ValueError("TLT must have 22 columns")
+    T = np.ascontiguousarray(T[:, :22])
+    rg = np.asarray(reconGeometry, dtype=np.float64).reshape(2, 3)
+    out = DeviceArray((nz, ny, nx), np.float32, ctx)
+    _lib.call("emc_weight_mask", ctx.handle, _lib.i3((nx, ny, nz)), float(SAMPLING),
+              _lib.d_arr(T.ravel()), T.shape[0], _lib.d_arr(np.asarray(xyzSubTomo, float).ravel()),
+              _lib.d_arr(rg[0]), _lib.d_arr(rg[1]), out.vp)
+    return out
+
+
+def mexCTF(doHalfGrid, doSqCTF, nX, nY, pixelSize, waveLength, CS, defocus1, defocus2, defocusAst,
+           AmpContrast, centered=False, ctx=None) -> DeviceArray:
+    """ctf = mexCTF(...) (mexFiles/mexCTF.cu:1-121): shape (nY, nx_out)."""
+    ctx = ctx or default_context()
+    nx, ny = int(nX), int(nY)
+    cx = nx // 2 + 1 if doHalfGrid else nx
+    out = DeviceArray((ny, cx), np.float32, ctx)
+    _lib.call("emc_ctf2d", ctx.handle, int(bool(doHalfGrid)), int(bool(doSqCTF)), nx, ny,
+              float(pixelSize), float(waveLength), float(CS), float(defocus1), float(defocus2),
+              float(defocusAst), float(AmpContrast), int(bool(centered)), out.vp)
+    return out

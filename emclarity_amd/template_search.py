@@ -1,0 +1,92 @@
+"""Template search chunk loop of BH_templateSearch3d_2 on the GPU.
+
+``template_search`` follows alignment/BH_templateSearch3d_2.m:397-398
+(symmetric padding of the tomogram), :480-535 (chunk cut-out), :650-982
+(theta -> chunk -> phi -> psi loop with the per-voxel max/argmax update) and
+:985-994 (crop of the padding). The per-chunk work runs in
+emc_template_search_chunk. The tomogram is taken as already preprocessed
+(the reference's per-chunk bandpass and local normalisation, :516-610, are
+not part of this path).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+from .device import DeviceArray, as_device
+from .sizing import BH_multi_padVal
+
+
+def chunk_geometry(size_tomo, size_chunk, valid_area, n_iters, tomo_pre, tomo_post, size_template):
+    """Chunk corners (0-based, padded tomogram), pad of the template into the
+    chunk (padBIN, :372) and the valid-area offset inside a chunk (vA, :403)."""
+    pad_bin = BH_multi_padVal(size_template, size_chunk)
+    vA = BH_multi_padVal(valid_area, size_chunk)
+    cuts = []
+    for ix in range(int(n_iters[0])):
+        for iy in range(int(n_iters[1])):
+            for iz in range(int(n_iters[2])):
+                cuts.append((ix * int(valid_area[0]), iy * int(valid_area[1]), iz * int(valid_area[2])))
+    return cuts, pad_bin[0], vA[0]
+
+
+def symmetric_pad(vol, pre, post):
+    """padarray(..., 'symmetric') pre and post on (z,y,x) arrays, sizes (x,y,z)."""
+    p = [(int(pre[2]), int(post[2])), (int(pre[1]), int(post[1])), (int(pre[0]), int(post[0]))]
+    return np.pad(vol, p, mode="symmetric")
+
+
+def template_search(tomogram, template, angles_pts, size_chunk, valid_area, n_iters, tomo_pre,
+                    tomo_post, ctx=None):
+    """Per-voxel max CC and angle id (1-based into angles_pts, 0 where no
+    orientation scored above 0 in a chunk other than the first, as the
+    reference initialises RESULTS with zeros). Returns (peak, angle) with the
+    tomogram's shape. ``angles_pts`` rows (phi, theta, psi) in the reference
+    order theta -> phi -> psi; blocks of one theta are passed per chunk call."""
+    tomogram = np.asarray(tomogram, dtype=np.float32)
+    template = np.asarray(template, dtype=np.float32)
+    nz, ny, nx = tomogram.shape
+    padded = symmetric_pad(tomogram, tomo_pre, tomo_post)
+    pnz, pny, pnx = padded.shape
+    sc = [int(s) for s in size_chunk]
+    va = [int(s) for s in valid_area]
+    td = (template.shape[2], template.shape[1], template.shape[0])
+    cuts, pad_low, v_low = chunk_geometry((pnx, pny, pnz), sc, va, n_iters, tomo_pre, tomo_post, td)
+    t_dev = as_device(template, np.float32, ctx)
+    ctx = t_dev.ctx
+    pts = np.ascontiguousarray(np.asarray(angles_pts, dtype=np.float64).reshape(-1, 3))
+    # group the flat list by theta, the outer loop of the reference (:650)
+    thetas = []
+    for i, th in enumerate(pts[:, 1]):
+        if not thetas or thetas[-1][0] != th:
+            thetas.append([th, i, i + 1])
+        else:
+            thetas[-1][2] = i + 1
+    chunks = []
+    for (cx, cy, cz) in cuts:
+        sub = padded[cz:cz + sc[2], cy:cy + sc[1], cx:cx + sc[0]]
+        if sub.shape != (sc[2], sc[1], sc[0]):
+            raise ValueError("chunk outside the padded tomogram")
+        chunks.append(DeviceArray.from_numpy(np.ascontiguousarray(sub), ctx=ctx))
+    vshape = (va[2], va[1], va[0])
+    peaks = [DeviceArray.zeros(vshape, np.float32, ctx) for _ in cuts]
+    angs = [DeviceArray.zeros(vshape, np.int32, ctx) for _ in cuts]
+    dp = _lib.C.POINTER(_lib.C.c_double)
+    first = True
+    for (_, a0, a1) in thetas:
+        for k, ch in enumerate(chunks):
+            _lib.call("emc_template_search_chunk", ctx.handle, ch.vp, _lib.i3(sc), t_dev.vp,
+                      _lib.i3(td), _lib.i3(pad_low), _lib.i3(v_low), _lib.i3(va),
+                      pts[a0:a1].ctypes.data_as(dp), int(a1 - a0), int(a0 + 1), int(first and k == 0),
+                      peaks[k].vp, angs[k].vp)
+        first = False
+    res_peak = np.zeros(padded.shape, np.float32)
+    res_ang = np.zeros(padded.shape, np.int32)
+    for (cx, cy, cz), p, a in zip(cuts, peaks, angs):
+        z0, y0, x0 = cz + v_low[2], cy + v_low[1], cx + v_low[0]
+        res_peak[z0:z0 + va[2], y0:y0 + va[1], x0:x0 + va[0]] = p.get()
+        res_ang[z0:z0 + va[2], y0:y0 + va[1], x0:x0 + va[0]] = a.get()
+    pre = [int(v) for v in tomo_pre]
+    res_peak = res_peak[pre[2]:pre[2] + nz, pre[1]:pre[1] + ny, pre[0]:pre[0] + nx]
+    res_ang = res_ang[pre[2]:pre[2] + nz, pre[1]:pre[1] + ny, pre[0]:pre[0] + nx]
+    return res_peak, res_ang

@@ -1,0 +1,277 @@
+/*
+ * emc_hip.h -- C ABI of the MI355X (gfx950) rotate-weight-correlate library.
+ *
+ * Every entry point replaces one MATLAB-facing function of emClarity's
+ * subtomogram-alignment / template-matching hot path; the reference
+ * interface is cited beside each declaration (paths relative to the
+ * emClarity source tree).
+ *
+ * Conventions
+ *   - Volumes are FP32, x fastest: linear index x + nx*(y + ny*z), the same
+ *     order as MATLAB's column-major arrays (mexFiles/mexXform3d.cu:86).
+ *   - dims[3] = {nx, ny, nz}.
+ *   - Half spectra (R2C output) are complex64 interleaved, (nx/2+1) x ny x nz,
+ *     x fastest (mexFiles/mexFFT.cu:70-86, 136-146).
+ *   - Pointers named d_* are device pointers (allocate with emc_malloc or any
+ *     allocator of the same HIP runtime); everything else is host memory.
+ *   - Calls are stream-ordered on the context's stream and return after the
+ *     work is queued unless stated otherwise ("synchronous" entries return
+ *     results to host memory and wait). Calls on one context are not
+ *     thread-safe; use one context per host thread.
+ *   - Errors never exit the process: every call returns an emc_status and
+ *     emc_last_error() gives the message of the last failing call on the
+ *     calling thread (the reference exits MATLAB on a CUDA error,
+ *     mexFiles/include/helper_cuda.h:541-548).
+ */
+#ifndef EMC_HIP_H
+#define EMC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  EMC_OK = 0,
+  EMC_ERR_INVALID = 1,   /* bad argument (null pointer, bad size, bad flag) */
+  EMC_ERR_HIP = 2,       /* HIP runtime error */
+  EMC_ERR_FFT = 3,       /* rocFFT error */
+  EMC_ERR_ALLOC = 4,     /* device allocation failed */
+  EMC_ERR_RANGE = 5,     /* data-dependent range error (e.g. COM box outside map) */
+  EMC_ERR_NODEVICE = 6   /* no HIP device */
+} emc_status;
+
+typedef struct emc_ctx emc_ctx;
+typedef struct emc_fft_plan emc_fft_plan;
+
+/* ---------------------------------------------------------------- context */
+
+const char* emc_last_error(void);
+const char* emc_version(void);
+
+/* Create a context on HIP device `device` with its own default-priority
+ * stream. Replaces the implicit per-MATLAB-process CUDA context
+ * (alignment/BH_alignRaw3d_v2.m:774-776 gpuDevice selection). */
+emc_status emc_ctx_create(int device, emc_ctx** out);
+emc_status emc_ctx_destroy(emc_ctx* ctx);
+emc_status emc_ctx_synchronize(emc_ctx* ctx);
+/* The hipStream_t of the context, as void*, for callers that interleave their
+ * own work on the same stream. */
+void* emc_ctx_stream(emc_ctx* ctx);
+
+/* Device memory helpers so a host without a GPU array type (C harness, MEX
+ * gateway on AMD, ctypes) can drive the library. Copies are synchronous. */
+emc_status emc_malloc(emc_ctx* ctx, size_t bytes, void** d_ptr);
+emc_status emc_free(emc_ctx* ctx, void* d_ptr);
+emc_status emc_memcpy_h2d(emc_ctx* ctx, void* d_dst, const void* src, size_t bytes);
+emc_status emc_memcpy_d2h(emc_ctx* ctx, void* dst, const void* d_src, size_t bytes);
+emc_status emc_memcpy_d2d(emc_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
+emc_status emc_memset_zero(emc_ctx* ctx, void* d_dst, size_t bytes);
+
+/* --------------------------------------------------------- host geometry */
+
+/* BH_defineMatrix (coordinates/BH_defineMatrix.m:51-126), FP64.
+ * convention: 0 Bah (ZXZ), 1 TILT (angles[0] about Y), 2 SPIDER, 3 Helical,
+ * 4 IMOD. direction: 0 'forward', 1 'inv', 2 'forwardVector', 3 'invVector'.
+ * R_colmajor receives the 3x3 matrix in MATLAB column-major order. */
+emc_status emc_define_matrix(const double angles_deg[3], int convention, int direction,
+                             double R_colmajor[9]);
+
+/* eulerSearch grid (coordinates/eulerSearch.m:38-177), random start off.
+ * symmetry: "C<n>", "D<n>", "O", "I". Writes up to `capacity` (phi,theta,psi)
+ * triples in the reference loop order theta->phi->psi and the total count
+ * in *n_out (call with capacity 0 to size the buffer). */
+emc_status emc_euler_search(const char* symmetry, double theta_max, double theta_step,
+                            double psi_max, double psi_step, double* angles_out,
+                            int64_t capacity, int64_t* n_out);
+
+/* ----------------------------------------------------- mexXform3d (A1/A2) */
+
+/* mexXform3d(size, vol, single R[9], single t[3], logical doFwd)
+ * (mexFiles/mexXform3d.cu:93-324, kernel :22-87). Trilinear rotate + shift
+ * about index n/2 with zero outside the volume; output has the input's dims.
+ * fwd != 0: p = R (u - t); fwd == 0: p = R u + t (u = index - n/2).
+ * R is the 9-vector MATLAB passes (column-major), read row-wise. Exact FP32
+ * trilinear weights (the reference's texture unit uses 8-bit fixed-point
+ * weights). d_out must not alias d_in. */
+emc_status emc_xform3d(emc_ctx* ctx, const float* d_in, const int dims[3],
+                       const float R_colmajor[9], const float t[3], int fwd, float* d_out);
+
+/* interpolator.interp3d with symmetry (testScripts/interpolator.m:99-154):
+ * out = (1/nsym) sum_i xform(vol, M * S_i, t, fwd), products in FP64 then
+ * cast to FP32 as the reference does (:110, :163). S holds nsym column-major
+ * 3x3 matrices (nsym <= 60). nsym == 1 with S == NULL means C1. */
+emc_status emc_interp3d(emc_ctx* ctx, const float* d_in, const int dims[3],
+                        const double M_colmajor[9], int nsym, const double* S_colmajor,
+                        const float t[3], int fwd, float* d_out);
+
+/* --------------------------------------------------- mexSF3D / mexCTF (A5-A7) */
+
+/* mexSF3D(half, sqCTF, size, pix[], lambda[], Cs[], df1[], df2[], astig[],
+ * amp[], nTilts, tilt[], exposure[], occupancy[]) (mexFiles/mexSF3D.cu:85-303;
+ * ctf kernel mexFiles/utils/ctf.cu:68-128). Per-tilt arrays are host arrays
+ * of length n_tilts; units as the reference: pixel and wavelength in
+ * Angstrom, Cs in mm, defocus in Angstrom, astigmatism angle in degrees,
+ * amplitude contrast as a fraction, tilt in degrees. d_out: nx*ny*nz FP32,
+ * full grid, centred. All tilts are handled in two launches. */
+emc_status emc_sf3d(emc_ctx* ctx, int half, int sq_ctf, const int dims[3], int n_tilts,
+                    const float* pixel, const float* wavelength, const float* cs_mm,
+                    const float* df1, const float* df2, const float* astig_deg,
+                    const float* amp_contrast, const float* tilt_deg,
+                    const float* exposure, const float* occupancy, float* d_out);
+
+/* BH_weightMaskMex (masking/BH_weightMaskMex.m:1-59): TLT table (n_tilts rows
+ * x 22 columns, row-major doubles as loaded from a *_ctf.tlt file), sampling,
+ * particle centre xyz and recon geometry rows (origin size [3], shift [3])
+ * -> SF3D - (min(SF3D) - 1e-4). */
+emc_status emc_weight_mask(emc_ctx* ctx, const int dims[3], double sampling,
+                           const double* tlt_rowmajor, int n_tilts,
+                           const double xyz_subtomo[3], const double recon_size[3],
+                           const double recon_shift[3], float* d_out);
+
+/* mexCTF(half, sq, nX, nY, pix, lambda, Cs, df1, df2, astig, amp [, centered])
+ * (mexFiles/mexCTF.cu:1-121, kernel mexFiles/utils/ctf.cu:12-65). Output
+ * (half ? nx/2+1 : nx) x ny FP32. The reference gateway accepts occupancy and
+ * exposure but launches the unweighted kernel (mexCTF.cu:101-111); so does
+ * this entry. */
+emc_status emc_ctf2d(emc_ctx* ctx, int half, int sq_ctf, int nx, int ny, float pixel,
+                     float wavelength, float cs_mm, float df1, float df2, float astig_deg,
+                     float amp_contrast, int centered, float* d_out);
+
+/* ------------------------------------------------------------ mexFFT (A8/A9) */
+
+/* mexFFT plan handles (mexFiles/mexFFT.cu:178-210): rank-3 FP32 plans for
+ * dims {nx,ny,nz} and `batch` contiguous volumes, backed by rocFFT. One
+ * handle holds the R2C, C2R and (on demand) C2C plans. No scaling, as the
+ * reference (do_scale=false, :37). */
+emc_status emc_fft_plan_create(emc_ctx* ctx, const int dims[3], int batch, emc_fft_plan** plan);
+emc_status emc_fft_plan_destroy(emc_fft_plan* plan);
+/* Real -> half spectrum (mexFFT real input, :98-100). */
+emc_status emc_fft_r2c(emc_fft_plan* plan, const float* d_in, void* d_out_c64);
+/* Half spectrum -> real (mexFFT complex input, nx_out = 2*nx_half - invTrim,
+ * :98-128). d_in may be overwritten, as cuFFT's C2R does. */
+emc_status emc_fft_c2r(emc_fft_plan* plan, void* d_in_c64, float* d_out);
+/* Full complex 3D FFT (MATLAB gpuArray fftn/ifftn, A9). direction -1 forward
+ * (unscaled), +1 inverse (unscaled; fftn/ifftn's 1/N is the caller's). */
+emc_status emc_fft_c2c(emc_fft_plan* plan, const void* d_in_c64, void* d_out_c64, int direction);
+
+/* ------------------------------------------ spectral helpers (A10-A12, A20) */
+
+/* BH_bandpass3d(SIZE, HIGH_THRESH, HIGH_CUT, LOW_CUT, 'GPU', PIXEL_SIZE)
+ * (masking/BH_bandpass3d.m:54-170), numeric pixel size. Full grid, FFT
+ * order; half != 0 writes only the first nx/2+1 x-planes
+ * (testScripts/fourierTransformer.m:313-319). */
+emc_status emc_bandpass3d(emc_ctx* ctx, const int dims[3], int half, double high_thresh,
+                          double high_cut, double low_cut, double pixel, float* d_out);
+
+/* BH_padZeros3d(IMG, padLow, padTop, 'GPU', precision[, extrapVal])
+ * (masking/BH_padZeros3d.m:37-229). Negative pads trim. taper != 0 applies
+ * the 'singleTaper' 7-voxel cosine edge. d_out has dims in + low + top. */
+emc_status emc_pad_zeros3d(emc_ctx* ctx, const float* d_in, const int dims[3],
+                           const int pad_low[3], const int pad_top[3], float extrap_val,
+                           int taper, float* d_out);
+
+/* BH_bandLimitCenterNormalize(IMG, BANDPASS, '', PAD, 'single')
+ * (masking/BH_bandLimitCenterNormalize.m:27-61) on the half spectrum: pad
+ * with the image mean, R2C, F(0)=0, x bandpass (NULL = none; full-grid FFT
+ * order FP32 volume of the padded size), scale so that the FULL-grid
+ * sum |F|^2 equals numel^2. Means and sums in FP64. `plan` must match the
+ * padded dims (batch 1). d_out: half spectrum of the padded size. */
+emc_status emc_band_limit_center_normalize(emc_ctx* ctx, emc_fft_plan* plan, const float* d_img,
+                                           const int dims[3], const int pad_low[3],
+                                           const int pad_top[3], const float* d_bandpass,
+                                           void* d_out_c64);
+
+/* ref_FT1 = conj(BH_bandLimitCenterNormalize(ref .* volMask, 1, '', padCalc))
+ * (alignment/BH_alignRaw3d_v2.m:607-608), the reference side of the
+ * rotational CC. d_vol_mask may be NULL (mask of ones). `plan` must match the
+ * padded dims. d_out: half spectrum of the padded size. */
+emc_status emc_ref_ft1(emc_ctx* ctx, emc_fft_plan* plan, const float* d_ref, const float* d_vol_mask,
+                       const int dims[3], const int pad_low[3], const int pad_top[3],
+                       void* d_out_c64);
+
+/* ------------------------------------------------ correlation (A13, A14) */
+
+/* BH_multi_xcf_Translational(rotPART_FT, REF_FT, peakMask, PEAK_COM)
+ * (statistics/BH_multi_xcf_Translational.m:1-109). Full-grid complex inputs
+ * (nx*ny*nz complex64, FFT order), peak mask FP32 centred (NULL = none),
+ * com[3] COM half-widths (0 = no COM). Synchronous: writes the shift
+ * (x,y,z) in pixels relative to ceil((N+1)/2) to peak_out[3] and the map
+ * maximum to *max_out (may be NULL). `plan` must match dims.
+ * d_work: 2*nx*ny*nz complex64 scratch. Returns EMC_ERR_RANGE when the
+ * COM box leaves the map, as the reference errors (:64-81). */
+emc_status emc_xcf_translational(emc_ctx* ctx, emc_fft_plan* plan, const void* d_a_c64,
+                                 const void* d_b_c64, const int dims[3], const float* d_peak_mask,
+                                 const int com[3], void* d_work, double peak_out[3],
+                                 double* max_out);
+
+/* BH_multi_xcf_Rotational(rotPART_FT, REF_FT, wdgMask, refWdg, wCCC=0)
+ * (statistics/BH_multi_xcf_Rotational.m:29-80), un-shelled form:
+ * CC = sum Re(Wr P R Wp) / sqrt(sum|R Wp|^2 sum|P Wr|^2) over full-grid
+ * complex64 inputs; weights are FP32 full grids (NULL = 1). FP64 sums.
+ * Synchronous. */
+emc_status emc_xcf_rotational(emc_ctx* ctx, const void* d_p_c64, const void* d_r_c64,
+                              const float* d_wp, const float* d_wr, const int dims[3],
+                              double* cc_out);
+
+/* -------------------------------------------- alignment loop (A15, A17) */
+
+/* One particle's plain grid search, the two-stage body of
+ * alignment/BH_alignRaw3d_v2.m:1119-1419 run on the GPU for every orientation
+ * of `angles_pts` ((phi, theta, psi) triples, degrees, n_orient of them):
+ *   stage 1: rotate the reference by RotMat' (shift = shift_val), the
+ *            reference weight and peak mask; band-limit-normalise both sides;
+ *            translational xcf with the particle wedge -> estPeak (:1282-1344);
+ *   stage 2: rotate the particle (inv, shift estPeak) and its SF3D wedge,
+ *            mask, band-limit-normalise, rotational CC (:1199-1256, :1371-1400).
+ * RotMat = angles_colmajor * BH_defineMatrix([phi,theta,psi-phi],'Bah','inv')
+ * (:1175-1176). Window W^3 = calc C^3 (padCalc zero), C1 symmetry, one
+ * reference. Inputs (device, FP32): particle W^3, ref W^3, ref_wgt_rot C^3
+ * centred, ref_wgt C^3 FFT order (NULL = 1), ref_ft1 half spectrum of C^3
+ * (conj(BH_bandLimitCenterNormalize(ref.*volMask,1,...)), :607), vol_mask W^3,
+ * peak_mask C^3, bandpass C^3 FFT order, wedge = SF3D C^3 centred.
+ * est0: the stage-1 input shift (shiftVAL in the coarse loop :1186, rXYZest
+ * in the refinement :1512, finalrXYZest in the final pass :1706); stage 2
+ * alone (mode 2) uses est0 as the particle shift (the initial-CCC call,
+ * :1222-1232, with angles_pts = {0,0,0}). mode: 1 stage 1 only, 2 stage 2
+ * only, 3 both. The refinement list passes (phi, theta, psi) with the
+ * matrix built from [phi, theta, psi - phi] in every mode (:1489-1515).
+ * Outputs (host): ccc_out[n_orient] (0 without stage 2) and
+ * shift_out[3*n_orient] (estPeak = stage-1 shift + est0, or est0).
+ * Synchronous; EMC_ERR_RANGE if a COM box leaves the map. */
+emc_status emc_align_grid(emc_ctx* ctx, int n, const float* d_particle, const float* d_ref,
+                          const float* d_ref_wgt_rot, const float* d_ref_wgt,
+                          const void* d_ref_ft1_c64, const float* d_vol_mask,
+                          const float* d_peak_mask, const float* d_bandpass, const float* d_wedge,
+                          const double angles_colmajor[9], const double est0[3],
+                          const int peak_com[3], const double* angles_pts, int n_orient, int mode,
+                          double* ccc_out, double* shift_out);
+
+/* ------------------------------------------------- template search (A16) */
+
+/* One tomogram chunk of alignment/BH_templateSearch3d_2.m:713-887 for the
+ * orientations angles_pts[n_orient] ((phi,theta,psi), global ids
+ * first_angle_id .. first_angle_id+n_orient-1):
+ *   tempRot = interp3d(template, [phi,theta,psi-phi], [1,1,1], 'forward');
+ *   zero-pad into the chunk at pad_low; ccf = C2R(swapQuadrants R2C(chunk)
+ *   conj(R2C(tempPAD))) cropped to the valid area [valid_low, valid_low +
+ *   valid_dims); ccf /= std(ccf) (N-1); per voxel, if peak < ccf then
+ *   peak = ccf, angle = id (strict <, :851-854, :883-887).
+ * d_chunk: chunk dims (FP32, already preprocessed); d_template: template
+ * dims; d_peak / d_angle: valid_dims FP32 / int32, updated in place. If
+ * init_first != 0 the first orientation initialises peak = ccf and angle = 1
+ * (the first chunk of a run, :802-823). Stream-ordered. */
+emc_status emc_template_search_chunk(emc_ctx* ctx, const float* d_chunk, const int chunk_dims[3],
+                                     const float* d_template, const int template_dims[3],
+                                     const int pad_low[3], const int valid_low[3],
+                                     const int valid_dims[3], const double* angles_pts,
+                                     int n_orient, int first_angle_id, int init_first,
+                                     float* d_peak, int32_t* d_angle);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EMC_HIP_H */

@@ -1,0 +1,742 @@
+"""FP64 CPU restatement of emClarity's rotate-weight-correlate path.
+
+TEST INFRASTRUCTURE ONLY. This module is the oracle: only ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
+import it, and only as the checker. The product (``emclarity_amd``) never
+imports it and has no CPU fallback.
+
+Every function restates one reference function in float64 numpy and cites
+the reference file:line it follows. Volumes are numpy arrays of shape
+``(nz, ny, nx)`` in C order, so the flat index is ``x + nx*(y + ny*z)``, the
+same linear index as the reference's column-major MATLAB arrays
+(``mexFiles/mexXform3d.cu:86``). Sizes are always given as ``(nx, ny, nz)``.
+
+How each function is pinned (the tests in ``tests/test_oracle_*.py``):
+  define_matrix        closed forms (orthonormal, det=+1, hand-computed 90 deg
+                       cases, inv == forward^T)
+  euler_search         orientation counts computed from the published grid
+                       formula (15,244 / 4,600 / 43,784 / 1,045 / 23,281 / 1,236)
+  xform3d              identity, integer shifts (exact translation), 90 deg
+                       rotations (exact permutations about n//2), brute-force
+                       trilinear on random 5^3/6^3 volumes, fwd/inv inverse pair
+  ctf2d / sf3d         zeros of sin(chi) at analytic k, single 0-deg tilt ==
+                       cosine slab x CTF^2, TLT fixtures from the reference docs
+  bandpass3d           closed form at sampled radii
+  band_limit_center_normalize   F(0)=0 and sum|F|^2 == numel^2
+  xcf_translational    circular shift -> exact integer peak; symmetric peak COM
+  xcf_rotational       CC(x,x)=1, Cauchy-Schwarz, scale invariance
+  align / template search loops   tiny brute-force cases against ground truth
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# --------------------------------------------------------------------------
+# A3  BH_defineMatrix  (coordinates/BH_defineMatrix.m:51-126)
+# --------------------------------------------------------------------------
+
+
+def _rz(a):
+    c, s = math.cos(a), math.sin(a)
+    return np.array([[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]])
+
+
+def _rx(a):
+    c, s = math.cos(a), math.sin(a)
+    return np.array([[1.0, 0.0, 0.0], [0.0, c, -s], [0.0, s, c]])
+
+
+def _ry(a):
+    c, s = math.cos(a), math.sin(a)
+    return np.array([[c, 0.0, s], [0.0, 1.0, 0.0], [-s, 0.0, c]])
+
+
+def define_matrix(angles, convention="Bah", direction="forward"):
+    """Rotation matrix from Euler angles in degrees.
+
+    BH_defineMatrix.m:53 (deg->rad), :85-86 ('forward' negates), :87-102
+    ('inv' reverses the order), :118-126 ('Bah' = Rz(a3) Rx(a2) Rz(a1)),
+    :130-132 ('TILT', one angle about Y), :140-148 SPIDER, :155-163 Helical,
+    :172-180 IMOD.
+    """
+    a = np.atleast_1d(np.asarray(angles, dtype=np.float64)) * (math.pi / 180.0)
+    d = direction.lower()
+    if d in ("forward", "invvector"):
+        a = -a
+    elif d in ("inv", "forwardvector"):
+        a = a[::-1].copy()
+    else:
+        raise ValueError(f"Direction must be forward or inv, not {direction}")
+    if convention == "Bah":
+        return _rz(a[2]) @ _rx(a[1]) @ _rz(a[0])
+    if convention == "TILT":
+        c, s = math.cos(a[0]), math.sin(a[0])
+        return np.array([[c, 0.0, s], [0.0, 1.0, 0.0], [-s, 0.0, c]])
+    if convention == "SPIDER":
+        return _rz(a[2]) @ _ry(a[1]) @ _rz(a[0])
+    if convention == "Helical":
+        return _rz(a[2]) @ _rx(a[1]) @ _ry(a[0])
+    if convention == "IMOD":
+        return _rz(a[2]) @ _ry(a[1]) @ _rz(a[0])
+    raise ValueError(f"Convention must be Bah,SPI,Helical, not {convention}")
+
+
+# --------------------------------------------------------------------------
+# A4  eulerSearch  (coordinates/eulerSearch.m:38-177)
+# --------------------------------------------------------------------------
+
+
+def matlab_colon(a, step, b):
+    """MATLAB ``a:step:b`` element count and values (float)."""
+    if step == 0 or (step > 0 and a > b) or (step < 0 and a < b):
+        return np.zeros(0)
+    n = int(math.floor((b - a) / step + 1e-10)) + 1
+    return a + step * np.arange(n, dtype=np.float64)
+
+
+@dataclass
+class EulerGrid:
+    theta: np.ndarray
+    phi: list
+    psi: np.ndarray
+    theta_step: float
+    psi_step: float
+    n_at_theta: np.ndarray = field(default=None)
+
+    def orientations(self):
+        """Flat (phi, theta, psi) list in the reference loop order theta->phi->psi
+        (BH_alignRaw3d_v2.m:1119-1170, BH_templateSearch3d_2.m:650-734)."""
+        out = []
+        for it, th in enumerate(self.theta):
+            for ph in self.phi[it]:
+                for ps in self.psi:
+                    out.append((ph, th, ps))
+        return np.array(out, dtype=np.float64).reshape(-1, 3)
+
+
+def euler_search(symmetry, theta_max, theta_step, psi_max, psi_step, random_start=False):
+    """Angular grid of eulerSearch (eulerSearch.m:38-124, limits :126-177).
+
+    ``random_start`` must be False for parity: the reference draws it from an
+    unseeded generator (eulerSearch.m:107-111).
+    """
+    bipolar = theta_max < 0
+    theta_max = abs(theta_max)
+    psi_max_full = 2.0 * psi_max                    # :58
+    sym = symmetry[0]
+    if sym == "C":
+        nsym = int(symmetry[1:])
+        psi_max_full = min(psi_max_full, 360.0 / nsym)
+        theta_max = min(180.0, theta_max)
+        phi_max = 360.0
+    elif sym == "D":
+        nsym = int(symmetry[1:])
+        psi_max_full = 360.0 / nsym                  # :143 min() of one value
+        theta_max = min(theta_max, 90.0)
+        phi_max = 360.0
+    elif sym == "O":
+        psi_max_full = min(psi_max_full, 90.0)
+        theta_max = min(theta_max, 54.7)
+        phi_max = 90.0
+    elif sym == "I":
+        psi_max_full = min(psi_max_full, 180.0)
+        theta_max = 31.7
+        phi_max = 180.0
+    else:
+        raise ValueError(symmetry)
+    psi = matlab_colon(-psi_max_full / 2.0, psi_step, psi_max_full / 2.0)  # :73
+    theta = matlab_colon(0.0, theta_step, theta_max)                        # :76
+    if theta.size == 0:
+        theta = np.zeros(1)
+    if bipolar:
+        theta = np.concatenate([theta, (180.0 - theta)[::-1]])
+    phis = []
+    for th in theta:                                                        # :92-116
+        if th == 0.0 or th == 180.0:
+            phi_step = phi_max
+        else:
+            phi_step = abs(theta_step / math.sin(math.radians(th)))
+            if phi_step > phi_max:
+                phi_step = phi_max
+            else:
+                phi_step = phi_max / math.floor(phi_max / phi_step + 0.5)
+        phis.append(matlab_colon(0.0, phi_step, phi_max - 1.0))
+    if psi.size == 0:
+        psi = np.zeros(1)
+    n_at = np.array([len(p) * len(psi) for p in phis])
+    return EulerGrid(theta, phis, psi, theta_step, psi_step, n_at)
+
+
+# --------------------------------------------------------------------------
+# A1/A2  mexXform3d + interpolator  (mexFiles/mexXform3d.cu:45-86,160-172,
+#        293-296; testScripts/interpolator.m:99-171)
+# --------------------------------------------------------------------------
+
+
+def _trilinear_zero(vol, px, py, pz):
+    """Trilinear sample of vol at index coordinates, zero outside [0,n-1]
+    (border addressing, mexXform3d.cu:269-271)."""
+    nz, ny, nx = vol.shape
+    x0 = np.floor(px).astype(np.int64)
+    y0 = np.floor(py).astype(np.int64)
+    z0 = np.floor(pz).astype(np.int64)
+    fx, fy, fz = px - x0, py - y0, pz - z0
+    out = np.zeros(px.shape, dtype=np.float64)
+    for dz in (0, 1):
+        wz = fz if dz else 1.0 - fz
+        zi = z0 + dz
+        okz = (zi >= 0) & (zi < nz)
+        for dy in (0, 1):
+            wy = fy if dy else 1.0 - fy
+            yi = y0 + dy
+            oky = okz & (yi >= 0) & (yi < ny)
+            for dx in (0, 1):
+                wx = fx if dx else 1.0 - fx
+                xi = x0 + dx
+                ok = oky & (xi >= 0) & (xi < nx)
+                v = np.zeros(px.shape, dtype=np.float64)
+                v[ok] = vol[zi[ok], yi[ok], xi[ok]]
+                out += wx * wy * wz * v
+    return out
+
+
+def xform3d(vol, M, t, fwd):
+    """Resample ``vol`` by the 3x3 matrix ``M`` (the matrix the caller passes,
+    read row-wise by the kernel: mexXform3d.cu:170-172) and shift ``t``.
+
+    fwd:  p = M (u - t)   (shift negated :160-164, added before rotation :49-54)
+    inv:  p = M u + t     (:59-64)
+    with u = index - n//2. The texture sample point in index units is
+    p + n//2 for both parities (normalised coords + the even-size half texel,
+    :68-74, :293-296); trilinear, zero outside.
+    """
+    vol = np.asarray(vol, dtype=np.float64)
+    nz, ny, nx = vol.shape
+    M = np.asarray(M, dtype=np.float64).reshape(3, 3)
+    t = np.asarray(t, dtype=np.float64).reshape(3)
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    u = np.stack([x - nx // 2, y - ny // 2, z - nz // 2]).astype(np.float64)
+    if fwd:
+        u = u - t[:, None, None, None]
+        p = np.einsum("ij,j...->i...", M, u)
+    else:
+        p = np.einsum("ij,j...->i...", M, u) + t[:, None, None, None]
+    return _trilinear_zero(vol, p[0] + nx // 2, p[1] + ny // 2, p[2] + nz // 2)
+
+
+def symmetry_matrices(symmetry, convention="Bah"):
+    """C_n operator list (interpolator.m:188-198). Other groups are tables in
+    the reference (:199-393); only C_n is restated here."""
+    if symmetry[0] != "C":
+        raise NotImplementedError(symmetry)
+    n = int(symmetry[1:])
+    return [define_matrix([0.0, 0.0, i * 360.0 / n], convention, "forward") for i in range(n)]
+
+
+def interp3d(vol, angles, shifts, convention="Bah", direction="forward", symmetry="C1"):
+    """interpolator.interp3d (interpolator.m:99-154): Euler triple -> matrix
+    (:159-161), sum over symmetry mates M*S_i (:108-143), / nSym (:145-147)."""
+    a = np.asarray(angles, dtype=np.float64)
+    M = define_matrix(a, convention, direction) if a.size == 3 else a.reshape(3, 3)
+    fwd = direction.lower() == "forward"
+    mats = symmetry_matrices(symmetry, convention)
+    out = None
+    for S in mats:
+        r = xform3d(vol, M @ S, shifts, fwd)
+        out = r if out is None else out + r
+    return out / len(mats) if len(mats) > 1 else out
+
+
+# --------------------------------------------------------------------------
+# A5/A7  CTF (mexFiles/utils/ctf.cu:12-128; include/core_headers.cuh:40-75)
+# --------------------------------------------------------------------------
+
+_EXP_A, _EXP_B, _EXP_C = 0.24499, -0.8325, 2.81410    # ctf.cu:5-8
+
+
+@dataclass
+class CtfParams:
+    """ctfParams constructor (core_headers.cuh:63-75)."""
+    pixel: float
+    wavelength: float
+    cs_mm: float
+    amp: float
+    df1: float
+    df2: float
+    astig_deg: float
+
+    @property
+    def amp_phase(self):
+        return math.atan(self.amp / math.sqrt(1.0 - self.amp ** 2))
+
+    @property
+    def dmean(self):
+        return 0.5 * (self.df1 + self.df2)
+
+    @property
+    def ddiff(self):
+        return 0.5 * (self.df1 - self.df2)
+
+    @property
+    def astig(self):
+        a = self.astig_deg
+        return a * math.pi / 180.0 - math.pi * float(np.rint(a / 180.0))
+
+    @property
+    def cs_term(self):
+        return math.pi * 0.5 * self.cs_mm * 1e7 * self.wavelength ** 3
+
+    @property
+    def df_term(self):
+        return math.pi * self.wavelength
+
+
+def ctf2d(nx, ny, p: CtfParams, half=False, sq=False, centered=True,
+          weighted=False, occupancy=1.0, exposure=0.0, spacing_from_full_nx=True):
+    """2D CTF image, shape (ny, nxh) with nxh = nx//2+1 for half grids
+    (kernels ctf.cu:12-65 plain, :68-128 weighted).
+
+    The Fourier voxel size along x is 1/(pixel*nx) in the mexCTF gateway
+    (mexCTF.cu:44, full nx) and 1/(pixel*ctf_dims.x) in mexSF3D
+    (mexSF3D.cu:148, :234-235, after the half-grid reduction);
+    ``spacing_from_full_nx`` selects between them (equal for full grids)."""
+    cx = nx // 2 + 1 if half else nx
+    o_x = 0 if half else cx // 2
+    o_y = ny // 2
+    fvx = 1.0 / (p.pixel * (nx if spacing_from_full_nx else cx))
+    fvy = 1.0 / (p.pixel * ny)
+    y, x = np.meshgrid(np.arange(ny), np.arange(cx), indexing="ij")
+    if centered:
+        y = y - o_y
+        if not half:
+            x = x - o_x
+    else:
+        y = np.where(y > o_y, y - ny, y)
+        if not half:
+            x = np.where(x > o_x, x - cx, x)
+    ky, kx = y * fvy, x * fvx
+    phi = np.arctan2(ky, kx)
+    r2 = kx * kx + ky * ky
+    chi = p.cs_term * r2 ** 2 - p.df_term * r2 * (p.dmean + p.ddiff * np.cos(2.0 * (phi - p.astig))) - p.amp_phase
+    v = np.sin(chi)
+    if sq:
+        v = v * v
+    if weighted:
+        rw = occupancy * np.where(x == 0, 0.2, np.abs(x).astype(np.float64))
+        with np.errstate(divide="ignore"):
+            pw = np.where(r2 > 0, np.power(np.where(r2 > 0, r2, 1.0), _EXP_B), np.inf)
+        v = v * rw * np.exp((-0.5 * exposure) / (1.0 * (_EXP_A * pw + _EXP_C)))
+    return v
+
+
+def _tex2d_linear_border(img, tu, tv):
+    """tex2D with normalised coords, linear filter, border 0 (texel centre
+    convention x_tex = tu*W - 0.5)."""
+    H, W = img.shape
+    xt = tu * W - 0.5
+    yt = tv * H - 0.5
+    x0 = np.floor(xt).astype(np.int64)
+    y0 = np.floor(yt).astype(np.int64)
+    fx, fy = xt - x0, yt - y0
+    out = np.zeros(np.broadcast(xt, yt).shape)
+    for dy in (0, 1):
+        wy = fy if dy else 1.0 - fy
+        yi = y0 + dy
+        for dx in (0, 1):
+            wx = fx if dx else 1.0 - fx
+            xi = x0 + dx
+            ok = (xi >= 0) & (xi < W) & (yi >= 0) & (yi < H)
+            v = np.zeros(out.shape)
+            xb, yb = np.broadcast_arrays(xi, yi)
+            okb = np.broadcast_to(ok, out.shape)
+            v[okb] = img[yb[okb], xb[okb]]
+            out += wx * wy * v
+    return out
+
+
+def sf3d(size, tilts, half=False, sq=True):
+    """mexSF3D: per-tilt weighted 2D CTF (ctf.cu:68-128) back-projected as a
+    cosine slab of half-thickness 4 px about the tilted plane and summed over
+    tilts (mexSF3D.cu:25-79 kernel, :125-277 gateway).
+
+    ``tilts`` is a list of dicts with keys pixel, wavelength, cs, df1, df2,
+    astig, amp, tilt, exposure, occupancy (one per tilt, the gateway's
+    per-tilt arrays :104-114). Returns (nz, ny, nx).
+    """
+    nx, ny, nz = (int(s) for s in size)
+    out = np.zeros((nz, ny, nx))
+    ssx = 0.5 if nx % 2 == 0 else 0.0
+    ssy = 0.5 if ny % 2 == 0 else 0.0
+    ssz = 0.5 if nz % 2 == 0 else 0.0
+    cx = nx // 2 + 1 if half else nx
+    for t in tilts:
+        p = CtfParams(t["pixel"], t["wavelength"], t["cs"], t["amp"], t["df1"], t["df2"], t["astig"])
+        img = ctf2d(nx, ny, p, half=half, sq=sq, centered=True, weighted=True,
+                    occupancy=t["occupancy"], exposure=t["exposure"], spacing_from_full_nx=False)
+        s = math.sin(math.radians(-t["tilt"]))                              # :134
+        c = math.cos(math.radians(-t["tilt"]))
+        x = np.arange(nx, dtype=np.float64)
+        y = np.arange(ny, dtype=np.float64)
+        z = np.arange(nz, dtype=np.float64)
+        u = x - nx / 2.0                                                    # :47 float division
+        tw_pre = -u * s + ssz                                               # :48
+        tv = (y - (ny // 2) + ssy) / ny + 0.5                               # :49
+        w = z - (nz // 2)                                                   # :53
+        tw = tw_pre[None, :] + w[:, None] * c                               # (nz, nx) :54
+        keep = ~((tw < -4.0) | (tw > 4.0))                                  # :56
+        zw = (0.5 + 0.5 * np.cos(tw * (math.pi / 4.0))) * 0.25              # :62
+        tu = (u[None, :] * c + w[:, None] * s + ssx) / nx + 0.5             # :64-72
+        # sample the CTF texture for every (z, y, x)
+        samp = _tex2d_linear_border(img, tu[:, None, :], tv[None, :, None])
+        out += np.where(keep[:, None, :], zw[:, None, :] * samp, 0.0)
+    return out
+
+
+def tlt_to_tilts(TLT, sampling, xyz_subtomo, recon_geometry, size):
+    """BH_weightMaskMex marshalling (masking/BH_weightMaskMex.m:7-51): TLT rows
+    (22 columns) -> per-tilt SF3D arguments, with the particle-height defocus
+    offset (:33-37) and occupancy = dose fraction x elastic fraction (:41-44)."""
+    TLT = np.asarray(TLT, dtype=np.float64)
+    TLT = TLT[np.argsort(TLT[:, 0], kind="stable")]                         # :12 sortrows(TLT,1)
+    tilt = TLT[:, 3]
+    pixel = TLT[:, 15] * sampling * 1e10
+    recon_shift = np.asarray(recon_geometry[1], dtype=np.float64)
+    origin = np.ceil((np.asarray(recon_geometry[0][:3], dtype=np.float64) + 1.0) / 2.0)
+    prj = np.asarray(xyz_subtomo, dtype=np.float64) - origin + recon_shift
+    cs = TLT[:, 16] * 1e3
+    wl = TLT[:, 17] * 1e10
+    phase = TLT[:, 18]
+    defocus = -TLT[:, 14] * 1e10
+    ddf = TLT[:, 11] * 1e10
+    dphi = TLT[:, 12] * 180.0 / math.pi
+    exposure = TLT[:, 10]
+    for i in range(TLT.shape[0]):
+        r = define_matrix([TLT[i, 3]], "TILT", "forwardVector")
+        pc = r @ prj
+        defocus[i] = defocus[i] - pc[2] * pixel[i]
+    dose = TLT[:, 13] / TLT[:, 13].mean()
+    elastic = np.exp(-75.0 / (np.cos(np.radians(TLT[:, 3])) * 400.0))
+    elastic = elastic / elastic.max()
+    occ = elastic * dose
+    return [dict(pixel=pixel[i], wavelength=wl[i], cs=cs[i], df1=defocus[i] + ddf[i],
+                 df2=defocus[i] - ddf[i], astig=dphi[i], amp=phase[i], tilt=tilt[i],
+                 exposure=exposure[i], occupancy=occ[i]) for i in range(TLT.shape[0])]
+
+
+def weight_mask_mex(size, sampling, TLT, xyz_subtomo, recon_geometry):
+    """BH_weightMaskMex (:1-59): SF3D then shift so min == 1e-4 (:55)."""
+    w = sf3d(size, tlt_to_tilts(TLT, sampling, xyz_subtomo, recon_geometry, size), half=False, sq=True)
+    return w - (w.min() - 1e-4)
+
+
+# --------------------------------------------------------------------------
+# A10 BH_bandpass3d (masking/BH_bandpass3d.m:54-170) and A20 radial grid
+# --------------------------------------------------------------------------
+
+
+def fft_radius(size):
+    """Radial frequency grid in FFT order, cycles/pixel
+    (BH_multi_gridCoordinates.m:209-215, :239-243, :303)."""
+    nx, ny, nz = size
+    def ax(n):
+        return np.concatenate([np.arange(0, n // 2 + 1), -np.arange((n - 1) // 2, 0, -1)]) / n
+    z, y, x = np.meshgrid(ax(nz), ax(ny), ax(nx), indexing="ij")
+    return np.sqrt(x * x + y * y + z * z)
+
+
+def bandpass3d(size, high_thresh, high_cut, low_cut, pixel):
+    """BH_bandpass3d for a numeric pixel size (:62-64, :102-115, :124-170).
+    Returns (nz, ny, nx) in FFT order."""
+    size = tuple(int(s) for s in size)
+    if not (0 <= high_thresh < 1):
+        raise ValueError("HIGH_THRESH must be between 0 and 1")
+    hc = pixel / high_cut if high_cut != 0 else 0.0
+    lc = pixel / low_cut
+    if lc > 0:
+        lroll = 2.0 / (min(size[:2]) if size[2] == 1 else min(size))
+    else:
+        lc = abs(lc)
+        lroll = math.sqrt(-((0.5 - lc) ** 2) / (2.0 * math.log(1e-3)))
+    hroll = math.sqrt(-(hc ** 2) / (2.0 * math.log(high_thresh))) if high_cut != 0 else 0.0
+    r = fft_radius(size)
+    g = lambda x, m, s: np.exp(-((x - m) ** 2) / (2.0 * s * s))
+    w = np.where(r < lc, 1.0, g(r, lc, lroll))
+    if hc != 0:
+        w = np.where(r <= hc, g(r, hc, hroll), w)
+    return w
+
+
+# --------------------------------------------------------------------------
+# A12 BH_padZeros3d (masking/BH_padZeros3d.m:37-229), A17 padVal
+# --------------------------------------------------------------------------
+
+
+def pad_val(size1, size2):
+    """BH_multi_padVal (:7-22): rows [fwdLow; fwdHigh; invLow; invHigh]."""
+    s1 = np.asarray(size1, dtype=np.int64)
+    s2 = np.asarray(size2, dtype=np.int64)
+    d = s2 - s1
+    od, oi = d % 2, s1 % 2
+    fwd = np.stack([d // 2 + oi * od, d // 2 + (1 - oi) * od])
+    d = s1 - s2
+    od, oi = d % 2, s2 % 2
+    inv = np.stack([d // 2 + oi * od, d // 2 + (1 - oi) * od])
+    return np.concatenate([fwd, inv])
+
+
+_TAPER = 0.5 + 0.5 * np.cos(np.arange(7) * math.pi / 7.0)        # :110-117
+
+
+def pad_zeros3d(img, pad_low, pad_top, extrap_val=0.0, taper=False):
+    """Pad/trim an (nz,ny,nx) volume; pads are given as (x,y,z).
+    Negative = trim (:79-93); constant fill value (:59-69, :154-156);
+    'singleTaper' 7-px cosine edge (:189-205)."""
+    img = np.array(img, dtype=np.float64, copy=True)
+    pl = np.asarray(pad_low, dtype=np.int64)
+    pt = np.asarray(pad_top, dtype=np.int64)
+    tl = np.where(pl < 0, -pl, 0)
+    tt = np.where(pt < 0, -pt, 0)
+    pl = np.where(pl >= 0, pl, 0)
+    pt = np.where(pt >= 0, pt, 0)
+    nz, ny, nx = img.shape
+    img = img[tl[2]:nz - tt[2], tl[1]:ny - tt[1], tl[0]:nx - tt[0]].copy()
+    if taper:
+        ft = _TAPER[::-1]
+        ev = extrap_val
+        # order follows :190-205: y-low, x-low, z-low, y-high, x-high, z-high
+        img[:, :7, :] = img[:, :7, :] * ft[None, :, None] + (ev * (1 - _TAPER))[::-1][None, :, None]
+        img[:, :, :7] = img[:, :, :7] * ft[None, None, :] + (ev * (1 - _TAPER))[::-1][None, None, :]
+        img[:7, :, :] = img[:7, :, :] * ft[:, None, None] + (ev * (1 - _TAPER))[::-1][:, None, None]
+        img[:, -7:, :] = img[:, -7:, :] * _TAPER[None, :, None] + (ev * (1 - _TAPER))[None, :, None]
+        img[:, :, -7:] = img[:, :, -7:] * _TAPER[None, None, :] + (ev * (1 - _TAPER))[None, None, :]
+        img[-7:, :, :] = img[-7:, :, :] * _TAPER[:, None, None] + (ev * (1 - _TAPER))[:, None, None]
+    nz, ny, nx = img.shape
+    out = np.full((nz + pl[2] + pt[2], ny + pl[1] + pt[1], nx + pl[0] + pt[0]), float(extrap_val))
+    out[pl[2]:pl[2] + nz, pl[1]:pl[1] + ny, pl[0]:pl[0] + nx] = img
+    return out
+
+
+# --------------------------------------------------------------------------
+# A11 BH_bandLimitCenterNormalize (masking/BH_bandLimitCenterNormalize.m:27-61)
+# --------------------------------------------------------------------------
+
+
+def band_limit_center_normalize(img, bandpass, pad):
+    """pad with mean(img) (:48) -> fftn -> F(0)=0 (:51) -> x bandpass (:55)
+    -> F / (sqrt(sum|F|^2)/numel) (:61). ``bandpass`` None means none;
+    ``pad`` is [[lowx,lowy,lowz],[topx,topy,topz]]."""
+    img = np.asarray(img, dtype=np.float64)
+    p = pad_zeros3d(img, pad[0], pad[1], extrap_val=float(img.mean()))
+    F = np.fft.fftn(p)
+    F.flat[0] = 0.0
+    if bandpass is not None:
+        F = F * bandpass
+    return F / (math.sqrt(float(np.sum(np.abs(F) ** 2))) / F.size)
+
+
+# --------------------------------------------------------------------------
+# A13 BH_multi_xcf_Translational (statistics/BH_multi_xcf_Translational.m:31-103)
+# --------------------------------------------------------------------------
+
+
+def matlab_round(x):
+    return np.where(np.asarray(x) >= 0, np.floor(np.asarray(x) + 0.5), np.ceil(np.asarray(x) - 0.5))
+
+
+def xcf_translational(A, B, peak_mask, peak_com, return_map=False):
+    """Returns the 1x3 shift (x,y,z) in pixels. ``peak_mask`` None = no mask."""
+    M = np.fft.fftshift(np.real(np.fft.ifftn(A * B)))                  # :31
+    min_val = M.min()                                                   # :33
+    if peak_mask is not None:
+        M = np.where(np.asarray(peak_mask) < 0.95, min_val, M)          # :34
+    nz, ny, nx = M.shape
+    idx = int(np.argmax(M.ravel()))                                     # :40 first max
+    max_val = M.ravel()[idx]
+    z, rem = divmod(idx, nx * ny)
+    y, x = divmod(rem, nx)
+    peak = np.array([x + 1, y + 1, z + 1], dtype=np.float64)           # :41-42 1-based
+    com = np.asarray(peak_com, dtype=np.int64).reshape(3)
+    if (not np.isnan(max_val)) and max_val != 0:                        # :45
+        for icom in (1, 2):                                             # :47
+            if np.all(com != 0):                                        # :49
+                pc = peak if icom == 1 else matlab_round(peak)          # :50-55
+                lo = (pc - com).astype(np.int64)
+                hi = (pc + com).astype(np.int64)
+                if np.any(lo < 1) or np.any(hi > np.array([nx, ny, nz])):
+                    raise IndexError("failed to box out COM calc")       # :64-81
+                box = M[lo[2] - 1:hi[2], lo[1] - 1:hi[1], lo[0] - 1:hi[0]] - min_val
+                gz, gy, gx = np.meshgrid(np.arange(-com[2], com[2] + 1), np.arange(-com[1], com[1] + 1),
+                                         np.arange(-com[0], com[0] + 1), indexing="ij")
+                s = box.sum()
+                cm = np.array([(box * gx).sum(), (box * gy).sum(), (box * gz).sum()]) / s   # :84-86
+                peak = pc + cm                                          # :88
+    else:
+        peak = np.zeros(3)                                              # :95
+    peak = peak - np.ceil((np.array([nx, ny, nz]) + 1) / 2.0)           # :97
+    peak = np.where(np.isnan(peak), 0.0, peak)                          # :99-103
+    if return_map:
+        return peak, M, max_val
+    return peak
+
+
+# --------------------------------------------------------------------------
+# A14 BH_multi_xcf_Rotational (statistics/BH_multi_xcf_Rotational.m:29-80)
+# --------------------------------------------------------------------------
+
+
+def xcf_rotational(P, R, Wp, Wr):
+    """CC = sum Re(Wr P R Wp) / sqrt(sum|R Wp|^2 * sum|P Wr|^2) (:73-78);
+    NaN first element of a weight -> weight 1 (:34-40)."""
+    Wr = np.asarray(Wr, dtype=np.float64)
+    Wp = np.asarray(Wp, dtype=np.float64)
+    if np.isnan(Wr.flat[0]):
+        Wr = np.float64(1.0)
+    if np.isnan(Wp.flat[0]):
+        Wp = np.float64(1.0)
+    num = np.sum(np.real(Wr * P * R * Wp))
+    den = np.sum(np.abs(R * Wp) ** 2) * np.sum(np.abs(P * Wr) ** 2)
+    return float(num / math.sqrt(den))
+
+
+# --------------------------------------------------------------------------
+# A15 the alignRaw per-particle loop (alignment/BH_alignRaw3d_v2.m:1119-1765)
+# --------------------------------------------------------------------------
+
+
+@dataclass
+class AlignInputs:
+    """Everything the per-particle loop reads (BH_alignRaw3d_v2.m:800-1110).
+    Volumes are (nz,ny,nx). W = window edge, C = calc edge."""
+    particle: np.ndarray            # iparticle after padZeros3d 'singleTaper' (W^3)
+    ref: np.ndarray                 # ref_FT2 (untrimmed reference, W^3)   :1108
+    ref_wgt_rot: np.ndarray         # refWgtROT (C^3 centred)              :1109
+    ref_wgt: object                 # refWGT (C^3 FFT order) or scalar     :1385
+    ref_ft1: np.ndarray             # conj(bandLimitCenterNormalize(ref*volMask,1,...)) :607
+    vol_mask: np.ndarray            # volMask (W^3)
+    peak_mask: np.ndarray           # peakMask (C^3)
+    bandpass: np.ndarray            # bandpassFilt (C^3 FFT order)
+    wedge: np.ndarray               # SF3D from BH_weightMaskMex (C^3 centred) :1093
+    angles: np.ndarray              # particle's current 3x3 matrix (col-major 9 in geometry)
+    shift_val: np.ndarray           # BH_isWindowValid SHIFTS (:1042)
+    pad_calc: np.ndarray            # [[low],[top]] W->C
+    peak_com: tuple = (3, 3, 3)
+    symmetry: str = "C1"
+
+
+def ref_ft1_of(ref, vol_mask, pad_calc):
+    """ref_FT1 = conj(bandLimitCenterNormalize(ref.*volMask, 1, ...)) (:607-608)."""
+    return np.conj(band_limit_center_normalize(ref * vol_mask, None, pad_calc))
+
+
+def align_orientation(inp: AlignInputs, rot_mat, est_peak0):
+    """One orientation of the two-stage body (:1182-1402). Returns
+    (ccc, est_peak) where est_peak is the stage-1 shift + est_peak0."""
+    R = rot_mat
+    # stage 1: rotate the reference (:1287-1336)
+    trim = inp.particle                                    # padWindow == 0
+    i_rot_ref = xform3d(inp.ref, R.T, est_peak0, True)
+    if inp.ref_wgt_rot is None:                            # no reference weight: 1
+        i_rot_wdg = np.ones(inp.ref.shape)
+    else:
+        i_rot_wdg = xform3d(inp.ref_wgt_rot, R.T, [0, 0, 0], True)
+    i_rot_mask = xform3d(inp.peak_mask, R.T, [0, 0, 0], True)
+    i_rot_ref = band_limit_center_normalize(i_rot_ref, None, inp.pad_calc)
+    rot_part = band_limit_center_normalize(trim, inp.bandpass, inp.pad_calc)
+    wedge_ifft = np.fft.ifftshift(inp.wedge)
+    peak = xcf_translational(rot_part * np.fft.ifftshift(i_rot_wdg), np.conj(i_rot_ref) * wedge_ifft,
+                             i_rot_mask, inp.peak_com)
+    est = peak + np.asarray(est_peak0, dtype=np.float64)   # :1341-1344
+    # stage 2: rotate the particle (:1199-1256, :1375-1387)
+    part = interp3d(inp.particle, R, est, "Bah", "inv", inp.symmetry)
+    wmask = interp3d(inp.wedge, R, [0, 0, 0], "Bah", "inv", inp.symmetry)
+    rot_part2 = band_limit_center_normalize(part * inp.vol_mask, inp.bandpass, inp.pad_calc)
+    ccc = xcf_rotational(rot_part2, inp.ref_ft1, np.fft.ifftshift(wmask), inp.ref_wgt)
+    return ccc, est
+
+
+def align_initial(inp: AlignInputs):
+    """getInitialCCC (:1222-1232, :1355-1369): CCC at the current pose."""
+    A = np.asarray(inp.angles, dtype=np.float64).reshape(3, 3)
+    part = interp3d(inp.particle, A, inp.shift_val, "Bah", "inv", inp.symmetry)
+    wmask = interp3d(inp.wedge, A, [0, 0, 0], "Bah", "inv", inp.symmetry)
+    rp = band_limit_center_normalize(part * inp.vol_mask, inp.bandpass, inp.pad_calc)
+    return xcf_rotational(rp, inp.ref_ft1, np.fft.ifftshift(wmask), inp.ref_wgt)
+
+
+def align_grid(inp: AlignInputs, grid_angles):
+    """Coarse grid (:1119-1419) over an explicit (phi, theta, psi) list with
+    the azimuthal randomiser zeroed. Returns (ccc[n], shifts[n,3])."""
+    A = np.asarray(inp.angles, dtype=np.float64).reshape(3, 3)
+    n = len(grid_angles)
+    ccc = np.zeros(n)
+    est = np.zeros((n, 3))
+    for i, (phi, theta, psi) in enumerate(grid_angles):
+        R = A @ define_matrix([phi, theta, psi - phi], "Bah", "inv")   # :1175-1176
+        ccc[i], est[i] = align_orientation(inp, R, inp.shift_val)
+    return ccc, est
+
+
+def sortrows_desc_first(values):
+    """Index of the row sortrows(...,-6) puts first: max value, earliest on
+    ties (MATLAB sortrows is stable)."""
+    v = np.asarray(values)
+    return int(np.flatnonzero(v == v.max())[0])
+
+
+# --------------------------------------------------------------------------
+# A16 template search (alignment/BH_templateSearch3d_2.m:632-982)
+# --------------------------------------------------------------------------
+
+
+def swap_quadrants_half(size):
+    """exp(-2 pi i (dU (nx//2+1) + dV (ny//2+1) + dW (nz//2+1))) on the half
+    grid (:632-641), dU etc. centred frequency grids (origin at n//2)."""
+    nx, ny, nz = size
+    def ax(n):
+        return np.arange(-(n // 2), (n - 1) // 2 + 1) / n
+    z, y, x = np.meshgrid(ax(nz), ax(ny), ax(nx), indexing="ij")
+    sq = np.exp(-2j * np.pi * (x * (nx // 2 + 1) + y * (ny // 2 + 1) + z * (nz // 2 + 1)))
+    return sq[:, :, : nx // 2 + 1]
+
+
+def matlab_std(x):
+    x = np.asarray(x, dtype=np.float64).ravel()
+    return float(np.std(x, ddof=1))
+
+
+def template_search_chunk(tomo_chunk, template, grid_angles, pad_bin, trim_valid,
+                          prev_peak=None, prev_angle=None, first_angle_id=1, init_from_first=True):
+    """Per chunk, per orientation (:713-887): tempRot = interp3d(template,
+    [phi,theta,psi-phi], [1,1,1], 'forward') (:744-747), zero-pad into the
+    chunk (:751-754), ccf = trim(real(C2R(swapQ R2C(chunk) conj(R2C(tempPAD)))))
+    (:759-761), / std (:764), strict-< max/argmax update (:851-854, :883-887).
+
+    Returns (peak, angle_id) over the valid area, angle ids 1-based."""
+    nz, ny, nx = tomo_chunk.shape
+    size = (nx, ny, nz)
+    tomo_fou = swap_quadrants_half(size) * np.fft.rfftn(tomo_chunk)
+    mag = None if prev_peak is None else prev_peak.copy()
+    ang = None if prev_angle is None else prev_angle.copy()
+    aid = first_angle_id
+    for (phi, theta, psi) in grid_angles:
+        trot = interp3d(template, [phi, theta, psi - phi], [1, 1, 1], "Bah", "forward")
+        tpad = pad_zeros3d(trot, pad_bin[0], pad_bin[1])
+        temp_fou = np.conj(np.fft.rfftn(tpad))
+        ccf = np.fft.irfftn(tomo_fou * temp_fou, s=(nz, ny, nx)) * (nx * ny * nz)   # unscaled C2R
+        ccf = pad_zeros3d(ccf, trim_valid[0], trim_valid[1])          # padVal rows: negative = trim
+        ccf = ccf / matlab_std(ccf)
+        if mag is None and init_from_first:
+            mag = ccf.copy()
+            ang = np.ones(ccf.shape)
+        else:
+            if mag is None:
+                mag = np.zeros(ccf.shape)
+                ang = np.zeros(ccf.shape)
+            rep = mag < ccf
+            mag = np.where(rep, ccf, mag)
+            ang = np.where(rep, aid, ang)
+        aid += 1
+    return mag, ang

@@ -1,0 +1,184 @@
+"""GPU parity of every drop-in primitive against the FP64 oracle, through the
+C ABI. Tolerances are stated per test; FP32 device arithmetic vs FP64."""
+import math
+
+import numpy as np
+import pytest
+
+from emclarity_amd import _lib
+from emclarity_amd.device import DeviceArray
+from emclarity_amd.fourier import FFTPlan, fourierTransformer, mexFFT
+from emclarity_amd.geometry import BH_defineMatrix
+from emclarity_amd.interpolator import interpolator, mexXform3d
+from emclarity_amd.masking import (BH_bandLimitCenterNormalize, BH_bandpass3d, BH_padZeros3d,
+                                   BH_weightMaskMex, mexCTF, mexSF3D)
+from emclarity_amd.xcf import BH_multi_xcf_Rotational, BH_multi_xcf_Translational
+from oracle import emc_oracle as O
+from tests.conftest import load_tlt
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
+
+
+# ------------------------------------------------------------ rotate (A1/A2)
+
+@pytest.mark.parametrize("n,fwd", [(32, True), (32, False), (33, True), (20, False)])
+def test_xform3d_matches_oracle(ctx, rng, n, fwd):
+    v = rng.standard_normal((n, n + 2, n - 1)).astype(np.float32)
+    M = BH_defineMatrix(rng.uniform(-180, 180, 3), "Bah", "forward")
+    t = rng.uniform(-3, 3, 3)
+    out = mexXform3d(v.shape[::-1], v, M, t, fwd).get()
+    ref = O.xform3d(v, M.astype(np.float32).astype(np.float64), t.astype(np.float32), fwd)
+    # FP32 coordinates: sample position error ~1e-5 px
+    assert np.max(np.abs(out - ref)) < 2e-4
+
+
+def test_xform3d_exact_cases(ctx, rng):
+    v = rng.standard_normal((16, 16, 16)).astype(np.float32)
+    # identity and integer shift are exact
+    assert np.array_equal(mexXform3d((16, 16, 16), v, np.eye(3), [0, 0, 0], True).get(), v)
+    s = mexXform3d((16, 16, 16), v, np.eye(3), [2, 0, 0], True).get()
+    assert np.array_equal(s[:, :, 2:], v[:, :, :-2]) and not s[:, :, :2].any()
+    # 90 degrees about z: exact permutation about n//2
+    Rz = np.array([[0, -1, 0], [1, 0, 0], [0, 0, 1.0]])
+    r = mexXform3d((16, 16, 16), v, Rz, [0, 0, 0], True).get()
+    assert np.array_equal(r, O.xform3d(v, Rz, [0, 0, 0], True).astype(np.float32))
+
+
+def test_interpolator_symmetry(ctx, rng):
+    v = rng.standard_normal((24, 24, 24)).astype(np.float32)
+    ip = interpolator(v, [0, 0, 0], [0, 0, 0], "Bah", "forward", "C1")
+    ang, sh = [12.0, 33.0, -70.0], [0.5, -1.0, 0.25]
+    out = ip.interp3d(ang, sh, "Bah", "inv", "C3").get()
+    ref = O.interp3d(v, ang, sh, "Bah", "inv", "C3")
+    assert np.max(np.abs(out - ref)) < 2e-4
+
+
+# ------------------------------------------------------------ CTF / SF3D (A5-A7)
+
+@pytest.mark.parametrize("half,centered,sq", [(False, True, True), (True, False, False),
+                                              (False, False, True), (True, True, False)])
+def test_mexctf_matches_oracle(ctx, half, centered, sq):
+    args = dict(pixelSize=2.17, waveLength=0.01968697, CS=2.7, defocus1=31000.0, defocus2=27000.0,
+                defocusAst=-60.0, AmpContrast=0.085)
+    out = mexCTF(half, sq, 64, 48, centered=centered, **args).get()
+    p = O.CtfParams(2.17, 0.01968697, 2.7, 0.085, 31000.0, 27000.0, -60.0)
+    ref = O.ctf2d(64, 48, p, half=half, sq=sq, centered=centered, spacing_from_full_nx=True)
+    assert out.shape == ref.shape and np.max(np.abs(out - ref)) < 2e-5
+
+
+def test_mexsf3d_matches_oracle(ctx):
+    tilts = [dict(pixel=2.5, wavelength=0.019687, cs=2.7, df1=30000.0 + 50 * t, df2=29000.0 + 50 * t,
+                  astig=15.0, amp=0.1, tilt=t, exposure=3.0 * abs(t) / 3 + 1.5,
+                  occupancy=math.cos(math.radians(t))) for t in np.arange(-60, 61, 6.0)]
+    n = 32
+    keys = ["pixel", "wavelength", "cs", "df1", "df2", "astig", "amp", "tilt", "exposure", "occupancy"]
+    arr = {k: np.array([t[k] for t in tilts], np.float32) for k in keys}
+    out = mexSF3D(False, True, (n, n, n), arr["pixel"], arr["wavelength"], arr["cs"], arr["df1"],
+                  arr["df2"], arr["astig"], arr["amp"], len(tilts), arr["tilt"], arr["exposure"],
+                  arr["occupancy"]).get()
+    ft = [{k: float(np.float32(t[k])) for k in keys} for t in tilts]
+    ref = O.sf3d((n, n, n), ft, half=False, sq=True)
+    assert rel(out, ref) < 1e-4
+
+
+def test_weight_mask_on_reference_tlt(ctx):
+    T = load_tlt("riboNew05_ali1_ctf.tlt")
+    size, xyz, rg = (32, 32, 32), [812.4, 1101.0, 150.2], [[3838, 3710, 300], [0, 0, 0]]
+    out = BH_weightMaskMex(size, 4, T, xyz, rg).get()
+    ref = O.weight_mask_mex(size, 4, T, xyz, rg)
+    assert rel(out, ref) < 1e-4
+    assert abs(float(out.min()) - 1e-4) < 1e-6
+
+
+# ------------------------------------------------------ spectral (A8-A12)
+
+def test_bandpass_matches_oracle(ctx):
+    out = BH_bandpass3d((48, 40, 32), 1e-3, 40.0, 8.0, "GPU", 2.5).get()
+    ref = O.bandpass3d((48, 40, 32), 1e-3, 40.0, 8.0, 2.5)
+    assert np.max(np.abs(out - ref)) < 1e-6
+    half = BH_bandpass3d((48, 40, 32), 1e-3, 40.0, 8.0, "GPU", 2.5, half=True).get()
+    assert np.array_equal(half, out[..., :25])
+
+
+def test_pad_zeros3d_matches_oracle(ctx, rng):
+    img = rng.standard_normal((20, 18, 22)).astype(np.float32)
+    out = BH_padZeros3d(img, [2, -1, 3], [-2, 4, 0], "GPU", "single", 0.75).get()
+    assert np.array_equal(out, O.pad_zeros3d(img, [2, -1, 3], [-2, 4, 0], 0.75).astype(np.float32))
+    out = BH_padZeros3d(img, [1, 1, 1], [1, 1, 1], "GPU", "singleTaper", 0.0).get()
+    assert np.max(np.abs(out - O.pad_zeros3d(img, [1, 1, 1], [1, 1, 1], 0.0, taper=True))) < 1e-6
+
+
+def test_fft_roundtrip_and_numpy(ctx, rng):
+    v = rng.standard_normal((24, 20, 32)).astype(np.float32)
+    F, plan = mexFFT(DeviceArray.from_numpy(v), 2)
+    Fh = F.get()
+    ref = np.fft.rfftn(v.astype(np.float64))
+    assert rel(Fh, ref) < 1e-5
+    back, _ = mexFFT(F, 2, plan)
+    assert rel(back.get() / v.size, v) < 1e-5
+    ft = fourierTransformer(v)
+    assert rel(ft.invFFT(ft.fwdFFT(v)).get() / v.size, v) < 1e-5
+    full = DeviceArray.from_numpy(np.fft.fftn(v).astype(np.complex64))
+    inv = plan.c2c(full, +1).get() / v.size
+    assert np.max(np.abs(inv.real - v)) < 1e-4
+
+
+def test_band_limit_center_normalize_matches_oracle(ctx, rng):
+    img = (rng.standard_normal((24, 24, 24)) + 2.0).astype(np.float32)
+    bp = BH_bandpass3d((28, 28, 28), 1e-3, 40.0, 8.0, "GPU", 2.5)
+    out = BH_bandLimitCenterNormalize(img, bp, "", [[2, 2, 2], [2, 2, 2]]).get()
+    ref = O.band_limit_center_normalize(img, O.bandpass3d((28, 28, 28), 1e-3, 40.0, 8.0, 2.5),
+                                        [[2, 2, 2], [2, 2, 2]])
+    assert rel(out, ref[..., :15]) < 2e-5
+
+
+# --------------------------------------------------------- correlation (A13/A14)
+
+def test_xcf_translational_matches_oracle(ctx, rng):
+    n = 32
+    x = rng.standard_normal((n, n, n))
+    ax = np.fft.fftfreq(n) * n
+    z, y, xx = np.meshgrid(ax, ax, ax, indexing="ij")
+    g = np.exp(-((xx - 2.3) ** 2 + (y + 1.6) ** 2 + (z - 0.4) ** 2) / (2 * 1.3 ** 2))
+    A = np.fft.fftn(g + 0.01 * x).astype(np.complex64)
+    B = np.ones((n, n, n), np.complex64)
+    mask = O.xform3d(np.ones((n, n, n)), np.eye(3), [0, 0, 0], True)
+    mask[:, :, : n // 2 - 6] = 0
+    p, mx = BH_multi_xcf_Translational(A, B, mask.astype(np.float32), [3, 3, 3], return_max=True)
+    ref, _, rmax = O.xcf_translational(A.astype(np.complex128), B.astype(np.complex128), mask,
+                                       [3, 3, 3], return_map=True)
+    assert np.max(np.abs(p - ref)) < 1e-4
+    assert abs(mx - rmax) < 1e-5 * abs(rmax)
+    # exact integer shift without COM
+    v = rng.standard_normal((n, n, n))
+    w = np.roll(v, (1, -4, 3), axis=(0, 1, 2))
+    p = BH_multi_xcf_Translational(np.fft.fftn(w).astype(np.complex64),
+                                   np.conj(np.fft.fftn(v)).astype(np.complex64), None, [0, 0, 0])
+    assert np.array_equal(p, [3.0, -4.0, 1.0])
+
+
+def test_xcf_translational_com_box_error(ctx):
+    n = 16
+    m = np.zeros((n, n, n))
+    m[n // 2 - 1, 0, 0] = 1.0
+    with pytest.raises(_lib.EmcError) as e:
+        BH_multi_xcf_Translational(np.fft.fftn(m).astype(np.complex64), np.ones((n, n, n), np.complex64),
+                                   None, [3, 3, 3])
+    assert e.value.status == 5
+
+
+def test_xcf_rotational_matches_oracle(ctx, rng):
+    n = 24
+    P = np.fft.fftn(rng.standard_normal((n, n, n))).astype(np.complex64)
+    R = np.conj(np.fft.fftn(rng.standard_normal((n, n, n)) + 0.5 * np.fft.ifftn(P).real)).astype(
+        np.complex64)
+    wp = rng.uniform(0, 1, (n, n, n)).astype(np.float32)
+    wr = rng.uniform(0, 1, (n, n, n)).astype(np.float32)
+    c = BH_multi_xcf_Rotational(P, R, wp, wr)
+    ref = O.xcf_rotational(P.astype(np.complex128), R.astype(np.complex128), wp, wr)
+    assert abs(c - ref) < 1e-5 * max(abs(ref), 1e-3)
+    assert abs(BH_multi_xcf_Rotational(P, np.conj(P), 1, 1) - 1.0) < 1e-6

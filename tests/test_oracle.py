@@ -1,0 +1,352 @@
+"""Pins of the FP64 oracle (CPU). Each test checks the oracle against a
+closed form, an invariant, an independent library routine or a hand-worked
+case, chosen so a plausible mistake (dropped term, wrong sign, swapped
+order, transposed operand, off-by-one centre) fails."""
+import math
+
+import numpy as np
+import pytest
+from scipy import ndimage
+
+from oracle import emc_oracle as O
+from tests.conftest import load_tlt
+
+
+# ------------------------------------------------------------ define_matrix
+
+@pytest.mark.parametrize("conv", ["Bah", "SPIDER", "Helical", "IMOD"])
+def test_define_matrix_orthonormal(conv, rng):
+    for _ in range(5):
+        a = rng.uniform(-180, 180, 3)
+        for d in ("forward", "inv"):
+            R = O.define_matrix(a, conv, d)
+            assert np.allclose(R @ R.T, np.eye(3), atol=1e-14)
+            assert abs(np.linalg.det(R) - 1.0) < 1e-14
+
+
+def test_define_matrix_hand_cases():
+    # forward [90,0,0] = Rz(-90): x -> -y
+    R = O.define_matrix([90, 0, 0], "Bah", "forward")
+    assert np.allclose(R, [[0, 1, 0], [-1, 0, 0], [0, 0, 1]], atol=1e-15)
+    # forward [90,90,0] = Rz(0) Rx(-90) Rz(-90); worked by hand. The swapped
+    # product Rz(-90) Rx(-90) gives [[0,0,1],[-1,0,0],[0,-1,0]] instead.
+    R = O.define_matrix([90, 90, 0], "Bah", "forward")
+    assert np.allclose(R, [[0, 1, 0], [0, 0, 1], [1, 0, 0]], atol=1e-15)
+    # TILT 'forwardVector' about Y
+    a = math.radians(30)
+    T = O.define_matrix([30], "TILT", "forwardVector")
+    assert np.allclose(T, [[math.cos(a), 0, math.sin(a)], [0, 1, 0], [-math.sin(a), 0, math.cos(a)]])
+
+
+def test_define_matrix_inv_is_forward_transpose(rng):
+    for _ in range(10):
+        a = rng.uniform(-180, 180, 3)
+        assert np.allclose(O.define_matrix(a, "Bah", "inv"), O.define_matrix(a, "Bah", "forward").T,
+                           atol=1e-15)
+
+
+# ------------------------------------------------------------- eulerSearch
+
+@pytest.mark.parametrize("args,count", [((180, 10, 180, 10), 15244), ((180, 15, 180, 15), 4600),
+                                        ((180, 7, 180, 7), 43784), ((15, 3, 15, 3), 1045),
+                                        ((15, 1, 15, 1), 23281), ((180, 10, 10, 10), 1236)])
+def test_euler_search_counts(args, count):
+    g = O.euler_search("C1", *args)
+    assert int(g.n_at_theta.sum()) == count
+    assert g.orientations().shape == (count, 3)
+
+
+def test_euler_search_small_hand_case():
+    g = O.euler_search("C1", 0, 0, 10, 5)
+    # theta=0 only, one phi (0), psi = -10:5:10
+    assert np.allclose(g.orientations(), [[0, 0, p] for p in (-10, -5, 0, 5, 10)])
+    g = O.euler_search("C4", 30, 15, 180, 30)
+    # C4 caps psi to +-45: psi = -45:30:45 -> -45,-15,15,45
+    assert np.allclose(g.psi, [-45, -15, 15, 45])
+    # theta 15: phi_step = 15/sin(15) = 57.96 -> 360/floor(6.21+0.5) = 60 -> 6 phis
+    assert len(g.phi[1]) == 6 and np.isclose(g.phi[1][1], 60.0)
+
+
+# ------------------------------------------------------------------ xform3d
+
+def test_xform_identity_and_integer_shift(rng):
+    v = rng.standard_normal((6, 7, 8))
+    assert np.array_equal(O.xform3d(v, np.eye(3), [0, 0, 0], True), v)
+    # forward: p = u - t, so out(x) = in(x - 1) along x
+    s = O.xform3d(v, np.eye(3), [1, 0, 0], True)
+    assert np.array_equal(s[:, :, 1:], v[:, :, :-1]) and not s[:, :, 0].any()
+    # inverse: p = u + t, so out(z) = in(z + 2)
+    s = O.xform3d(v, np.eye(3), [0, 0, 2], False)
+    assert np.array_equal(s[:-2], v[2:]) and not s[-2:].any()
+
+
+@pytest.mark.parametrize("n", [7, 8])
+def test_xform_90deg_permutations(n, rng):
+    v = rng.standard_normal((n, n, n))
+    c = n // 2
+    z, y, x = np.meshgrid(range(n), range(n), range(n), indexing="ij")
+    # rows of M act on u = index - n//2; sample at M u + c
+    cases = {
+        "z": (np.array([[0, -1, 0], [1, 0, 0], [0, 0, 1.0]]), (c - (y - c), c + (x - c), z)),
+        "x": (np.array([[1, 0, 0], [0, 0, -1], [0, 1, 0.0]]), (x, c - (z - c), c + (y - c))),
+        "y": (np.array([[0, 0, 1], [0, 1, 0], [-1, 0, 0.0]]), (c + (z - c), y, c - (x - c))),
+    }
+    for M, (px, py, pz) in cases.values():
+        out = O.xform3d(v, M, [0, 0, 0], True)
+        ok = (px >= 0) & (px < n) & (py >= 0) & (py < n) & (pz >= 0) & (pz < n)
+        exp = np.zeros_like(v)
+        exp[ok] = v[pz[ok], py[ok], px[ok]]
+        assert np.array_equal(out, exp)
+
+
+def test_xform_matches_scipy_linear(rng):
+    """Independent routine: scipy map_coordinates order 1, 'grid-constant'
+    (samples outside the grid are 0 and are interpolated, texture border)."""
+    v = rng.standard_normal((5, 6, 7))
+    M = O.define_matrix([23, 47, -71], "Bah", "forward")
+    t = np.array([0.3, -0.7, 1.1])
+    out = O.xform3d(v, M, t, True)
+    nz, ny, nx = v.shape
+    z, y, x = np.meshgrid(range(nz), range(ny), range(nx), indexing="ij")
+    u = np.stack([x - nx // 2, y - ny // 2, z - nz // 2]).reshape(3, -1) - t[:, None]
+    p = M @ u
+    coords = np.stack([p[2] + nz // 2, p[1] + ny // 2, p[0] + nx // 2])
+    ref = ndimage.map_coordinates(v, coords, order=1, mode="grid-constant", cval=0.0).reshape(v.shape)
+    assert np.allclose(out, ref, atol=1e-12)
+
+
+def test_xform_fwd_inv_sign_identity(rng):
+    """fwd (M, t) samples the same points as inv (M, -M t)."""
+    v = rng.standard_normal((8, 8, 8))
+    M = O.define_matrix([10, 70, 200], "Bah", "inv")
+    t = np.array([1.25, -0.5, 2.0])
+    assert np.allclose(O.xform3d(v, M, t, True), O.xform3d(v, M, -M @ t, False), atol=1e-12)
+
+
+def test_interp3d_symmetry_average_invariant(rng):
+    v = rng.standard_normal((9, 9, 9))
+    M = O.define_matrix([15, 30, 45], "Bah", "forward")
+    S = O.symmetry_matrices("C4")
+    a = O.interp3d(v, M, [0, 0, 0], "Bah", "forward", "C4")
+    b = O.interp3d(v, M @ S[1], [0, 0, 0], "Bah", "forward", "C4")
+    assert np.allclose(a, b, atol=1e-10)
+
+
+# ------------------------------------------------------------------- CTF
+
+def test_ctf_zero_at_analytic_frequency():
+    lam, df, nx = 0.019687, 30000.0, 64
+    k1 = math.sqrt(1.0 / (lam * df))          # sin(-pi lam df k^2) = 0 at k^2 = 1/(lam df)
+    pix = 5.0 / (nx * k1)                     # puts k1 on the 5th Fourier pixel
+    p = O.CtfParams(pix, lam, 0.0, 0.0, df, df, 0.0)
+    img = O.ctf2d(nx, nx, p, half=False, sq=False, centered=True)
+    assert abs(img[nx // 2, nx // 2 + 5]) < 1e-9 and abs(img[nx // 2 + 5, nx // 2]) < 1e-9
+    assert abs(img[nx // 2, nx // 2 + 3]) > 0.1
+    # DC: sin(-amp_phase) = 0 for amp 0
+    assert img[nx // 2, nx // 2] == 0.0
+
+
+def test_ctf_astigmatism_convention_and_weights():
+    base = dict(pixel=2.0, wavelength=0.019687, cs_mm=2.7, amp=0.1, df1=32000.0, df2=28000.0)
+    a = O.ctf2d(32, 32, O.CtfParams(astig_deg=200.0, **base), sq=True)
+    b = O.ctf2d(32, 32, O.CtfParams(astig_deg=20.0, **base), sq=True)
+    assert np.allclose(a, b, atol=1e-12)       # -90..90 wrap of the angle
+    w = O.ctf2d(32, 32, O.CtfParams(astig_deg=20.0, **base), sq=True, weighted=True,
+                occupancy=0.7, exposure=0.0)
+    x = np.arange(32) - 16
+    rw = 0.7 * np.where(x == 0, 0.2, np.abs(x))
+    assert np.allclose(w, b * rw[None, :], atol=1e-12)
+    # exposure filter at one point: exp(-E/2 / (0.24499 k^-1.665 + 2.8141))
+    e = O.ctf2d(32, 32, O.CtfParams(astig_deg=20.0, **base), sq=True, weighted=True,
+                occupancy=1.0, exposure=40.0)
+    k = 5 / (2.0 * 32)
+    f = math.exp(-20.0 / (0.24499 * k ** -1.665 + 2.8141))
+    assert math.isclose(e[16, 21], b[16, 21] * 5 * f, rel_tol=1e-9)
+
+
+def _tilt(t, df=30000.0):
+    return dict(pixel=2.5, wavelength=0.019687, cs=2.7, df1=df, df2=df, astig=0.0, amp=0.1, tilt=t,
+                exposure=0.0, occupancy=1.0)
+
+
+def test_sf3d_single_zero_tilt_is_slab_times_ctf():
+    n = 16
+    w = O.sf3d((n, n, n), [_tilt(0.0)])
+    img = O.ctf2d(n, n, O.CtfParams(2.5, 0.019687, 2.7, 0.1, 30000.0, 30000.0, 0.0), sq=True,
+                  weighted=True, occupancy=1.0, exposure=0.0, spacing_from_full_nx=False)
+    z = np.arange(n) - n // 2 + 0.5          # even nz: tw = w + 1/2
+    zw = np.where(np.abs(z) <= 4, 0.25 * (0.5 + 0.5 * np.cos(np.pi * z / 4)), 0.0)
+    assert np.allclose(w, zw[:, None, None] * img[None, :, :], atol=1e-12)
+
+
+def test_sf3d_missing_wedge_is_empty():
+    n = 32
+    tilts = [_tilt(t) for t in np.arange(-60, 61, 3.0)]
+    w = O.sf3d((n, n, n), tilts)
+    # along kz at kx = 0 nothing is sampled beyond the slab reach of a 60 deg
+    # tilt: |0.5 + w cos60| > 4  <=>  w >= 8 or w <= -10
+    col = w[:, :, n // 2]
+    assert not col[n // 2 + 8:].any() and not col[: n // 2 - 9].any()
+    assert col[n // 2].any()
+
+
+def test_weight_mask_on_reference_tlt_fixture():
+    T = load_tlt("riboNew11_ali1_ctf.tlt")
+    w = O.weight_mask_mex((16, 16, 16), 4, T, [100.0, 120.0, 40.0], [[400, 400, 200], [0, 0, 0]])
+    assert np.isfinite(w).all() and abs(w.min() - 1e-4) < 1e-15
+    tilts = O.tlt_to_tilts(T, 4, [100.0, 120.0, 40.0], [[400, 400, 200], [0, 0, 0]], (16, 16, 16))
+    # the particle sits above the tilt axis: defocus changes with the tilt
+    # by -(x sin a ... ) ; at the 0-deg-like tilt it equals the mean defocus
+    # shifted by z*pixel (hand form of BH_weightMaskMex.m:33-37)
+    i0 = int(np.argmin([abs(t["tilt"]) for t in tilts]))
+    row = T[np.argsort(T[:, 0], kind="stable")][i0]
+    pix = row[15] * 4 * 1e10
+    prj = np.array([100.0, 120.0, 40.0]) - np.ceil((np.array([400, 400, 200]) + 1) / 2)
+    a = math.radians(row[3])
+    zc = -math.sin(a) * prj[0] + math.cos(a) * prj[2]
+    assert math.isclose(tilts[i0]["df1"], -row[14] * 1e10 - zc * pix + row[11] * 1e10, rel_tol=1e-12)
+
+
+# --------------------------------------------------------------- bandpass
+
+def test_bandpass_closed_form():
+    n = 32
+    bp = O.bandpass3d((n, n, n), 1e-3, 40.0, 8.0, 2.5)
+    # DC equals HIGH_THRESH: highRoll^2 = -hc^2 / (2 ln thr)
+    assert math.isclose(bp[0, 0, 0], 1e-3, rel_tol=1e-12)
+    lc, hc = 2.5 / 8.0, 2.5 / 40.0
+    r = O.fft_radius((n, n, n))
+    inside = (r > hc) & (r < lc)
+    assert np.all(bp[inside] == 1.0)
+    # one lowRoll beyond lowCut: exp(-1/2)
+    lroll = 2.0 / n
+    sel = np.isclose(r, lc + lroll)
+    if sel.any():
+        assert np.allclose(bp[sel], math.exp(-0.5))
+    # a voxel on the x axis at radius 12/32
+    assert math.isclose(bp[0, 0, 12], math.exp(-((12 / 32 - lc) ** 2) / (2 * lroll ** 2)), rel_tol=1e-12)
+
+
+# ----------------------------------------------- band-limit / normalise / pad
+
+def test_blcn_invariants(rng):
+    img = rng.standard_normal((8, 8, 8)) + 3.0
+    F = O.band_limit_center_normalize(img, None, [[0, 0, 0], [0, 0, 0]])
+    assert F.flat[0] == 0
+    assert math.isclose(np.sum(np.abs(F) ** 2), F.size ** 2, rel_tol=1e-12)
+    # padding with the mean: the back-transform is (padded - mean) up to scale
+    img = rng.standard_normal((6, 6, 6))
+    F = O.band_limit_center_normalize(img, None, [[1, 1, 1], [1, 1, 1]])
+    back = np.real(np.fft.ifftn(F))
+    padded = np.full((8, 8, 8), img.mean())
+    padded[1:7, 1:7, 1:7] = img
+    d = padded - padded.mean()
+    assert np.corrcoef(back.ravel(), d.ravel())[0, 1] > 1 - 1e-12
+
+
+def test_pad_zeros3d_trim_and_taper(rng):
+    img = rng.standard_normal((10, 10, 10))
+    p = O.pad_zeros3d(img, [2, 1, 0], [0, 3, 1], extrap_val=0.5)
+    assert p.shape == (11, 14, 12)
+    assert np.array_equal(p[0:10, 1:11, 2:12], img) and np.all(p[10] == 0.5)
+    t = O.pad_zeros3d(img, [-1, -1, -1], [-1, -1, -1])
+    assert np.array_equal(t, img[1:-1, 1:-1, 1:-1])
+    ones = np.ones((16, 16, 16))
+    tp = O.pad_zeros3d(ones, [0, 0, 0], [0, 0, 0], taper=True)
+    e = 0.5 + 0.5 * math.cos(6 * math.pi / 7)
+    assert math.isclose(tp[8, 8, 0], e) and math.isclose(tp[0, 0, 0], e ** 3)
+    assert tp[8, 8, 8] == 1.0
+
+
+# ----------------------------------------------------------- translational
+
+def test_xcf_translational_integer_shift(rng):
+    n = 16
+    x = rng.standard_normal((n, n, n))
+    s = (2, -3, 1)                                  # (x, y, z)
+    y = np.roll(x, (s[2], s[1], s[0]), axis=(0, 1, 2))
+    p = O.xcf_translational(np.fft.fftn(y), np.conj(np.fft.fftn(x)), None, (0, 0, 0))
+    assert np.array_equal(p, np.array(s, dtype=float))
+
+
+def test_xcf_translational_com_half_pixel():
+    n = 24
+    ax = np.fft.fftfreq(n) * n                       # circular coordinates
+    z, y, x = np.meshgrid(ax, ax, ax, indexing="ij")
+    g = np.exp(-((x - 1.5) ** 2 + (y + 2.0) ** 2 + z ** 2) / (2 * 0.8 ** 2))
+    p = O.xcf_translational(np.fft.fftn(g), np.ones((n, n, n)), None, (3, 3, 3))
+    assert np.allclose(p, [1.5, -2.0, 0.0], atol=0.02)
+
+
+def test_xcf_translational_mask_and_empty():
+    n = 16
+    m = np.zeros((n, n, n))
+    m[0, 0, 3] = 5.0                                 # strong peak at +3 x
+    m[0, 0, n - 2] = 1.0                             # weak peak at -2 x
+    mask = np.ones((n, n, n))
+    mask[n // 2, n // 2, n // 2 + 3] = 0.0           # mask out the strong one
+    p = O.xcf_translational(np.fft.fftn(m), np.ones((n, n, n)), mask, (0, 0, 0))
+    assert np.allclose(p, [-2, 0, 0])
+    z = O.xcf_translational(np.zeros((n, n, n)), np.zeros((n, n, n)), None, (3, 3, 3))
+    assert np.allclose(z, -(n // 2 + 1))             # [0,0,0] - ceil((N+1)/2) (:95-97)
+
+
+def test_xcf_translational_com_box_error():
+    n = 16
+    m = np.zeros((n, n, n))
+    m[n // 2 - 1, 0, 0] = 1.0                        # fftshifted to the map edge
+    with pytest.raises(IndexError):
+        O.xcf_translational(np.fft.fftn(m), np.ones((n, n, n)), None, (3, 3, 3))
+
+
+# ------------------------------------------------------------- rotational
+
+def test_xcf_rotational_invariants(rng):
+    P = np.fft.fftn(rng.standard_normal((8, 8, 8)))
+    assert math.isclose(O.xcf_rotational(P, np.conj(P), np.ones(P.shape), 1.0), 1.0, rel_tol=1e-12)
+    Q = np.fft.fftn(rng.standard_normal((8, 8, 8)))
+    w = rng.uniform(0, 1, P.shape)
+    c = O.xcf_rotational(P, np.conj(Q), w, 1.0)
+    assert abs(c) < 1
+    assert math.isclose(O.xcf_rotational(3 * P, np.conj(Q), w, 1.0), c, rel_tol=1e-12)
+    wr = np.full(P.shape, np.nan)
+    assert math.isclose(O.xcf_rotational(P, np.conj(Q), w, wr), c, rel_tol=1e-12)
+
+
+# ------------------------------------------------------------- template search
+
+def test_swap_quadrants_is_shift_and_sign():
+    """swapQuadrants = circular shift by n/2+1 and (-1)^(n/2+1) per axis."""
+    n = 8
+    sq = O.swap_quadrants_half((n, n, n))
+    v = np.random.default_rng(3).standard_normal((n, n, n))
+    a = np.fft.irfftn(sq * np.fft.rfftn(v), s=(n, n, n))
+    sh = n // 2 + 1
+    b = np.roll(v, (sh, sh, sh), axis=(0, 1, 2)) * (-1) ** (3 * (n // 2 + 1))
+    assert np.allclose(a, b, atol=1e-12)
+
+
+def test_template_search_finds_inserted_template():
+    rng = np.random.default_rng(7)
+    # compact asymmetric template: every 90-degree rotation (plus the [1,1,1]
+    # shift) stays inside the 8^3 box, so no orientation loses energy
+    ax = np.arange(8) - 4
+    zz, yy, xx = np.meshgrid(ax, ax, ax, indexing="ij")
+    t = np.where(xx ** 2 + yy ** 2 + zz ** 2 <= 4, rng.uniform(0.2, 1.0, (8, 8, 8)), 0.0)
+    grid = O.euler_search("C1", 90, 90, 90, 90).orientations()[:6]
+    k = 4
+    phi, theta, psi = grid[k]
+    rot = O.interp3d(t, [phi, theta, psi - phi], [1, 1, 1], "Bah", "forward")
+    chunk = rng.standard_normal((24, 24, 24)) * 0.01
+    c = np.array([12, 11, 13])                       # (x, y, z)
+    chunk[c[2] - 4:c[2] + 4, c[1] - 4:c[1] + 4, c[0] - 4:c[0] + 4] += rot
+    pad = O.pad_val((8, 8, 8), (24, 24, 24))
+    valid = O.pad_val((24, 24, 24), (16, 16, 16))
+    # the reference negates the tomogram in preprocessing (:586); for chunk
+    # sizes = 0 mod 4 swapQuadrants carries (-1)^3 which cancels it
+    mag, ang = O.template_search_chunk(-chunk, t, grid, pad[0:2], valid[0:2])
+    i = np.unravel_index(np.argmax(mag), mag.shape)
+    assert ang[i] == k + 1
+    # the peak lands on the centre of the inserted copy: its box centre c
+    # plus the [1,1,1] forward shift of the rotated template; valid area at 4
+    assert np.array_equal(np.array(i)[::-1] + 4, c + 1)
