@@ -55,6 +55,46 @@ void* pinned(emc_ctx* ctx, size_t bytes) {
   return ctx->h_pinned;
 }
 
+// ------------------------------------------------------------- timing
+
+static hipEvent_t take_event(emc_ctx* c) {
+  if (!c->event_pool.empty()) {
+    hipEvent_t e = c->event_pool.back();
+    c->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  EMC_HIP(hipEventCreate(&e));
+  return e;
+}
+
+KTimer::KTimer(emc_ctx* ctx, const char* name) : c(ctx), on(ctx && ctx->timing) {
+  if (!on) return;
+  p.name = name;
+  p.a = take_event(c);
+  p.b = take_event(c);
+  EMC_HIP(hipEventRecord(p.a, c->stream));
+}
+
+KTimer::~KTimer() {
+  if (!on) return;
+  if (hipEventRecord(p.b, c->stream) == hipSuccess) c->pending.push_back(p);
+}
+
+static void fold_timing(emc_ctx* c) {
+  EMC_HIP(hipStreamSynchronize(c->stream));
+  for (auto& p : c->pending) {
+    float ms = 0.0f;
+    EMC_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+    auto& t = c->ktimes[p.name];
+    t.first += ms;
+    t.second += 1;
+    c->event_pool.push_back(p.a);
+    c->event_pool.push_back(p.b);
+  }
+  c->pending.clear();
+}
+
 // ------------------------------------------------------------- geometry
 
 static void rz(double a, double m[3][3]) {
@@ -230,6 +270,11 @@ emc_status emc_ctx_destroy(emc_ctx* ctx) {
     use_device(ctx);
     (void)hipStreamSynchronize(ctx->stream);
     for (auto& kv : ctx->plans) emc_fft_plan_destroy(kv.second);
+    for (auto& p : ctx->pending) {
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     for (auto& kv : ctx->scratch)
       if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
@@ -246,6 +291,33 @@ emc_status emc_ctx_synchronize(emc_ctx* ctx) {
 }
 
 void* emc_ctx_stream(emc_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+emc_status emc_timing_enable(emc_ctx* ctx, int on) {
+  return guard([&] {
+    use_device(ctx);
+    fold_timing(ctx);
+    ctx->timing = on != 0;
+  });
+}
+
+emc_status emc_timing_reset(emc_ctx* ctx) {
+  return guard([&] {
+    use_device(ctx);
+    fold_timing(ctx);
+    ctx->ktimes.clear();
+  });
+}
+
+emc_status emc_timing_read(emc_ctx* ctx, const char* name, double* ms_total, int64_t* launches) {
+  return guard([&] {
+    use_device(ctx);
+    EMC_REQUIRE(name && ms_total && launches, "null pointer");
+    fold_timing(ctx);
+    auto it = ctx->ktimes.find(name);
+    *ms_total = it == ctx->ktimes.end() ? 0.0 : it->second.first;
+    *launches = it == ctx->ktimes.end() ? 0 : it->second.second;
+  });
+}
 
 emc_status emc_malloc(emc_ctx* ctx, size_t bytes, void** d_ptr) {
   return guard([&] {

@@ -84,7 +84,29 @@ struct emc_ctx {
   std::map<std::tuple<int, int, int, int>, emc_fft_plan*> plans;
   double* h_pinned = nullptr;  // small pinned host buffer for results
   size_t h_pinned_bytes = 0;
+  // optional per-kernel timing with HIP events on the context stream
+  bool timing = false;
+  struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> event_pool;
+  std::map<std::string, std::pair<double, int64_t>> ktimes;  // name -> (ms, launches)
 };
+
+namespace emc {
+
+// Brackets one kernel launch with HIP events when the context's timing is on.
+struct KTimer {
+  emc_ctx* c;
+  emc_ctx::Pending p;
+  bool on;
+  KTimer(emc_ctx* ctx, const char* name);
+  ~KTimer();
+};
+
+}  // namespace emc
 
 namespace emc {
 

@@ -78,12 +78,15 @@ def ctf3d(n: int, pixel_A: float, defocus_um: float, kv=300.0, cs_mm=2.7, amp=0.
 
 
 def degrade(vol, rng, max_tilt=60.0, pixel_A=2.5, defocus_um=3.0, snr=0.1, ctf=True):
-    """Wedge x CTF in Fourier space, then white noise at the SNR."""
+    """Wedge x CTF in Fourier space, then white noise at the SNR. The CTF is
+    applied phase-flipped (|CTF|), as in the CTF-corrected tomograms the
+    alignment runs on: a raw sin(chi) inverts the low-frequency contrast and
+    makes every cross-correlation against the reference negative."""
     n = vol.shape[0]
     F = np.fft.fftn(vol)
     W = wedge_mask(n, max_tilt)
     if ctf:
-        W = W * ctf3d(n, pixel_A, defocus_um)
+        W = W * np.abs(ctf3d(n, pixel_A, defocus_um))
     sig = np.real(np.fft.ifftn(F * W))
     if snr is None or snr <= 0:
         return sig

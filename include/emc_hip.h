@@ -61,6 +61,14 @@ emc_status emc_ctx_synchronize(emc_ctx* ctx);
  * own work on the same stream. */
 void* emc_ctx_stream(emc_ctx* ctx);
 
+/* Kernel timing with HIP events on the context stream (for roofline reports):
+ * enable/disable, reset, and read the accumulated milliseconds and launch
+ * count of one named kernel ("xform3d", "align_fused", ...). Reading
+ * synchronises the stream. */
+emc_status emc_timing_enable(emc_ctx* ctx, int on);
+emc_status emc_timing_reset(emc_ctx* ctx);
+emc_status emc_timing_read(emc_ctx* ctx, const char* name, double* ms_total, int64_t* launches);
+
 /* Device memory helpers so a host without a GPU array type (C harness, MEX
  * gateway on AMD, ctypes) can drive the library. Copies are synchronous. */
 emc_status emc_malloc(emc_ctx* ctx, size_t bytes, void** d_ptr);

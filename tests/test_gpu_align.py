@@ -13,12 +13,13 @@ from oracle import emc_oracle as O
 pytestmark = pytest.mark.gpu
 
 
-def _inputs(n, seed, euler, snr):
-    case = S.make_align_case(seed, n=n, euler=euler, snr=snr, n_blobs=16, blob_radius=0.22 * n)
+def _inputs(n, seed, euler, snr, n_blobs=16, blob_radius=None):
+    case = S.make_align_case(seed, n=n, euler=euler, snr=snr, n_blobs=n_blobs,
+                             blob_radius=blob_radius or 0.22 * n)
     T = S.tilt_table(60.0, 3.0, 3.0, 2.5)
     wedge = BH_weightMaskMex((n, n, n), 1, T, [n // 2 + 1] * 3, [[n, n, n], [0, 0, 0]]).get()
     bp = O.bandpass3d((n, n, n), 1e-3, 40.0, 8.0, 2.5).astype(np.float32)
-    peak_mask = S.sphere_mask(n, 5.0)
+    peak_mask = S.sphere_mask(n, n // 4)
     vol_mask = S.sphere_mask(n, 0.4 * n)
     ref_wgt_rot = np.ones((n, n, n), np.float32)
     return case, wedge, bp, peak_mask, vol_mask, ref_wgt_rot
@@ -44,7 +45,11 @@ def test_align_grid_matches_oracle(ctx, n):
     # CC values to 1e-4 relative, shifts to 1e-3 px, best orientation exact
     assert np.max(np.abs(ccc - rc) / np.abs(rc)) < 1e-4
     assert np.max(np.abs(est - re)) < 1e-3
-    assert int(np.argmax(ccc)) == int(np.argmax(rc)) == 0
+    # best orientation index exact (the oracle's top-2 margin is well above
+    # the FP32 error here)
+    top2 = np.sort(rc)[-2:]
+    assert top2[1] - top2[0] > 1e-3 * top2[1]
+    assert int(np.argmax(ccc)) == int(np.argmax(rc))
     # stage 2 alone at the identity pose = the initial CCC of the reference
     c0, _ = al.grid([[0.0, 0.0, 0.0]], shift_val, mode=2)
     assert abs(c0[0] - O.align_initial(inp)) < 1e-4 * abs(O.align_initial(inp))
@@ -56,7 +61,7 @@ def test_align_particle_recovers_ground_truth(ctx):
     and the shift within 0.5 px."""
     n = 64
     truth = (60.0, 20.0, 40.0)
-    case, wedge, bp, pm, vm, rwr = _inputs(n, 5, truth, snr=0.1)
+    case, wedge, bp, pm, vm, rwr = _inputs(n, 5, truth, snr=0.1, n_blobs=40, blob_radius=0.3 * n)
     grid = eulerSearch("C1", 40, 10, 60, 10)
     assert any(np.allclose(g, truth) for g in grid)
     al = ParticleAligner(case.particle, case.ref, wedge, bp, pm, vol_mask=vm, ref_wgt_rot=rwr)
