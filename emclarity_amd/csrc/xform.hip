@@ -111,6 +111,7 @@ XformArgs xform_args(const int in_dims[3], const float* Rrow, int nsym, const fl
 void launch_xform(emc_ctx* ctx, const float* in, float* out, const XformArgs& a) {
   dim3 block(64, 4, 1);
   dim3 grid(ceil_div(a.ox, 64), ceil_div(a.oy, 4), a.oz);
+  KTimer kt(ctx, "xform3d");
   hipLaunchKernelGGL(xform3d_kernel, grid, block, 0, ctx->stream, in, out, a);
   EMC_HIP(hipGetLastError());
 }
