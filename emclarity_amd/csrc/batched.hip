@@ -1,0 +1,561 @@
+// Batched alignment of many subtomograms x many orientations in one launch,
+// and the hand-written adjoint (score gradient w.r.t. 3 Euler angles and 3
+// shifts) used for gradient refinement.
+//
+// This path has no counterpart in the reference: it is the batched form of
+// the rotate-weight-correlate chain that alignment/BH_alignRaw3d_v2.m runs
+// one orientation and one particle at a time (:1119-1419). The reference
+// rotates in real space (mexXform3d), then FFTs, multiplies and inverse-FFTs
+// the full volume for every orientation. Here the reference is rotated in
+// Fourier space instead: its spectrum, oversampled 2x by zero padding, is
+// computed once per box size and stays resident in L2 / MALL; for each
+// (particle, orientation) one workgroup
+//   1. samples the rotated reference spectrum F(2 R^T k) trilinearly for
+//      every frequency k inside the bandpass support (half grid),
+//   2. multiplies by the particle's prepared spectrum
+//      Q = P_centred * bandpass^2 * wedge, accumulates the normalisation
+//      sum |F|^2 * wedge^2 * bandpass^2 in FP64,
+//   3. runs a pruned inverse DFT that evaluates the correlation only on the
+//      (2r+1)^3 cube of candidate shifts (z, then y, then the Hermitian x
+//      sum), entirely in registers and LDS,
+//   4. takes the peak inside the sphere |d| <= r with parabolic sub-pixel
+//      refinement, and writes the normalised CC and the shift.
+// No FFT, no per-orientation volume in HBM: per unit HBM traffic is a few
+// hundred bytes of results; the work is FP32 FMA bound (the roofline used
+// in the benchmark is the FP32 vector peak).
+//
+// CC definition (Parseval form of the normalised real-space CC of the
+// band-passed particle and the wedge-weighted rotated reference):
+//   a(k) = P(k) b(k),   c_o(k) = F(2 R_o^T k) S(k) b(k)
+//   CC_o(s) = sum_k Re[a(k) conj(c_o(k)) e^{2 pi i k.s/n}]
+//             / sqrt(sum_k |a|^2 * sum_k |c_o|^2)
+// with b the bandpass, S the particle's SF3D wedge symmetrised over k/-k so
+// that the product is Hermitian and the half grid is exact.
+#include <cmath>
+#include <cstring>
+
+#include "ops.hpp"
+
+namespace emc {
+
+void define_matrix(const double ang_deg[3], int convention, int direction, double out_colmajor[9]);
+
+constexpr int kBatchThreads = 256;
+constexpr int kMaxN = 128;
+
+// --------------------------------------------------------------- preparation
+
+// Reference of edge n, zero-padded to 2n with its centre n/2 moved to the
+// origin (ifftshift of the centred padding).
+__global__ void pad_origin_kernel(const float* __restrict__ ref, float* __restrict__ out, int n) {
+  const int m = 2 * n;
+  const int X = blockIdx.x * blockDim.x + threadIdx.x;
+  const int Y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int Z = blockIdx.z;
+  if (X >= m || Y >= m) return;
+  // signed offset of this output voxel from the origin
+  const int ux = X < n ? X : X - m, uy = Y < n ? Y : Y - m, uz = Z < n ? Z : Z - m;
+  const int x = ux + n / 2, y = uy + n / 2, z = uz + n / 2;
+  float v = 0.0f;
+  if ((unsigned)x < (unsigned)n && (unsigned)y < (unsigned)n && (unsigned)z < (unsigned)n)
+    v = ref[((int64_t)z * n + y) * n + x];
+  out[((int64_t)Z * m + Y) * m + X] = v;
+}
+
+// ifftshift of an n^3 volume (centre n/2 -> origin)
+__global__ void ifftshift_kernel(const float* __restrict__ in, float* __restrict__ out, int n) {
+  const int X = blockIdx.x * blockDim.x + threadIdx.x;
+  const int Y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int Z = blockIdx.z;
+  if (X >= n || Y >= n) return;
+  const int x = shifted_index(X, n), y = shifted_index(Y, n), z = shifted_index(Z, n);
+  out[((int64_t)Z * n + Y) * n + X] = in[((int64_t)z * n + y) * n + x];
+}
+
+// Q, N in the fused kernel's layout [kz][kx][ky] (ky fastest, coalesced for
+// threads running along ky); per-block partials of A = sum_full |P b|^2.
+__global__ void __launch_bounds__(kRedThreads) prep_particle_kernel(
+    const float2* __restrict__ P, const float* __restrict__ bp, const float* __restrict__ wedge_c,
+    int n, float2* __restrict__ Q, float* __restrict__ N, double* __restrict__ part) {
+  const int cx = n / 2 + 1;
+  const int64_t nh = (int64_t)cx * n * n;
+  double v[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nh;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int x = (int)(i % cx);
+    const int64_t yz = i / cx;
+    const int y = (int)(yz % n), z = (int)(yz / n);
+    const float b = bp[((int64_t)z * n + y) * n + x];
+    const int64_t cp = ((int64_t)shifted_index(z, n) * n + shifted_index(y, n)) * n + shifted_index(x, n);
+    const int mx = mirror_index(x, n), my = mirror_index(y, n), mz = mirror_index(z, n);
+    const int64_t cm = ((int64_t)shifted_index(mz, n) * n + shifted_index(my, n)) * n + shifted_index(mx, n);
+    const float S = wedge_c ? 0.5f * (wedge_c[cp] + wedge_c[cm]) : 1.0f;
+    float2 p = P[i];
+    if (i == 0) p = make_float2(0.0f, 0.0f);  // mean removed
+    const float w = b * b * S;
+    const int64_t o = ((int64_t)z * cx + x) * n + y;
+    Q[o] = make_float2(p.x * w, p.y * w);
+    N[o] = (i == 0) ? 0.0f : S * S * b * b;
+    const double m = (x == 0 || (n % 2 == 0 && x == n / 2)) ? 1.0 : 2.0;
+    v[0] += m * ((double)p.x * p.x + (double)p.y * p.y) * (double)b * b;
+  }
+  block_sum<1>(v);
+  if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+
+// ---------------------------------------------------------- rotation matrices
+
+// R = A * BH_defineMatrix([phi, theta, psi - phi], 'Bah', 'inv') in FP64 on
+// the device (alignment/BH_alignRaw3d_v2.m:1175-1176), stored as the FP32
+// sampling matrix M = 2 R^T (row-major) per (particle, orientation).
+__device__ inline void rz_d(double a, double m[3][3]) {
+  double s, c;
+  sincos(a, &s, &c);
+  m[0][0] = c; m[0][1] = -s; m[0][2] = 0;
+  m[1][0] = s; m[1][1] = c;  m[1][2] = 0;
+  m[2][0] = 0; m[2][1] = 0;  m[2][2] = 1;
+}
+__device__ inline void rx_d(double a, double m[3][3]) {
+  double s, c;
+  sincos(a, &s, &c);
+  m[0][0] = 1; m[0][1] = 0; m[0][2] = 0;
+  m[1][0] = 0; m[1][1] = c; m[1][2] = -s;
+  m[2][0] = 0; m[2][1] = s; m[2][2] = c;
+}
+__device__ inline void mm_d(const double a[3][3], const double b[3][3], double o[3][3]) {
+  double t[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) t[i][j] = a[i][0] * b[0][j] + a[i][1] * b[1][j] + a[i][2] * b[2][j];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) o[i][j] = t[i][j];
+}
+
+// Local matrix of (phi, theta, psi): inv reverses [phi, theta, psi-phi]
+// -> Rz(phi) Rx(theta) Rz(psi - phi).
+__device__ inline void local_mat(double phi, double theta, double psi, double L[3][3]) {
+  const double d2r = 3.14159265358979323846 / 180.0;
+  double a[3][3], b[3][3], c[3][3];
+  rz_d(phi * d2r, a);
+  rx_d(theta * d2r, b);
+  rz_d((psi - phi) * d2r, c);
+  mm_d(a, b, L);
+  mm_d(L, c, L);
+}
+
+__global__ void compose_kernel(const double* __restrict__ start_cm, const double* __restrict__ pts,
+                               int n_part, int n_orient, float* __restrict__ M) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= n_part * n_orient) return;
+  const int p = u / n_orient, o = u % n_orient;
+  double A[3][3], L[3][3], R[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) A[r][c] = start_cm[9 * p + c * 3 + r];
+  local_mat(pts[3 * o], pts[3 * o + 1], pts[3 * o + 2], L);
+  mm_d(A, L, R);
+  float* m = M + 9 * (int64_t)u;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) m[3 * r + c] = (float)(2.0 * R[c][r]);  // 2 R^T
+}
+
+// ------------------------------------------------------------- sampling
+
+// F2: half grid of the 2n-padded reference, x in [0, n], y, z in [0, 2n).
+// Texels with x < 0 come from the Hermitian mirror.
+__device__ inline float2 f2_texel(const float2* __restrict__ F2, int n, int tx, int ty, int tz) {
+  const int m = 2 * n;
+  bool cj = false;
+  if (tx < 0) {
+    tx = -tx;
+    ty = -ty;
+    tz = -tz;
+    cj = true;
+  }
+  ty = ty < 0 ? ty + m : ty;
+  tz = tz < 0 ? tz + m : tz;
+  const float2 v = F2[((int64_t)tz * m + ty) * (n + 1) + tx];
+  return cj ? make_float2(v.x, -v.y) : v;
+}
+
+struct Tri {
+  float2 f;
+  float2 g[3];  // d f / d j_x, j_y, j_z
+};
+
+template <bool GRAD>
+__device__ inline Tri f2_sample(const float2* __restrict__ F2, int n, float jx, float jy, float jz) {
+  const float xf = floorf(jx), yf = floorf(jy), zf = floorf(jz);
+  const float fx = jx - xf, fy = jy - yf, fz = jz - zf;
+  const int x0 = (int)xf, y0 = (int)yf, z0 = (int)zf;
+  float2 c[2][2][2];
+#pragma unroll
+  for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+    for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+      for (int dx = 0; dx < 2; ++dx) c[dz][dy][dx] = f2_texel(F2, n, x0 + dx, y0 + dy, z0 + dz);
+  Tri t;
+  auto lerp2 = [](float2 a, float2 b, float w) {
+    return make_float2(a.x + w * (b.x - a.x), a.y + w * (b.y - a.y));
+  };
+  const float2 c00 = lerp2(c[0][0][0], c[0][0][1], fx), c01 = lerp2(c[0][1][0], c[0][1][1], fx);
+  const float2 c10 = lerp2(c[1][0][0], c[1][0][1], fx), c11 = lerp2(c[1][1][0], c[1][1][1], fx);
+  const float2 c0 = lerp2(c00, c01, fy), c1 = lerp2(c10, c11, fy);
+  t.f = lerp2(c0, c1, fz);
+  if (GRAD) {
+    // d/dz
+    t.g[2] = make_float2(c1.x - c0.x, c1.y - c0.y);
+    // d/dy: lerp over z of (c?1 - c?0)
+    const float2 dy0 = make_float2(c01.x - c00.x, c01.y - c00.y);
+    const float2 dy1 = make_float2(c11.x - c10.x, c11.y - c10.y);
+    t.g[1] = lerp2(dy0, dy1, fz);
+    // d/dx: bilinear over (y, z) of (c??1 - c??0)
+    float2 dx[2][2];
+#pragma unroll
+    for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+      for (int dyy = 0; dyy < 2; ++dyy)
+        dx[dz][dyy] = make_float2(c[dz][dyy][1].x - c[dz][dyy][0].x, c[dz][dyy][1].y - c[dz][dyy][0].y);
+    t.g[0] = lerp2(lerp2(dx[0][0], dx[0][1], fy), lerp2(dx[1][0], dx[1][1], fy), fz);
+  }
+  return t;
+}
+
+__device__ inline int signed_freq(int i, int n) { return i <= n / 2 ? i : i - n; }
+
+// ------------------------------------------------------------ fused search
+
+template <int RR>
+__global__ void __launch_bounds__(kBatchThreads) align_fused_kernel(
+    const float2* __restrict__ F2, int n, const float2* __restrict__ Q, const float* __restrict__ Nw,
+    const double* __restrict__ A, const float* __restrict__ Mall, int n_orient, int n_units,
+    int units_per_xcd, float kmax2, float* __restrict__ score, float* __restrict__ shift) {
+  constexpr int D = 2 * RR + 1;
+  __shared__ float2 tw[kMaxN][D];              // e^{2 pi i k d / n}
+  __shared__ float2 T1[kBatchThreads][D];      // z-pruned column results
+  __shared__ float map[D * D * D];
+  __shared__ float Ms[9];
+  __shared__ double red[kBatchThreads / kWave];
+
+  // XCD-aware unit: workgroups are dealt round-robin over the 8 XCDs;
+  // consecutive units (the orientations of one particle) land on one XCD so
+  // they share the particle's spectrum in that XCD's L2.
+  const int unit = (blockIdx.x % 8) * units_per_xcd + blockIdx.x / 8;
+  if (unit >= n_units) return;  // uniform per block
+  const int p = unit / n_orient;
+  const int cx = n / 2 + 1;
+  const int64_t nh = (int64_t)cx * n * n;
+  const float2* Qp = Q + (int64_t)p * nh;
+  const float* Np = Nw + (int64_t)p * nh;
+  const int t = threadIdx.x;
+
+  for (int i = t; i < n * D; i += kBatchThreads) {
+    const int k = i / D, d = i % D - RR;
+    double s, c;
+    sincospi(2.0 * (double)k * (double)d / (double)n, &s, &c);
+    tw[k][i % D] = make_float2((float)c, (float)s);
+  }
+  for (int i = t; i < D * D * D; i += kBatchThreads) map[i] = 0.0f;
+  if (t < 9) Ms[t] = Mall[9 * (int64_t)unit + t];
+  __syncthreads();
+  const float m00 = Ms[0], m01 = Ms[1], m02 = Ms[2], m10 = Ms[3], m11 = Ms[4], m12 = Ms[5],
+              m20 = Ms[6], m21 = Ms[7], m22 = Ms[8];
+
+  const int planes = kBatchThreads / n;        // kx planes per pass
+  const int q = t / n, yy = t % n;             // this thread's plane and ky column
+  const int ky = signed_freq(yy, n);
+  const int kmax = (int)floorf(sqrtf(kmax2));
+  double dsum = 0.0;
+
+  for (int kx0 = 0; kx0 <= n / 2; kx0 += planes) {
+    const int kx = kx0 + q;
+    float2 acc[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) acc[d] = make_float2(0.0f, 0.0f);
+    const bool col = (q < planes) && (kx <= n / 2) && ((float)(kx * kx + ky * ky) <= kmax2);
+    // kz loop bounds uniform over the block: [-kmax, kmax] clipped to the grid
+    const int kzlo = -min(kmax, n / 2 - (n % 2 == 0 ? 1 : 0)), kzhi = min(kmax, (n - 1) / 2);
+    float dpart = 0.0f;
+    for (int kz = kzlo; kz <= kzhi; ++kz) {
+      const int zz = kz < 0 ? kz + n : kz;
+      const float r2 = (float)(kx * kx + ky * ky + kz * kz);
+      if (col && r2 <= kmax2) {
+        const int64_t o = ((int64_t)zz * cx + kx) * n + yy;
+        const float2 qv = Qp[o];
+        const float w = Np[o];
+        const float fkx = (float)kx, fky = (float)ky, fkz = (float)kz;
+        const float jx = m00 * fkx + m01 * fky + m02 * fkz;
+        const float jy = m10 * fkx + m11 * fky + m12 * fkz;
+        const float jz = m20 * fkx + m21 * fky + m22 * fkz;
+        const float2 f = f2_sample<false>(F2, n, jx, jy, jz).f;
+        // X = Q conj(f)
+        const float xr = qv.x * f.x + qv.y * f.y, xi = qv.y * f.x - qv.x * f.y;
+        dpart += (f.x * f.x + f.y * f.y) * w;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          const float2 e = tw[zz][d];
+          acc[d].x += xr * e.x - xi * e.y;
+          acc[d].y += xr * e.y + xi * e.x;
+        }
+      }
+    }
+    const double mxk = (kx == 0 || (n % 2 == 0 && kx == n / 2)) ? 1.0 : 2.0;
+    if (col) dsum += mxk * (double)dpart;
+#pragma unroll
+    for (int d = 0; d < D; ++d) T1[t][d] = acc[d];
+    __syncthreads();
+    // y-pruned sums and the Hermitian x expansion; each thread owns (dy, dz)
+    for (int o = t; o < D * D; o += kBatchThreads) {
+      const int dy = o / D, dz = o % D;
+      float vals[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) vals[d] = 0.0f;
+      for (int qq = 0; qq < planes; ++qq) {
+        const int kxx = kx0 + qq;
+        if (kxx > n / 2) break;
+        float sr = 0.0f, si = 0.0f;
+        for (int y2 = 0; y2 < n; ++y2) {
+          const float2 a = T1[qq * n + y2][dz];
+          const float2 e = tw[y2][dy];
+          sr += a.x * e.x - a.y * e.y;
+          si += a.x * e.y + a.y * e.x;
+        }
+        const float mk = (kxx == 0 || (n % 2 == 0 && kxx == n / 2)) ? 1.0f : 2.0f;
+#pragma unroll
+        for (int dx = 0; dx < D; ++dx) {
+          const float2 e = tw[kxx][dx];
+          vals[dx] += mk * (sr * e.x - si * e.y);
+        }
+      }
+#pragma unroll
+      for (int dx = 0; dx < D; ++dx) map[(dz * D + dy) * D + dx] += vals[dx];
+    }
+    __syncthreads();
+  }
+
+  // normalisation sum (FP64)
+  {
+    double v = wave_sum(dsum);
+    if (t % kWave == 0) red[t / kWave] = v;
+  }
+  // peak inside the sphere |d| <= RR, first index on ties (z, y, x order)
+  unsigned long long key = 0ull;
+  for (int i = t; i < D * D * D; i += kBatchThreads) {
+    const int dx = i % D - RR, dy = (i / D) % D - RR, dz = i / (D * D) - RR;
+    if (dx * dx + dy * dy + dz * dz > RR * RR) continue;
+    const unsigned long long k = argmax_key(map[i], (uint32_t)i);
+    key = k > key ? k : key;
+  }
+  key = block_max_u64(key);  // contains a barrier; valid in thread 0
+  if (t == 0) {
+    double dt = 0.0;
+    for (int w = 0; w < kBatchThreads / kWave; ++w) dt += red[w];
+    const int best = (int)key_index(key);
+    const float bv = map[best];
+    const int bx = best % D, by = (best / D) % D, bz = best / (D * D);
+    // parabolic sub-pixel refinement per axis
+    auto para = [&](int i0, int stride, int coord) {
+      if (coord <= 0 || coord >= D - 1) return 0.0f;
+      const float a = map[i0 - stride], b = map[i0], c = map[i0 + stride];
+      const float den = a - 2.0f * b + c;
+      if (!(den < 0.0f)) return 0.0f;
+      float d = 0.5f * (a - c) / den;
+      return fminf(0.5f, fmaxf(-0.5f, d));
+    };
+    const float sx = (float)(bx - RR) + para(best, 1, bx);
+    const float sy = (float)(by - RR) + para(best, D, by);
+    const float sz = (float)(bz - RR) + para(best, D * D, bz);
+    const double norm = sqrt(A[p] * dt);
+    score[unit] = norm > 0.0 ? (float)((double)bv / norm) : 0.0f;
+    shift[3 * (int64_t)unit + 0] = sx;
+    shift[3 * (int64_t)unit + 1] = sy;
+    shift[3 * (int64_t)unit + 2] = sz;
+  }
+}
+
+// ------------------------------------------------------------ adjoint
+
+// Score and gradient at one pose per particle:
+//   num(s, R) = sum_k Re[Q(k) conj(f(k)) e^{2 pi i k.s/n}],  f = F(M k), M = 2 R^T
+//   D(R) = sum_k |f|^2 N(k),  score = num / sqrt(A D)
+// d num / d s_i = -sum_k Im[Q conj(f) e] 2 pi k_i / n
+// d num / d a   =  sum_k Re[Q conj(grad f . dM_a k) e]
+// d D / d a     =  2 sum_k Re[conj(f) (grad f . dM_a k)] N
+// The derivative of the trilinear interpolant is taken inside each cell
+// (piecewise constant across cells). Output per particle: [score,
+// d/dphi, d/dtheta, d/dpsi (per radian), d/dsx, d/dsy, d/dsz].
+__global__ void __launch_bounds__(kBatchThreads) align_eval_grad_kernel(
+    const float2* __restrict__ F2, int n, const float2* __restrict__ Q, const float* __restrict__ Nw,
+    const double* __restrict__ A, const float* __restrict__ MdM, const float* __restrict__ shifts,
+    float kmax2, double* __restrict__ out) {
+  const int p = blockIdx.x;
+  const int cx = n / 2 + 1;
+  const int64_t nh = (int64_t)cx * n * n;
+  const float2* Qp = Q + (int64_t)p * nh;
+  const float* Np = Nw + (int64_t)p * nh;
+  __shared__ float Ms[36];
+  if (threadIdx.x < 36) Ms[threadIdx.x] = MdM[36 * (int64_t)p + threadIdx.x];
+  __syncthreads();
+  const float s0 = shifts[3 * p], s1 = shifts[3 * p + 1], s2 = shifts[3 * p + 2];
+  double v[11];
+#pragma unroll
+  for (int i = 0; i < 11; ++i) v[i] = 0.0;
+  for (int64_t i = (int64_t)threadIdx.x; i < nh; i += blockDim.x) {
+    const int yy = (int)(i % n);
+    const int64_t zx = i / n;
+    const int kx = (int)(zx % cx), zz = (int)(zx / cx);
+    const int ky = signed_freq(yy, n), kz = signed_freq(zz, n);
+    if ((float)(kx * kx + ky * ky + kz * kz) > kmax2) continue;
+    if (kx == 0 && ky == 0 && kz == 0) continue;
+    const float fk[3] = {(float)kx, (float)ky, (float)kz};
+    float j[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) j[r] = Ms[3 * r] * fk[0] + Ms[3 * r + 1] * fk[1] + Ms[3 * r + 2] * fk[2];
+    const Tri tr = f2_sample<true>(F2, n, j[0], j[1], j[2]);
+    const float2 qv = Qp[i];
+    const float w = Np[i];
+    const double m = (kx == 0 || (n % 2 == 0 && kx == n / 2)) ? 1.0 : 2.0;
+    double sn, cs;
+    sincospi(2.0 * ((double)fk[0] * s0 + (double)fk[1] * s1 + (double)fk[2] * s2) / (double)n, &sn, &cs);
+    // X = Q conj(f); Y = X e
+    const double xr = (double)qv.x * tr.f.x + (double)qv.y * tr.f.y;
+    const double xi = (double)qv.y * tr.f.x - (double)qv.x * tr.f.y;
+    const double yr = xr * cs - xi * sn, yi = xr * sn + xi * cs;
+    v[0] += m * yr;
+    const double tp = 2.0 * 3.14159265358979323846 / (double)n;
+    v[1] += -m * yi * tp * fk[0];
+    v[2] += -m * yi * tp * fk[1];
+    v[3] += -m * yi * tp * fk[2];
+    v[4] += m * ((double)tr.f.x * tr.f.x + (double)tr.f.y * tr.f.y) * w;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float* dM = Ms + 9 * (a + 1);
+      float dj[3];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) dj[r] = dM[3 * r] * fk[0] + dM[3 * r + 1] * fk[1] + dM[3 * r + 2] * fk[2];
+      const double dfr = (double)tr.g[0].x * dj[0] + (double)tr.g[1].x * dj[1] + (double)tr.g[2].x * dj[2];
+      const double dfi = (double)tr.g[0].y * dj[0] + (double)tr.g[1].y * dj[1] + (double)tr.g[2].y * dj[2];
+      // Re[Q conj(df) e]
+      const double zr = (double)qv.x * dfr + (double)qv.y * dfi;
+      const double zi = (double)qv.y * dfr - (double)qv.x * dfi;
+      v[5 + a] += m * (zr * cs - zi * sn);
+      v[8 + a] += m * 2.0 * ((double)tr.f.x * dfr + (double)tr.f.y * dfi) * w;
+    }
+  }
+  block_sum<11>(v);
+  if (threadIdx.x == 0) {
+    const double num = v[0], Dn = v[4];
+    const double norm = sqrt(A[p] * Dn);
+    double* o = out + 7 * (int64_t)p;
+    o[0] = num / norm;
+    for (int a = 0; a < 3; ++a) o[1 + a] = v[5 + a] / norm - num * v[8 + a] / (2.0 * Dn * norm);
+    for (int i = 0; i < 3; ++i) o[4 + i] = v[1 + i] / norm;
+  }
+}
+
+template <int RR>
+void launch_fused(emc_ctx* ctx, const float2* F2, int n, const float2* Q, const float* N,
+                  const double* A, const float* M, int n_orient, int n_units, float kmax2,
+                  float* score, float* shift) {
+  const int per = ceil_div(n_units, 8);
+  KTimer kt(ctx, "align_fused");
+  hipLaunchKernelGGL(align_fused_kernel<RR>, dim3(8 * per), dim3(kBatchThreads), 0, ctx->stream, F2, n,
+                     Q, N, A, M, n_orient, n_units, per, kmax2, score, shift);
+  EMC_HIP(hipGetLastError());
+}
+
+}  // namespace emc
+
+using namespace emc;
+
+extern "C" {
+
+emc_status emc_batch_reference_spectrum(emc_ctx* ctx, int n, const float* d_ref, void* d_F2) {
+  return guard([&] {
+    use_device(ctx);
+    EMC_REQUIRE(n >= 8 && n <= kMaxN, "box size must be 8..%d", kMaxN);
+    EMC_REQUIRE(d_ref && d_F2, "null pointer");
+    const int m = 2 * n;
+    float* pad = (float*)scratch(ctx, "bat_pad", sizeof(float) * (size_t)m * m * m);
+    hipLaunchKernelGGL(pad_origin_kernel, dim3(ceil_div(m, 64), ceil_div(m, 4), m), dim3(64, 4, 1), 0,
+                       ctx->stream, d_ref, pad, n);
+    EMC_HIP(hipGetLastError());
+    fft_r2c(cached_plan(ctx, m, m, m, 1), pad, d_F2);
+  });
+}
+
+emc_status emc_batch_prepare_particle(emc_ctx* ctx, int n, const float* d_particle,
+                                      const float* d_bandpass, const float* d_wedge, void* d_Q,
+                                      float* d_N, double* d_A) {
+  return guard([&] {
+    use_device(ctx);
+    EMC_REQUIRE(n >= 8 && n <= kMaxN, "box size must be 8..%d", kMaxN);
+    EMC_REQUIRE(d_particle && d_bandpass && d_Q && d_N && d_A, "null pointer");
+    const int64_t nv = (int64_t)n * n * n, nh = (int64_t)(n / 2 + 1) * n * n;
+    float* sh = (float*)scratch(ctx, "bat_shift", sizeof(float) * nv);
+    float2* P = (float2*)scratch(ctx, "bat_P", sizeof(float2) * nh);
+    hipLaunchKernelGGL(ifftshift_kernel, dim3(ceil_div(n, 64), ceil_div(n, 4), n), dim3(64, 4, 1), 0,
+                       ctx->stream, d_particle, sh, n);
+    fft_r2c(cached_plan(ctx, n, n, n, 1), sh, P);
+    const int nb = 256;
+    double* part = (double*)scratch(ctx, "bat_part", sizeof(double) * nb);
+    hipLaunchKernelGGL(prep_particle_kernel, dim3(nb), dim3(kRedThreads), 0, ctx->stream,
+                       (const float2*)P, d_bandpass, d_wedge, n, (float2*)d_Q, d_N, part);
+    hipLaunchKernelGGL(fold_partials_kernel<1>, dim3(1), dim3(kRedThreads), 0, ctx->stream,
+                       (const double*)part, nb, d_A);
+    EMC_HIP(hipGetLastError());
+  });
+}
+
+emc_status emc_align_batched(emc_ctx* ctx, int n, int n_part, const void* d_F2, const void* d_Q,
+                             const float* d_N, const double* d_A, const double* d_start_cm,
+                             const double* angles_pts, int n_orient, int peak_radius,
+                             double support_radius, float* d_score, float* d_shift) {
+  return guard([&] {
+    use_device(ctx);
+    EMC_REQUIRE(n >= 8 && n <= kMaxN, "box size must be 8..%d", kMaxN);
+    EMC_REQUIRE(n_part > 0 && n_orient > 0, "empty batch");
+    EMC_REQUIRE(d_F2 && d_Q && d_N && d_A && d_start_cm && angles_pts && d_score && d_shift,
+                "null pointer");
+    EMC_REQUIRE(support_radius > 0 && support_radius < 0.5 * n, "support radius must be in (0, n/2)");
+    EMC_REQUIRE(peak_radius == 4 || peak_radius == 6 || peak_radius == 8, "peak radius must be 4, 6 or 8");
+    const int64_t units64 = (int64_t)n_part * n_orient;
+    EMC_REQUIRE(units64 < ((int64_t)1 << 31) / 8, "too many units in one call");
+    const int units = (int)units64;
+    double* d_pts = (double*)scratch(ctx, "bat_pts", sizeof(double) * 3 * n_orient);
+    float* d_M = (float*)scratch(ctx, "bat_M", sizeof(float) * 9 * (size_t)units);
+    double* h = (double*)pinned(ctx, sizeof(double) * 3 * n_orient);
+    EMC_HIP(hipStreamSynchronize(ctx->stream));  // the pinned buffer may be in use
+    std::memcpy(h, angles_pts, sizeof(double) * 3 * n_orient);
+    EMC_HIP(hipMemcpyAsync(d_pts, h, sizeof(double) * 3 * n_orient, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(compose_kernel, dim3(ceil_div(units, 256)), dim3(256), 0, ctx->stream, d_start_cm,
+                       (const double*)d_pts, n_part, n_orient, d_M);
+    EMC_HIP(hipGetLastError());
+    const float kmax2 = (float)(support_radius * support_radius);
+    const float2* F2 = (const float2*)d_F2;
+    const float2* Q = (const float2*)d_Q;
+    switch (peak_radius) {
+      case 4: launch_fused<4>(ctx, F2, n, Q, d_N, d_A, d_M, n_orient, units, kmax2, d_score, d_shift); break;
+      case 6: launch_fused<6>(ctx, F2, n, Q, d_N, d_A, d_M, n_orient, units, kmax2, d_score, d_shift); break;
+      default: launch_fused<8>(ctx, F2, n, Q, d_N, d_A, d_M, n_orient, units, kmax2, d_score, d_shift); break;
+    }
+  });
+}
+
+emc_status emc_align_eval_grad(emc_ctx* ctx, int n, int n_part, const void* d_F2, const void* d_Q,
+                               const float* d_N, const double* d_A, const float* d_MdM,
+                               const float* d_shift, double support_radius, double* d_out) {
+  return guard([&] {
+    use_device(ctx);
+    EMC_REQUIRE(n >= 8 && n <= kMaxN, "box size must be 8..%d", kMaxN);
+    EMC_REQUIRE(n_part > 0, "empty batch");
+    EMC_REQUIRE(d_F2 && d_Q && d_N && d_A && d_MdM && d_shift && d_out, "null pointer");
+    EMC_REQUIRE(support_radius > 0 && support_radius < 0.5 * n, "support radius must be in (0, n/2)");
+    KTimer kt(ctx, "align_eval_grad");
+    hipLaunchKernelGGL(align_eval_grad_kernel, dim3(n_part), dim3(kBatchThreads), 0, ctx->stream,
+                       (const float2*)d_F2, n, (const float2*)d_Q, d_N, d_A, d_MdM, d_shift,
+                       (float)(support_radius * support_radius), d_out);
+    EMC_HIP(hipGetLastError());
+  });
+}
+
+}  // extern "C"

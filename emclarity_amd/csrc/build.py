@@ -65,7 +65,22 @@ def build(jobs: int | None = None, verbose: bool = False) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_harness()
     return LIB
+
+
+HARNESS_SRC = PKG / "harness" / "emc_harness.c"
+HARNESS_BIN = PKG / "lib" / "emc_harness"
+
+
+def build_harness() -> Path:
+    """The plain C harness, linked against the library (gcc, no HIP headers)."""
+    cmd = ["gcc", "-O2", "-Wall", f"-I{ROOT / 'include'}", str(HARNESS_SRC), f"-L{LIB.parent}",
+           "-lemc_hip", f"-Wl,-rpath,{LIB.parent}", "-lm", "-o", str(HARNESS_BIN)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"harness build failed:\n{r.stdout}\n{r.stderr}")
+    return HARNESS_BIN
 
 
 if __name__ == "__main__":

@@ -84,7 +84,7 @@ def degrade(vol, rng, max_tilt=60.0, pixel_A=2.5, defocus_um=3.0, snr=0.1, ctf=T
     makes every cross-correlation against the reference negative."""
     n = vol.shape[0]
     F = np.fft.fftn(vol)
-    W = wedge_mask(n, max_tilt)
+    W = wedge_mask(n, max_tilt) if max_tilt is not None else np.ones(F.shape)
     if ctf:
         W = W * np.abs(ctf3d(n, pixel_A, defocus_um))
     sig = np.real(np.fft.ifftn(F * W))

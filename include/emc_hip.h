@@ -257,6 +257,53 @@ emc_status emc_align_grid(emc_ctx* ctx, int n, const float* d_particle, const fl
                           const int peak_com[3], const double* angles_pts, int n_orient, int mode,
                           double* ccc_out, double* shift_out);
 
+/* ------------------------------- batched alignment and its adjoint (new) */
+
+/* The batched form of the alignment chain (many particles x many
+ * orientations per launch) and its hand-written adjoint. No single reference
+ * function corresponds: they replace the per-orientation, per-particle loop
+ * of alignment/BH_alignRaw3d_v2.m:1119-1419 for callers that align
+ * thousands of particles. The reference is rotated in Fourier space
+ * (2x-oversampled spectrum, trilinear), the correlation is evaluated only on
+ * the candidate-shift cube by a pruned inverse DFT. Box sizes 8..128.
+ *
+ * CC_o(s) = sum_k Re[a conj(c_o) e^{2 pi i k.s/n}] / sqrt(sum|a|^2 sum|c_o|^2)
+ *   a = P_centred * bandpass, c_o = F_ref(R_o^T k) * wedge_sym * bandpass,
+ * summed over the half grid inside `support_radius` (index units). */
+
+/* Reference spectrum for one box size: d_F2 receives the half spectrum of
+ * the reference zero-padded to 2n with its centre n/2 at the origin:
+ * (n+1) x 2n x 2n complex64. */
+emc_status emc_batch_reference_spectrum(emc_ctx* ctx, int n, const float* d_ref, void* d_F2);
+
+/* Per-particle preparation: d_particle n^3 (centred), d_bandpass n^3 FFT
+ * order, d_wedge n^3 centred SF3D (NULL = 1). Writes Q (half grid complex64,
+ * layout [kz][kx][ky]), N (same layout, float) and *d_A (one double). */
+emc_status emc_batch_prepare_particle(emc_ctx* ctx, int n, const float* d_particle,
+                                      const float* d_bandpass, const float* d_wedge, void* d_Q,
+                                      float* d_N, double* d_A);
+
+/* Score every (particle, orientation): particles p = 0..n_part-1 with their
+ * Q/N/A stacked contiguously, start matrices d_start_cm (n_part x 9 doubles,
+ * column-major), local angles (phi, theta, psi) in angles_pts (host), R =
+ * start * BH_defineMatrix([phi,theta,psi-phi],'Bah','inv') composed in FP64
+ * on the device. peak_radius 4, 6 or 8 (shift search sphere). Outputs
+ * (device): d_score[n_part*n_orient] and d_shift[3*n_part*n_orient]
+ * (x,y,z, parabolic sub-pixel), orientation fastest. Stream-ordered. */
+emc_status emc_align_batched(emc_ctx* ctx, int n, int n_part, const void* d_F2, const void* d_Q,
+                             const float* d_N, const double* d_A, const double* d_start_cm,
+                             const double* angles_pts, int n_orient, int peak_radius,
+                             double support_radius, float* d_score, float* d_shift);
+
+/* Adjoint: score and gradient at one pose per particle. d_MdM: per particle
+ * 36 floats = M = 2 R^T and dM/dphi, dM/dtheta, dM/dpsi (row-major, angles
+ * in radians); d_shift: 3 floats per particle. d_out: 7 doubles per particle
+ * = [CC, dCC/dphi, dCC/dtheta, dCC/dpsi, dCC/dsx, dCC/dsy, dCC/dsz].
+ * Stream-ordered. */
+emc_status emc_align_eval_grad(emc_ctx* ctx, int n, int n_part, const void* d_F2, const void* d_Q,
+                               const float* d_N, const double* d_A, const float* d_MdM,
+                               const float* d_shift, double support_radius, double* d_out);
+
 /* ------------------------------------------------- template search (A16) */
 
 /* One tomogram chunk of alignment/BH_templateSearch3d_2.m:713-887 for the

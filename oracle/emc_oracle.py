@@ -740,3 +740,163 @@ def template_search_chunk(tomo_chunk, template, grid_angles, pad_bin, trim_valid
             ang = np.where(rep, aid, ang)
         aid += 1
     return mag, ang
+
+
+# --------------------------------------------------------------------------
+# Batched Fourier-space alignment score and its gradient (this package's
+# batched path; no reference counterpart: it replaces the per-orientation
+# loop of BH_alignRaw3d_v2.m:1119-1419 for many particles). Restated here in
+# FP64 on the FULL frequency grid with numpy's inverse FFT, independently of
+# the device's half-grid pruned DFT.
+# --------------------------------------------------------------------------
+
+
+def batch_reference_spectrum(ref):
+    """rfftn of the reference zero-padded to 2n with its centre n//2 at the
+    origin; shape (2n, 2n, n+1) [z, y, x]."""
+    ref = np.asarray(ref, dtype=np.float64)
+    n = ref.shape[0]
+    m = 2 * n
+    pad = np.zeros((m, m, m))
+    idx = (np.arange(n) - n // 2) % m
+    pad[np.ix_(idx, idx, idx)] = ref
+    return np.fft.rfftn(pad)
+
+
+def _f2_texel(F2, n, tx, ty, tz):
+    m = 2 * n
+    neg = tx < 0
+    x = np.where(neg, -tx, tx)
+    y = np.where(neg, -ty, ty) % m
+    z = np.where(neg, -tz, tz) % m
+    v = F2[z, y, x]
+    return np.where(neg, np.conj(v), v)
+
+
+def batch_sample(F2, n, j):
+    """Trilinear sample of the 2n spectrum at index coordinates j (3, ...),
+    with the Hermitian mirror for negative x. Returns (f, grad[3])."""
+    jx, jy, jz = j
+    x0, y0, z0 = np.floor(jx).astype(int), np.floor(jy).astype(int), np.floor(jz).astype(int)
+    fx, fy, fz = jx - x0, jy - y0, jz - z0
+    c = {}
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):
+                c[dz, dy, dx] = _f2_texel(F2, n, x0 + dx, y0 + dy, z0 + dz)
+    lerp = lambda a, b, w: a + w * (b - a)
+    c00 = lerp(c[0, 0, 0], c[0, 0, 1], fx)
+    c01 = lerp(c[0, 1, 0], c[0, 1, 1], fx)
+    c10 = lerp(c[1, 0, 0], c[1, 0, 1], fx)
+    c11 = lerp(c[1, 1, 0], c[1, 1, 1], fx)
+    c0, c1 = lerp(c00, c01, fy), lerp(c10, c11, fy)
+    f = lerp(c0, c1, fz)
+    gz = c1 - c0
+    gy = lerp(c01 - c00, c11 - c10, fz)
+    dxv = {(dz, dy): c[dz, dy, 1] - c[dz, dy, 0] for dz in (0, 1) for dy in (0, 1)}
+    gx = lerp(lerp(dxv[0, 0], dxv[0, 1], fy), lerp(dxv[1, 0], dxv[1, 1], fy), fz)
+    return f, (gx, gy, gz)
+
+
+@dataclass
+class BatchParticle:
+    a: np.ndarray        # P_centred * bandpass, full grid FFT order, DC 0
+    w: np.ndarray        # wedge_sym * bandpass (real), full grid
+    A: float             # sum |a|^2
+    kvec: np.ndarray     # (3, m) integer frequencies inside the support
+    sel: tuple           # indices of those frequencies in the full grid
+
+
+def batch_prepare(particle, bandpass, wedge, support_radius):
+    p = np.asarray(particle, dtype=np.float64)
+    n = p.shape[0]
+    P = np.fft.fftn(np.fft.ifftshift(p))
+    P.flat[0] = 0.0
+    bp = np.asarray(bandpass, dtype=np.float64)
+    if wedge is None:
+        S = np.ones_like(bp)
+    else:
+        Wi = np.fft.ifftshift(np.asarray(wedge, dtype=np.float64))   # FFT order
+        Wm = np.roll(Wi[::-1, ::-1, ::-1], 1, axis=(0, 1, 2))        # W(-k)
+        S = 0.5 * (Wi + Wm)
+    f = np.fft.fftfreq(n) * n
+    kz, ky, kx = np.meshgrid(f, f, f, indexing="ij")
+    inside = (kx ** 2 + ky ** 2 + kz ** 2 <= support_radius ** 2)
+    inside.flat[0] = False
+    sel = np.nonzero(inside)
+    kvec = np.stack([kx[sel], ky[sel], kz[sel]])
+    a = P * bp
+    # the particle norm runs over the whole grid (the bandpass sets its support)
+    return BatchParticle(a, S * bp, float(np.sum(np.abs(a) ** 2)), kvec, sel)
+
+
+def batch_corr_terms(F2, n, bp_: BatchParticle, R):
+    """c(k) = F(2 R^T k) w(k) on the support, and a(k)."""
+    M = 2.0 * np.asarray(R, dtype=np.float64).T
+    j = M @ bp_.kvec
+    f, g = batch_sample(F2, n, j)
+    return f, g, M
+
+
+def batch_score_map(F2, n, bp_: BatchParticle, R, peak_radius):
+    """(score, shift) of one (particle, orientation): normalised CC over the
+    candidate shifts |d| <= r, parabolic sub-pixel refinement (the device's
+    estimator)."""
+    f, _, _ = batch_corr_terms(F2, n, bp_, R)
+    c = f * bp_.w[bp_.sel]
+    X = np.zeros((n, n, n), dtype=np.complex128)
+    X[bp_.sel] = bp_.a[bp_.sel] * np.conj(c)
+    M = np.real(np.fft.ifftn(X)) * X.size                # sum_k Re[X e^{+2 pi i k.d/n}]
+    D = float(np.sum(np.abs(c) ** 2))
+    r = peak_radius
+    best, bv = None, -np.inf
+    for dz in range(-r, r + 1):
+        for dy in range(-r, r + 1):
+            for dx in range(-r, r + 1):
+                if dx * dx + dy * dy + dz * dz > r * r:
+                    continue
+                v = M[dz % n, dy % n, dx % n]
+                if v > bv:
+                    bv, best = v, (dx, dy, dz)
+    sh = []
+    for ax in range(3):
+        d = list(best)
+        if abs(d[ax]) >= r:
+            sh.append(float(d[ax]))
+            continue
+        lo, hi = list(d), list(d)
+        lo[ax] -= 1
+        hi[ax] += 1
+        a_ = M[lo[2] % n, lo[1] % n, lo[0] % n]
+        c_ = M[hi[2] % n, hi[1] % n, hi[0] % n]
+        den = a_ - 2 * bv + c_
+        off = 0.0 if not den < 0 else min(0.5, max(-0.5, 0.5 * (a_ - c_) / den))
+        sh.append(d[ax] + off)
+    return bv / math.sqrt(bp_.A * D), np.array(sh)
+
+
+def batch_eval(F2, n, bp_: BatchParticle, R, s, dR=None):
+    """Score at (R, s) and, with dR (3 derivative matrices of R), the
+    gradient [d/dparam_1..3, d/ds_x, d/ds_y, d/ds_z] (FP64, full grid)."""
+    f, g, M = batch_corr_terms(F2, n, bp_, R)
+    w = bp_.w[bp_.sel]
+    a = bp_.a[bp_.sel]
+    k = bp_.kvec
+    e = np.exp(2j * np.pi * (k.T @ np.asarray(s, dtype=np.float64)) / n)
+    Y = a * np.conj(f * w) * e
+    num = float(np.sum(Y.real))
+    D = float(np.sum(np.abs(f * w) ** 2))
+    norm = math.sqrt(bp_.A * D)
+    score = num / norm
+    if dR is None:
+        return score
+    grad = np.zeros(6)
+    for i in range(3):
+        grad[3 + i] = float(np.sum((-Y.imag) * 2 * np.pi * k[i] / n)) / norm
+    for q in range(3):
+        dj = 2.0 * np.asarray(dR[q], dtype=np.float64).T @ k
+        df = g[0] * dj[0] + g[1] * dj[1] + g[2] * dj[2]
+        dnum = float(np.sum(np.real(a * np.conj(df * w) * e)))
+        dD = float(np.sum(2.0 * np.real(np.conj(f * w) * df * w)))
+        grad[q] = dnum / norm - num * dD / (2.0 * D * norm)
+    return score, grad

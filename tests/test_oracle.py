@@ -350,3 +350,61 @@ def test_template_search_finds_inserted_template():
     # the peak lands on the centre of the inserted copy: its box centre c
     # plus the [1,1,1] forward shift of the rotated template; valid area at 4
     assert np.array_equal(np.array(i)[::-1] + 4, c + 1)
+
+
+# ------------------------------------------- batched Fourier-space scoring
+
+def _batched_case(n=24):
+    from emclarity_amd import synthetic as S
+    c = S.make_align_case(4, n=n, euler=(40.0, 30.0, 20.0), snr=None, n_blobs=10, blob_radius=0.22 * n,
+                          ctf=False, max_tilt=None)
+    bp = O.bandpass3d((n, n, n), 1e-3, 40.0, 8.0, 2.5)
+    F2 = O.batch_reference_spectrum(c.ref)
+    return c, bp, F2
+
+
+def test_batch_spectrum_fold_and_rotation_identity():
+    c, bp, F2 = _batched_case()
+    n = c.ref.shape[0]
+    # at even index coordinates the 2x spectrum equals the unpadded spectrum
+    # of the centred reference (zero padding = sinc interpolation)
+    direct = np.fft.fftn(np.fft.ifftshift(c.ref.astype(np.float64)))
+    k = np.array([[3, -2, 5], [0, 4, -7], [-6, 1, 2]]).T
+    f, _ = O.batch_sample(F2, n, 2.0 * k)
+    assert np.allclose(f, direct[k[2] % n, k[1] % n, k[0] % n], rtol=1e-10, atol=1e-9)
+    # Hermitian fold: F(-j) = conj F(j) at a non-integer point
+    j = np.array([[1.3], [-2.7], [0.4]])
+    fa, _ = O.batch_sample(F2, n, j)
+    fb, _ = O.batch_sample(F2, n, -j)
+    assert np.allclose(fa, np.conj(fb), rtol=1e-12)
+
+
+def test_batch_score_prefers_truth_and_gradient_matches_fd():
+    c, bp, F2 = _batched_case()
+    n = c.ref.shape[0]
+    bpp = O.batch_prepare(c.particle, bp, None, 9.0)
+    Rt = c.rotmat
+    st, sht = O.batch_score_map(F2, n, bpp, Rt, 4)
+    so, _ = O.batch_score_map(F2, n, bpp, O.define_matrix([0, 0, 0], "Bah", "inv"), 4)
+    assert st > so and st > 0.95
+    assert np.max(np.abs(sht - c.shift)) < 0.25
+    # analytic gradient vs central differences (FP64): rotation increments
+    # about x, y, z and the shifts
+    G = [np.array([[0, 0, 0], [0, 0, -1], [0, 1, 0.0]]), np.array([[0, 0, 1], [0, 0, 0], [-1, 0, 0.0]]),
+         np.array([[0, -1, 0], [1, 0, 0], [0, 0, 0.0]])]
+    R0 = Rt @ O.define_matrix([5, 3, 2], "Bah", "forward")
+    s0 = c.shift + np.array([0.3, -0.2, 0.1])
+    _, g = O.batch_eval(F2, n, bpp, R0, s0, [R0 @ Gi for Gi in G])
+    h = 1e-5
+    for i in range(3):
+        w = np.zeros(3)
+        w[i] = h
+        Rp = R0 @ (np.eye(3) + np.sin(h) * G[i] + (1 - np.cos(h)) * G[i] @ G[i])
+        Rm = R0 @ (np.eye(3) - np.sin(h) * G[i] + (1 - np.cos(h)) * G[i] @ G[i])
+        fd = (O.batch_eval(F2, n, bpp, Rp, s0) - O.batch_eval(F2, n, bpp, Rm, s0)) / (2 * h)
+        assert abs(fd - g[i]) < 1e-4 * max(abs(g[i]), 1e-3)
+    for i in range(3):
+        e = np.zeros(3)
+        e[i] = 1e-5
+        fd = (O.batch_eval(F2, n, bpp, R0, s0 + e) - O.batch_eval(F2, n, bpp, R0, s0 - e)) / 2e-5
+        assert abs(fd - g[3 + i]) < 1e-5 * max(abs(g[3 + i]), 1e-3)
