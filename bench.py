@@ -1,17 +1,16 @@
 #!/usr/bin/env python3
 """Benchmark: subtomogram alignments/s on one MI355X.
 
-Workload (one "step" = one batch of particles through the alignment):
-mixed-size subtomograms (64^3/96^3/128^3 in 50/30/20 proportion, FP32,
-2.5 A/px), each with its own missing wedge (max tilt U[50,70] deg) and
-defocus (U[1.5,4.0] um), SNR 0.1, true pose inside a +-15 deg local search of
-1,045 orientations around a perturbed start. Inputs are resident in HBM
-before the timed region. ``value`` counts particles aligned per second; the
-accuracy block reports how many met <=1 deg and <=0.5 px against ground truth.
-
-Paths: ``plain`` is the drop-in per-particle grid search (the two-stage
-alignRaw body for every orientation, then the reference's local refinement
-and final translational pass).
+Workload (one "step" = one batch of particles through alignment): mixed-size
+subtomograms (64^3/96^3/128^3 in 50/30/20 proportion, FP32, 2.5 A/px), each
+with its own missing wedge (max tilt U[50,70] deg) and defocus (U[1.5,4.0]
+um), SNR 0.1, true pose inside a +-15 deg local search around a perturbed
+start. Particles and wedges are resident in HBM before the timed region.
+``value`` counts particles aligned per second by the batched + adjoint path;
+the accuracy block reports how many met <=1 deg and <=0.5 px against ground
+truth. The plain GPU path (the drop-in two-stage grid search over the
+1,045-orientation local grid, then the reference's refinement) is timed on a
+subset of the same particles for the speed-up at equal accuracy.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
 """
@@ -30,20 +29,39 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "subtomo alignments/s at <=1 deg & <=0.5 px error; batched+adjoint vs plain GPU"
+FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=10, help="particles per step")
-    ap.add_argument("--path", default="plain", choices=["plain"])
+    ap.add_argument("--batch", type=int, default=100, help="particles per step (batched path)")
+    ap.add_argument("--plain", type=int, default=10, help="particles timed on the plain path (0 = skip)")
+    ap.add_argument("--refine-iters", type=int, default=12)
     ap.add_argument("--cpu-sample", type=float, default=20.0,
                     help="seconds budget of the CPU oracle sample (0 = skip)")
     ap.add_argument("--seed", type=int, default=2024)
     return ap.parse_args()
+
+
+def accuracy_rows(particles, rows):
+    """Plain-path rows [ref, pIdx, phi, theta, psi-phi, CCC, 1, dx, dy, dz]."""
+    from emclarity_amd.workload import local_matrix
+    Rs = [p.start @ local_matrix(r[2], r[3], r[4] + r[2]) for p, r in zip(particles, rows)]
+    return accuracy(particles, Rs, [np.asarray(r[7:10]) for r in rows])
+
+
+def accuracy(particles, Rs, shifts):
+    from emclarity_amd.workload import angle_error_deg
+    errs = np.array([(angle_error_deg(R, p.truth_R), float(np.max(np.abs(s - p.shift))))
+                     for p, R, s in zip(particles, Rs, shifts)])
+    ok = int(np.sum((errs[:, 0] <= 1.0) & (errs[:, 1] <= 0.5)))
+    return {"within_1deg_0p5px": ok, "of": len(particles), "fraction": round(ok / len(particles), 4),
+            "median_angle_err_deg": round(float(np.median(errs[:, 0])), 4),
+            "median_shift_err_px": round(float(np.median(errs[:, 1])), 4)}
 
 
 def phi_steps(grid):
@@ -54,71 +72,72 @@ def phi_steps(grid):
     return steps
 
 
-class PlainBatch:
-    """Per-particle plain GPU grid search over a batch (device-resident)."""
+class PlainRun:
+    """The drop-in per-particle grid search (device-resident inputs)."""
 
-    def __init__(self, particles, classes, grid):
+    def __init__(self, particles, classes, wedges, grid):
         from emclarity_amd.align_raw import ParticleAligner
-        from emclarity_amd.masking import BH_weightMaskMex
         self.grid = grid
         self.psteps = phi_steps(grid)
-        self.aligners = []
-        for p in particles:
-            sc = classes[p.n]
-            n = p.n
-            wedge = BH_weightMaskMex((n, n, n), 1, p.tlt, [n // 2 + 1] * 3, [[n, n, n], [0, 0, 0]])
-            self.aligners.append(ParticleAligner(p.data, sc.ref, wedge, sc.bandpass, sc.peak_mask,
-                                                 vol_mask=sc.vol_mask, angles=p.start))
+        self.aligners = [ParticleAligner(p.data, classes[p.n].ref, w, classes[p.n].bandpass,
+                                         classes[p.n].peak_mask, vol_mask=classes[p.n].vol_mask,
+                                         angles=p.start) for p, w in zip(particles, wedges)]
 
     def run(self):
         from emclarity_amd.align_raw import align_particle
-        out = []
-        for al in self.aligners:
-            r = align_particle(al, self.grid, phi_inc_of=lambda th: self.psteps.get(float(th), 0.0),
-                               theta_inc=3.0, psi_inc=3.0, refine=True)
-            out.append(r.row)
-        return out
+        return [align_particle(al, self.grid, phi_inc_of=lambda th: self.psteps.get(float(th), 0.0),
+                               theta_inc=3.0, psi_inc=3.0, refine=True).row for al in self.aligners]
 
 
-def accuracy(particles, rows):
-    from emclarity_amd.workload import angle_error_deg, local_matrix
-    ok = 0
-    errs = []
-    for p, row in zip(particles, rows):
-        R = p.start @ local_matrix(row[2], row[3], row[4] + row[2])
-        ea = angle_error_deg(R, p.truth_R)
-        es = float(np.max(np.abs(np.asarray(row[7:10]) - p.shift)))
-        errs.append((ea, es))
-        ok += int(ea <= 1.0 and es <= 0.5)
-    e = np.array(errs)
-    return {"within_1deg_0p5px": ok, "of": len(rows), "median_angle_err_deg": float(np.median(e[:, 0])),
-            "median_shift_err_px": float(np.median(e[:, 1]))}
+class BatchedRun:
+    """Batched Fourier-space search over a coarse local grid + adjoint
+    refinement, one BatchAligner per box size (particles, wedges resident)."""
+
+    def __init__(self, particles, classes, wedges, grid, refine_iters):
+        self.groups = {}
+        for i, p in enumerate(particles):
+            self.groups.setdefault(p.n, []).append(i)
+        self.particles, self.classes, self.wedges = particles, classes, wedges
+        self.grid, self.iters = grid, refine_iters
+
+    def run(self):
+        from emclarity_amd.batched import BatchAligner, align_batch
+        R = [None] * len(self.particles)
+        S = [None] * len(self.particles)
+        self.units = {}
+        for n, idx in self.groups.items():
+            sc = self.classes[n]
+            al = BatchAligner(n, sc.ref, sc.bandpass, peak_radius=6)
+            al.set_particles([self.particles[i].data for i in idx], [self.wedges[i] for i in idx],
+                             [self.particles[i].start for i in idx])
+            r, s, _, _ = align_batch(al, self.grid, refine_iters=self.iters)
+            self.units[n] = (len(idx) * len(self.grid), al.support)
+            for k, i in enumerate(idx):
+                R[i], S[i] = r[k], s[k]
+        return R, S
 
 
-def cpu_baseline(budget_s, classes, particles):
-    """FP64 oracle (CPU restatement of the two-stage body) on a bounded
-    sample: a few orientations per size class, extrapolated to the 1,045 +
-    refinement orientations each particle needs."""
+def cpu_baseline(budget_s, classes, particles, host, wedges):
+    """FP64 oracle (CPU restatement of the reference's two-stage body) on a
+    bounded sample: orientations per size class within the time budget,
+    extrapolated to the 1,045 + refinement orientations a particle needs."""
     from oracle import emc_oracle as O
-    from emclarity_amd.masking import BH_weightMaskMex
-    per_orient = {}
+    per_orient, counts = {}, {}
     t_all = time.perf_counter()
-    counts = {}
     for n in (64, 96, 128):
-        p = next(q for q in particles if q.n == n)
-        sc = classes[n]
-        wedge = BH_weightMaskMex((n, n, n), 1, p.tlt, [n // 2 + 1] * 3, [[n, n, n], [0, 0, 0]]).get()
-        inp = O.AlignInputs(particle=p.data.astype(np.float64), ref=sc.ref.astype(np.float64),
+        i = next(k for k, q in enumerate(particles) if q.n == n)
+        p, sc = particles[i], classes[n]
+        inp = O.AlignInputs(particle=host[i].astype(np.float64), ref=sc.ref.astype(np.float64),
                             ref_wgt_rot=None, ref_wgt=1.0,
                             ref_ft1=O.ref_ft1_of(sc.ref.astype(np.float64), sc.vol_mask, [[0] * 3, [0] * 3]),
                             vol_mask=sc.vol_mask.astype(np.float64), peak_mask=sc.peak_mask.astype(np.float64),
-                            bandpass=sc.bandpass.astype(np.float64), wedge=wedge.astype(np.float64),
+                            bandpass=sc.bandpass.astype(np.float64), wedge=wedges[i].get().astype(np.float64),
                             angles=p.start, shift_val=np.zeros(3), pad_calc=[[0] * 3, [0] * 3])
         k = 0
         t0 = time.perf_counter()
         share = budget_s * {64: 0.2, 96: 0.3, 128: 0.5}[n]
         while True:
-            R = p.start @ O.define_matrix([3.0 * k, 3.0, 0.0 - 3.0 * k], "Bah", "inv")
+            R = p.start @ O.define_matrix([3.0 * k, 3.0, -3.0 * k], "Bah", "inv")
             O.align_orientation(inp, R, np.zeros(3))
             k += 1
             if time.perf_counter() - t0 > share:
@@ -127,20 +146,36 @@ def cpu_baseline(budget_s, classes, particles):
         counts[n] = k
     n_orient = 1045 + 45 + 1
     mix = {64: 0.5, 96: 0.3, 128: 0.2}
-    sec_per_particle = sum(mix[n] * per_orient[n] * n_orient for n in mix)
-    return {"value": 1.0 / sec_per_particle, "unit": "alignments/s", "cores": 1, "kind": "port",
-            "sample": (f"FP64 numpy oracle, two-stage alignRaw body: {counts[64]}/{counts[96]}/{counts[128]} "
-                       f"orientations at 64/96/128^3 ({time.perf_counter() - t_all:.1f} s), extrapolated to "
-                       f"{n_orient} orientations per particle, 50/30/20 mix; single thread")}
+    sec = sum(mix[n] * per_orient[n] * n_orient for n in mix)
+    return {"value": round(1.0 / sec, 6), "unit": "alignments/s", "cores": 1, "kind": "port",
+            "sample": (f"FP64 numpy oracle of the reference's two-stage alignRaw body: "
+                       f"{counts[64]}/{counts[96]}/{counts[128]} orientations at 64/96/128^3 "
+                       f"({time.perf_counter() - t_all:.1f} s), extrapolated to {n_orient} orientations "
+                       f"per particle, 50/30/20 mix; single thread")}
 
 
 def kernel_time(ctx, name):
     import ctypes as C
     from emclarity_amd import _lib
-    ms = C.c_double()
-    cnt = C.c_int64()
+    ms, cnt = C.c_double(), C.c_int64()
     _lib.call("emc_timing_read", ctx.handle, name.encode(), C.byref(ms), C.byref(cnt))
     return ms.value, cnt.value
+
+
+def timed(ctx, fn, steps, label):
+    from emclarity_amd import _lib
+    _lib.call("emc_timing_reset", ctx.handle)
+    _lib.call("emc_timing_enable", ctx.handle, 1)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    out = None
+    for s in range(steps):
+        out = fn()
+        print(f"[bench] {label} step {s + 1}/{steps} done", file=sys.stderr, flush=True)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    _lib.call("emc_timing_enable", ctx.handle, 0)
+    return out, dt
 
 
 def main():
@@ -149,8 +184,10 @@ def main():
         print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported (every box here has one GPU)",
               file=sys.stderr)
         sys.exit(2)
-    from emclarity_amd import _lib
-    from emclarity_amd.device import default_context
+    from emclarity_amd.batched import fused_flops_per_unit
+    from emclarity_amd.device import DeviceArray, default_context
+    from emclarity_amd.geometry import eulerSearch
+    from emclarity_amd.masking import BH_weightMaskMex
     from emclarity_amd.workload import local_grid, make_particle, make_size_class, mixed_sizes
 
     ctx = default_context()
@@ -160,46 +197,64 @@ def main():
         classes[n], blobs[n] = make_size_class(n)
     sizes = mixed_sizes(a.batch)
     particles = [make_particle(rng, classes[n], blobs[n]) for n in sizes]
-    grid = local_grid()
-    print(f"[bench] {len(particles)} particles {dict(zip(*np.unique(sizes, return_counts=True)))}, "
-          f"{len(grid)} orientations, path={a.path}", file=sys.stderr, flush=True)
-    runner = PlainBatch(particles, classes, grid)
-    for w in range(a.warmup):
-        rows = runner.run()
-        print(f"[bench] warmup {w + 1}/{a.warmup} done", file=sys.stderr, flush=True)
-    _lib.call("emc_timing_reset", ctx.handle)
-    _lib.call("emc_timing_enable", ctx.handle, 1)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for s in range(a.steps):
-        rows = runner.run()
-        print(f"[bench] step {s + 1}/{a.steps} done", file=sys.stderr, flush=True)
-    ctx.synchronize()
-    dt = time.perf_counter() - t0
-    _lib.call("emc_timing_enable", ctx.handle, 0)
-    acc = accuracy(particles, rows)
-    value = len(particles) * a.steps / dt
+    wedges = [BH_weightMaskMex((p.n,) * 3, 1, p.tlt, [p.n // 2 + 1] * 3, [[p.n] * 3, [0, 0, 0]])
+              for p in particles]
+    host = [p.data for p in particles]
+    for p in particles:
+        p.data = DeviceArray.from_numpy(p.data)   # resident before timing
+    coarse = eulerSearch("C1", 15, 6, 15, 6)
+    print(f"[bench] {len(particles)} particles {dict(zip(*[x.tolist() for x in np.unique(sizes, return_counts=True)]))}, "
+          f"coarse grid {len(coarse)} orientations + {a.refine_iters} adjoint iterations",
+          file=sys.stderr, flush=True)
 
-    # roofline of the rotate kernel (the plain path's most frequent kernel):
-    # algorithmic bytes per launch = 4 N_in + 4 N_out (+ 4 N for the volMask
-    # multiplier of the stage-2 particle rotation, averaged in by count)
-    ms, cnt = kernel_time(ctx, "xform3d")
-    vox = sum(p.n ** 3 for p in particles) / len(particles)
-    bytes_per_launch = 8.0 * vox * (1.0 + 0.25 / 5)   # 1 of 5 launches reads volMask too
-    achieved = bytes_per_launch * cnt / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "xform3d", "avg_us": round(ms * 1e3 / max(cnt, 1), 3), "launches": cnt}
-    cpu = cpu_baseline(a.cpu_sample, classes, particles) if a.cpu_sample > 0 else None
-    line = {"metric": METRIC, "value": round(value, 4), "unit": "alignments/s", "n_gpus": 1,
+    run = BatchedRun(particles, classes, wedges, coarse, a.refine_iters)
+    for w in range(a.warmup):
+        run.run()
+    (Rb, Sb), dt = timed(ctx, run.run, a.steps, "batched")
+    value = len(particles) * a.steps / dt
+    acc = accuracy(particles, Rb, Sb)
+
+    # roofline of the fused search kernel: FP32 vector arithmetic (it reads
+    # almost nothing from HBM: the reference spectrum and particle spectra
+    # stay in L2/MALL)
+    ms, cnt = kernel_time(ctx, "align_fused")
+    flops = sum(fused_flops_per_unit(n, sup, 6) * u for n, (u, sup) in run.units.items())
+    avg_us = ms * 1e3 / max(cnt, 1)
+    achieved = flops * a.steps / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
+    # the same units through the real-space chain would move
+    # B_unit = 4 W^3 + 16 C^3 + 40 H bytes each (H = C^2 (C/2+1))
+    b_unit = {n: 4 * n ** 3 + 16 * n ** 3 + 40 * n * n * (n // 2 + 1) for n in run.units}
+    eq_gbs = sum(b_unit[n] * u for n, (u, _) in run.units.items()) * a.steps / (ms * 1e-3) / 1e9
+    roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None, "kernel": "align_fused",
+            "avg_us": round(avg_us, 2), "launches": cnt, "flops_per_launch": round(flops / max(cnt / a.steps, 1)),
+            "note": "FP32 vector peak; the kernel's HBM traffic is near zero, see DESIGN.md",
+            "realspace_chain_equivalent_GBps": round(eq_gbs, 1)}
+
+    plain = None
+    if a.plain > 0:
+        sub = []
+        for n, k in ((64, 5), (96, 3), (128, 2)):
+            sub += [i for i, p in enumerate(particles) if p.n == n][: max(1, round(a.plain * k / 10))]
+        pp = [particles[i] for i in sub]
+        pr = PlainRun(pp, classes, [wedges[i] for i in sub], local_grid())
+        pr.run()
+        rows, pdt = timed(ctx, pr.run, 1, "plain")
+        plain = {"value": round(len(pp) / pdt, 4), "unit": "alignments/s", "particles": len(pp),
+                 "orientations": 1045, "accuracy": accuracy_rows(pp, rows)}
+
+    cpu = cpu_baseline(a.cpu_sample, classes, particles, host, wedges) if a.cpu_sample > 0 else None
+    line = {"metric": METRIC, "value": round(value, 3), "unit": "alignments/s", "n_gpus": 1,
             "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(dt * 1e3 / a.steps, 2),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
             "data": "synthetic",
-            "config": {"workload": "batched alignment, mixed 64/96/128^3 (50/30/20), per-particle wedge "
-                                   "U[50,70] deg + defocus U[1.5,4.0] um, SNR 0.1, 1,045-orientation local "
-                                   "search + refinement", "particles_per_step": len(particles),
-                       "path": a.path, "orientations": int(len(grid))},
-            "accuracy": acc, "roofline": roof, "cpu_baseline": cpu}
+            "config": {"workload": "batched alignment: mixed 64/96/128^3 subtomograms (50/30/20), own wedge "
+                                   "U[50,70] deg + defocus U[1.5,4.0] um, SNR 0.1, +-15 deg local search",
+                       "particles_per_step": len(particles), "path": "batched+adjoint",
+                       "coarse_orientations": int(len(coarse)), "refine_iterations": a.refine_iters},
+            "accuracy": acc, "plain_gpu": plain,
+            "speedup_vs_plain": round(value / plain["value"], 2) if plain else None,
+            "roofline": roof, "cpu_baseline": cpu}
     print(json.dumps(line))
 
 

@@ -188,6 +188,30 @@ class BatchAligner:
         return R, s, f
 
 
+def support_voxels(n, support):
+    """Half-grid frequencies inside the support sphere (what the fused
+    kernel evaluates per unit)."""
+    f = np.fft.fftfreq(n) * n
+    kz, ky = np.meshgrid(f, f, indexing="ij")
+    kx = np.arange(n // 2 + 1)
+    r2 = kz[:, :, None] ** 2 + ky[:, :, None] ** 2 + kx[None, None, :] ** 2
+    return int(np.count_nonzero(r2 <= support * support))
+
+
+def fused_flops_per_unit(n, support, peak_radius):
+    """FP32 operations of one (particle, orientation) in align_fused_kernel,
+    counted from its arithmetic (an FMA counts 2):
+      per support frequency: j = M k (9 FMA), trilinear (7 complex lerps =
+      14 FMA), X = Q conj(f) (4 FMA-equivalents), norm term (2), pruned z-DFT
+      (D complex MACs = 4 D FMA);
+      per kx plane: y-DFT (D^2 n complex MACs) and x expansion (D^3 real FMA-pairs)."""
+    D = 2 * peak_radius + 1
+    v = support_voxels(n, support)
+    per_k = 2 * (9 + 14 + 4 + 2 + 4 * D)
+    planes = n // 2 + 1
+    return v * per_k + planes * (2 * 4 * D * D * n + 2 * 2 * D ** 3)
+
+
 def align_batch(aligner: BatchAligner, grid_pts, refine_iters=12):
     """Coarse search over grid_pts, best per particle (first maximum), then
     adjoint refinement. Returns (R (P,3,3), shift (P,3), cc (P,), grid best
