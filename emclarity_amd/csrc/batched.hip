@@ -31,6 +31,7 @@
 //             / sqrt(sum_k |a|^2 * sum_k |c_o|^2)
 // with b the bandpass, S the particle's SF3D wedge symmetrised over k/-k so
 // that the product is Hermitian and the half grid is exact.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -381,13 +382,18 @@ __global__ void __launch_bounds__(kBatchThreads) align_fused_kernel(
 // d num / d a   =  sum_k Re[Q conj(grad f . dM_a k) e]
 // d D / d a     =  2 sum_k Re[conj(f) (grad f . dM_a k)] N
 // The derivative of the trilinear interpolant is taken inside each cell
-// (piecewise constant across cells). Output per particle: [score,
-// d/dphi, d/dtheta, d/dpsi (per radian), d/dsx, d/dsy, d/dsz].
+// (piecewise constant across cells). The support frequencies come from a
+// compact list (index into the Q/N layout, packed signed k); each particle's
+// list is split over gridDim.x workgroups that write FP64 partial sums,
+// folded by adjoint_fold_kernel into [score, d/dphi, d/dtheta, d/dpsi
+// (per radian), d/dsx, d/dsy, d/dsz].
+constexpr int kGradTerms = 11;
+
 __global__ void __launch_bounds__(kBatchThreads) align_eval_grad_kernel(
     const float2* __restrict__ F2, int n, const float2* __restrict__ Q, const float* __restrict__ Nw,
-    const double* __restrict__ A, const float* __restrict__ MdM, const float* __restrict__ shifts,
-    float kmax2, double* __restrict__ out) {
-  const int p = blockIdx.x;
+    const float* __restrict__ MdM, const float* __restrict__ shifts, const int2* __restrict__ supp,
+    int n_supp, double* __restrict__ part) {
+  const int p = blockIdx.y;
   const int cx = n / 2 + 1;
   const int64_t nh = (int64_t)cx * n * n;
   const float2* Qp = Q + (int64_t)p * nh;
@@ -396,52 +402,71 @@ __global__ void __launch_bounds__(kBatchThreads) align_eval_grad_kernel(
   if (threadIdx.x < 36) Ms[threadIdx.x] = MdM[36 * (int64_t)p + threadIdx.x];
   __syncthreads();
   const float s0 = shifts[3 * p], s1 = shifts[3 * p + 1], s2 = shifts[3 * p + 2];
-  double v[11];
+  const float inv_n2 = 2.0f / (float)n, tp = 6.283185307179586f / (float)n;
+  float v[kGradTerms];
 #pragma unroll
-  for (int i = 0; i < 11; ++i) v[i] = 0.0;
-  for (int64_t i = (int64_t)threadIdx.x; i < nh; i += blockDim.x) {
-    const int yy = (int)(i % n);
-    const int64_t zx = i / n;
-    const int kx = (int)(zx % cx), zz = (int)(zx / cx);
-    const int ky = signed_freq(yy, n), kz = signed_freq(zz, n);
-    if ((float)(kx * kx + ky * ky + kz * kz) > kmax2) continue;
-    if (kx == 0 && ky == 0 && kz == 0) continue;
-    const float fk[3] = {(float)kx, (float)ky, (float)kz};
+  for (int i = 0; i < kGradTerms; ++i) v[i] = 0.0f;
+  const int per = (n_supp + gridDim.x - 1) / gridDim.x;
+  const int i0 = blockIdx.x * per, i1 = min(n_supp, i0 + per);
+  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    const int2 e = supp[i];
+    const float fk[3] = {(float)((e.y & 255) - 128), (float)(((e.y >> 8) & 255) - 128),
+                         (float)(((e.y >> 16) & 255) - 128)};
     float j[3];
 #pragma unroll
     for (int r = 0; r < 3; ++r) j[r] = Ms[3 * r] * fk[0] + Ms[3 * r + 1] * fk[1] + Ms[3 * r + 2] * fk[2];
     const Tri tr = f2_sample<true>(F2, n, j[0], j[1], j[2]);
-    const float2 qv = Qp[i];
-    const float w = Np[i];
-    const double m = (kx == 0 || (n % 2 == 0 && kx == n / 2)) ? 1.0 : 2.0;
-    double sn, cs;
-    sincospi(2.0 * ((double)fk[0] * s0 + (double)fk[1] * s1 + (double)fk[2] * s2) / (double)n, &sn, &cs);
+    const float2 qv = Qp[e.x];
+    const float w = Np[e.x];
+    const float m = (e.y >> 24) ? 2.0f : 1.0f;   // Hermitian multiplicity of the x plane
+    float sn, cs;
+    sincospif((fk[0] * s0 + fk[1] * s1 + fk[2] * s2) * inv_n2, &sn, &cs);
     // X = Q conj(f); Y = X e
-    const double xr = (double)qv.x * tr.f.x + (double)qv.y * tr.f.y;
-    const double xi = (double)qv.y * tr.f.x - (double)qv.x * tr.f.y;
-    const double yr = xr * cs - xi * sn, yi = xr * sn + xi * cs;
+    const float xr = qv.x * tr.f.x + qv.y * tr.f.y;
+    const float xi = qv.y * tr.f.x - qv.x * tr.f.y;
+    const float yr = xr * cs - xi * sn, yi = xr * sn + xi * cs;
     v[0] += m * yr;
-    const double tp = 2.0 * 3.14159265358979323846 / (double)n;
-    v[1] += -m * yi * tp * fk[0];
-    v[2] += -m * yi * tp * fk[1];
-    v[3] += -m * yi * tp * fk[2];
-    v[4] += m * ((double)tr.f.x * tr.f.x + (double)tr.f.y * tr.f.y) * w;
+    v[1] -= m * yi * tp * fk[0];
+    v[2] -= m * yi * tp * fk[1];
+    v[3] -= m * yi * tp * fk[2];
+    v[4] += m * (tr.f.x * tr.f.x + tr.f.y * tr.f.y) * w;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       const float* dM = Ms + 9 * (a + 1);
       float dj[3];
 #pragma unroll
       for (int r = 0; r < 3; ++r) dj[r] = dM[3 * r] * fk[0] + dM[3 * r + 1] * fk[1] + dM[3 * r + 2] * fk[2];
-      const double dfr = (double)tr.g[0].x * dj[0] + (double)tr.g[1].x * dj[1] + (double)tr.g[2].x * dj[2];
-      const double dfi = (double)tr.g[0].y * dj[0] + (double)tr.g[1].y * dj[1] + (double)tr.g[2].y * dj[2];
+      const float dfr = tr.g[0].x * dj[0] + tr.g[1].x * dj[1] + tr.g[2].x * dj[2];
+      const float dfi = tr.g[0].y * dj[0] + tr.g[1].y * dj[1] + tr.g[2].y * dj[2];
       // Re[Q conj(df) e]
-      const double zr = (double)qv.x * dfr + (double)qv.y * dfi;
-      const double zi = (double)qv.y * dfr - (double)qv.x * dfi;
+      const float zr = qv.x * dfr + qv.y * dfi;
+      const float zi = qv.y * dfr - qv.x * dfi;
       v[5 + a] += m * (zr * cs - zi * sn);
-      v[8 + a] += m * 2.0 * ((double)tr.f.x * dfr + (double)tr.f.y * dfi) * w;
+      v[8 + a] += m * 2.0f * (tr.f.x * dfr + tr.f.y * dfi) * w;
     }
   }
-  block_sum<11>(v);
+  double d[kGradTerms];
+#pragma unroll
+  for (int i = 0; i < kGradTerms; ++i) d[i] = (double)v[i];
+  block_sum<kGradTerms>(d);
+  if (threadIdx.x == 0) {
+    double* o = part + ((int64_t)p * gridDim.x + blockIdx.x) * kGradTerms;
+#pragma unroll
+    for (int i = 0; i < kGradTerms; ++i) o[i] = d[i];
+  }
+}
+
+__global__ void __launch_bounds__(kRedThreads) adjoint_fold_kernel(const double* __restrict__ part, int g,
+                                                                   const double* __restrict__ A,
+                                                                   double* __restrict__ out) {
+  const int p = blockIdx.x;
+  double v[kGradTerms];
+#pragma unroll
+  for (int i = 0; i < kGradTerms; ++i) v[i] = 0.0;
+  for (int b = threadIdx.x; b < g; b += blockDim.x)
+#pragma unroll
+    for (int i = 0; i < kGradTerms; ++i) v[i] += part[((int64_t)p * g + b) * kGradTerms + i];
+  block_sum<kGradTerms>(v);
   if (threadIdx.x == 0) {
     const double num = v[0], Dn = v[4];
     const double norm = sqrt(A[p] * Dn);
@@ -450,6 +475,42 @@ __global__ void __launch_bounds__(kBatchThreads) align_eval_grad_kernel(
     for (int a = 0; a < 3; ++a) o[1 + a] = v[5 + a] / norm - num * v[8 + a] / (2.0 * Dn * norm);
     for (int i = 0; i < 3; ++i) o[4 + i] = v[1 + i] / norm;
   }
+}
+
+// Compact support list of the half grid, cached per (n, radius) in the
+// context: x = index in the [kz][kx][ky] layout of Q/N, y = packed signed
+// (kx, ky, kz) + 128 in bytes 0..2 and the Hermitian multiplicity flag
+// (x plane not self-mirrored) in byte 3. DC excluded.
+const int2* support_list(emc_ctx* ctx, int n, double radius, int* count) {
+  char key[64];
+  snprintf(key, sizeof(key), "supp_%d_%.6f", n, radius);
+  auto it = ctx->scratch.find(key);
+  if (it != ctx->scratch.end() && it->second.ptr) {
+    *count = (int)(it->second.bytes / sizeof(int2));  // allocated with the exact size
+    return (const int2*)it->second.ptr;
+  }
+  const int cx = n / 2 + 1;
+  const double r2 = radius * radius;
+  std::vector<int2> v;
+  for (int zz = 0; zz < n; ++zz)
+    for (int kx = 0; kx < cx; ++kx)
+      for (int yy = 0; yy < n; ++yy) {
+        const int ky = yy <= n / 2 ? yy : yy - n, kz = zz <= n / 2 ? zz : zz - n;
+        if ((double)(kx * kx + ky * ky + kz * kz) > r2 || (kx == 0 && ky == 0 && kz == 0)) continue;
+        const int herm = (kx == 0 || (n % 2 == 0 && kx == n / 2)) ? 0 : 1;
+        int2 e;
+        e.x = (zz * cx + kx) * n + yy;
+        e.y = (kx + 128) | ((ky + 128) << 8) | ((kz + 128) << 16) | (herm << 24);
+        v.push_back(e);
+      }
+  // pad to >= 32 entries so the scratch allocation is exactly the list size;
+  // padding points at DC, where Q = N = 0, so it adds nothing
+  while (v.size() < 32) v.push_back(int2{0, 128 | (128 << 8) | (128 << 16)});
+  *count = (int)v.size();
+  int2* d = (int2*)scratch(ctx, key, sizeof(int2) * v.size());
+  EMC_HIP(hipMemcpyAsync(d, v.data(), sizeof(int2) * v.size(), hipMemcpyHostToDevice, ctx->stream));
+  EMC_HIP(hipStreamSynchronize(ctx->stream));
+  return d;
 }
 
 template <int RR>
@@ -550,10 +611,20 @@ emc_status emc_align_eval_grad(emc_ctx* ctx, int n, int n_part, const void* d_F2
     EMC_REQUIRE(n_part > 0, "empty batch");
     EMC_REQUIRE(d_F2 && d_Q && d_N && d_A && d_MdM && d_shift && d_out, "null pointer");
     EMC_REQUIRE(support_radius > 0 && support_radius < 0.5 * n, "support radius must be in (0, n/2)");
-    KTimer kt(ctx, "align_eval_grad");
-    hipLaunchKernelGGL(align_eval_grad_kernel, dim3(n_part), dim3(kBatchThreads), 0, ctx->stream,
-                       (const float2*)d_F2, n, (const float2*)d_Q, d_N, d_A, d_MdM, d_shift,
-                       (float)(support_radius * support_radius), d_out);
+    int ns = 0;
+    const int2* supp = support_list(ctx, n, support_radius, &ns);
+    EMC_REQUIRE(ns > 0, "empty support");
+    // split every particle's support over enough workgroups to fill the
+    // device (>= ~4 per CU over the whole batch), at least 512 frequencies each
+    int g = std::max(1, std::min((ns + 511) / 512, (1024 + n_part - 1) / n_part));
+    double* part = (double*)scratch(ctx, "grad_part", sizeof(double) * kGradTerms * (size_t)g * n_part);
+    {
+      KTimer kt(ctx, "align_eval_grad");
+      hipLaunchKernelGGL(align_eval_grad_kernel, dim3(g, n_part), dim3(kBatchThreads), 0, ctx->stream,
+                         (const float2*)d_F2, n, (const float2*)d_Q, d_N, d_MdM, d_shift, supp, ns, part);
+    }
+    hipLaunchKernelGGL(adjoint_fold_kernel, dim3(n_part), dim3(kRedThreads), 0, ctx->stream,
+                       (const double*)part, g, d_A, d_out);
     EMC_HIP(hipGetLastError());
   });
 }
