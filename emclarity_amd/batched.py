@@ -66,30 +66,43 @@ def euler_derivatives(A, phi, theta, psi):
 
 
 def support_radius_of(bandpass, n, thresh=1e-4):
-    """Largest frequency radius (index units) where the bandpass exceeds
-    `thresh`, capped below n/2."""
+    """Largest frequency radius (index units) where the bandpass (FFT order)
+    exceeds `thresh`, capped below n/2."""
     bp = np.asarray(bandpass)
-    f = np.fft.fftfreq(n) * n
-    kz, ky, kx = np.meshgrid(f, f, f, indexing="ij")
-    r = np.sqrt(kx ** 2 + ky ** 2 + kz ** 2)
-    rs = float(r[bp > thresh].max()) if np.any(bp > thresh) else n / 4
+    f2 = (np.fft.fftfreq(n) * n).astype(np.int32) ** 2
+    r2 = f2[:, None, None] + f2[None, :, None] + f2[None, None, :]
+    m = bp > thresh
+    rs = math.sqrt(float(r2[m].max())) if m.any() else n / 4
     return min(rs + 0.5, n / 2 - 1.01)
 
 
-class BatchAligner:
-    """Particles of one box size n against one reference."""
+TABLE_FORMATS = {"c64": 0, "f16": 1}
 
-    def __init__(self, n, ref, bandpass, support_radius=None, peak_radius=6, ctx=None):
+
+class BatchAligner:
+    """Particles of one box size n against one reference. ``table`` is the
+    reference table format: "c64" (complex64) or "f16" (half-precision
+    parts, half the gather bytes)."""
+
+    def __init__(self, n, ref, bandpass, support_radius=None, peak_radius=6, ctx=None, table="c64"):
         self.ctx = ctx or default_context()
         self.n = int(n)
         self.peak_radius = int(peak_radius)
-        bp_host = bandpass.get() if isinstance(bandpass, DeviceArray) else np.asarray(bandpass)
-        self.support = float(support_radius) if support_radius else support_radius_of(bp_host, n)
-        self.bandpass = as_device(bp_host, np.float32, self.ctx)
+        if table not in TABLE_FORMATS:
+            raise ValueError(f"table must be one of {sorted(TABLE_FORMATS)}")
+        self.table = table
+        self._fmt = TABLE_FORMATS[table]
+        if support_radius:
+            self.support = float(support_radius)
+        else:
+            bp_host = bandpass.get() if isinstance(bandpass, DeviceArray) else np.asarray(bandpass)
+            self.support = support_radius_of(bp_host, n)
+        self.bandpass = as_device(bandpass, np.float32, self.ctx)
         n2 = 2 * self.n
-        self.F2 = DeviceArray((n2, n2, self.n + 1), np.complex64, self.ctx)
+        texel = np.complex64 if table == "c64" else np.float32   # f16: two halves per texel
+        self.F2 = DeviceArray((n2 + 2, n2 + 2, self.n + 2), texel, self.ctx)
         r = as_device(ref, np.float32, self.ctx)
-        _lib.call("emc_batch_reference_spectrum", self.ctx.handle, self.n, r.vp, self.F2.vp)
+        _lib.call("emc_batch_reference_spectrum", self.ctx.handle, self.n, r.vp, self._fmt, self.F2.vp)
         self.n_part = 0
 
     @property
@@ -104,16 +117,21 @@ class BatchAligner:
         self.Q = DeviceArray((P, self.nh), np.complex64, self.ctx)
         self.N = DeviceArray((P, self.nh), np.float32, self.ctx)
         self.A = DeviceArray((P,), np.float64, self.ctx)
+        # adjoint inputs (36 matrix + 3 shift floats per particle) and outputs,
+        # reused by every eval_grad call on this particle set
+        self._h_in = np.zeros(39 * P, np.float32)   # [P x 36 matrices | P x 3 shifts]
+        self._d_in = DeviceArray((39 * P,), np.float32, self.ctx)
+        self._d_out = DeviceArray((P, 7), np.float64, self.ctx)
         self.starts = np.stack([np.asarray(s, dtype=np.float64).reshape(3, 3) for s in starts])
         self.d_starts = DeviceArray.from_numpy(
             np.ascontiguousarray(self.starts.transpose(0, 2, 1).reshape(P, 9)), ctx=self.ctx)
-        for i, (p, w) in enumerate(zip(particles, wedges)):
-            pd = as_device(p, np.float32, self.ctx)
-            wd = None if w is None else as_device(w, np.float32, self.ctx)
-            _lib.call("emc_batch_prepare_particle", self.ctx.handle, self.n, pd.vp, self.bandpass.vp,
-                      wd.vp if wd is not None else None,
-                      C.c_void_p(self.Q.ptr + i * self.nh * 8), C.c_void_p(self.N.ptr + i * self.nh * 4),
-                      C.c_void_p(self.A.ptr + i * 8))
+        # device copies of host inputs stay referenced until the call returns
+        pd = [as_device(p, np.float32, self.ctx) for p in particles]
+        wd = [None if w is None else as_device(w, np.float32, self.ctx) for w in wedges]
+        hp = (C.c_void_p * P)(*[a.ptr for a in pd])
+        hw = (C.c_void_p * P)(*[None if a is None else a.ptr for a in wd])
+        _lib.call("emc_batch_prepare_particles", self.ctx.handle, self.n, P, hp, self.bandpass.vp, hw,
+                  self.Q.vp, self.N.vp, self.A.vp)
 
     # ------------------------------------------------------------ search
     def search(self, angles_pts):
@@ -122,7 +140,7 @@ class BatchAligner:
         O = pts.shape[0]
         score = DeviceArray((self.n_part, O), np.float32, self.ctx)
         shift = DeviceArray((self.n_part, O, 3), np.float32, self.ctx)
-        _lib.call("emc_align_batched", self.ctx.handle, self.n, self.n_part, self.F2.vp, self.Q.vp,
+        _lib.call("emc_align_batched", self.ctx.handle, self.n, self.n_part, self._fmt, self.F2.vp, self.Q.vp,
                   self.N.vp, self.A.vp, self.d_starts.vp, pts.ctypes.data_as(C.POINTER(C.c_double)), O,
                   self.peak_radius, self.support, score.vp, shift.vp)
         return score.get(), shift.get()
@@ -132,15 +150,18 @@ class BatchAligner:
         """CC and gradient at per-particle poses: R (P,3,3), dR (P,3,3,3)
         derivatives w.r.t. three rotation parameters, s (P,3)."""
         P = self.n_part
-        MdM = np.zeros((P, 4, 3, 3), np.float32)
-        MdM[:, 0] = 2.0 * np.transpose(R, (0, 2, 1))
-        MdM[:, 1:] = 2.0 * np.transpose(dR, (0, 1, 3, 2))
-        d_m = DeviceArray.from_numpy(MdM.reshape(P, 36), ctx=self.ctx)
-        d_s = DeviceArray.from_numpy(np.asarray(s, np.float32).reshape(P, 3), ctx=self.ctx)
-        out = DeviceArray((P, 7), np.float64, self.ctx)
-        _lib.call("emc_align_eval_grad", self.ctx.handle, self.n, P, self.F2.vp, self.Q.vp, self.N.vp,
-                  self.A.vp, d_m.vp, d_s.vp, self.support, out.vp)
-        o = out.get()
+        h = self._h_in
+        hm = h[:36 * P].reshape(P, 36)
+        hm[:, :9] = 2.0 * np.transpose(R, (0, 2, 1)).reshape(P, 9)
+        hm[:, 9:] = 2.0 * np.transpose(dR, (0, 1, 3, 2)).reshape(P, 27)
+        h[36 * P:] = np.asarray(s).reshape(3 * P)
+        _lib.call("emc_memcpy_h2d", self.ctx.handle, self._d_in.vp, h.ctypes.data_as(C.c_void_p),
+                  C.c_size_t(h.nbytes))
+        d_m = self._d_in.ptr
+        d_s = self._d_in.ptr + 36 * 4 * P
+        _lib.call("emc_align_eval_grad", self.ctx.handle, self.n, P, self._fmt, self.F2.vp, self.Q.vp, self.N.vp,
+                  self.A.vp, C.c_void_p(d_m), C.c_void_p(d_s), self.support, self._d_out.vp)
+        o = self._d_out.get()
         return o[:, 0], o[:, 1:]
 
     def _eval_local(self, R, s):
@@ -159,9 +180,10 @@ class BatchAligner:
         gw = np.linalg.norm(g[:, :3], axis=1) + 1e-12
         gs = np.linalg.norm(g[:, 3:], axis=1) + 1e-12
         H = np.zeros((P, 6, 6))
-        for i in range(P):
-            H[i] = np.diag([step_angle / gw[i]] * 3 + [step_shift / gs[i]] * 3)
+        H[:, [0, 1, 2], [0, 1, 2]] = (step_angle / gw)[:, None]
+        H[:, [3, 4, 5], [3, 4, 5]] = (step_shift / gs)[:, None]
         H0 = H.copy()
+        I6 = np.eye(6)
         t = np.ones(P)
         for _ in range(iters):
             d = np.einsum("pij,pj->pi", H, g)
@@ -174,14 +196,15 @@ class BatchAligner:
             st = s + step[:, 3:]
             ft, gt = self._eval_local(Rt, st)
             ok = ft > f + 1e-4 * t * np.einsum("pi,pi->p", d, g)
-            for i in np.flatnonzero(ok):
-                y = g[i] - gt[i]             # gradient of -CC: (-gt) - (-g)
-                sv = step[i]
-                sy = float(sv @ y)
-                if sy > 1e-18:
-                    rho = 1.0 / sy
-                    I = np.eye(6)
-                    H[i] = (I - rho * np.outer(sv, y)) @ H[i] @ (I - rho * np.outer(y, sv)) + rho * np.outer(sv, sv)
+            # BFGS update of the inverse Hessian of -CC where the step was taken
+            y = g - gt                       # gradient of -CC: (-gt) - (-g)
+            sy = np.einsum("pi,pi->p", step, y)
+            upd = ok & (sy > 1e-18)
+            if np.any(upd):
+                sv, yv = step[upd], y[upd]
+                rho = (1.0 / sy[upd])[:, None, None]
+                V = I6 - rho * sv[:, :, None] * yv[:, None, :]
+                H[upd] = V @ H[upd] @ np.transpose(V, (0, 2, 1)) + rho * sv[:, :, None] * sv[:, None, :]
             R[ok], s[ok], f[ok], g[ok] = Rt[ok], st[ok], ft[ok], gt[ok]
             t[ok] = np.minimum(1.0, t[ok] * 2.0)
             t[~ok] *= 0.3

@@ -34,6 +34,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 
 #include "ops.hpp"
 
@@ -63,23 +64,39 @@ __global__ void pad_origin_kernel(const float* __restrict__ ref, float* __restri
   out[((int64_t)Z * m + Y) * m + X] = v;
 }
 
-// ifftshift of an n^3 volume (centre n/2 -> origin)
-__global__ void ifftshift_kernel(const float* __restrict__ in, float* __restrict__ out, int n) {
+// Particles of one preparation chunk (kernel-argument pointer table).
+constexpr int kPrepChunk = 16;
+struct PrepPtrs {
+  const float* part[kPrepChunk];
+  const float* wedge[kPrepChunk];  // centred SF3D or null (= 1)
+};
+
+// ifftshift of each chunk volume (centre n/2 -> origin) into a contiguous
+// batch; blockIdx.z = z + n * particle.
+__global__ void ifftshift_many_kernel(PrepPtrs pp, float* __restrict__ out, int n) {
   const int X = blockIdx.x * blockDim.x + threadIdx.x;
   const int Y = blockIdx.y * blockDim.y + threadIdx.y;
-  const int Z = blockIdx.z;
+  const int Z = blockIdx.z % n, b = blockIdx.z / n;
   if (X >= n || Y >= n) return;
   const int x = shifted_index(X, n), y = shifted_index(Y, n), z = shifted_index(Z, n);
-  out[((int64_t)Z * n + Y) * n + X] = in[((int64_t)z * n + y) * n + x];
+  out[(((int64_t)b * n + Z) * n + Y) * n + X] = pp.part[b][((int64_t)z * n + y) * n + x];
 }
 
 // Q, N in the fused kernel's layout [kz][kx][ky] (ky fastest, coalesced for
 // threads running along ky); per-block partials of A = sum_full |P b|^2.
+// blockIdx.y = particle of the chunk; spectra, Q, N and partials are
+// stacked per particle.
 __global__ void __launch_bounds__(kRedThreads) prep_particle_kernel(
-    const float2* __restrict__ P, const float* __restrict__ bp, const float* __restrict__ wedge_c,
-    int n, float2* __restrict__ Q, float* __restrict__ N, double* __restrict__ part) {
+    const float2* __restrict__ P, const float* __restrict__ bp, PrepPtrs pp, int n,
+    float2* __restrict__ Q, float* __restrict__ N, double* __restrict__ part) {
   const int cx = n / 2 + 1;
   const int64_t nh = (int64_t)cx * n * n;
+  const int b = blockIdx.y;
+  const float* __restrict__ wedge_c = pp.wedge[b];
+  P += b * nh;
+  Q += b * nh;
+  N += b * nh;
+  part += (int64_t)b * gridDim.x;
   double v[1] = {0.0};
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nh;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -160,63 +177,152 @@ __global__ void compose_kernel(const double* __restrict__ start_cm, const double
 
 // ------------------------------------------------------------- sampling
 
-// F2: half grid of the 2n-padded reference, x in [0, n], y, z in [0, 2n).
-// Texels with x < 0 come from the Hermitian mirror.
-__device__ inline float2 f2_texel(const float2* __restrict__ F2, int n, int tx, int ty, int tz) {
-  const int m = 2 * n;
-  bool cj = false;
-  if (tx < 0) {
-    tx = -tx;
-    ty = -ty;
-    tz = -tz;
-    cj = true;
+// Reference table formats: complex64 texels, or complex with fp16 parts
+// (half the bytes per gather). The fp16 table holds F * s with s = 2^(14-e),
+// max |Re|,|Im| of F < 2^e, so every texel is below 2^14; the normalised
+// correlation and its gradient do not depend on s.
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+struct TabC64 {
+  using T = float2;
+  struct alignas(8) Pair {
+    float2 a, b;
+  };
+  __device__ static void store(T* dst, float2 v, float) { *dst = v; }
+  __device__ static void pair(const T* p, float2& a, float2& b) {
+    const Pair v = *reinterpret_cast<const Pair*>(p);  // x-adjacent texels, one 16-byte load
+    a = v.a;
+    b = v.b;
   }
-  ty = ty < 0 ? ty + m : ty;
-  tz = tz < 0 ? tz + m : tz;
-  const float2 v = F2[((int64_t)tz * m + ty) * (n + 1) + tx];
-  return cj ? make_float2(v.x, -v.y) : v;
+};
+
+struct TabF16 {
+  using T = half2_t;
+  struct alignas(4) Pair {
+    half2_t a, b;
+  };
+  __device__ static void store(T* dst, float2 v, float s) {
+    *dst = half2_t{(_Float16)(v.x * s), (_Float16)(v.y * s)};
+  }
+  __device__ static void pair(const T* p, float2& a, float2& b) {
+    const Pair v = *reinterpret_cast<const Pair*>(p);  // one 8-byte load
+    a = make_float2((float)v.a.x, (float)v.a.y);
+    b = make_float2((float)v.b.x, (float)v.b.y);
+  }
+};
+
+// Scale of the fp16 table: per-block maxima of max(|Re|, |Im|) (as
+// order-preserving bits of non-negative floats), then s = 2^(14 - e).
+__global__ void __launch_bounds__(kRedThreads) f2_maxabs_kernel(const float2* __restrict__ raw, int64_t count,
+                                                                unsigned long long* __restrict__ part) {
+  unsigned long long m = 0ull;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float2 v = raw[i];
+    const float a = fmaxf(fabsf(v.x), fabsf(v.y));
+    const unsigned long long u = (a == a) ? (unsigned long long)__float_as_uint(a) : 0ull;
+    m = u > m ? u : m;
+  }
+  m = block_max_u64(m);
+  if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+
+__global__ void __launch_bounds__(kRedThreads) f2_scale_kernel(const unsigned long long* __restrict__ part,
+                                                               int nb, float* __restrict__ scale) {
+  unsigned long long m = 0ull;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) m = part[i] > m ? part[i] : m;
+  m = block_max_u64(m);
+  if (threadIdx.x == 0) {
+    const float a = __uint_as_float((uint32_t)m);
+    int e = 0;
+    if (a > 0.0f && a <= 3.0e38f) (void)frexpf(a, &e);  // a = f 2^e, f in [0.5, 1)
+    scale[0] = (a > 0.0f && a <= 3.0e38f) ? ldexpf(1.0f, 14 - e) : 1.0f;
+  }
+}
+
+// Centred reference table F2c of the 2n-padded spectrum: tx in [0, n+1],
+// ty, tz in [-n, n+1], element [(tz + n) (2n+2) + (ty + n)] (n+2) + tx.
+// Texels beyond the raw half grid repeat it periodically (period 2n) or by
+// Hermitian symmetry (tx = n+1), so a folded sample with |j| <= n reads
+// its 2x2x2 footprint without wrapping.
+template <class Tab>
+__global__ void f2_centre_kernel(const float2* __restrict__ raw, int n, const float* __restrict__ scale,
+                                 typename Tab::T* __restrict__ F2c) {
+  const int X = n + 2, Y = 2 * n + 2, m = 2 * n;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int yi = blockIdx.y * blockDim.y + threadIdx.y;
+  const int zi = blockIdx.z;
+  if (x >= X || yi >= Y) return;
+  const int ty = yi - n, tz = zi - n;
+  float2 v;
+  if (x <= n) {
+    const int y = ty < 0 ? ty + m : (ty >= m ? ty - m : ty);
+    const int z = tz < 0 ? tz + m : (tz >= m ? tz - m : tz);
+    v = raw[((int64_t)z * m + y) * (n + 1) + x];
+  } else {  // tx = n + 1 = -(n - 1) mod 2n: conj of (n - 1, -ty, -tz)
+    int y = -ty, z = -tz;
+    y = y < 0 ? y + m : (y >= m ? y - m : y);
+    z = z < 0 ? z + m : (z >= m ? z - m : z);
+    const float2 w = raw[((int64_t)z * m + y) * (n + 1) + (n - 1)];
+    v = make_float2(w.x, -w.y);
+  }
+  Tab::store(F2c + ((int64_t)zi * Y + yi) * X + x, v, scale ? scale[0] : 1.0f);
 }
 
 struct Tri {
   float2 f;
-  float2 g[3];  // d f / d j_x, j_y, j_z
+  float2 g[3];  // d f / d j (per index unit)
 };
 
-template <bool GRAD>
-__device__ inline Tri f2_sample(const float2* __restrict__ F2, int n, float jx, float jy, float jz) {
+// Trilinear sample of the centred table at j (index units of the padded
+// grid). The trilinear interpolant of a Hermitian table is Hermitian,
+// I(-j) = conj I(j), so a sample with jx < 0 is folded as a whole. The
+// clamp keeps every read inside the table for any j; samples with
+// |j| < n are never clamped.
+template <bool GRAD, class Tab>
+__device__ inline Tri f2_sample(const typename Tab::T* __restrict__ F2c, int n, float jx, float jy, float jz) {
+  const bool cj = jx < 0.0f;
+  if (cj) {
+    jx = -jx;
+    jy = -jy;
+    jz = -jz;
+  }
+  const float fn = (float)n;
+  jx = fminf(jx, fn);
+  jy = fminf(fmaxf(jy, -fn), fn);
+  jz = fminf(fmaxf(jz, -fn), fn);
   const float xf = floorf(jx), yf = floorf(jy), zf = floorf(jz);
   const float fx = jx - xf, fy = jy - yf, fz = jz - zf;
-  const int x0 = (int)xf, y0 = (int)yf, z0 = (int)zf;
-  float2 c[2][2][2];
-#pragma unroll
-  for (int dz = 0; dz < 2; ++dz)
-#pragma unroll
-    for (int dy = 0; dy < 2; ++dy)
-#pragma unroll
-      for (int dx = 0; dx < 2; ++dx) c[dz][dy][dx] = f2_texel(F2, n, x0 + dx, y0 + dy, z0 + dz);
-  Tri t;
+  const int X = n + 2, XY = (2 * n + 2) * (n + 2);
+  const typename Tab::T* p = F2c + ((((int)zf + n) * (2 * n + 2) + ((int)yf + n)) * X + (int)xf);
+  float2 t000, t001, t010, t011, t100, t101, t110, t111;  // t[z][y][x]
+  Tab::pair(p, t000, t001);
+  Tab::pair(p + X, t010, t011);
+  Tab::pair(p + XY, t100, t101);
+  Tab::pair(p + XY + X, t110, t111);
   auto lerp2 = [](float2 a, float2 b, float w) {
     return make_float2(a.x + w * (b.x - a.x), a.y + w * (b.y - a.y));
   };
-  const float2 c00 = lerp2(c[0][0][0], c[0][0][1], fx), c01 = lerp2(c[0][1][0], c[0][1][1], fx);
-  const float2 c10 = lerp2(c[1][0][0], c[1][0][1], fx), c11 = lerp2(c[1][1][0], c[1][1][1], fx);
+  const float2 c00 = lerp2(t000, t001, fx), c01 = lerp2(t010, t011, fx);
+  const float2 c10 = lerp2(t100, t101, fx), c11 = lerp2(t110, t111, fx);
   const float2 c0 = lerp2(c00, c01, fy), c1 = lerp2(c10, c11, fy);
+  Tri t;
   t.f = lerp2(c0, c1, fz);
+  if (cj) t.f.y = -t.f.y;
   if (GRAD) {
-    // d/dz
     t.g[2] = make_float2(c1.x - c0.x, c1.y - c0.y);
-    // d/dy: lerp over z of (c?1 - c?0)
     const float2 dy0 = make_float2(c01.x - c00.x, c01.y - c00.y);
     const float2 dy1 = make_float2(c11.x - c10.x, c11.y - c10.y);
     t.g[1] = lerp2(dy0, dy1, fz);
-    // d/dx: bilinear over (y, z) of (c??1 - c??0)
-    float2 dx[2][2];
+    const float2 d00 = make_float2(t001.x - t000.x, t001.y - t000.y);
+    const float2 d01 = make_float2(t011.x - t010.x, t011.y - t010.y);
+    const float2 d10 = make_float2(t101.x - t100.x, t101.y - t100.y);
+    const float2 d11 = make_float2(t111.x - t110.x, t111.y - t110.y);
+    t.g[0] = lerp2(lerp2(d00, d01, fy), lerp2(d10, d11, fy), fz);
+    // d/dj of conj(I(-j)) = -conj(I'(-j))
+    if (cj)
 #pragma unroll
-    for (int dz = 0; dz < 2; ++dz)
-#pragma unroll
-      for (int dyy = 0; dyy < 2; ++dyy)
-        dx[dz][dyy] = make_float2(c[dz][dyy][1].x - c[dz][dyy][0].x, c[dz][dyy][1].y - c[dz][dyy][0].y);
-    t.g[0] = lerp2(lerp2(dx[0][0], dx[0][1], fy), lerp2(dx[1][0], dx[1][1], fy), fz);
+      for (int i = 0; i < 3; ++i) t.g[i].x = -t.g[i].x;
   }
   return t;
 }
@@ -225,16 +331,18 @@ __device__ inline int signed_freq(int i, int n) { return i <= n / 2 ? i : i - n;
 
 // ------------------------------------------------------------ fused search
 
-template <int RR>
+template <int RR, class Tab>
 __global__ void __launch_bounds__(kBatchThreads) align_fused_kernel(
-    const float2* __restrict__ F2, int n, const float2* __restrict__ Q, const float* __restrict__ Nw,
+    const typename Tab::T* __restrict__ F2, int n, const float2* __restrict__ Q, const float* __restrict__ Nw,
     const double* __restrict__ A, const float* __restrict__ Mall, int n_orient, int n_units,
-    int units_per_xcd, float kmax2, float* __restrict__ score, float* __restrict__ shift) {
+    int units_per_xcd, const int* __restrict__ sched, float* __restrict__ score,
+    float* __restrict__ shift) {
   constexpr int D = 2 * RR + 1;
   __shared__ float2 tw[kMaxN][D];              // e^{2 pi i k d / n}
   __shared__ float2 T1[kBatchThreads][D];      // z-pruned column results
   __shared__ float map[D * D * D];
   __shared__ float Ms[9];
+  __shared__ int kyslot[kBatchThreads];        // FFT-order ky of each column slot
   __shared__ double red[kBatchThreads / kWave];
 
   // XCD-aware unit: workgroups are dealt round-robin over the 8 XCDs;
@@ -261,70 +369,77 @@ __global__ void __launch_bounds__(kBatchThreads) align_fused_kernel(
   const float m00 = Ms[0], m01 = Ms[1], m02 = Ms[2], m10 = Ms[3], m11 = Ms[4], m12 = Ms[5],
               m20 = Ms[6], m21 = Ms[7], m22 = Ms[8];
 
-  const int planes = kBatchThreads / n;        // kx planes per pass
-  const int q = t / n, yy = t % n;             // this thread's plane and ky column
-  const int ky = signed_freq(yy, n);
-  const int kmax = (int)floorf(sqrtf(kmax2));
+  // Compact schedule (support_schedule): ngroups passes over at most
+  // kBatchThreads support columns each; per column (kx, ky, h) with h the
+  // largest |kz| inside the support sphere; a piece is a run of slots of one
+  // kx inside a group.
+  const int ngroups = sched[0], npieces = sched[1];
+  const int4* groups = reinterpret_cast<const int4*>(sched + 4);
+  const int4* pieces = groups + ngroups;
+  const int* cols = reinterpret_cast<const int*>(pieces + npieces);
   double dsum = 0.0;
 
-  for (int kx0 = 0; kx0 <= n / 2; kx0 += planes) {
-    const int kx = kx0 + q;
+  for (int g = 0; g < ngroups; ++g) {
+    const int4 G = groups[g];  // (first column, column count, first piece, piece count)
+    const bool active = t < G.y;
+    const int packed = active ? cols[G.x + t] : 0;
+    const int kx = packed & 255, ky = ((packed >> 8) & 255) - 128;
+    const int h = active ? ((packed >> 16) & 255) : -1;
+    const int yy = ky < 0 ? ky + n : ky;
+    kyslot[t] = yy;
+    int hw = h;  // kz bound uniform over the wavefront (twiddle reads stay broadcast)
+    for (int o = kWave / 2; o > 0; o >>= 1) hw = max(hw, __shfl_xor(hw, o, kWave));
     float2 acc[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) acc[d] = make_float2(0.0f, 0.0f);
-    const bool col = (q < planes) && (kx <= n / 2) && ((float)(kx * kx + ky * ky) <= kmax2);
-    // kz loop bounds uniform over the block: [-kmax, kmax] clipped to the grid
-    const int kzlo = -min(kmax, n / 2 - (n % 2 == 0 ? 1 : 0)), kzhi = min(kmax, (n - 1) / 2);
     float dpart = 0.0f;
-    for (int kz = kzlo; kz <= kzhi; ++kz) {
+    const float fkx = (float)kx, fky = (float)ky;
+    const float bx = m00 * fkx + m01 * fky, by = m10 * fkx + m11 * fky, bz = m20 * fkx + m21 * fky;
+    for (int kz = -hw; kz <= hw; ++kz) {
       const int zz = kz < 0 ? kz + n : kz;
-      const float r2 = (float)(kx * kx + ky * ky + kz * kz);
-      if (col && r2 <= kmax2) {
-        const int64_t o = ((int64_t)zz * cx + kx) * n + yy;
+      if (kz >= -h && kz <= h) {
+        const int o = (zz * cx + kx) * n + yy;
         const float2 qv = Qp[o];
         const float w = Np[o];
-        const float fkx = (float)kx, fky = (float)ky, fkz = (float)kz;
-        const float jx = m00 * fkx + m01 * fky + m02 * fkz;
-        const float jy = m10 * fkx + m11 * fky + m12 * fkz;
-        const float jz = m20 * fkx + m21 * fky + m22 * fkz;
-        const float2 f = f2_sample<false>(F2, n, jx, jy, jz).f;
+        const float fkz = (float)kz;
+        const float2 f = f2_sample<false, Tab>(F2, n, bx + m02 * fkz, by + m12 * fkz, bz + m22 * fkz).f;
         // X = Q conj(f)
         const float xr = qv.x * f.x + qv.y * f.y, xi = qv.y * f.x - qv.x * f.y;
         dpart += (f.x * f.x + f.y * f.y) * w;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
           const float2 e = tw[zz][d];
-          acc[d].x += xr * e.x - xi * e.y;
-          acc[d].y += xr * e.y + xi * e.x;
+          acc[d].x = fmaf(xr, e.x, fmaf(-xi, e.y, acc[d].x));
+          acc[d].y = fmaf(xr, e.y, fmaf(xi, e.x, acc[d].y));
         }
       }
     }
-    const double mxk = (kx == 0 || (n % 2 == 0 && kx == n / 2)) ? 1.0 : 2.0;
-    if (col) dsum += mxk * (double)dpart;
+    if (active) dsum += ((kx == 0 || (n % 2 == 0 && kx == n / 2)) ? 1.0 : 2.0) * (double)dpart;
 #pragma unroll
     for (int d = 0; d < D; ++d) T1[t][d] = acc[d];
     __syncthreads();
-    // y-pruned sums and the Hermitian x expansion; each thread owns (dy, dz)
+    // y-pruned sums over each plane's columns and the Hermitian x
+    // expansion; each thread owns (dy, dz)
     for (int o = t; o < D * D; o += kBatchThreads) {
       const int dy = o / D, dz = o % D;
       float vals[D];
 #pragma unroll
       for (int d = 0; d < D; ++d) vals[d] = 0.0f;
-      for (int qq = 0; qq < planes; ++qq) {
-        const int kxx = kx0 + qq;
-        if (kxx > n / 2) break;
+      for (int pc = G.z; pc < G.z + G.w; ++pc) {
+        const int4 pz = pieces[pc];  // (kx, first slot, end slot, 0)
+        const int kxx = pz.x, s0 = pz.y, s1 = pz.z;
         float sr = 0.0f, si = 0.0f;
-        for (int y2 = 0; y2 < n; ++y2) {
-          const float2 a = T1[qq * n + y2][dz];
-          const float2 e = tw[y2][dy];
-          sr += a.x * e.x - a.y * e.y;
-          si += a.x * e.y + a.y * e.x;
+        for (int sl = s0; sl < s1; ++sl) {
+          const float2 a = T1[sl][dz];
+          const float2 e = tw[kyslot[sl]][dy];
+          sr = fmaf(a.x, e.x, fmaf(-a.y, e.y, sr));
+          si = fmaf(a.x, e.y, fmaf(a.y, e.x, si));
         }
         const float mk = (kxx == 0 || (n % 2 == 0 && kxx == n / 2)) ? 1.0f : 2.0f;
 #pragma unroll
         for (int dx = 0; dx < D; ++dx) {
           const float2 e = tw[kxx][dx];
-          vals[dx] += mk * (sr * e.x - si * e.y);
+          vals[dx] = fmaf(mk * sr, e.x, fmaf(-mk * si, e.y, vals[dx]));
         }
       }
 #pragma unroll
@@ -389,8 +504,9 @@ __global__ void __launch_bounds__(kBatchThreads) align_fused_kernel(
 // (per radian), d/dsx, d/dsy, d/dsz].
 constexpr int kGradTerms = 11;
 
+template <class Tab>
 __global__ void __launch_bounds__(kBatchThreads) align_eval_grad_kernel(
-    const float2* __restrict__ F2, int n, const float2* __restrict__ Q, const float* __restrict__ Nw,
+    const typename Tab::T* __restrict__ F2, int n, const float2* __restrict__ Q, const float* __restrict__ Nw,
     const float* __restrict__ MdM, const float* __restrict__ shifts, const int2* __restrict__ supp,
     int n_supp, double* __restrict__ part) {
   const int p = blockIdx.y;
@@ -415,7 +531,7 @@ __global__ void __launch_bounds__(kBatchThreads) align_eval_grad_kernel(
     float j[3];
 #pragma unroll
     for (int r = 0; r < 3; ++r) j[r] = Ms[3 * r] * fk[0] + Ms[3 * r + 1] * fk[1] + Ms[3 * r + 2] * fk[2];
-    const Tri tr = f2_sample<true>(F2, n, j[0], j[1], j[2]);
+    const Tri tr = f2_sample<true, Tab>(F2, n, j[0], j[1], j[2]);
     const float2 qv = Qp[e.x];
     const float w = Np[e.x];
     const float m = (e.y >> 24) ? 2.0f : 1.0f;   // Hermitian multiplicity of the x plane
@@ -513,15 +629,88 @@ const int2* support_list(emc_ctx* ctx, int n, double radius, int* count) {
   return d;
 }
 
-template <int RR>
-void launch_fused(emc_ctx* ctx, const float2* F2, int n, const float2* Q, const float* N,
-                  const double* A, const float* M, int n_orient, int n_units, float kmax2,
-                  float* score, float* shift) {
+template <int RR, class Tab>
+void launch_fused(emc_ctx* ctx, const void* F2, int n, const float2* Q, const float* N,
+                  const double* A, const float* M, int n_orient, int n_units,
+                  const int* sched, float* score, float* shift) {
   const int per = ceil_div(n_units, 8);
   KTimer kt(ctx, "align_fused");
-  hipLaunchKernelGGL(align_fused_kernel<RR>, dim3(8 * per), dim3(kBatchThreads), 0, ctx->stream, F2, n,
-                     Q, N, A, M, n_orient, n_units, per, kmax2, score, shift);
+  hipLaunchKernelGGL((align_fused_kernel<RR, Tab>), dim3(8 * per), dim3(kBatchThreads), 0, ctx->stream,
+                     (const typename Tab::T*)F2, n, Q, N, A, M, n_orient, n_units, per, sched, score, shift);
   EMC_HIP(hipGetLastError());
+}
+
+template <class Tab>
+void launch_fused_r(emc_ctx* ctx, int peak_radius, const void* F2, int n, const float2* Q, const float* N,
+                    const double* A, const float* M, int n_orient, int n_units, const int* sched,
+                    float* score, float* shift) {
+  switch (peak_radius) {
+    case 4: launch_fused<4, Tab>(ctx, F2, n, Q, N, A, M, n_orient, n_units, sched, score, shift); break;
+    case 6: launch_fused<6, Tab>(ctx, F2, n, Q, N, A, M, n_orient, n_units, sched, score, shift); break;
+    default: launch_fused<8, Tab>(ctx, F2, n, Q, N, A, M, n_orient, n_units, sched, score, shift); break;
+  }
+}
+
+// Column schedule of the fused search for (n, radius), cached in the
+// context: [ngroups, npieces, 0, 0, groups (int4 x ngroups: first column,
+// column count, first piece, piece count), pieces (int4: kx, first slot, end
+// slot, 0), columns]. A column is a (kx, ky) with kx^2 + ky^2 <= r^2, packed
+// kx | (ky + 128) << 8 | h << 16, h = max |kz| with the frequency inside the
+// sphere (the same float test as the kernel). Columns are tiled in 16 x 16
+// (kx, ky) patches, consecutive patches merged while they fit one
+// workgroup: the units running together on an XCD (nearby orientations)
+// then read a compact region of the reference table per pass.
+const int* support_schedule(emc_ctx* ctx, int n, float kmax2) {
+  char key[64];
+  snprintf(key, sizeof(key), "sched_%d_%.6f", n, (double)kmax2);
+  auto it = ctx->scratch.find(key);
+  if (it != ctx->scratch.end() && it->second.ptr) return (const int*)it->second.ptr;
+  constexpr int kTile = 16;
+  std::map<std::pair<int, int>, std::vector<int>> tiles;  // ordered by (kx tile, ky tile)
+  for (int kx = 0; kx <= n / 2; ++kx) {
+    if ((float)(kx * kx) > kmax2) break;
+    for (int ky = -(n / 2 - 1); ky <= n / 2 - 1; ++ky) {
+      if ((float)(kx * kx + ky * ky) > kmax2) continue;
+      int h = 0;
+      while (h + 1 <= n / 2 - 1 && (float)(kx * kx + ky * ky + (h + 1) * (h + 1)) <= kmax2) ++h;
+      tiles[{kx / kTile, (ky + 128) / kTile}].push_back(kx | ((ky + 128) << 8) | (h << 16));
+    }
+  }
+  std::vector<int> cols, groups, pieces;
+  std::vector<int> cur;
+  auto flush = [&]() {
+    if (cur.empty()) return;
+    std::sort(cur.begin(), cur.end(), [](int a, int b) {
+      const int ka = a & 255, kb = b & 255;
+      return ka != kb ? ka < kb : ((a >> 8) & 255) < ((b >> 8) & 255);
+    });
+    const int c0 = (int)cols.size(), p0 = (int)pieces.size() / 4;
+    for (size_t i = 0; i < cur.size(); ++i) {
+      if (i == 0 || (cur[i] & 255) != (cur[i - 1] & 255)) {
+        if (i > 0) pieces[pieces.size() - 2] = (int)i;
+        pieces.insert(pieces.end(), {cur[i] & 255, (int)i, (int)cur.size(), 0});
+      }
+      cols.push_back(cur[i]);
+    }
+    groups.insert(groups.end(), {c0, (int)cur.size(), p0, (int)pieces.size() / 4 - p0});
+    cur.clear();
+  };
+  for (auto& kv : tiles) {
+    if (cur.size() + kv.second.size() > (size_t)kBatchThreads) flush();
+    cur.insert(cur.end(), kv.second.begin(), kv.second.end());
+  }
+  flush();
+  for (size_t i = 1; i < groups.size(); i += 4)
+    EMC_REQUIRE(groups[i] <= kBatchThreads, "support tile wider than one workgroup");
+  const int ng = (int)groups.size() / 4, np = (int)pieces.size() / 4;
+  std::vector<int> buf = {ng, np, 0, 0};
+  buf.insert(buf.end(), groups.begin(), groups.end());
+  buf.insert(buf.end(), pieces.begin(), pieces.end());
+  buf.insert(buf.end(), cols.begin(), cols.end());
+  int* d = (int*)scratch(ctx, key, sizeof(int) * buf.size());
+  EMC_HIP(hipMemcpyAsync(d, buf.data(), sizeof(int) * buf.size(), hipMemcpyHostToDevice, ctx->stream));
+  EMC_HIP(hipStreamSynchronize(ctx->stream));
+  return d;
 }
 
 }  // namespace emc
@@ -530,44 +719,85 @@ using namespace emc;
 
 extern "C" {
 
-emc_status emc_batch_reference_spectrum(emc_ctx* ctx, int n, const float* d_ref, void* d_F2) {
+emc_status emc_batch_reference_spectrum(emc_ctx* ctx, int n, const float* d_ref, int table_format,
+                                        void* d_F2) {
   return guard([&] {
     use_device(ctx);
     EMC_REQUIRE(n >= 8 && n <= kMaxN, "box size must be 8..%d", kMaxN);
     EMC_REQUIRE(d_ref && d_F2, "null pointer");
+    EMC_REQUIRE(table_format == EMC_TABLE_C64 || table_format == EMC_TABLE_F16, "unknown table format %d",
+                table_format);
     const int m = 2 * n;
     float* pad = (float*)scratch(ctx, "bat_pad", sizeof(float) * (size_t)m * m * m);
+    float2* raw = (float2*)scratch(ctx, "bat_F2raw", sizeof(float2) * (size_t)(n + 1) * m * m);
     hipLaunchKernelGGL(pad_origin_kernel, dim3(ceil_div(m, 64), ceil_div(m, 4), m), dim3(64, 4, 1), 0,
                        ctx->stream, d_ref, pad, n);
     EMC_HIP(hipGetLastError());
-    fft_r2c(cached_plan(ctx, m, m, m, 1), pad, d_F2);
+    fft_r2c(cached_plan(ctx, m, m, m, 1), pad, raw);
+    const dim3 grid(ceil_div(n + 2, 64), ceil_div(m + 2, 4), m + 2), block(64, 4, 1);
+    if (table_format == EMC_TABLE_C64) {
+      hipLaunchKernelGGL(f2_centre_kernel<TabC64>, grid, block, 0, ctx->stream, (const float2*)raw, n,
+                         (const float*)nullptr, (float2*)d_F2);
+    } else {
+      const int nb = 256;
+      auto* part = (unsigned long long*)scratch(ctx, "bat_f2max", sizeof(unsigned long long) * nb + 256);
+      float* sc = (float*)(part + nb);
+      hipLaunchKernelGGL(f2_maxabs_kernel, dim3(nb), dim3(kRedThreads), 0, ctx->stream, (const float2*)raw,
+                         (int64_t)(n + 1) * m * m, part);
+      hipLaunchKernelGGL(f2_scale_kernel, dim3(1), dim3(kRedThreads), 0, ctx->stream,
+                         (const unsigned long long*)part, nb, sc);
+      hipLaunchKernelGGL(f2_centre_kernel<TabF16>, grid, block, 0, ctx->stream, (const float2*)raw, n,
+                         (const float*)sc, (half2_t*)d_F2);
+    }
+    EMC_HIP(hipGetLastError());
+  });
+}
+
+emc_status emc_batch_prepare_particles(emc_ctx* ctx, int n, int n_part, const float* const* h_particles,
+                                       const float* d_bandpass, const float* const* h_wedges, void* d_Q,
+                                       float* d_N, double* d_A) {
+  return guard([&] {
+    use_device(ctx);
+    EMC_REQUIRE(n >= 8 && n <= kMaxN, "box size must be 8..%d", kMaxN);
+    EMC_REQUIRE(n_part >= 0, "n_part must be >= 0");
+    if (n_part == 0) return;
+    EMC_REQUIRE(h_particles && d_bandpass && d_Q && d_N && d_A, "null pointer");
+    for (int i = 0; i < n_part; ++i) EMC_REQUIRE(h_particles[i], "null particle pointer %d", i);
+    const int64_t nv = (int64_t)n * n * n, nh = (int64_t)(n / 2 + 1) * n * n;
+    float* sh = (float*)scratch(ctx, "bat_shift", sizeof(float) * nv * kPrepChunk);
+    float2* P = (float2*)scratch(ctx, "bat_P", sizeof(float2) * nh * kPrepChunk);
+    const int nb = 256;
+    double* part = (double*)scratch(ctx, "bat_part", sizeof(double) * nb * kPrepChunk);
+    // chunks of 16, 8, 4, 2, 1 particles: a handful of batched FFT plans per size
+    for (int i0 = 0; i0 < n_part;) {
+      int B = kPrepChunk;
+      while (B > n_part - i0) B /= 2;
+      PrepPtrs pp{};
+      for (int b = 0; b < B; ++b) {
+        pp.part[b] = h_particles[i0 + b];
+        pp.wedge[b] = h_wedges ? h_wedges[i0 + b] : nullptr;
+      }
+      hipLaunchKernelGGL(ifftshift_many_kernel, dim3(ceil_div(n, 64), ceil_div(n, 4), n * B), dim3(64, 4, 1),
+                         0, ctx->stream, pp, sh, n);
+      EMC_HIP(hipGetLastError());
+      fft_r2c(cached_plan(ctx, n, n, n, B), sh, P);
+      hipLaunchKernelGGL(prep_particle_kernel, dim3(nb, B), dim3(kRedThreads), 0, ctx->stream,
+                         (const float2*)P, d_bandpass, pp, n, (float2*)d_Q + i0 * nh, d_N + i0 * nh, part);
+      hipLaunchKernelGGL(fold_partials_kernel<1>, dim3(B), dim3(kRedThreads), 0, ctx->stream,
+                         (const double*)part, nb, d_A + i0);
+      EMC_HIP(hipGetLastError());
+      i0 += B;
+    }
   });
 }
 
 emc_status emc_batch_prepare_particle(emc_ctx* ctx, int n, const float* d_particle,
                                       const float* d_bandpass, const float* d_wedge, void* d_Q,
                                       float* d_N, double* d_A) {
-  return guard([&] {
-    use_device(ctx);
-    EMC_REQUIRE(n >= 8 && n <= kMaxN, "box size must be 8..%d", kMaxN);
-    EMC_REQUIRE(d_particle && d_bandpass && d_Q && d_N && d_A, "null pointer");
-    const int64_t nv = (int64_t)n * n * n, nh = (int64_t)(n / 2 + 1) * n * n;
-    float* sh = (float*)scratch(ctx, "bat_shift", sizeof(float) * nv);
-    float2* P = (float2*)scratch(ctx, "bat_P", sizeof(float2) * nh);
-    hipLaunchKernelGGL(ifftshift_kernel, dim3(ceil_div(n, 64), ceil_div(n, 4), n), dim3(64, 4, 1), 0,
-                       ctx->stream, d_particle, sh, n);
-    fft_r2c(cached_plan(ctx, n, n, n, 1), sh, P);
-    const int nb = 256;
-    double* part = (double*)scratch(ctx, "bat_part", sizeof(double) * nb);
-    hipLaunchKernelGGL(prep_particle_kernel, dim3(nb), dim3(kRedThreads), 0, ctx->stream,
-                       (const float2*)P, d_bandpass, d_wedge, n, (float2*)d_Q, d_N, part);
-    hipLaunchKernelGGL(fold_partials_kernel<1>, dim3(1), dim3(kRedThreads), 0, ctx->stream,
-                       (const double*)part, nb, d_A);
-    EMC_HIP(hipGetLastError());
-  });
+  return emc_batch_prepare_particles(ctx, n, 1, &d_particle, d_bandpass, &d_wedge, d_Q, d_N, d_A);
 }
 
-emc_status emc_align_batched(emc_ctx* ctx, int n, int n_part, const void* d_F2, const void* d_Q,
+emc_status emc_align_batched(emc_ctx* ctx, int n, int n_part, int table_format, const void* d_F2, const void* d_Q,
                              const float* d_N, const double* d_A, const double* d_start_cm,
                              const double* angles_pts, int n_orient, int peak_radius,
                              double support_radius, float* d_score, float* d_shift) {
@@ -579,6 +809,8 @@ emc_status emc_align_batched(emc_ctx* ctx, int n, int n_part, const void* d_F2, 
                 "null pointer");
     EMC_REQUIRE(support_radius > 0 && support_radius < 0.5 * n, "support radius must be in (0, n/2)");
     EMC_REQUIRE(peak_radius == 4 || peak_radius == 6 || peak_radius == 8, "peak radius must be 4, 6 or 8");
+    EMC_REQUIRE(table_format == EMC_TABLE_C64 || table_format == EMC_TABLE_F16, "unknown table format %d",
+                table_format);
     const int64_t units64 = (int64_t)n_part * n_orient;
     EMC_REQUIRE(units64 < ((int64_t)1 << 31) / 8, "too many units in one call");
     const int units = (int)units64;
@@ -592,17 +824,19 @@ emc_status emc_align_batched(emc_ctx* ctx, int n, int n_part, const void* d_F2, 
                        (const double*)d_pts, n_part, n_orient, d_M);
     EMC_HIP(hipGetLastError());
     const float kmax2 = (float)(support_radius * support_radius);
-    const float2* F2 = (const float2*)d_F2;
     const float2* Q = (const float2*)d_Q;
-    switch (peak_radius) {
-      case 4: launch_fused<4>(ctx, F2, n, Q, d_N, d_A, d_M, n_orient, units, kmax2, d_score, d_shift); break;
-      case 6: launch_fused<6>(ctx, F2, n, Q, d_N, d_A, d_M, n_orient, units, kmax2, d_score, d_shift); break;
-      default: launch_fused<8>(ctx, F2, n, Q, d_N, d_A, d_M, n_orient, units, kmax2, d_score, d_shift); break;
-    }
+    const int* sched = support_schedule(ctx, n, kmax2);
+    if (table_format == EMC_TABLE_C64)
+      launch_fused_r<TabC64>(ctx, peak_radius, d_F2, n, Q, d_N, d_A, d_M, n_orient, units, sched, d_score,
+                             d_shift);
+    else
+      launch_fused_r<TabF16>(ctx, peak_radius, d_F2, n, Q, d_N, d_A, d_M, n_orient, units, sched, d_score,
+                             d_shift);
   });
 }
 
-emc_status emc_align_eval_grad(emc_ctx* ctx, int n, int n_part, const void* d_F2, const void* d_Q,
+emc_status emc_align_eval_grad(emc_ctx* ctx, int n, int n_part, int table_format, const void* d_F2,
+                               const void* d_Q,
                                const float* d_N, const double* d_A, const float* d_MdM,
                                const float* d_shift, double support_radius, double* d_out) {
   return guard([&] {
@@ -610,6 +844,8 @@ emc_status emc_align_eval_grad(emc_ctx* ctx, int n, int n_part, const void* d_F2
     EMC_REQUIRE(n >= 8 && n <= kMaxN, "box size must be 8..%d", kMaxN);
     EMC_REQUIRE(n_part > 0, "empty batch");
     EMC_REQUIRE(d_F2 && d_Q && d_N && d_A && d_MdM && d_shift && d_out, "null pointer");
+    EMC_REQUIRE(table_format == EMC_TABLE_C64 || table_format == EMC_TABLE_F16, "unknown table format %d",
+                table_format);
     EMC_REQUIRE(support_radius > 0 && support_radius < 0.5 * n, "support radius must be in (0, n/2)");
     int ns = 0;
     const int2* supp = support_list(ctx, n, support_radius, &ns);
@@ -620,8 +856,14 @@ emc_status emc_align_eval_grad(emc_ctx* ctx, int n, int n_part, const void* d_F2
     double* part = (double*)scratch(ctx, "grad_part", sizeof(double) * kGradTerms * (size_t)g * n_part);
     {
       KTimer kt(ctx, "align_eval_grad");
-      hipLaunchKernelGGL(align_eval_grad_kernel, dim3(g, n_part), dim3(kBatchThreads), 0, ctx->stream,
-                         (const float2*)d_F2, n, (const float2*)d_Q, d_N, d_MdM, d_shift, supp, ns, part);
+      if (table_format == EMC_TABLE_C64)
+        hipLaunchKernelGGL(align_eval_grad_kernel<TabC64>, dim3(g, n_part), dim3(kBatchThreads), 0,
+                           ctx->stream, (const float2*)d_F2, n, (const float2*)d_Q, d_N, d_MdM, d_shift, supp,
+                           ns, part);
+      else
+        hipLaunchKernelGGL(align_eval_grad_kernel<TabF16>, dim3(g, n_part), dim3(kBatchThreads), 0,
+                           ctx->stream, (const half2_t*)d_F2, n, (const float2*)d_Q, d_N, d_MdM, d_shift,
+                           supp, ns, part);
     }
     hipLaunchKernelGGL(adjoint_fold_kernel, dim3(n_part), dim3(kRedThreads), 0, ctx->stream,
                        (const double*)part, g, d_A, d_out);

@@ -2,6 +2,7 @@
 // (BH_defineMatrix, eulerSearch) of libemc_hip.so.
 #include <cmath>
 #include <cstring>
+#include <iterator>
 
 #include "emc_internal.hpp"
 
@@ -231,6 +232,23 @@ void euler_search(const char* sym, double theta_max, double theta_step, double p
   }
 }
 
+constexpr size_t kCacheLimit = size_t(16) << 30;  // bytes of free blocks kept per context
+
+// Block size class: 512 B granules below 1 MiB, 2 MiB granules above.
+size_t block_size(size_t bytes) {
+  const size_t g = bytes < (size_t(1) << 20) ? 512 : (size_t(2) << 20);
+  return (std::max<size_t>(bytes, 1) + g - 1) / g * g;
+}
+
+// Return every cached block to HIP (after the stream drains).
+void release_cache(emc_ctx* ctx) {
+  if (ctx->cached.empty()) return;
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& kv : ctx->cached) (void)hipFree(kv.second);
+  ctx->cached.clear();
+  ctx->cached_bytes = 0;
+}
+
 }  // namespace emc
 
 using namespace emc;
@@ -277,6 +295,7 @@ emc_status emc_ctx_destroy(emc_ctx* ctx) {
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     for (auto& kv : ctx->scratch)
       if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+    release_cache(ctx);
     if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -319,22 +338,50 @@ emc_status emc_timing_read(emc_ctx* ctx, const char* name, double* ms_total, int
   });
 }
 
+
 emc_status emc_malloc(emc_ctx* ctx, size_t bytes, void** d_ptr) {
   return guard([&] {
     use_device(ctx);
     EMC_REQUIRE(d_ptr != nullptr, "null output pointer");
-    if (hipMalloc(d_ptr, bytes ? bytes : 1) != hipSuccess) {
-      (void)hipGetLastError();
-      fail(EMC_ERR_ALLOC, "cannot allocate %zu bytes", bytes);
+    const size_t sz = block_size(bytes);
+    auto range = ctx->cached.equal_range(sz);
+    if (range.first != range.second) {  // most recently freed block of this size
+      auto it = std::prev(range.second);
+      *d_ptr = it->second;
+      ctx->cached.erase(it);
+      ctx->cached_bytes -= sz;
+    } else {
+      void* p = nullptr;
+      if (hipMalloc(&p, sz) != hipSuccess) {
+        (void)hipGetLastError();
+        release_cache(ctx);
+        if (hipMalloc(&p, sz) != hipSuccess) {
+          (void)hipGetLastError();
+          fail(EMC_ERR_ALLOC, "cannot allocate %zu bytes", bytes);
+        }
+      }
+      *d_ptr = p;
     }
+    ctx->live[*d_ptr] = sz;
   });
 }
 
 emc_status emc_free(emc_ctx* ctx, void* d_ptr) {
   return guard([&] {
     use_device(ctx);
-    EMC_HIP(hipStreamSynchronize(ctx->stream));
-    if (d_ptr) EMC_HIP(hipFree(d_ptr));
+    if (!d_ptr) return;
+    auto it = ctx->live.find(d_ptr);
+    if (it == ctx->live.end()) {  // not from emc_malloc: plain HIP free
+      for (auto& kv : ctx->cached)
+        if (kv.second == d_ptr) fail(EMC_ERR_INVALID, "emc_free: block %p was already freed", d_ptr);
+      EMC_HIP(hipStreamSynchronize(ctx->stream));
+      EMC_HIP(hipFree(d_ptr));
+      return;
+    }
+    ctx->cached.emplace(it->second, d_ptr);
+    ctx->cached_bytes += it->second;
+    ctx->live.erase(it);
+    if (ctx->cached_bytes > kCacheLimit) release_cache(ctx);
   });
 }
 

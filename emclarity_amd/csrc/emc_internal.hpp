@@ -82,6 +82,11 @@ struct emc_ctx {
   std::map<std::string, emc::Scratch> scratch;
   // internal FFT plans keyed by (nx, ny, nz, batch)
   std::map<std::tuple<int, int, int, int>, emc_fft_plan*> plans;
+  // caching allocator behind emc_malloc / emc_free: freed blocks are kept
+  // by size and reused in the order of this context's stream
+  std::multimap<size_t, void*> cached;  // size -> free block
+  std::map<void*, size_t> live;         // block handed out -> size
+  size_t cached_bytes = 0;
   double* h_pinned = nullptr;  // small pinned host buffer for results
   size_t h_pinned_bytes = 0;
   // optional per-kernel timing with HIP events on the context stream
@@ -208,11 +213,14 @@ __device__ inline float block_min(float v) {
 }
 
 // Fold kRedBlocks-style partials of K doubles (layout part[K*b + k]) in a
-// fixed order into out[0..K-1]. Launch with one block of kRedThreads.
+// fixed order into out[0..K-1]. One block of kRedThreads per reduction;
+// block i folds part + i*K*nb into out + i*K.
 template <int K>
 __global__ void __launch_bounds__(kRedThreads) fold_partials_kernel(const double* __restrict__ part,
                                                                     int nb,
                                                                     double* __restrict__ out) {
+  part += (int64_t)blockIdx.x * K * nb;
+  out += (int64_t)blockIdx.x * K;
   double v[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) v[k] = 0.0;

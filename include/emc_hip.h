@@ -70,7 +70,13 @@ emc_status emc_timing_reset(emc_ctx* ctx);
 emc_status emc_timing_read(emc_ctx* ctx, const char* name, double* ms_total, int64_t* launches);
 
 /* Device memory helpers so a host without a GPU array type (C harness, MEX
- * gateway on AMD, ctypes) can drive the library. Copies are synchronous. */
+ * gateway on AMD, ctypes) can drive the library. Copies are synchronous.
+ * emc_free returns an emc_malloc block to the context's cache without
+ * waiting: a later emc_malloc on the same context may hand it out again,
+ * ordered after the work already queued on the context's stream (work the
+ * caller queued on other streams must be finished first). Blocks not from
+ * emc_malloc are freed after the stream drains. emc_ctx_destroy releases
+ * the cache. */
 emc_status emc_malloc(emc_ctx* ctx, size_t bytes, void** d_ptr);
 emc_status emc_free(emc_ctx* ctx, void* d_ptr);
 emc_status emc_memcpy_h2d(emc_ctx* ctx, void* d_dst, const void* src, size_t bytes);
@@ -271,10 +277,20 @@ emc_status emc_align_grid(emc_ctx* ctx, int n, const float* d_particle, const fl
  *   a = P_centred * bandpass, c_o = F_ref(R_o^T k) * wedge_sym * bandpass,
  * summed over the half grid inside `support_radius` (index units). */
 
-/* Reference spectrum for one box size: d_F2 receives the half spectrum of
- * the reference zero-padded to 2n with its centre n/2 at the origin:
- * (n+1) x 2n x 2n complex64. */
-emc_status emc_batch_reference_spectrum(emc_ctx* ctx, int n, const float* d_ref, void* d_F2);
+/* Reference table formats: EMC_TABLE_C64 = complex64 texels (8 bytes);
+ * EMC_TABLE_F16 = complex with IEEE half parts (4 bytes), holding F * 2^s
+ * with the power of two chosen so every part is below 2^14 (the normalised
+ * correlation and its gradient do not depend on that scale). */
+enum { EMC_TABLE_C64 = 0, EMC_TABLE_F16 = 1 };
+
+/* Reference spectrum for one box size: the half spectrum F of the reference
+ * zero-padded to 2n with its centre n/2 at the origin, stored centred with
+ * a guard texel: d_F2 holds (2n+2) x (2n+2) x (n+2) texels of table_format,
+ * element [(kz + n)(2n+2) + (ky + n)](n+2) + kx = F(kx, ky, kz) for kx in
+ * [0, n+1], ky, kz in [-n, n+1] (indices taken mod 2n; kx = n+1 by Hermitian
+ * symmetry). */
+emc_status emc_batch_reference_spectrum(emc_ctx* ctx, int n, const float* d_ref, int table_format,
+                                        void* d_F2);
 
 /* Per-particle preparation: d_particle n^3 (centred), d_bandpass n^3 FFT
  * order, d_wedge n^3 centred SF3D (NULL = 1). Writes Q (half grid complex64,
@@ -283,6 +299,14 @@ emc_status emc_batch_prepare_particle(emc_ctx* ctx, int n, const float* d_partic
                                       const float* d_bandpass, const float* d_wedge, void* d_Q,
                                       float* d_N, double* d_A);
 
+/* The same preparation for n_part particles of one box size in one call
+ * (batched FFTs): h_particles / h_wedges are host arrays of n_part device
+ * pointers (h_wedges NULL or an entry NULL = no wedge); particle i writes
+ * d_Q + i*nh, d_N + i*nh and d_A[i], nh = (n/2+1) n^2. */
+emc_status emc_batch_prepare_particles(emc_ctx* ctx, int n, int n_part, const float* const* h_particles,
+                                       const float* d_bandpass, const float* const* h_wedges, void* d_Q,
+                                       float* d_N, double* d_A);
+
 /* Score every (particle, orientation): particles p = 0..n_part-1 with their
  * Q/N/A stacked contiguously, start matrices d_start_cm (n_part x 9 doubles,
  * column-major), local angles (phi, theta, psi) in angles_pts (host), R =
@@ -290,7 +314,7 @@ emc_status emc_batch_prepare_particle(emc_ctx* ctx, int n, const float* d_partic
  * on the device. peak_radius 4, 6 or 8 (shift search sphere). Outputs
  * (device): d_score[n_part*n_orient] and d_shift[3*n_part*n_orient]
  * (x,y,z, parabolic sub-pixel), orientation fastest. Stream-ordered. */
-emc_status emc_align_batched(emc_ctx* ctx, int n, int n_part, const void* d_F2, const void* d_Q,
+emc_status emc_align_batched(emc_ctx* ctx, int n, int n_part, int table_format, const void* d_F2, const void* d_Q,
                              const float* d_N, const double* d_A, const double* d_start_cm,
                              const double* angles_pts, int n_orient, int peak_radius,
                              double support_radius, float* d_score, float* d_shift);
@@ -300,7 +324,8 @@ emc_status emc_align_batched(emc_ctx* ctx, int n, int n_part, const void* d_F2, 
  * in radians); d_shift: 3 floats per particle. d_out: 7 doubles per particle
  * = [CC, dCC/dphi, dCC/dtheta, dCC/dpsi, dCC/dsx, dCC/dsy, dCC/dsz].
  * Stream-ordered. */
-emc_status emc_align_eval_grad(emc_ctx* ctx, int n, int n_part, const void* d_F2, const void* d_Q,
+emc_status emc_align_eval_grad(emc_ctx* ctx, int n, int n_part, int table_format, const void* d_F2,
+                               const void* d_Q,
                                const float* d_N, const double* d_A, const float* d_MdM,
                                const float* d_shift, double support_radius, double* d_out);
 

@@ -751,16 +751,30 @@ def template_search_chunk(tomo_chunk, template, grid_angles, pad_bin, trim_valid
 # --------------------------------------------------------------------------
 
 
-def batch_reference_spectrum(ref):
+def batch_reference_spectrum(ref, table="c64"):
     """rfftn of the reference zero-padded to 2n with its centre n//2 at the
-    origin; shape (2n, 2n, n+1) [z, y, x]."""
+    origin; shape (2n, 2n, n+1) [z, y, x].
+
+    table "f16": the device's half-precision table. Real and imaginary parts
+    are scaled by 2^(14-e), where max(|Re|, |Im|) < 2^e (frexp), rounded to
+    IEEE half (nearest even) and scaled back. The normalised score does not
+    depend on the power-of-two scale, only on the rounding."""
     ref = np.asarray(ref, dtype=np.float64)
     n = ref.shape[0]
     m = 2 * n
     pad = np.zeros((m, m, m))
     idx = (np.arange(n) - n // 2) % m
     pad[np.ix_(idx, idx, idx)] = ref
-    return np.fft.rfftn(pad)
+    F = np.fft.rfftn(pad)
+    if table == "f16":
+        a = max(np.max(np.abs(F.real)), np.max(np.abs(F.imag)))
+        e = int(np.frexp(a)[1]) if a > 0 else 0
+        sc = 2.0 ** (14 - e)
+        half = lambda x: (x * sc).astype(np.float16).astype(np.float64) / sc
+        F = half(F.real) + 1j * half(F.imag)
+    elif table != "c64":
+        raise ValueError(table)
+    return F
 
 
 def _f2_texel(F2, n, tx, ty, tz):

@@ -14,7 +14,8 @@ from oracle import emc_oracle as O
 pytestmark = pytest.mark.gpu
 
 
-def test_batched_search_matches_oracle(ctx):
+@pytest.mark.parametrize("table", ["c64", "f16"])
+def test_batched_search_matches_oracle(ctx, table):
     n = 32
     bp = O.bandpass3d((n, n, n), 1e-3, 40.0, 8.0, 2.5)
     truth = (40.0, 30.0, 20.0)
@@ -23,12 +24,12 @@ def test_batched_search_matches_oracle(ctx):
     w = BH_weightMaskMex((n, n, n), 1, T, [n // 2 + 1] * 3, [[n, n, n], [0, 0, 0]]).get()
     # second particle: the same data seen from a different start orientation
     A2 = local_matrix(20.0, 10.0, 5.0)
-    al = BatchAligner(n, c.ref, bp.astype(np.float32), peak_radius=6)
+    al = BatchAligner(n, c.ref, bp.astype(np.float32), peak_radius=6, table=table)
     al.set_particles([c.particle, c.particle], [w, w], [np.eye(3), A2])
     pts = np.array([truth, (0.0, 0.0, 0.0), (40.0, 30.0, 30.0), (80.0, 30.0, 20.0), (200.0, 60.0, -40.0),
                     (45.0, 32.0, 18.0), (10.0, 45.0, 120.0)])
     score, shift = al.search(pts)
-    F2 = O.batch_reference_spectrum(c.ref)
+    F2 = O.batch_reference_spectrum(c.ref, table)
     bpp = O.batch_prepare(c.particle, bp, w, al.support)
     for pi, A in enumerate([np.eye(3), A2]):
         rs = []
@@ -44,15 +45,16 @@ def test_batched_search_matches_oracle(ctx):
             assert int(np.argmax(score[pi])) == int(np.argmax(rs))
 
 
-def test_adjoint_matches_oracle_and_finite_differences(ctx):
+@pytest.mark.parametrize("table", ["c64", "f16"])
+def test_adjoint_matches_oracle_and_finite_differences(ctx, table):
     n = 32
     bp = O.bandpass3d((n, n, n), 1e-3, 40.0, 8.0, 2.5)
     c = S.make_align_case(11, n=n, euler=(40.0, 30.0, 20.0), snr=1.0, n_blobs=16, blob_radius=0.22 * n)
     w = BH_weightMaskMex((n, n, n), 1, S.tilt_table(60.0, 3.0, 3.0, 2.5), [n // 2 + 1] * 3,
                          [[n, n, n], [0, 0, 0]]).get()
-    al = BatchAligner(n, c.ref, bp.astype(np.float32), peak_radius=6)
+    al = BatchAligner(n, c.ref, bp.astype(np.float32), peak_radius=6, table=table)
     al.set_particles([c.particle], [w], [np.eye(3)])
-    F2 = O.batch_reference_spectrum(c.ref)
+    F2 = O.batch_reference_spectrum(c.ref, table)
     bpp = O.batch_prepare(c.particle, bp, w, al.support)
     # Euler-angle gradient at an off-grid pose
     ang = (43.0, 27.5, 22.0)
@@ -86,7 +88,8 @@ def test_adjoint_matches_oracle_and_finite_differences(ctx):
     assert np.max(np.abs(g[0] - fd)) < 0.05 * np.max(np.abs(fd)), (g[0], fd)
 
 
-def test_batched_refinement_accuracy(ctx):
+@pytest.mark.parametrize("table", ["c64", "f16"])
+def test_batched_refinement_accuracy(ctx, table):
     """64^3 particles with their own wedge and defocus, SNR 0.1: coarse
     6-degree local grid + adjoint refinement reaches <=1 deg and <=0.5 px."""
     n = 64
@@ -96,7 +99,7 @@ def test_batched_refinement_accuracy(ctx):
     parts = [make_particle(rng, sc, blobs) for _ in range(4)]
     wedges = [BH_weightMaskMex((n, n, n), 1, p.tlt, [n // 2 + 1] * 3, [[n, n, n], [0, 0, 0]]).get()
               for p in parts]
-    al = BatchAligner(n, sc.ref, sc.bandpass, peak_radius=6)
+    al = BatchAligner(n, sc.ref, sc.bandpass, peak_radius=6, table=table)
     al.set_particles([p.data for p in parts], wedges, [p.start for p in parts])
     grid = eulerSearch("C1", 15, 6, 15, 6)
     R, s, cc, _ = align_batch(al, grid, refine_iters=12)
@@ -106,3 +109,39 @@ def test_batched_refinement_accuracy(ctx):
         es = float(np.max(np.abs(s[i] - p.shift)))
         ok += int(ea <= 1.0 and es <= 0.5)
     assert ok >= 3, [(angle_error_deg(R[i], p.truth_R), s[i] - p.shift) for i, p in enumerate(parts)]
+
+
+def test_batched_preparation_matches_single(ctx):
+    """emc_batch_prepare_particles (chunks 16 + 2 + 1, some without wedge)
+    against emc_batch_prepare_particle per particle, and A against the
+    oracle's preparation."""
+    import ctypes as C
+    from emclarity_amd import _lib
+    from emclarity_amd.device import DeviceArray
+    n, P = 16, 19
+    rng = np.random.default_rng(5)
+    bp = O.bandpass3d((n, n, n), 1e-3, 40.0, 8.0, 2.5)
+    w = BH_weightMaskMex((n, n, n), 1, S.tilt_table(60.0, 3.0, 3.0, 2.5), [n // 2 + 1] * 3,
+                         [[n, n, n], [0, 0, 0]]).get()
+    vols = [rng.standard_normal((n, n, n)).astype(np.float32) for _ in range(P)]
+    wedges = [None if i % 5 == 3 else w for i in range(P)]
+    al = BatchAligner(n, np.zeros((n, n, n), np.float32), bp.astype(np.float32))
+    al.set_particles(vols, wedges, [np.eye(3)] * P)
+    Qb, Nb, Ab = al.Q.get(), al.N.get(), al.A.get()
+    nh = al.nh
+    d_bp = DeviceArray.from_numpy(bp.astype(np.float32))
+    for i in range(P):
+        Q1 = DeviceArray((nh,), np.complex64)
+        N1 = DeviceArray((nh,), np.float32)
+        A1 = DeviceArray((1,), np.float64)
+        pd = DeviceArray.from_numpy(vols[i])
+        wd = None if wedges[i] is None else DeviceArray.from_numpy(wedges[i].astype(np.float32))
+        _lib.call("emc_batch_prepare_particle", ctx.handle, n, pd.vp, d_bp.vp,
+                  None if wd is None else wd.vp, Q1.vp, N1.vp, A1.vp)
+        q1, n1, a1 = Q1.get(), N1.get(), A1.get()[0]
+        scale = np.max(np.abs(q1))
+        assert np.max(np.abs(Qb[i] - q1)) <= 1e-5 * scale, i
+        assert np.array_equal(Nb[i], n1), i
+        assert abs(Ab[i] - a1) <= 1e-6 * a1, (i, Ab[i], a1)
+        ref = O.batch_prepare(vols[i].astype(np.float64), bp, wedges[i], al.support)
+        assert abs(Ab[i] - ref.A) <= 1e-5 * ref.A, (i, Ab[i], ref.A)

@@ -182,3 +182,24 @@ def test_xcf_rotational_matches_oracle(ctx, rng):
     ref = O.xcf_rotational(P.astype(np.complex128), R.astype(np.complex128), wp, wr)
     assert abs(c - ref) < 1e-5 * max(abs(ref), 1e-3)
     assert abs(BH_multi_xcf_Rotational(P, np.conj(P), 1, 1) - 1.0) < 1e-6
+
+
+def test_device_block_cache(ctx):
+    """emc_free keeps emc_malloc blocks for reuse (same size class, same
+    address), data written after reuse is intact, and a second free of the
+    same block is rejected."""
+    import ctypes as C
+    a = DeviceArray.from_numpy(np.arange(1000, dtype=np.float32))
+    p = a.ptr
+    a.free()
+    b = DeviceArray.from_numpy(np.full(900, 7.0, np.float32))   # same 512-byte class
+    assert b.ptr == p
+    assert np.array_equal(b.get(), np.full(900, 7.0, np.float32))
+    big = DeviceArray.from_numpy(np.ones(3 << 20, np.float32))
+    q = big.ptr
+    big.free()
+    c = DeviceArray(((3 << 20) - 100,), np.float32)                # same 2 MiB class
+    assert c.ptr == q
+    c.free()
+    with pytest.raises(_lib.EmcError):
+        _lib.call("emc_free", ctx.handle, C.c_void_p(q))

@@ -408,3 +408,26 @@ def test_batch_score_prefers_truth_and_gradient_matches_fd():
         e[i] = 1e-5
         fd = (O.batch_eval(F2, n, bpp, R0, s0 + e) - O.batch_eval(F2, n, bpp, R0, s0 - e)) / 2e-5
         assert abs(fd - g[3 + i]) < 1e-5 * max(abs(g[3 + i]), 1e-3)
+
+
+def test_batch_f16_table_rounding_and_scale_invariance():
+    """The half-precision table: every part within half an fp16 ulp of the
+    FP64 spectrum (relative 2^-11, or the subnormal spacing), the score
+    unchanged by a power-of-two scale of the table, and within 1e-4 of the
+    complex64 table's score."""
+    c, bp, F2 = _batched_case()
+    n = c.ref.shape[0]
+    H = O.batch_reference_spectrum(c.ref, "f16")
+    a = max(np.max(np.abs(F2.real)), np.max(np.abs(F2.imag)))
+    sc = 2.0 ** (14 - int(np.frexp(a)[1]))
+    for x, h in ((F2.real, H.real), (F2.imag, H.imag)):
+        assert np.all(np.abs(h - x) <= np.maximum(2.0 ** -11 * np.abs(x), 2.0 ** -25 / sc) * (1 + 1e-12))
+        assert np.max(np.abs(x)) * sc < 2.0 ** 14
+    assert not np.array_equal(H, F2)   # rounding happened
+    bpp = O.batch_prepare(c.particle, bp, None, 9.0)
+    R = c.rotmat @ O.define_matrix([3, 2, 1], "Bah", "forward")
+    s64, sh64 = O.batch_score_map(F2, n, bpp, R, 4)
+    s16, sh16 = O.batch_score_map(H, n, bpp, R, 4)
+    s16x, sh16x = O.batch_score_map(8.0 * H, n, bpp, R, 4)
+    assert abs(s16 - s16x) < 1e-12 and np.allclose(sh16, sh16x, atol=1e-12)
+    assert abs(s16 - s64) < 1e-4 * abs(s64) and np.max(np.abs(sh16 - sh64)) < 1e-3
