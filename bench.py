@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--table", choices=["c64", "f16"], default="c64",
                     help="reference table format of the batched path")
+    ap.add_argument("--no-variant", dest="variant", action="store_false",
+                    help="skip the run with the other reference table format")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="one untimed device-to-device copy of this size first: a known-size "
                          "kernel that calibrates HBM byte counters (FETCH_SIZE, WRITE_SIZE)")
@@ -167,6 +169,25 @@ def cpu_baseline(budget_s, classes, particles, host, wedges):
                        f"per particle, 50/30/20 mix; single thread")}
 
 
+COUNTERS = os.path.join(ROOT, "profiles", "align_fused_counters.md")
+
+
+def counter_traffic(table):
+    """HBM bytes per align_fused launch (FETCH_SIZE + WRITE_SIZE, calibrated
+    against a copy of known size) from the committed counter summary, which
+    carries one line 'traffic <table> <bytes per launch>' per table format."""
+    try:
+        with open(COUNTERS) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic" and parts[1] == table:
+                    return {"bytes_per_launch": int(float(parts[2])),
+                            "source": os.path.relpath(COUNTERS, ROOT)}
+    except OSError:
+        pass
+    return None
+
+
 def kernel_time(ctx, name):
     import ctypes as C
     from emclarity_amd import _lib
@@ -197,7 +218,7 @@ def main():
         print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported (every box here has one GPU)",
               file=sys.stderr)
         sys.exit(2)
-    from emclarity_amd.batched import fused_flops_per_unit
+    from emclarity_amd.batched import fused_flops_per_unit, fused_samples_per_unit
     from emclarity_amd.device import DeviceArray, default_context
     from emclarity_amd.geometry import eulerSearch
     from emclarity_amd.masking import BH_weightMaskMex
@@ -225,29 +246,53 @@ def main():
           f"coarse grid {len(coarse)} orientations + {a.refine_iters} adjoint iterations",
           file=sys.stderr, flush=True)
 
-    run = BatchedRun(particles, classes, wedges, coarse, a.refine_iters, a.table)
-    for w in range(a.warmup):
-        run.run()
-    (Rb, Sb), dt = timed(ctx, run.run, a.steps, "batched")
-    value = len(particles) * a.steps / dt
-    acc = accuracy(particles, Rb, Sb)
+    def batched(table):
+        run = BatchedRun(particles, classes, wedges, coarse, a.refine_iters, table)
+        for w in range(a.warmup):
+            run.run()
+        (Rb, Sb), dt = timed(ctx, run.run, a.steps, f"batched ({table} table)")
+        ms, cnt = kernel_time(ctx, "align_fused")
+        flops = sum(fused_flops_per_unit(n, sup, 6) * u for n, (u, sup) in run.units.items())
+        samples = sum(fused_samples_per_unit(n, sup) * u for n, (u, sup) in run.units.items())
+        return {"value": len(particles) * a.steps / dt, "dt": dt, "acc": accuracy(particles, Rb, Sb),
+                "ms": ms, "launches": cnt, "flops": flops * a.steps, "samples": samples * a.steps,
+                "units": run.units}
 
-    # roofline of the fused search kernel: FP32 vector arithmetic (it reads
-    # almost nothing from HBM: the reference spectrum and particle spectra
-    # stay in L2/MALL)
-    ms, cnt = kernel_time(ctx, "align_fused")
-    flops = sum(fused_flops_per_unit(n, sup, 6) * u for n, (u, sup) in run.units.items())
+    main_run = batched(a.table)
+    other = "f16" if a.table == "c64" else "c64"
+    alt = batched(other) if a.variant else None
+    value, dt, acc = main_run["value"], main_run["dt"], main_run["acc"]
+
+    # roofline of the fused search kernel against the FP32 peak; its limit in
+    # practice is the trilinear gather traffic past L2, measured by the HBM
+    # byte counters (copy-calibrated) in the committed counter summary
+    ms, cnt = main_run["ms"], main_run["launches"]
     avg_us = ms * 1e3 / max(cnt, 1)
-    achieved = flops * a.steps / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
+    achieved = main_run["flops"] / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
+    traffic = counter_traffic(a.table)
     # the same units through the real-space chain would move
     # B_unit = 4 W^3 + 16 C^3 + 40 H bytes each (H = C^2 (C/2+1))
-    b_unit = {n: 4 * n ** 3 + 16 * n ** 3 + 40 * n * n * (n // 2 + 1) for n in run.units}
-    eq_gbs = sum(b_unit[n] * u for n, (u, _) in run.units.items()) * a.steps / (ms * 1e-3) / 1e9
+    units = main_run["units"]
+    b_unit = {n: 4 * n ** 3 + 16 * n ** 3 + 40 * n * n * (n // 2 + 1) for n in units}
+    eq_gbs = sum(b_unit[n] * u for n, (u, _) in units.items()) * a.steps / (ms * 1e-3) / 1e9
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None, "kernel": "align_fused",
-            "avg_us": round(avg_us, 2), "launches": cnt, "flops_per_launch": round(flops / max(cnt / a.steps, 1)),
-            "note": "FP32 vector peak; the kernel's HBM traffic is near zero, see DESIGN.md",
+            "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+            "traffic": traffic["bytes_per_launch"] if traffic else None,
+            "kernel": f"align_fused<6, {a.table} table>", "avg_us": round(avg_us, 2), "launches": cnt,
+            "flops_per_launch": round(main_run["flops"] / max(cnt, 1)),
+            "gsamples_per_s": round(main_run["samples"] / (ms * 1e-3) / 1e9, 2),
+            "traffic_GBps": round(traffic["bytes_per_launch"] / (avg_us * 1e-6) / 1e9, 1) if traffic else None,
+            "traffic_source": traffic["source"] if traffic else None,
+            "note": ("FP32 arithmetic vs the 157.3 TFLOP/s FP32 peak (packed FP32 is disabled in this build, "
+                     "so 78.6 is attainable); the kernel is limited by trilinear gathers that miss L2"),
             "realspace_chain_equivalent_GBps": round(eq_gbs, 1)}
+    variants = None
+    if alt:
+        variants = {f"{other}_table": {
+            "value": round(alt["value"], 3), "ms_per_step": round(alt["dt"] * 1e3 / a.steps, 2),
+            "accuracy": alt["acc"], "align_fused_avg_us": round(alt["ms"] * 1e3 / max(alt["launches"], 1), 2),
+            "achieved_tflops": round(alt["flops"] / (alt["ms"] * 1e-3) / 1e12, 3) if alt["ms"] > 0 else None,
+            "traffic": (counter_traffic(other) or {}).get("bytes_per_launch")}}
 
     plain = None
     if a.plain > 0:
@@ -271,7 +316,7 @@ def main():
                        "particles_per_step": len(particles), "path": "batched+adjoint",
                        "reference_table": a.table,
                        "coarse_orientations": int(len(coarse)), "refine_iterations": a.refine_iters},
-            "accuracy": acc, "plain_gpu": plain,
+            "accuracy": acc, "variants": variants, "plain_gpu": plain,
             "speedup_vs_plain": round(value / plain["value"], 2) if plain else None,
             "roofline": roof, "cpu_baseline": cpu}
     print(json.dumps(line))

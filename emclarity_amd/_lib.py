@@ -64,6 +64,10 @@ SIGNATURES = {
                              _d, C.c_int, C.c_int, _d, _d]),
     "emc_template_search_chunk": (_st, [_vp, _vp, _i3, _vp, _i3, _i3, _i3, _i3, _d, C.c_int, C.c_int,
                                         C.c_int, _vp, _vp]),
+    "emc_moving_average": (_st, [_vp, _vp, _i3, C.c_int, _vp]),
+    "emc_local_normalize": (_st, [_vp, _vp, _i3, C.c_int, _vp]),
+    "emc_tsearch_preprocess_chunk": (_st, [_vp, _vp, _i3, _d, C.c_int, C.c_float, _vp, _vp, _d]),
+    "emc_scale": (_st, [_vp, _vp, C.c_int64, C.c_double]),
     "emc_batch_reference_spectrum": (_st, [_vp, C.c_int, _vp, C.c_int, _vp]),
     "emc_batch_prepare_particle": (_st, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "emc_batch_prepare_particles": (_st, [_vp, C.c_int, C.c_int, C.POINTER(C.c_void_p), _vp,

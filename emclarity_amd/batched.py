@@ -221,18 +221,56 @@ def support_voxels(n, support):
     return int(np.count_nonzero(r2 <= support * support))
 
 
+def fused_schedule(n, support, tile=16, width=256):
+    """Column schedule of the fused search kernel (restated from its host
+    builder): columns (kx, ky, h) of the half disk kx^2 + ky^2 <= r^2, tiled
+    in tile x tile (kx, ky) patches, consecutive patches merged up to `width`
+    columns per pass. Returns (passes, pieces): each pass is a list of
+    columns; pieces counts the runs of one kx inside the passes."""
+    k2 = np.float32(support * support)
+    tiles = {}
+    for kx in range(0, n // 2 + 1):
+        if np.float32(kx * kx) > k2:
+            break
+        for ky in range(-(n // 2 - 1), n // 2):
+            if np.float32(kx * kx + ky * ky) > k2:
+                continue
+            h = 0
+            while h + 1 <= n // 2 - 1 and np.float32(kx * kx + ky * ky + (h + 1) ** 2) <= k2:
+                h += 1
+            tiles.setdefault((kx // tile, (ky + 128) // tile), []).append((kx, ky, h))
+    passes, cur = [], []
+    for key in sorted(tiles):
+        if len(cur) + len(tiles[key]) > width:
+            passes.append(cur)
+            cur = []
+        cur += tiles[key]
+    if cur:
+        passes.append(cur)
+    pieces = sum(len({c[0] for c in g}) for g in passes)
+    return passes, pieces
+
+
 def fused_flops_per_unit(n, support, peak_radius):
     """FP32 operations of one (particle, orientation) in align_fused_kernel,
     counted from its arithmetic (an FMA counts 2):
-      per support frequency: j = M k (9 FMA), trilinear (7 complex lerps =
-      14 FMA), X = Q conj(f) (4 FMA-equivalents), norm term (2), pruned z-DFT
-      (D complex MACs = 4 D FMA);
-      per kx plane: y-DFT (D^2 n complex MACs) and x expansion (D^3 real FMA-pairs)."""
+      per support frequency: j = M k from the column base (3 FMA), trilinear
+      (7 complex lerps, 28), X = Q conj(f) (6), norm term (4), z-pruned DFT
+      (D complex MACs, 8 D);
+      per column: y-pruned DFT (D^2 complex MACs, 8 D^2);
+      per plane piece: Hermitian x expansion (D^3 real MACs of a complex
+      product, 4 D^3)."""
     D = 2 * peak_radius + 1
-    v = support_voxels(n, support)
-    per_k = 2 * (9 + 14 + 4 + 2 + 4 * D)
-    planes = n // 2 + 1
-    return v * per_k + planes * (2 * 4 * D * D * n + 2 * 2 * D ** 3)
+    passes, pieces = fused_schedule(n, support)
+    cols = sum(len(g) for g in passes)
+    v = sum(2 * h + 1 for g in passes for (_, _, h) in g)
+    return v * (6 + 28 + 6 + 4 + 8 * D) + cols * 8 * D * D + pieces * 4 * D ** 3
+
+
+def fused_samples_per_unit(n, support):
+    """Trilinear reference samples one (particle, orientation) takes."""
+    passes, _ = fused_schedule(n, support)
+    return sum(2 * h + 1 for g in passes for (_, _, h) in g)
 
 
 def align_batch(aligner: BatchAligner, grid_pts, refine_iters=12):

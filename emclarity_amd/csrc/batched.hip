@@ -20,9 +20,14 @@
 //      sum), entirely in registers and LDS,
 //   4. takes the peak inside the sphere |d| <= r with parabolic sub-pixel
 //      refinement, and writes the normalised CC and the shift.
-// No FFT, no per-orientation volume in HBM: per unit HBM traffic is a few
-// hundred bytes of results; the work is FP32 FMA bound (the roofline used
-// in the benchmark is the FP32 vector peak).
+// No FFT and no per-orientation volume: a unit writes a few bytes of
+// results. What it does read is the reference table, by trilinear gathers
+// (a 2x2x2 footprint per frequency) that the units running together on an
+// XCD share only in part; the traffic past L2 from those gathers, not the
+// FP32 arithmetic, is what bounds the kernel. The reference table can be
+// complex64 or fp16 parts (half the gather bytes), and the frequencies are
+// visited in compact (kx, ky) patches so that nearby orientations read
+// nearby parts of the table at the same time.
 //
 // CC definition (Parseval form of the normalised real-space CC of the
 // band-passed particle and the wedge-weighted rotated reference):
@@ -189,8 +194,9 @@ struct TabC64 {
     float2 a, b;
   };
   __device__ static void store(T* dst, float2 v, float) { *dst = v; }
-  __device__ static void pair(const T* p, float2& a, float2& b) {
-    const Pair v = *reinterpret_cast<const Pair*>(p);  // x-adjacent texels, one 16-byte load
+  // x-adjacent texels, one 16-byte load
+  __device__ static Pair load(const T* p) { return *reinterpret_cast<const Pair*>(p); }
+  __device__ static void unpack(const Pair& v, float2& a, float2& b) {
     a = v.a;
     b = v.b;
   }
@@ -204,8 +210,9 @@ struct TabF16 {
   __device__ static void store(T* dst, float2 v, float s) {
     *dst = half2_t{(_Float16)(v.x * s), (_Float16)(v.y * s)};
   }
-  __device__ static void pair(const T* p, float2& a, float2& b) {
-    const Pair v = *reinterpret_cast<const Pair*>(p);  // one 8-byte load
+  // x-adjacent texels, one 8-byte load
+  __device__ static Pair load(const T* p) { return *reinterpret_cast<const Pair*>(p); }
+  __device__ static void unpack(const Pair& v, float2& a, float2& b) {
     a = make_float2((float)v.a.x, (float)v.a.y);
     b = make_float2((float)v.b.x, (float)v.b.y);
   }
@@ -275,14 +282,24 @@ struct Tri {
 };
 
 // Trilinear sample of the centred table at j (index units of the padded
-// grid). The trilinear interpolant of a Hermitian table is Hermitian,
-// I(-j) = conj I(j), so a sample with jx < 0 is folded as a whole. The
-// clamp keeps every read inside the table for any j; samples with
-// |j| < n are never clamped.
-template <bool GRAD, class Tab>
-__device__ inline Tri f2_sample(const typename Tab::T* __restrict__ F2c, int n, float jx, float jy, float jz) {
-  const bool cj = jx < 0.0f;
-  if (cj) {
+// grid), split into fetch (the 2x2x2 footprint as four x-pairs) and
+// interpolation so a caller can keep several fetches in flight. The
+// trilinear interpolant of a Hermitian table is Hermitian, I(-j) = conj I(j),
+// so a sample with jx < 0 is folded as a whole. The clamp keeps every read
+// inside the table for any j; samples with |j| < n are never clamped.
+template <class Tab>
+struct Foot {
+  typename Tab::Pair t[4];  // (z, y) = (0,0), (0,1), (1,0), (1,1); each an x-pair
+  float fx, fy, fz;
+  bool cj;
+};
+
+template <class Tab>
+__device__ inline Foot<Tab> f2_fetch(const typename Tab::T* __restrict__ F2c, int n, float jx, float jy,
+                                     float jz) {
+  Foot<Tab> q;
+  q.cj = jx < 0.0f;
+  if (q.cj) {
     jx = -jx;
     jy = -jy;
     jz = -jz;
@@ -292,14 +309,26 @@ __device__ inline Tri f2_sample(const typename Tab::T* __restrict__ F2c, int n, 
   jy = fminf(fmaxf(jy, -fn), fn);
   jz = fminf(fmaxf(jz, -fn), fn);
   const float xf = floorf(jx), yf = floorf(jy), zf = floorf(jz);
-  const float fx = jx - xf, fy = jy - yf, fz = jz - zf;
+  q.fx = jx - xf;
+  q.fy = jy - yf;
+  q.fz = jz - zf;
   const int X = n + 2, XY = (2 * n + 2) * (n + 2);
   const typename Tab::T* p = F2c + ((((int)zf + n) * (2 * n + 2) + ((int)yf + n)) * X + (int)xf);
+  q.t[0] = Tab::load(p);
+  q.t[1] = Tab::load(p + X);
+  q.t[2] = Tab::load(p + XY);
+  q.t[3] = Tab::load(p + XY + X);
+  return q;
+}
+
+template <bool GRAD, class Tab>
+__device__ inline Tri f2_interp(const Foot<Tab>& q) {
   float2 t000, t001, t010, t011, t100, t101, t110, t111;  // t[z][y][x]
-  Tab::pair(p, t000, t001);
-  Tab::pair(p + X, t010, t011);
-  Tab::pair(p + XY, t100, t101);
-  Tab::pair(p + XY + X, t110, t111);
+  Tab::unpack(q.t[0], t000, t001);
+  Tab::unpack(q.t[1], t010, t011);
+  Tab::unpack(q.t[2], t100, t101);
+  Tab::unpack(q.t[3], t110, t111);
+  const float fx = q.fx, fy = q.fy, fz = q.fz;
   auto lerp2 = [](float2 a, float2 b, float w) {
     return make_float2(a.x + w * (b.x - a.x), a.y + w * (b.y - a.y));
   };
@@ -308,7 +337,7 @@ __device__ inline Tri f2_sample(const typename Tab::T* __restrict__ F2c, int n, 
   const float2 c0 = lerp2(c00, c01, fy), c1 = lerp2(c10, c11, fy);
   Tri t;
   t.f = lerp2(c0, c1, fz);
-  if (cj) t.f.y = -t.f.y;
+  if (q.cj) t.f.y = -t.f.y;
   if (GRAD) {
     t.g[2] = make_float2(c1.x - c0.x, c1.y - c0.y);
     const float2 dy0 = make_float2(c01.x - c00.x, c01.y - c00.y);
@@ -320,16 +349,35 @@ __device__ inline Tri f2_sample(const typename Tab::T* __restrict__ F2c, int n, 
     const float2 d11 = make_float2(t111.x - t110.x, t111.y - t110.y);
     t.g[0] = lerp2(lerp2(d00, d01, fy), lerp2(d10, d11, fy), fz);
     // d/dj of conj(I(-j)) = -conj(I'(-j))
-    if (cj)
+    if (q.cj)
 #pragma unroll
       for (int i = 0; i < 3; ++i) t.g[i].x = -t.g[i].x;
   }
   return t;
 }
 
+template <bool GRAD, class Tab>
+__device__ inline Tri f2_sample(const typename Tab::T* __restrict__ F2c, int n, float jx, float jy, float jz) {
+  return f2_interp<GRAD, Tab>(f2_fetch<Tab>(F2c, n, jx, jy, jz));
+}
+
 __device__ inline int signed_freq(int i, int n) { return i <= n / 2 ? i : i - n; }
 
 // ------------------------------------------------------------ fused search
+
+// One frequency of a column: X = Q conj(f) into the z-pruned sums
+// acc[d] += X e^{2 pi i kz d / n} (twiddle row e), and the norm term.
+template <int D>
+__device__ inline void accumulate_z(float2 (&acc)[D], float& dpart, float2 qv, float w, float2 f,
+                                    const float2* __restrict__ e) {
+  const float xr = qv.x * f.x + qv.y * f.y, xi = qv.y * f.x - qv.x * f.y;
+  dpart += (f.x * f.x + f.y * f.y) * w;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    acc[d].x = fmaf(xr, e[d].x, fmaf(-xi, e[d].y, acc[d].x));
+    acc[d].y = fmaf(xr, e[d].y, fmaf(xi, e[d].x, acc[d].y));
+  }
+}
 
 template <int RR, class Tab>
 __global__ void __launch_bounds__(kBatchThreads) align_fused_kernel(
@@ -382,7 +430,7 @@ __global__ void __launch_bounds__(kBatchThreads) align_fused_kernel(
   for (int g = 0; g < ngroups; ++g) {
     const int4 G = groups[g];  // (first column, column count, first piece, piece count)
     const bool active = t < G.y;
-    const int packed = active ? cols[G.x + t] : 0;
+    const int packed = active ? cols[G.x + t] : (128 << 8);  // idle slot: k = (0, 0), h = -1
     const int kx = packed & 255, ky = ((packed >> 8) & 255) - 128;
     const int h = active ? ((packed >> 16) & 255) : -1;
     const int yy = ky < 0 ? ky + n : ky;
@@ -395,23 +443,23 @@ __global__ void __launch_bounds__(kBatchThreads) align_fused_kernel(
     float dpart = 0.0f;
     const float fkx = (float)kx, fky = (float)ky;
     const float bx = m00 * fkx + m01 * fky, by = m10 * fkx + m11 * fky, bz = m20 * fkx + m21 * fky;
-    for (int kz = -hw; kz <= hw; ++kz) {
-      const int zz = kz < 0 ? kz + n : kz;
-      if (kz >= -h && kz <= h) {
-        const int o = (zz * cx + kx) * n + yy;
-        const float2 qv = Qp[o];
-        const float w = Np[o];
-        const float fkz = (float)kz;
-        const float2 f = f2_sample<false, Tab>(F2, n, bx + m02 * fkz, by + m12 * fkz, bz + m22 * fkz).f;
-        // X = Q conj(f)
-        const float xr = qv.x * f.x + qv.y * f.y, xi = qv.y * f.x - qv.x * f.y;
-        dpart += (f.x * f.x + f.y * f.y) * w;
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          const float2 e = tw[zz][d];
-          acc[d].x = fmaf(xr, e.x, fmaf(-xi, e.y, acc[d].x));
-          acc[d].y = fmaf(xr, e.y, fmaf(xi, e.x, acc[d].y));
-        }
+    // two kz per iteration: both samples' loads are issued in one block
+    // before either is consumed (accumulation order stays kz, kz + 1). A
+    // lane at its column end also fetches kz + 1 = h + 1: its Q index is
+    // valid (kz + 1 <= n/2) and the table read is clamped; it is not summed.
+    for (int kz = -hw; kz <= hw; kz += 2) {
+      const int kz1 = kz + 1;
+      const bool pa = kz >= -h && kz <= h, pb = kz1 >= -h && kz1 <= h;
+      if (pa || pb) {
+        const int za = kz < 0 ? kz + n : kz, zb = kz1 < 0 ? kz1 + n : kz1;
+        const int oa = (za * cx + kx) * n + yy, ob = (zb * cx + kx) * n + yy;
+        const float2 qa = Qp[oa], qb = Qp[ob];
+        const float wa = Np[oa], wb = Np[ob];
+        const float fa = (float)kz, fb = (float)kz1;
+        const Foot<Tab> ta = f2_fetch<Tab>(F2, n, bx + m02 * fa, by + m12 * fa, bz + m22 * fa);
+        const Foot<Tab> tb = f2_fetch<Tab>(F2, n, bx + m02 * fb, by + m12 * fb, bz + m22 * fb);
+        if (pa) accumulate_z<D>(acc, dpart, qa, wa, f2_interp<false, Tab>(ta).f, tw[za]);
+        if (pb) accumulate_z<D>(acc, dpart, qb, wb, f2_interp<false, Tab>(tb).f, tw[zb]);
       }
     }
     if (active) dsum += ((kx == 0 || (n % 2 == 0 && kx == n / 2)) ? 1.0 : 2.0) * (double)dpart;

@@ -107,3 +107,35 @@ def mexCTF(doHalfGrid, doSqCTF, nX, nY, pixelSize, waveLength, CS, defocus1, def
               float(pixelSize), float(waveLength), float(CS), float(defocus1), float(defocus2),
               float(defocusAst), float(AmpContrast), int(bool(centered)), out.vp)
     return out
+
+
+def BH_mask3d(SHAPE, SIZE, RADIUS, CENTER, ctx=None) -> DeviceArray:
+    """masking/BH_mask3d.m for SHAPE 'rectangle' at CENTER [0,0,0] (the
+    valid-area mask of template search, BH_templateSearch3d_2.m:402): a
+    window of ones 2*RADIUS+13 per axis (:102-108), padded to SIZE with the
+    7-voxel cosine taper and the half-voxel origin shift of even sizes
+    (:244-280), then tapered at the volume edge (:462-466). Other shapes are
+    not part of this path."""
+    if str(SHAPE).lower() != "rectangle" or any(float(c) != 0.0 for c in CENTER):
+        raise NotImplementedError("only BH_mask3d('rectangle', SIZE, RADIUS, [0,0,0]) is provided")
+    from .sizing import BH_multi_padVal
+    size = np.asarray(SIZE, dtype=np.int64)
+    win = (2 * np.asarray(RADIUS, dtype=np.float64) + 13).astype(np.int64)
+    pv = BH_multi_padVal(win, size)
+    shift = 1 - size % 2
+    pre = pv[0] + shift
+    post = pv[1] - shift
+    ones = DeviceArray.from_numpy(np.ones(tuple(int(w) for w in win[::-1]), np.float32), ctx=ctx)
+    # negative pads trim (BH_padZeros3d), as the reference trims before padding
+    w = BH_padZeros3d(ones, np.minimum(pre, 0), np.minimum(post, 0), ctx=ctx)
+    w = BH_padZeros3d(w, np.maximum(pre, 0), np.maximum(post, 0), "GPU", "singleTaper", ctx=ctx)
+    return BH_padZeros3d(w, [0, 0, 0], [0, 0, 0], "GPU", "singleTaper", ctx=ctx)
+
+
+def BH_movingAverage_2(img, featureSize, ctx=None) -> DeviceArray:
+    """statistics/BH_movingAverage_2.m: Gaussian (sigma 3, featureSize taps)
+    moving average with symmetric padding (emc_moving_average)."""
+    v = as_device(img, np.float32, ctx)
+    out = DeviceArray(v.shape, np.float32, v.ctx)
+    _lib.call("emc_moving_average", v.ctx.handle, v.vp, _lib.i3(v.dims), int(featureSize), out.vp)
+    return out

@@ -4,9 +4,12 @@
 (symmetric padding of the tomogram), :480-535 (chunk cut-out), :650-982
 (theta -> chunk -> phi -> psi loop with the per-voxel max/argmax update) and
 :985-994 (crop of the padding). The per-chunk work runs in
-emc_template_search_chunk. The tomogram is taken as already preprocessed
-(the reference's per-chunk bandpass and local normalisation, :516-610, are
-not part of this path).
+emc_template_search_chunk. ``preprocess_chunks`` is the reference's chunk
+pre-processing (:480-611: standardise, bandpass, local normalisation over
+the stats radius, sign and valid-area mask, global variance), one
+emc_tsearch_preprocess_chunk call per chunk; ``template_search`` runs it
+first when given its parameters, else it takes the tomogram as already
+pre-processed.
 """
 from __future__ import annotations
 
@@ -36,13 +39,53 @@ def symmetric_pad(vol, pre, post):
     return np.pad(vol, p, mode="symmetric")
 
 
+def preprocess_chunks(padded, corners, size_chunk, valid_calc, bp_vals, pixel, stats_radius,
+                      should_be_ctf=1, ctx=None):
+    """BH_templateSearch3d_2.m:480-611 on the padded tomogram (numpy, z,y,x):
+    each chunk (x,y,z corner in ``corners``) is standardised, band-passed
+    (bp_vals = HIGH_THRESH, HIGH_CUT, LOW_CUT at ``pixel``), locally
+    normalised over ``stats_radius`` voxels, multiplied by -shouldBeCTF and
+    the valid-area mask BH_mask3d('rectangle', sizeChunk, validCalc/2)
+    (:402). The reference accumulates the square of each chunk's masked sum
+    (:590), so globalVariance = sum(s_i^2)/N - (sum(s_i)/N)^2 with N = number
+    of chunks * numel(chunk) (:603); every chunk is divided by its square
+    root (:605-611). Returns (device chunks, globalVariance)."""
+    from .masking import BH_mask3d
+    sc = [int(s) for s in size_chunk]
+    mask = BH_mask3d("rectangle", sc, np.asarray(valid_calc, dtype=np.float64) / 2.0, [0, 0, 0], ctx=ctx)
+    ctx = mask.ctx
+    bp = np.ascontiguousarray([float(bp_vals[0]), float(bp_vals[1]), float(bp_vals[2]), float(pixel)])
+    dp = _lib.C.POINTER(_lib.C.c_double)
+    out, fx, fx2, fn = [], 0.0, 0.0, 0.0
+    for (cx, cy, cz) in corners:
+        sub = padded[cz:cz + sc[2], cy:cy + sc[1], cx:cx + sc[0]]
+        if sub.shape != (sc[2], sc[1], sc[0]):
+            raise ValueError("chunk outside the padded tomogram")
+        raw = DeviceArray.from_numpy(np.ascontiguousarray(sub, dtype=np.float32), ctx=ctx)
+        res = DeviceArray(raw.shape, np.float32, ctx)
+        tmp = _lib.C.c_double()
+        _lib.call("emc_tsearch_preprocess_chunk", ctx.handle, raw.vp, _lib.i3(sc), bp.ctypes.data_as(dp),
+                  int(stats_radius), float(-should_be_ctf), mask.vp, res.vp, _lib.C.byref(tmp))
+        fx += tmp.value
+        fx2 += tmp.value ** 2
+        fn += float(np.prod(sc))
+        out.append(res)
+    gvar = fx2 / fn - (fx / fn) ** 2
+    for c in out:
+        _lib.call("emc_scale", ctx.handle, c.vp, int(np.prod(sc)), 1.0 / np.sqrt(gvar))
+    return out, gvar
+
+
 def template_search(tomogram, template, angles_pts, size_chunk, valid_area, n_iters, tomo_pre,
-                    tomo_post, ctx=None):
+                    tomo_post, ctx=None, preprocess=None):
     """Per-voxel max CC and angle id (1-based into angles_pts, 0 where no
     orientation scored above 0 in a chunk other than the first, as the
     reference initialises RESULTS with zeros). Returns (peak, angle) with the
     tomogram's shape. ``angles_pts`` rows (phi, theta, psi) in the reference
-    order theta -> phi -> psi; blocks of one theta are passed per chunk call."""
+    order theta -> phi -> psi; blocks of one theta are passed per chunk call.
+    ``preprocess``: None (the tomogram is already pre-processed) or a dict
+    with valid_calc, bp_vals, pixel, stats_radius and should_be_ctf for
+    ``preprocess_chunks``."""
     tomogram = np.asarray(tomogram, dtype=np.float32)
     template = np.asarray(template, dtype=np.float32)
     nz, ny, nx = tomogram.shape
@@ -62,12 +105,15 @@ def template_search(tomogram, template, angles_pts, size_chunk, valid_area, n_it
             thetas.append([th, i, i + 1])
         else:
             thetas[-1][2] = i + 1
-    chunks = []
-    for (cx, cy, cz) in cuts:
-        sub = padded[cz:cz + sc[2], cy:cy + sc[1], cx:cx + sc[0]]
-        if sub.shape != (sc[2], sc[1], sc[0]):
-            raise ValueError("chunk outside the padded tomogram")
-        chunks.append(DeviceArray.from_numpy(np.ascontiguousarray(sub), ctx=ctx))
+    if preprocess is not None:
+        chunks, _ = preprocess_chunks(padded, cuts, sc, ctx=ctx, **preprocess)
+    else:
+        chunks = []
+        for (cx, cy, cz) in cuts:
+            sub = padded[cz:cz + sc[2], cy:cy + sc[1], cx:cx + sc[0]]
+            if sub.shape != (sc[2], sc[1], sc[0]):
+                raise ValueError("chunk outside the padded tomogram")
+            chunks.append(DeviceArray.from_numpy(np.ascontiguousarray(sub), ctx=ctx))
     vshape = (va[2], va[1], va[0])
     peaks = [DeviceArray.zeros(vshape, np.float32, ctx) for _ in cuts]
     angs = [DeviceArray.zeros(vshape, np.int32, ctx) for _ in cuts]

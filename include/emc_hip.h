@@ -350,6 +350,36 @@ emc_status emc_template_search_chunk(emc_ctx* ctx, const float* d_chunk, const i
                                      int n_orient, int first_angle_id, int init_first,
                                      float* d_peak, int32_t* d_angle);
 
+/* -------------------------------- template-search pre-processing (next) */
+
+/* statistics/BH_movingAverage_2.m:1-25: symmetric pad by ceil(F/2), the
+ * separable normalised Gaussian EMC_gaussianKernel([1,F], 3)
+ * (masking/EMC_gaussianKernel.m) through convn(.., 'same') along x, y, z
+ * (testScripts/EMC_convn.m), crop. F = feature_size >= 2 with ceil(F/2) <=
+ * every dimension. d_out may equal d_in. Stream-ordered. */
+emc_status emc_moving_average(emc_ctx* ctx, const float* d_in, const int dims[3], int feature_size,
+                              float* d_out);
+
+/* alignment/BH_templateSearch3d_2.m:558-561: (x - avg) ./ sqrt(avg2 - avg.^2)
+ * with avg, avg2 the moving averages of x and x.^2 (emc_moving_average).
+ * d_out may equal d_in. Stream-ordered. */
+emc_status emc_local_normalize(emc_ctx* ctx, const float* d_in, const int dims[3], int feature_size,
+                               float* d_out);
+
+/* One chunk of alignment/BH_templateSearch3d_2.m:532-589: x = (x - mean) / rms;
+ * bandpass x = C2R(R2C(x) .* BH_bandpass3d half grid) / N with bp = {HIGH_THRESH,
+ * HIGH_CUT, LOW_CUT, pixel} (fourierTransformer fwdFFT(x,0,0,bp) then
+ * invFFT(.., 2)); local normalisation over stats_radius; d_out = sign * x .*
+ * d_valid_mask (sign = -shouldBeCTF; mask NULL = 1). *h_masked_sum (if not
+ * NULL) = sum(d_out(mask > 0.1)), the chunk's term of the global variance
+ * (:589-592). Synchronous when h_masked_sum is given. */
+emc_status emc_tsearch_preprocess_chunk(emc_ctx* ctx, const float* d_in, const int dims[3], const double bp[4],
+                                        int stats_radius, float sign, const float* d_valid_mask, float* d_out,
+                                        double* h_masked_sum);
+
+/* d[i] *= s (the global-variance scaling of the chunk stack, :605-611). */
+emc_status emc_scale(emc_ctx* ctx, float* d, int64_t count, double s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -914,3 +914,130 @@ def batch_eval(F2, n, bp_: BatchParticle, R, s, dR=None):
         dD = float(np.sum(2.0 * np.real(np.conj(f * w) * df * w)))
         grad[q] = dnum / norm - num * dD / (2.0 * D * norm)
     return score, grad
+
+
+# --------------------------------------------------------------------------
+# Template-search pre-processing (alignment/BH_templateSearch3d_2.m:480-611)
+# --------------------------------------------------------------------------
+
+
+def gaussian_taps(F, sigma=3.0):
+    """EMC_gaussianKernel([1, F], sigma) (masking/EMC_gaussianKernel.m: row
+    vector on EMC_coordVectors origin 0, i.e. -F/2+0.5 .. F/2-0.5), sum 1."""
+    v = np.arange(F) - (F - 1) / 2.0
+    k = np.exp(-0.5 * (v / sigma) ** 2)
+    return k / k.sum()
+
+
+def matlab_conv_same(u, v):
+    """MATLAB conv(u, v, 'same'): the full convolution from (1-based) index
+    floor(numel(v)/2) + 1, numel(u) samples."""
+    full = np.convolve(np.asarray(u, dtype=np.float64), np.asarray(v, dtype=np.float64))
+    s = len(v) // 2
+    return full[s:s + len(u)]
+
+
+def _conv_same_axis(a, k, axis):
+    """convn(a, k along `axis`, 'same') (zero outside a)."""
+    F, n = len(k), a.shape[axis]
+    s = F // 2
+    ap = np.pad(a, [(F, F) if ax == axis else (0, 0) for ax in range(a.ndim)])
+    out = np.zeros_like(a)
+    for m in range(F):
+        sl = [slice(None)] * a.ndim
+        start = F + s - m                # out[i] = sum_m k[m] a[i + s - m]
+        sl[axis] = slice(start, start + n)
+        out += k[m] * ap[tuple(sl)]
+    return out
+
+
+def moving_average(img, F):
+    """BH_movingAverage_2 (statistics/BH_movingAverage_2.m:1-25): padarray
+    'symmetric' by ceil(F/2) (:10-11), EMC_convn with the separable kernel
+    (testScripts/EMC_convn.m:54-56: 'same' along x, y, z), crop back with
+    EMC_limits/EMC_resize (:18-19; equal left/right crops)."""
+    pad = (F + 1) // 2
+    a = np.pad(np.asarray(img, dtype=np.float64), pad, mode="symmetric")
+    k = gaussian_taps(F)
+    k = k / k.sum()                                                  # :14
+    for axis in (2, 1, 0):                                           # x, y, z of an (nz,ny,nx) array
+        a = _conv_same_axis(a, k, axis)
+    return a[pad:-pad, pad:-pad, pad:-pad]
+
+
+def local_normalize(img, F):
+    """BH_templateSearch3d_2.m:558-561."""
+    x = np.asarray(img, dtype=np.float64)
+    avg = moving_average(x, F)
+    avg2 = moving_average(x * x, F)
+    with np.errstate(invalid="ignore"):
+        return (x - avg) / np.sqrt(avg2 - avg ** 2)
+
+
+def mask3d_rectangle(size, radius):
+    """BH_mask3d('rectangle', size, radius, [0,0,0]) (masking/BH_mask3d.m:
+    :102-108 window of ones 2r+13 per axis, :244-280 pad to size with the
+    singleTaper and the even-size origin shift, :462-466 final singleTaper).
+    size and radius are (x, y, z); returns (nz, ny, nx)."""
+    size = np.asarray(size, dtype=np.int64)
+    win_xyz = (2 * np.asarray(radius, dtype=np.float64) + 13).astype(np.int64)
+    win = np.ones(win_xyz[::-1])
+    pv = pad_val(win_xyz, size)
+    shift = 1 - size % 2
+    pre = pv[0] + shift
+    post = pv[1] - shift
+    pre_trim = np.where(pre < 0, -pre, 0)
+    post_trim = np.where(post < 0, -post, 0)
+    pre = np.where(pre >= 0, pre, 0)
+    post = np.where(post >= 0, post, 0)
+    nz, ny, nx = win.shape
+    win = win[pre_trim[2]:nz - post_trim[2], pre_trim[1]:ny - post_trim[1], pre_trim[0]:nx - post_trim[0]]
+    win = pad_zeros3d(win, pre, post, taper=True)
+    return pad_zeros3d(win, [0, 0, 0], [0, 0, 0], taper=True)
+
+
+def tsearch_preprocess_chunk(chunk, bp_vals, pixel, stats_radius, sign, valid_mask=None):
+    """One chunk of BH_templateSearch3d_2.m:532-589: returns (out, tmp_sum).
+    bp_vals = (HIGH_THRESH, HIGH_CUT, LOW_CUT); sign = -shouldBeCTF."""
+    x = np.asarray(chunk, dtype=np.float64)
+    x = x - x.mean()                                                  # :535
+    x = x / np.sqrt(np.mean(x * x))                                   # :536 (rms)
+    nz, ny, nx = x.shape
+    bp = bandpass3d((nx, ny, nz), bp_vals[0], bp_vals[1], bp_vals[2], pixel)
+    # fwdFFT(x,0,0,[bp,pixel]) then invFFT(.., 2) (fourierTransformer.m:92-158, :294-328)
+    x = np.fft.irfftn(np.fft.rfftn(x) * bp[:, :, :nx // 2 + 1], s=x.shape)
+    x = local_normalize(x, stats_radius)                              # :558-561
+    m = np.ones_like(x) if valid_mask is None else np.asarray(valid_mask, dtype=np.float64)
+    out = (sign * x) * m                                              # :586
+    return out, float(np.sum(out[m > 0.1]))                           # :589
+
+
+def tsearch_preprocess(tomogram, size_chunk, valid_area, n_iters, tomo_pre, tomo_post, valid_calc,
+                       bp_vals, pixel, stats_radius, should_be_ctf=1):
+    """BH_templateSearch3d_2.m:397-398 (symmetric padding) and :480-611: the
+    stack of pre-processed chunks (chunk corners in the padded tomogram) and
+    the global variance. The reference accumulates the square of each chunk's
+    masked SUM (:590), so globalVariance = sum(tmp_sum^2)/N - (sum(tmp_sum)/N)^2
+    with N = chunks * numel(chunk); every chunk is divided by its square root."""
+    t = np.asarray(tomogram, dtype=np.float64)
+    p = [(int(tomo_pre[2]), int(tomo_post[2])), (int(tomo_pre[1]), int(tomo_post[1])),
+         (int(tomo_pre[0]), int(tomo_post[0]))]
+    t = np.pad(t, p, mode="symmetric")
+    sc = [int(s) for s in size_chunk]
+    va = [int(s) for s in valid_area]
+    vmask = mask3d_rectangle(sc, np.asarray(valid_calc, dtype=np.float64) / 2.0)   # :402
+    chunks, corners = [], []
+    fx = fx2 = fn = 0.0
+    for ix in range(int(n_iters[0])):
+        for iy in range(int(n_iters[1])):
+            for iz in range(int(n_iters[2])):
+                cx, cy, cz = ix * va[0], iy * va[1], iz * va[2]
+                sub = t[cz:cz + sc[2], cy:cy + sc[1], cx:cx + sc[0]]
+                out, s = tsearch_preprocess_chunk(sub, bp_vals, pixel, stats_radius, -float(should_be_ctf), vmask)
+                fx += s
+                fx2 += s * s
+                fn += float(np.prod(sc))
+                chunks.append(out)
+                corners.append((cx, cy, cz))
+    gvar = fx2 / fn - (fx / fn) ** 2                                  # :603
+    return [c / math.sqrt(gvar) for c in chunks], corners, gvar, vmask

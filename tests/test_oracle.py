@@ -431,3 +431,80 @@ def test_batch_f16_table_rounding_and_scale_invariance():
     s16x, sh16x = O.batch_score_map(8.0 * H, n, bpp, R, 4)
     assert abs(s16 - s16x) < 1e-12 and np.allclose(sh16, sh16x, atol=1e-12)
     assert abs(s16 - s64) < 1e-4 * abs(s64) and np.max(np.abs(sh16 - sh64)) < 1e-3
+
+
+# ------------------------------------------ template-search pre-processing
+
+def test_matlab_conv_same_known_answers():
+    # MATLAB: conv([1 2 3],[1 1],'same') = [3 5 3]; conv([1 2 3 4],[1 2 1],'same') = [4 8 12 11]
+    assert np.array_equal(O.matlab_conv_same([1, 2, 3], [1, 1]), [3, 5, 3])
+    assert np.array_equal(O.matlab_conv_same([1, 2, 3, 4], [1, 2, 1]), [4, 8, 12, 11])
+    # the example of MATLAB's conv documentation (even-length kernel)
+    assert np.array_equal(O.matlab_conv_same([-1, 2, 3, -2, 0, 1, 2], [2, 4, -1, 1]), [15, 5, -9, 7, 6, 7, -1])
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal((5, 6, 7))
+    k = rng.standard_normal(4)
+    ref = np.apply_along_axis(O.matlab_conv_same, 1, a, k)
+    assert np.allclose(O._conv_same_axis(a, k, 1), ref, atol=1e-12)
+
+
+@pytest.mark.parametrize("F", [4, 5, 8, 9, 12])
+def test_moving_average_matches_reflect_correlation(F):
+    """BH_movingAverage_2 == separable correlation with the normalised
+    Gaussian under symmetric (scipy 'reflect') boundaries; an even kernel is
+    centred half a voxel high (MATLAB 'same'): scipy origin -1."""
+    from scipy.ndimage import correlate1d
+    rng = np.random.default_rng(F)
+    a = rng.standard_normal((12, 14, 10))
+    k = O.gaussian_taps(F)
+    assert abs(k.sum() - 1) < 1e-12 and np.allclose(k, k[::-1])
+    ref = a
+    for ax in (2, 1, 0):
+        ref = correlate1d(ref, k, axis=ax, mode="reflect", origin=-1 if F % 2 == 0 else 0)
+    assert np.allclose(O.moving_average(a, F), ref, atol=1e-12)
+    assert np.allclose(O.moving_average(np.full((8, 9, 10), 2.5), F), 2.5)
+
+
+def test_local_normalize_invariance():
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal((10, 11, 12))
+    assert np.allclose(O.local_normalize(3.0 * a - 7.0, 6), O.local_normalize(a, 6), atol=1e-9)
+
+
+def test_preprocess_bandpass_half_grid_equals_full():
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal((12, 10, 16))
+    bp = O.bandpass3d((16, 10, 12), 1e-3, 40.0, 8.0, 2.5)
+    half = np.fft.irfftn(np.fft.rfftn(x) * bp[:, :, :9], s=x.shape)
+    full = np.real(np.fft.ifftn(np.fft.fftn(x) * bp))
+    assert np.allclose(half, full, atol=1e-12)
+
+
+def test_mask3d_rectangle_edges():
+    """size 40, radius 12: window 37, padVal low/high 2/1 plus the even-size
+    shift -> 3/0; the window's taper times the volume-edge taper."""
+    m = O.mask3d_rectangle((40, 36, 32), (12, 10, 8))
+    t = 0.5 + 0.5 * np.cos(np.arange(7) * np.pi / 7)
+    line = m[16, 18, :]
+    assert np.all(line[:3] == 0)
+    assert np.isclose(line[3], t[6] * t[3]) and np.isclose(line[6], t[3] * t[0])
+    assert np.all(line[10:33] == 1.0)
+    assert np.isclose(line[39], t[6] * t[6]) and np.isclose(line[33], t[0] * t[0])
+    # separable: the volume is the outer product of its central lines
+    lz, ly, lx = m[:, 18, 20], m[16, :, 20], m[16, 18, :]
+    assert np.allclose(m, lz[:, None, None] * ly[None, :, None] * lx[None, None, :])
+
+
+def test_tsearch_global_variance_squares_chunk_sums():
+    rng = np.random.default_rng(6)
+    tomo = rng.standard_normal((30, 36, 40))
+    chunks, corners, gvar, mask = O.tsearch_preprocess(tomo, (24, 24, 24), (16, 16, 16), (2, 2, 1),
+                                                       (4, 4, 2), (4, 4, 2), (20, 20, 20), (1e-3, 600.0, 28.0),
+                                                       10.0, 8, 1)
+    sums = []
+    for (cx, cy, cz) in corners:
+        t = np.pad(tomo, [(2, 2), (4, 4), (4, 4)], mode="symmetric")[cz:cz + 24, cy:cy + 24, cx:cx + 24]
+        sums.append(O.tsearch_preprocess_chunk(t, (1e-3, 600.0, 28.0), 10.0, 8, -1.0, mask)[1])
+    N = len(sums) * 24 ** 3
+    assert np.isclose(gvar, sum(s * s for s in sums) / N - (sum(sums) / N) ** 2, rtol=1e-12)
+    assert len(chunks) == 4 and corners == [(0, 0, 0), (0, 16, 0), (16, 0, 0), (16, 16, 0)]
