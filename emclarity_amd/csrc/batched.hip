@@ -188,33 +188,37 @@ __global__ void compose_kernel(const double* __restrict__ start_cm, const double
 // correlation and its gradient do not depend on s.
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 
+// A table entry holds the texels (x, y) and (x, y+1); entries x0 and x0+1
+// are adjacent, so one z-plane of a 2x2x2 footprint is one contiguous quad.
 struct TabC64 {
   using T = float2;
-  struct alignas(8) Pair {
-    float2 a, b;
+  struct alignas(16) Quad {  // (x0,y0), (x0,y1), (x1,y0), (x1,y1): 32 bytes
+    float2 a, b, c, d;
   };
   __device__ static void store(T* dst, float2 v, float) { *dst = v; }
-  // x-adjacent texels, one 16-byte load
-  __device__ static Pair load(const T* p) { return *reinterpret_cast<const Pair*>(p); }
-  __device__ static void unpack(const Pair& v, float2& a, float2& b) {
-    a = v.a;
-    b = v.b;
+  __device__ static Quad load(const T* p) { return *reinterpret_cast<const Quad*>(p); }
+  __device__ static void unpack(const Quad& q, float2& x0y0, float2& x0y1, float2& x1y0, float2& x1y1) {
+    x0y0 = q.a;
+    x0y1 = q.b;
+    x1y0 = q.c;
+    x1y1 = q.d;
   }
 };
 
 struct TabF16 {
   using T = half2_t;
-  struct alignas(4) Pair {
-    half2_t a, b;
+  struct alignas(8) Quad {  // 16 bytes, one load
+    half2_t a, b, c, d;
   };
   __device__ static void store(T* dst, float2 v, float s) {
     *dst = half2_t{(_Float16)(v.x * s), (_Float16)(v.y * s)};
   }
-  // x-adjacent texels, one 8-byte load
-  __device__ static Pair load(const T* p) { return *reinterpret_cast<const Pair*>(p); }
-  __device__ static void unpack(const Pair& v, float2& a, float2& b) {
-    a = make_float2((float)v.a.x, (float)v.a.y);
-    b = make_float2((float)v.b.x, (float)v.b.y);
+  __device__ static Quad load(const T* p) { return *reinterpret_cast<const Quad*>(p); }
+  __device__ static void unpack(const Quad& q, float2& x0y0, float2& x0y1, float2& x1y0, float2& x1y1) {
+    x0y0 = make_float2((float)q.a.x, (float)q.a.y);
+    x0y1 = make_float2((float)q.b.x, (float)q.b.y);
+    x1y0 = make_float2((float)q.c.x, (float)q.c.y);
+    x1y1 = make_float2((float)q.d.x, (float)q.d.y);
   }
 };
 
@@ -247,33 +251,40 @@ __global__ void __launch_bounds__(kRedThreads) f2_scale_kernel(const unsigned lo
   }
 }
 
-// Centred reference table F2c of the 2n-padded spectrum: tx in [0, n+1],
-// ty, tz in [-n, n+1], element [(tz + n) (2n+2) + (ty + n)] (n+2) + tx.
-// Texels beyond the raw half grid repeat it periodically (period 2n) or by
-// Hermitian symmetry (tx = n+1), so a folded sample with |j| <= n reads
-// its 2x2x2 footprint without wrapping.
+// Texel (tx, ty, tz) of the 2n-padded half spectrum for tx in [0, n+1],
+// ty, tz in [-n, n+1]: indices repeat with period 2n, tx = n + 1 =
+// -(n - 1) mod 2n comes from the Hermitian mirror (n - 1, -ty, -tz).
+__device__ inline float2 raw_texel(const float2* __restrict__ raw, int n, int tx, int ty, int tz) {
+  const int m = 2 * n;
+  bool cj = false;
+  if (tx > n) {
+    tx = m - tx;
+    ty = -ty;
+    tz = -tz;
+    cj = true;
+  }
+  ty = ty < 0 ? ty + m : (ty >= m ? ty - m : ty);
+  tz = tz < 0 ? tz + m : (tz >= m ? tz - m : tz);
+  const float2 v = raw[((int64_t)tz * m + ty) * (n + 1) + tx];
+  return cj ? make_float2(v.x, -v.y) : v;
+}
+
+// Centred reference table F2c: entries (x, y, z) for x in [0, n+1], y in
+// [-n, n], z in [-n, n+1], entry index [(z + n)(2n+1) + (y + n)](n+2) + x,
+// each the texel pair (x, y, z), (x, y+1, z). A folded sample with |j| <= n
+// reads its footprint without wrapping: two quads, planes z0 and z0+1.
 template <class Tab>
 __global__ void f2_centre_kernel(const float2* __restrict__ raw, int n, const float* __restrict__ scale,
                                  typename Tab::T* __restrict__ F2c) {
-  const int X = n + 2, Y = 2 * n + 2, m = 2 * n;
+  const int X = n + 2, Y = 2 * n + 1;
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int yi = blockIdx.y * blockDim.y + threadIdx.y;
   const int zi = blockIdx.z;
   if (x >= X || yi >= Y) return;
-  const int ty = yi - n, tz = zi - n;
-  float2 v;
-  if (x <= n) {
-    const int y = ty < 0 ? ty + m : (ty >= m ? ty - m : ty);
-    const int z = tz < 0 ? tz + m : (tz >= m ? tz - m : tz);
-    v = raw[((int64_t)z * m + y) * (n + 1) + x];
-  } else {  // tx = n + 1 = -(n - 1) mod 2n: conj of (n - 1, -ty, -tz)
-    int y = -ty, z = -tz;
-    y = y < 0 ? y + m : (y >= m ? y - m : y);
-    z = z < 0 ? z + m : (z >= m ? z - m : z);
-    const float2 w = raw[((int64_t)z * m + y) * (n + 1) + (n - 1)];
-    v = make_float2(w.x, -w.y);
-  }
-  Tab::store(F2c + ((int64_t)zi * Y + yi) * X + x, v, scale ? scale[0] : 1.0f);
+  const float s = scale ? scale[0] : 1.0f;
+  typename Tab::T* e = F2c + 2 * (((int64_t)zi * Y + yi) * X + x);
+  Tab::store(e, raw_texel(raw, n, x, yi - n, zi - n), s);
+  Tab::store(e + 1, raw_texel(raw, n, x, yi - n + 1, zi - n), s);
 }
 
 struct Tri {
@@ -282,14 +293,14 @@ struct Tri {
 };
 
 // Trilinear sample of the centred table at j (index units of the padded
-// grid), split into fetch (the 2x2x2 footprint as four x-pairs) and
+// grid), split into fetch (the 2x2x2 footprint as two quads) and
 // interpolation so a caller can keep several fetches in flight. The
 // trilinear interpolant of a Hermitian table is Hermitian, I(-j) = conj I(j),
 // so a sample with jx < 0 is folded as a whole. The clamp keeps every read
 // inside the table for any j; samples with |j| < n are never clamped.
 template <class Tab>
 struct Foot {
-  typename Tab::Pair t[4];  // (z, y) = (0,0), (0,1), (1,0), (1,1); each an x-pair
+  typename Tab::Quad t[2];  // planes z0, z0+1
   float fx, fy, fz;
   bool cj;
 };
@@ -312,22 +323,18 @@ __device__ inline Foot<Tab> f2_fetch(const typename Tab::T* __restrict__ F2c, in
   q.fx = jx - xf;
   q.fy = jy - yf;
   q.fz = jz - zf;
-  const int X = n + 2, XY = (2 * n + 2) * (n + 2);
-  const typename Tab::T* p = F2c + ((((int)zf + n) * (2 * n + 2) + ((int)yf + n)) * X + (int)xf);
+  const int X = n + 2, Y = 2 * n + 1;
+  const typename Tab::T* p = F2c + 2 * ((((int)zf + n) * Y + ((int)yf + n)) * X + (int)xf);
   q.t[0] = Tab::load(p);
-  q.t[1] = Tab::load(p + X);
-  q.t[2] = Tab::load(p + XY);
-  q.t[3] = Tab::load(p + XY + X);
+  q.t[1] = Tab::load(p + 2 * X * Y);
   return q;
 }
 
 template <bool GRAD, class Tab>
 __device__ inline Tri f2_interp(const Foot<Tab>& q) {
   float2 t000, t001, t010, t011, t100, t101, t110, t111;  // t[z][y][x]
-  Tab::unpack(q.t[0], t000, t001);
-  Tab::unpack(q.t[1], t010, t011);
-  Tab::unpack(q.t[2], t100, t101);
-  Tab::unpack(q.t[3], t110, t111);
+  Tab::unpack(q.t[0], t000, t010, t001, t011);
+  Tab::unpack(q.t[1], t100, t110, t101, t111);
   const float fx = q.fx, fy = q.fy, fz = q.fz;
   auto lerp2 = [](float2 a, float2 b, float w) {
     return make_float2(a.x + w * (b.x - a.x), a.y + w * (b.y - a.y));
@@ -379,129 +386,145 @@ __device__ inline void accumulate_z(float2 (&acc)[D], float& dpart, float2 qv, f
   }
 }
 
+// One pass of the column schedule for every (particle, orientation) unit:
+// the z-pruned DFT of the pass's support columns (registers), then the
+// y-pruned DFT per plane piece and the Hermitian x expansion (LDS), added to
+// the unit's shift-cube map and normalisation sum in global memory. One
+// launch per pass keeps all resident units of an XCD on the same pass, so
+// their reads of the reference table (nearby orientations of one particle
+// on nearby patches) share that XCD's L2; the map round trip per pass is
+// 4 D^3 bytes each way.
 template <int RR, class Tab>
-__global__ void __launch_bounds__(kBatchThreads) align_fused_kernel(
+__global__ void __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(RR <= 6 ? 4 : 3))) align_pass_kernel(
     const typename Tab::T* __restrict__ F2, int n, const float2* __restrict__ Q, const float* __restrict__ Nw,
-    const double* __restrict__ A, const float* __restrict__ Mall, int n_orient, int n_units,
-    int units_per_xcd, const int* __restrict__ sched, float* __restrict__ score,
-    float* __restrict__ shift) {
+    const float* __restrict__ Mall, int n_orient, int unit0, int n_units, int units_per_xcd,
+    const int* __restrict__ sched, int g, const float2* __restrict__ twg, float* __restrict__ maps,
+    double* __restrict__ dsums) {
   constexpr int D = 2 * RR + 1;
   __shared__ float2 tw[kMaxN][D];              // e^{2 pi i k d / n}
   __shared__ float2 T1[kBatchThreads][D];      // z-pruned column results
-  __shared__ float map[D * D * D];
   __shared__ float Ms[9];
-  __shared__ int kyslot[kBatchThreads];        // FFT-order ky of each column slot
+  __shared__ uint8_t kyslot[kBatchThreads];    // FFT-order ky of each column slot (< 128)
   __shared__ double red[kBatchThreads / kWave];
 
   // XCD-aware unit: workgroups are dealt round-robin over the 8 XCDs;
-  // consecutive units (the orientations of one particle) land on one XCD so
-  // they share the particle's spectrum in that XCD's L2.
-  const int unit = (blockIdx.x % 8) * units_per_xcd + blockIdx.x / 8;
-  if (unit >= n_units) return;  // uniform per block
+  // consecutive units (the orientations of one particle) land on one XCD.
+  // lu indexes this launch's units (maps, dsums), unit all units.
+  const int lu = (blockIdx.x % 8) * units_per_xcd + blockIdx.x / 8;
+  if (lu >= n_units) return;  // uniform per block
+  const int unit = unit0 + lu;
+  const int t = threadIdx.x;
   const int p = unit / n_orient;
   const int cx = n / 2 + 1;
   const int64_t nh = (int64_t)cx * n * n;
   const float2* Qp = Q + (int64_t)p * nh;
   const float* Np = Nw + (int64_t)p * nh;
-  const int t = threadIdx.x;
-
-  for (int i = t; i < n * D; i += kBatchThreads) {
-    const int k = i / D, d = i % D - RR;
-    double s, c;
-    sincospi(2.0 * (double)k * (double)d / (double)n, &s, &c);
-    tw[k][i % D] = make_float2((float)c, (float)s);
-  }
-  for (int i = t; i < D * D * D; i += kBatchThreads) map[i] = 0.0f;
+  for (int i = t; i < n * D; i += kBatchThreads) tw[i / D][i % D] = twg[i];
   if (t < 9) Ms[t] = Mall[9 * (int64_t)unit + t];
-  __syncthreads();
-  const float m00 = Ms[0], m01 = Ms[1], m02 = Ms[2], m10 = Ms[3], m11 = Ms[4], m12 = Ms[5],
-              m20 = Ms[6], m21 = Ms[7], m22 = Ms[8];
 
-  // Compact schedule (support_schedule): ngroups passes over at most
-  // kBatchThreads support columns each; per column (kx, ky, h) with h the
-  // largest |kz| inside the support sphere; a piece is a run of slots of one
-  // kx inside a group.
+  // Column schedule (support_schedule): per pass at most kBatchThreads
+  // support columns (kx, ky, h), h the largest |kz| inside the support
+  // sphere; a piece is a run of slots of one kx inside a pass.
   const int ngroups = sched[0], npieces = sched[1];
   const int4* groups = reinterpret_cast<const int4*>(sched + 4);
   const int4* pieces = groups + ngroups;
   const int* cols = reinterpret_cast<const int*>(pieces + npieces);
-  double dsum = 0.0;
-
-  for (int g = 0; g < ngroups; ++g) {
-    const int4 G = groups[g];  // (first column, column count, first piece, piece count)
-    const bool active = t < G.y;
-    const int packed = active ? cols[G.x + t] : (128 << 8);  // idle slot: k = (0, 0), h = -1
-    const int kx = packed & 255, ky = ((packed >> 8) & 255) - 128;
-    const int h = active ? ((packed >> 16) & 255) : -1;
-    const int yy = ky < 0 ? ky + n : ky;
-    kyslot[t] = yy;
-    int hw = h;  // kz bound uniform over the wavefront (twiddle reads stay broadcast)
-    for (int o = kWave / 2; o > 0; o >>= 1) hw = max(hw, __shfl_xor(hw, o, kWave));
-    float2 acc[D];
+  const int4 G = groups[g];  // (first column, column count, first piece, piece count)
+  const bool active = t < G.y;
+  const int packed = active ? cols[G.x + t] : (128 << 8);  // idle slot: k = (0, 0), h = -1
+  const int kx = packed & 255, ky = ((packed >> 8) & 255) - 128;
+  const int h = active ? ((packed >> 16) & 255) : -1;
+  const int yy = ky < 0 ? ky + n : ky;
+  kyslot[t] = (uint8_t)yy;
+  __syncthreads();
+  const float m00 = Ms[0], m01 = Ms[1], m02 = Ms[2], m10 = Ms[3], m11 = Ms[4], m12 = Ms[5],
+              m20 = Ms[6], m21 = Ms[7], m22 = Ms[8];
+  int hw = h;  // kz bound uniform over the wavefront (twiddle reads stay broadcast)
+  for (int o = kWave / 2; o > 0; o >>= 1) hw = max(hw, __shfl_xor(hw, o, kWave));
+  float2 acc[D];
 #pragma unroll
-    for (int d = 0; d < D; ++d) acc[d] = make_float2(0.0f, 0.0f);
-    float dpart = 0.0f;
-    const float fkx = (float)kx, fky = (float)ky;
-    const float bx = m00 * fkx + m01 * fky, by = m10 * fkx + m11 * fky, bz = m20 * fkx + m21 * fky;
-    // two kz per iteration: both samples' loads are issued in one block
-    // before either is consumed (accumulation order stays kz, kz + 1). A
-    // lane at its column end also fetches kz + 1 = h + 1: its Q index is
-    // valid (kz + 1 <= n/2) and the table read is clamped; it is not summed.
-    for (int kz = -hw; kz <= hw; kz += 2) {
-      const int kz1 = kz + 1;
-      const bool pa = kz >= -h && kz <= h, pb = kz1 >= -h && kz1 <= h;
-      if (pa || pb) {
-        const int za = kz < 0 ? kz + n : kz, zb = kz1 < 0 ? kz1 + n : kz1;
-        const int oa = (za * cx + kx) * n + yy, ob = (zb * cx + kx) * n + yy;
-        const float2 qa = Qp[oa], qb = Qp[ob];
-        const float wa = Np[oa], wb = Np[ob];
-        const float fa = (float)kz, fb = (float)kz1;
-        const Foot<Tab> ta = f2_fetch<Tab>(F2, n, bx + m02 * fa, by + m12 * fa, bz + m22 * fa);
-        const Foot<Tab> tb = f2_fetch<Tab>(F2, n, bx + m02 * fb, by + m12 * fb, bz + m22 * fb);
-        if (pa) accumulate_z<D>(acc, dpart, qa, wa, f2_interp<false, Tab>(ta).f, tw[za]);
-        if (pb) accumulate_z<D>(acc, dpart, qb, wb, f2_interp<false, Tab>(tb).f, tw[zb]);
-      }
+  for (int d = 0; d < D; ++d) acc[d] = make_float2(0.0f, 0.0f);
+  float dpart = 0.0f;
+  const float fkx = (float)kx, fky = (float)ky;
+  const float bx = m00 * fkx + m01 * fky, by = m10 * fkx + m11 * fky, bz = m20 * fkx + m21 * fky;
+  // two kz per iteration: both samples' loads are issued in one block
+  // before either is consumed (accumulation order stays kz, kz + 1). A lane
+  // at its column end also fetches kz + 1 = h + 1: its Q index is valid
+  // (kz + 1 <= n/2) and the table read is clamped; it is not summed.
+  for (int kz = -hw; kz <= hw; kz += 2) {
+    const int kz1 = kz + 1;
+    const bool pa = kz >= -h && kz <= h, pb = kz1 >= -h && kz1 <= h;
+    if (pa || pb) {
+      const int za = kz < 0 ? kz + n : kz, zb = kz1 < 0 ? kz1 + n : kz1;
+      const int oa = (za * cx + kx) * n + yy, ob = (zb * cx + kx) * n + yy;
+      const float2 qa = Qp[oa], qb = Qp[ob];
+      const float wa = Np[oa], wb = Np[ob];
+      const float fa = (float)kz, fb = (float)kz1;
+      const Foot<Tab> ta = f2_fetch<Tab>(F2, n, bx + m02 * fa, by + m12 * fa, bz + m22 * fa);
+      const Foot<Tab> tb = f2_fetch<Tab>(F2, n, bx + m02 * fb, by + m12 * fb, bz + m22 * fb);
+      if (pa) accumulate_z<D>(acc, dpart, qa, wa, f2_interp<false, Tab>(ta).f, tw[za]);
+      if (pb) accumulate_z<D>(acc, dpart, qb, wb, f2_interp<false, Tab>(tb).f, tw[zb]);
     }
-    if (active) dsum += ((kx == 0 || (n % 2 == 0 && kx == n / 2)) ? 1.0 : 2.0) * (double)dpart;
-#pragma unroll
-    for (int d = 0; d < D; ++d) T1[t][d] = acc[d];
-    __syncthreads();
-    // y-pruned sums over each plane's columns and the Hermitian x
-    // expansion; each thread owns (dy, dz)
-    for (int o = t; o < D * D; o += kBatchThreads) {
-      const int dy = o / D, dz = o % D;
-      float vals[D];
-#pragma unroll
-      for (int d = 0; d < D; ++d) vals[d] = 0.0f;
-      for (int pc = G.z; pc < G.z + G.w; ++pc) {
-        const int4 pz = pieces[pc];  // (kx, first slot, end slot, 0)
-        const int kxx = pz.x, s0 = pz.y, s1 = pz.z;
-        float sr = 0.0f, si = 0.0f;
-        for (int sl = s0; sl < s1; ++sl) {
-          const float2 a = T1[sl][dz];
-          const float2 e = tw[kyslot[sl]][dy];
-          sr = fmaf(a.x, e.x, fmaf(-a.y, e.y, sr));
-          si = fmaf(a.x, e.y, fmaf(a.y, e.x, si));
-        }
-        const float mk = (kxx == 0 || (n % 2 == 0 && kxx == n / 2)) ? 1.0f : 2.0f;
-#pragma unroll
-        for (int dx = 0; dx < D; ++dx) {
-          const float2 e = tw[kxx][dx];
-          vals[dx] = fmaf(mk * sr, e.x, fmaf(-mk * si, e.y, vals[dx]));
-        }
-      }
-#pragma unroll
-      for (int dx = 0; dx < D; ++dx) map[(dz * D + dy) * D + dx] += vals[dx];
-    }
-    __syncthreads();
   }
-
-  // normalisation sum (FP64)
+  const double dsum = active ? ((kx == 0 || (n % 2 == 0 && kx == n / 2)) ? 1.0 : 2.0) * (double)dpart : 0.0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) T1[t][d] = acc[d];
   {
-    double v = wave_sum(dsum);
+    const double v = wave_sum(dsum);
     if (t % kWave == 0) red[t / kWave] = v;
   }
-  // peak inside the sphere |d| <= RR, first index on ties (z, y, x order)
+  __syncthreads();
+  // y-pruned sums over each plane piece and the Hermitian x expansion; each
+  // thread owns (dy, dz) and its row of D map entries
+  float* mp = maps + (int64_t)lu * (D * D * D);
+  for (int o = t; o < D * D; o += kBatchThreads) {
+    const int dy = o / D, dz = o % D;
+    float vals[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) vals[d] = 0.0f;
+    for (int pc = G.z; pc < G.z + G.w; ++pc) {
+      const int4 pz = pieces[pc];  // (kx, first slot, end slot, 0)
+      const int kxx = pz.x, s0 = pz.y, s1 = pz.z;
+      float sr = 0.0f, si = 0.0f;
+      for (int sl = s0; sl < s1; ++sl) {
+        const float2 a = T1[sl][dz];
+        const float2 e = tw[kyslot[sl]][dy];
+        sr = fmaf(a.x, e.x, fmaf(-a.y, e.y, sr));
+        si = fmaf(a.x, e.y, fmaf(a.y, e.x, si));
+      }
+      const float mk = (kxx == 0 || (n % 2 == 0 && kxx == n / 2)) ? 1.0f : 2.0f;
+#pragma unroll
+      for (int dx = 0; dx < D; ++dx) {
+        const float2 e = tw[kxx][dx];
+        vals[dx] = fmaf(mk * sr, e.x, fmaf(-mk * si, e.y, vals[dx]));
+      }
+    }
+    float* row = mp + (dz * D + dy) * D;
+#pragma unroll
+    for (int dx = 0; dx < D; ++dx) row[dx] = (g == 0 ? 0.0f : row[dx]) + vals[dx];
+  }
+  if (t == 0) {
+    double v = 0.0;
+    for (int w = 0; w < kBatchThreads / kWave; ++w) v += red[w];
+    dsums[lu] = (g == 0 ? 0.0 : dsums[lu]) + v;
+  }
+}
+
+// Peak of each unit's shift-cube map inside the sphere |d| <= RR (first
+// index on ties, z, y, x order), parabolic sub-pixel refinement per axis,
+// normalised score.
+template <int RR>
+__global__ void __launch_bounds__(kBatchThreads) align_final_kernel(const float* __restrict__ maps,
+                                                                   const double* __restrict__ dsums,
+                                                                   const double* __restrict__ A, int n_orient,
+                                                                   int unit0, float* __restrict__ score,
+                                                                   float* __restrict__ shift) {
+  constexpr int D = 2 * RR + 1;
+  __shared__ float map[D * D * D];
+  const int lu = blockIdx.x, unit = unit0 + lu, t = threadIdx.x;
+  const float* mp = maps + (int64_t)lu * (D * D * D);
+  for (int i = t; i < D * D * D; i += kBatchThreads) map[i] = mp[i];
+  __syncthreads();
   unsigned long long key = 0ull;
   for (int i = t; i < D * D * D; i += kBatchThreads) {
     const int dx = i % D - RR, dy = (i / D) % D - RR, dz = i / (D * D) - RR;
@@ -511,12 +534,9 @@ __global__ void __launch_bounds__(kBatchThreads) align_fused_kernel(
   }
   key = block_max_u64(key);  // contains a barrier; valid in thread 0
   if (t == 0) {
-    double dt = 0.0;
-    for (int w = 0; w < kBatchThreads / kWave; ++w) dt += red[w];
     const int best = (int)key_index(key);
     const float bv = map[best];
     const int bx = best % D, by = (best / D) % D, bz = best / (D * D);
-    // parabolic sub-pixel refinement per axis
     auto para = [&](int i0, int stride, int coord) {
       if (coord <= 0 || coord >= D - 1) return 0.0f;
       const float a = map[i0 - stride], b = map[i0], c = map[i0 + stride];
@@ -528,12 +548,23 @@ __global__ void __launch_bounds__(kBatchThreads) align_fused_kernel(
     const float sx = (float)(bx - RR) + para(best, 1, bx);
     const float sy = (float)(by - RR) + para(best, D, by);
     const float sz = (float)(bz - RR) + para(best, D * D, bz);
-    const double norm = sqrt(A[p] * dt);
+    const double norm = sqrt(A[unit / n_orient] * dsums[lu]);
     score[unit] = norm > 0.0 ? (float)((double)bv / norm) : 0.0f;
     shift[3 * (int64_t)unit + 0] = sx;
     shift[3 * (int64_t)unit + 1] = sy;
     shift[3 * (int64_t)unit + 2] = sz;
   }
+}
+
+// e^{2 pi i k d / n} for k in [0, n), d in [-RR, RR] (FP64 sincospi).
+__global__ void twiddle_kernel(int n, int RR, float2* __restrict__ tw) {
+  const int D = 2 * RR + 1;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * D) return;
+  const int k = i / D, d = i % D - RR;
+  double s, c;
+  sincospi(2.0 * (double)k * (double)d / (double)n, &s, &c);
+  tw[i] = make_float2((float)c, (float)s);
 }
 
 // ------------------------------------------------------------ adjoint
@@ -679,23 +710,48 @@ const int2* support_list(emc_ctx* ctx, int n, double radius, int* count) {
 
 template <int RR, class Tab>
 void launch_fused(emc_ctx* ctx, const void* F2, int n, const float2* Q, const float* N,
-                  const double* A, const float* M, int n_orient, int n_units,
-                  const int* sched, float* score, float* shift) {
-  const int per = ceil_div(n_units, 8);
-  KTimer kt(ctx, "align_fused");
-  hipLaunchKernelGGL((align_fused_kernel<RR, Tab>), dim3(8 * per), dim3(kBatchThreads), 0, ctx->stream,
-                     (const typename Tab::T*)F2, n, Q, N, A, M, n_orient, n_units, per, sched, score, shift);
-  EMC_HIP(hipGetLastError());
+                  const double* A, const float* M, int n_orient, int n_units, const int* sched, int ngroups,
+                  float* score, float* shift) {
+  constexpr int D = 2 * RR + 1;
+  char key[32];
+  snprintf(key, sizeof(key), "bat_tw_%d_%d", n, RR);
+  auto it = ctx->scratch.find(key);
+  const bool fresh = it == ctx->scratch.end() || !it->second.ptr;
+  float2* tw = (float2*)scratch(ctx, key, sizeof(float2) * n * D);
+  if (fresh) {
+    hipLaunchKernelGGL(twiddle_kernel, dim3(ceil_div(n * D, 256)), dim3(256), 0, ctx->stream, n, RR, tw);
+    EMC_HIP(hipGetLastError());
+  }
+  // units in chunks so the per-unit maps stay bounded (<= 1.3 GB at r = 8)
+  constexpr int kChunkUnits = 65536;
+  const int cu = std::min(n_units, kChunkUnits);
+  float* maps = (float*)scratch(ctx, "bat_maps", sizeof(float) * D * D * D * (size_t)cu);
+  double* dsums = (double*)scratch(ctx, "bat_dsums", sizeof(double) * (size_t)cu);
+  for (int u0 = 0; u0 < n_units; u0 += kChunkUnits) {
+    const int nu = std::min(kChunkUnits, n_units - u0);
+    const int per = ceil_div(nu, 8);
+    for (int g = 0; g < ngroups; ++g) {
+      KTimer kt(ctx, "align_fused");
+      hipLaunchKernelGGL((align_pass_kernel<RR, Tab>), dim3(8 * per), dim3(kBatchThreads), 0, ctx->stream,
+                         (const typename Tab::T*)F2, n, Q, N, M, n_orient, u0, nu, per, sched, g,
+                         (const float2*)tw, maps, dsums);
+      EMC_HIP(hipGetLastError());
+    }
+    KTimer kt(ctx, "align_final");
+    hipLaunchKernelGGL(align_final_kernel<RR>, dim3(nu), dim3(kBatchThreads), 0, ctx->stream,
+                       (const float*)maps, (const double*)dsums, A, n_orient, u0, score, shift);
+    EMC_HIP(hipGetLastError());
+  }
 }
 
 template <class Tab>
 void launch_fused_r(emc_ctx* ctx, int peak_radius, const void* F2, int n, const float2* Q, const float* N,
-                    const double* A, const float* M, int n_orient, int n_units, const int* sched,
+                    const double* A, const float* M, int n_orient, int n_units, const int* sched, int ngroups,
                     float* score, float* shift) {
   switch (peak_radius) {
-    case 4: launch_fused<4, Tab>(ctx, F2, n, Q, N, A, M, n_orient, n_units, sched, score, shift); break;
-    case 6: launch_fused<6, Tab>(ctx, F2, n, Q, N, A, M, n_orient, n_units, sched, score, shift); break;
-    default: launch_fused<8, Tab>(ctx, F2, n, Q, N, A, M, n_orient, n_units, sched, score, shift); break;
+    case 4: launch_fused<4, Tab>(ctx, F2, n, Q, N, A, M, n_orient, n_units, sched, ngroups, score, shift); break;
+    case 6: launch_fused<6, Tab>(ctx, F2, n, Q, N, A, M, n_orient, n_units, sched, ngroups, score, shift); break;
+    default: launch_fused<8, Tab>(ctx, F2, n, Q, N, A, M, n_orient, n_units, sched, ngroups, score, shift); break;
   }
 }
 
@@ -708,11 +764,15 @@ void launch_fused_r(emc_ctx* ctx, int peak_radius, const void* F2, int n, const 
 // (kx, ky) patches, consecutive patches merged while they fit one
 // workgroup: the units running together on an XCD (nearby orientations)
 // then read a compact region of the reference table per pass.
-const int* support_schedule(emc_ctx* ctx, int n, float kmax2) {
+const int* support_schedule(emc_ctx* ctx, int n, float kmax2, int* ngroups) {
   char key[64];
   snprintf(key, sizeof(key), "sched_%d_%.6f", n, (double)kmax2);
   auto it = ctx->scratch.find(key);
-  if (it != ctx->scratch.end() && it->second.ptr) return (const int*)it->second.ptr;
+  auto mt = ctx->meta.find(key);
+  if (it != ctx->scratch.end() && it->second.ptr && mt != ctx->meta.end()) {
+    *ngroups = mt->second;
+    return (const int*)it->second.ptr;
+  }
   constexpr int kTile = 16;
   std::map<std::pair<int, int>, std::vector<int>> tiles;  // ordered by (kx tile, ky tile)
   for (int kx = 0; kx <= n / 2; ++kx) {
@@ -758,6 +818,8 @@ const int* support_schedule(emc_ctx* ctx, int n, float kmax2) {
   int* d = (int*)scratch(ctx, key, sizeof(int) * buf.size());
   EMC_HIP(hipMemcpyAsync(d, buf.data(), sizeof(int) * buf.size(), hipMemcpyHostToDevice, ctx->stream));
   EMC_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->meta[key] = ng;
+  *ngroups = ng;
   return d;
 }
 
@@ -782,7 +844,7 @@ emc_status emc_batch_reference_spectrum(emc_ctx* ctx, int n, const float* d_ref,
                        ctx->stream, d_ref, pad, n);
     EMC_HIP(hipGetLastError());
     fft_r2c(cached_plan(ctx, m, m, m, 1), pad, raw);
-    const dim3 grid(ceil_div(n + 2, 64), ceil_div(m + 2, 4), m + 2), block(64, 4, 1);
+    const dim3 grid(ceil_div(n + 2, 64), ceil_div(m + 1, 4), m + 2), block(64, 4, 1);
     if (table_format == EMC_TABLE_C64) {
       hipLaunchKernelGGL(f2_centre_kernel<TabC64>, grid, block, 0, ctx->stream, (const float2*)raw, n,
                          (const float*)nullptr, (float2*)d_F2);
@@ -873,12 +935,13 @@ emc_status emc_align_batched(emc_ctx* ctx, int n, int n_part, int table_format, 
     EMC_HIP(hipGetLastError());
     const float kmax2 = (float)(support_radius * support_radius);
     const float2* Q = (const float2*)d_Q;
-    const int* sched = support_schedule(ctx, n, kmax2);
+    int ngroups = 0;
+    const int* sched = support_schedule(ctx, n, kmax2, &ngroups);
     if (table_format == EMC_TABLE_C64)
-      launch_fused_r<TabC64>(ctx, peak_radius, d_F2, n, Q, d_N, d_A, d_M, n_orient, units, sched, d_score,
+      launch_fused_r<TabC64>(ctx, peak_radius, d_F2, n, Q, d_N, d_A, d_M, n_orient, units, sched, ngroups, d_score,
                              d_shift);
     else
-      launch_fused_r<TabF16>(ctx, peak_radius, d_F2, n, Q, d_N, d_A, d_M, n_orient, units, sched, d_score,
+      launch_fused_r<TabF16>(ctx, peak_radius, d_F2, n, Q, d_N, d_A, d_M, n_orient, units, sched, ngroups, d_score,
                              d_shift);
   });
 }
@@ -899,8 +962,8 @@ emc_status emc_align_eval_grad(emc_ctx* ctx, int n, int n_part, int table_format
     const int2* supp = support_list(ctx, n, support_radius, &ns);
     EMC_REQUIRE(ns > 0, "empty support");
     // split every particle's support over enough workgroups to fill the
-    // device (>= ~4 per CU over the whole batch), at least 512 frequencies each
-    int g = std::max(1, std::min((ns + 511) / 512, (1024 + n_part - 1) / n_part));
+    // device (~16 per CU over the whole batch), at least 256 frequencies each
+    int g = std::max(1, std::min((ns + 255) / 256, (4096 + n_part - 1) / n_part));
     double* part = (double*)scratch(ctx, "grad_part", sizeof(double) * kGradTerms * (size_t)g * n_part);
     {
       KTimer kt(ctx, "align_eval_grad");

@@ -80,6 +80,8 @@ struct emc_ctx {
   hipStream_t stream = nullptr;
   // named scratch slots (grown on demand, freed with the context)
   std::map<std::string, emc::Scratch> scratch;
+  // small host-side facts about cached scratch contents (e.g. pass counts)
+  std::map<std::string, int> meta;
   // internal FFT plans keyed by (nx, ny, nz, batch)
   std::map<std::tuple<int, int, int, int>, emc_fft_plan*> plans;
   // caching allocator behind emc_malloc / emc_free: freed blocks are kept
