@@ -170,45 +170,18 @@ class BatchAligner:
         return self.eval_grad(R, dR, s)
 
     def refine(self, R0, s0, iters=12, step_angle=0.02, step_shift=0.3):
-        """Batched BFGS ascent of the CC over (rotation increment, shift) for
-        every particle, one adjoint launch per iteration. Returns (R, s, cc)."""
+        """Batched quasi-Newton ascent of the CC over (rotation increment,
+        shift) for every particle, on the device (emc_align_refine: iters + 1
+        adjoint evaluations, one result copy). Returns (R, s, cc)."""
         P = self.n_part
-        R = np.array(R0, dtype=np.float64).reshape(P, 3, 3)
-        s = np.array(s0, dtype=np.float64).reshape(P, 3)
-        f, g = self._eval_local(R, s)
-        # initial inverse-Hessian scale: first step ~ (step_angle rad, step_shift px)
-        gw = np.linalg.norm(g[:, :3], axis=1) + 1e-12
-        gs = np.linalg.norm(g[:, 3:], axis=1) + 1e-12
-        H = np.zeros((P, 6, 6))
-        H[:, [0, 1, 2], [0, 1, 2]] = (step_angle / gw)[:, None]
-        H[:, [3, 4, 5], [3, 4, 5]] = (step_shift / gs)[:, None]
-        H0 = H.copy()
-        I6 = np.eye(6)
-        t = np.ones(P)
-        for _ in range(iters):
-            d = np.einsum("pij,pj->pi", H, g)
-            bad = np.einsum("pi,pi->p", d, g) <= 0
-            if np.any(bad):
-                H[bad] = H0[bad]
-                d[bad] = np.einsum("pij,pj->pi", H0[bad], g[bad])
-            step = t[:, None] * d
-            Rt = R @ rodrigues(step[:, :3])
-            st = s + step[:, 3:]
-            ft, gt = self._eval_local(Rt, st)
-            ok = ft > f + 1e-4 * t * np.einsum("pi,pi->p", d, g)
-            # BFGS update of the inverse Hessian of -CC where the step was taken
-            y = g - gt                       # gradient of -CC: (-gt) - (-g)
-            sy = np.einsum("pi,pi->p", step, y)
-            upd = ok & (sy > 1e-18)
-            if np.any(upd):
-                sv, yv = step[upd], y[upd]
-                rho = (1.0 / sy[upd])[:, None, None]
-                V = I6 - rho * sv[:, :, None] * yv[:, None, :]
-                H[upd] = V @ H[upd] @ np.transpose(V, (0, 2, 1)) + rho * sv[:, :, None] * sv[:, None, :]
-            R[ok], s[ok], f[ok], g[ok] = Rt[ok], st[ok], ft[ok], gt[ok]
-            t[ok] = np.minimum(1.0, t[ok] * 2.0)
-            t[~ok] *= 0.3
-        return R, s, f
+        R = DeviceArray.from_numpy(np.ascontiguousarray(np.asarray(R0, dtype=np.float64).reshape(P, 9)),
+                                   ctx=self.ctx)
+        s = DeviceArray.from_numpy(np.ascontiguousarray(np.asarray(s0, dtype=np.float64).reshape(P, 3)),
+                                   ctx=self.ctx)
+        cc = DeviceArray((P,), np.float64, self.ctx)
+        _lib.call("emc_align_refine", self.ctx.handle, self.n, P, self._fmt, self.F2.vp, self.Q.vp, self.N.vp,
+                  self.A.vp, R.vp, s.vp, int(iters), float(step_angle), float(step_shift), self.support, cc.vp)
+        return R.get().reshape(P, 3, 3), s.get(), cc.get()
 
 
 def support_voxels(n, support):

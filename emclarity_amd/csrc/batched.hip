@@ -823,6 +823,223 @@ const int* support_schedule(emc_ctx* ctx, int n, float kmax2, int* ngroups) {
   return d;
 }
 
+// Score + gradient launch shared by emc_align_eval_grad and the refinement.
+void eval_grad_impl(emc_ctx* ctx, int n, int n_part, int table_format, const void* d_F2, const void* d_Q,
+                    const float* d_N, const double* d_A, const float* d_MdM, const float* d_shift,
+                    double support_radius, double* d_out) {
+  int ns = 0;
+  const int2* supp = support_list(ctx, n, support_radius, &ns);
+  EMC_REQUIRE(ns > 0, "empty support");
+  // split every particle's support over enough workgroups to fill the
+  // device (~16 per CU over the whole batch), at least 256 frequencies each
+  int g = std::max(1, std::min((ns + 255) / 256, (4096 + n_part - 1) / n_part));
+  double* part = (double*)scratch(ctx, "grad_part", sizeof(double) * kGradTerms * (size_t)g * n_part);
+  {
+    KTimer kt(ctx, "align_eval_grad");
+    if (table_format == EMC_TABLE_C64)
+      hipLaunchKernelGGL(align_eval_grad_kernel<TabC64>, dim3(g, n_part), dim3(kBatchThreads), 0, ctx->stream,
+                         (const float2*)d_F2, n, (const float2*)d_Q, d_N, d_MdM, d_shift, supp, ns, part);
+    else
+      hipLaunchKernelGGL(align_eval_grad_kernel<TabF16>, dim3(g, n_part), dim3(kBatchThreads), 0, ctx->stream,
+                         (const half2_t*)d_F2, n, (const float2*)d_Q, d_N, d_MdM, d_shift, supp, ns, part);
+  }
+  hipLaunchKernelGGL(adjoint_fold_kernel, dim3(n_part), dim3(kRedThreads), 0, ctx->stream, (const double*)part,
+                     g, d_A, d_out);
+  EMC_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------ refinement
+//
+// Batched quasi-Newton ascent of the CC over (rotation increment w, shift s)
+// per particle, all on the device (FP64, one thread per particle): the
+// candidate pose is R exp([w]x), s + ds with (w, ds) = t H g; the step is
+// taken when the CC rises by at least 1e-4 t (H g).g (Armijo), then H gets
+// the BFGS update of the inverse Hessian of -CC and t doubles (capped at 1),
+// else t shrinks by 0.3; a non-ascent direction resets H to its start
+// diag(step_angle/|g_w|, step_shift/|g_s|). Derivatives are w.r.t.
+// right-multiplied rotations, dR/dw_a = R G_a (no gimbal lock).
+
+struct RefineState {
+  double *R, *s, *f, *g, *H, *H0, *t, *Rt, *st, *step, *dir;  // per particle: 9,3,1,6,36,36,1,9,3,6,6
+  float *MdM, *sf;                                             // eval inputs: 36 + 3 floats
+  double* out;                                                 // eval outputs: 7
+  static size_t bytes(int P) { return (size_t)P * (sizeof(double) * (9 + 3 + 1 + 6 + 36 + 36 + 1 + 9 + 3 + 6 + 6 + 7) +
+                                                   sizeof(float) * (36 + 3)) + 256; }
+  void bind(char* b, int P) {
+    double* d = (double*)b;
+    R = d; d += 9 * P;
+    s = d; d += 3 * P;
+    f = d; d += P;
+    g = d; d += 6 * P;
+    H = d; d += 36 * P;
+    H0 = d; d += 36 * P;
+    t = d; d += P;
+    Rt = d; d += 9 * P;
+    st = d; d += 3 * P;
+    step = d; d += 6 * P;
+    dir = d; d += 6 * P;
+    out = d; d += 7 * P;
+    float* q = (float*)d;
+    MdM = q; q += 36 * P;
+    sf = q;
+  }
+};
+
+// exp([w]x) (Rodrigues), row-major
+__device__ inline void rodrigues_d(const double w[3], double E[9]) {
+  const double th = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  double k[3] = {0.0, 0.0, 0.0};
+  if (th > 1e-15)
+    for (int i = 0; i < 3; ++i) k[i] = w[i] / th;
+  const double K[9] = {0.0, -k[2], k[1], k[2], 0.0, -k[0], -k[1], k[0], 0.0};
+  const double sn = sin(th), c = 1.0 - cos(th);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double kk = 0.0;
+      for (int l = 0; l < 3; ++l) kk += K[3 * i + l] * K[3 * l + j];
+      E[3 * i + j] = (i == j ? 1.0 : 0.0) + sn * K[3 * i + j] + c * kk;
+    }
+}
+
+// Eval inputs of pose (R, s): M = 2 R^T and dM_a = 2 (R G_a)^T, G_a the
+// generators of rotations about x, y, z; FP32 as the adjoint kernel reads.
+__device__ inline void refine_eval_inputs(const double* R, const double* s, float* MdM, float* sf) {
+  // R G_x = [0, R02, -R01; ...], R G_y = [-R02, 0, R00; ...], R G_z = [R01, -R00, 0; ...]
+  double D[4][9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) D[0][3 * i + j] = R[3 * i + j];
+  for (int i = 0; i < 3; ++i) {
+    const double r0 = R[3 * i], r1 = R[3 * i + 1], r2 = R[3 * i + 2];
+    D[1][3 * i] = 0.0, D[1][3 * i + 1] = r2, D[1][3 * i + 2] = -r1;
+    D[2][3 * i] = -r2, D[2][3 * i + 1] = 0.0, D[2][3 * i + 2] = r0;
+    D[3][3 * i] = r1, D[3][3 * i + 1] = -r0, D[3][3 * i + 2] = 0.0;
+  }
+  for (int m = 0; m < 4; ++m)
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) MdM[9 * m + 3 * i + j] = (float)(2.0 * D[m][3 * j + i]);
+  for (int i = 0; i < 3; ++i) sf[i] = (float)s[i];
+}
+
+// Next candidate: dir = H g (H reset to H0 if that is not an ascent
+// direction), step = t dir, Rt = R exp([step_w]x), st = s + step_s.
+__device__ inline void refine_candidate(RefineState& S, int p) {
+  double* H = S.H + 36 * p;
+  const double* g = S.g + 6 * p;
+  double d[6], dg = 0.0;
+  for (int i = 0; i < 6; ++i) {
+    d[i] = 0.0;
+    for (int j = 0; j < 6; ++j) d[i] += H[6 * i + j] * g[j];
+    dg += d[i] * g[i];
+  }
+  if (dg <= 0.0) {
+    const double* H0 = S.H0 + 36 * p;
+    for (int i = 0; i < 36; ++i) H[i] = H0[i];
+    for (int i = 0; i < 6; ++i) {
+      d[i] = 0.0;
+      for (int j = 0; j < 6; ++j) d[i] += H0[6 * i + j] * g[j];
+    }
+  }
+  const double t = S.t[p];
+  double w[3], E[9];
+  for (int i = 0; i < 6; ++i) {
+    S.dir[6 * p + i] = d[i];
+    S.step[6 * p + i] = t * d[i];
+  }
+  for (int i = 0; i < 3; ++i) w[i] = t * d[i];
+  rodrigues_d(w, E);
+  const double* R = S.R + 9 * p;
+  double* Rt = S.Rt + 9 * p;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      Rt[3 * i + j] = R[3 * i] * E[j] + R[3 * i + 1] * E[3 + j] + R[3 * i + 2] * E[6 + j];
+  for (int i = 0; i < 3; ++i) S.st[3 * p + i] = S.s[3 * p + i] + t * d[3 + i];
+  refine_eval_inputs(Rt, S.st + 3 * p, S.MdM + 36 * p, S.sf + 3 * p);
+}
+
+__global__ void refine_begin_kernel(RefineState S, const double* __restrict__ R0, const double* __restrict__ s0,
+                                    int P) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  for (int i = 0; i < 9; ++i) S.R[9 * p + i] = R0[9 * p + i];
+  for (int i = 0; i < 3; ++i) S.s[3 * p + i] = s0[3 * p + i];
+  refine_eval_inputs(S.R + 9 * p, S.s + 3 * p, S.MdM + 36 * p, S.sf + 3 * p);
+}
+
+__global__ void refine_first_kernel(RefineState S, int P, double step_angle, double step_shift) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const double* o = S.out + 7 * p;
+  S.f[p] = o[0];
+  for (int i = 0; i < 6; ++i) S.g[6 * p + i] = o[1 + i];
+  const double* g = S.g + 6 * p;
+  const double gw = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]) + 1e-12;
+  const double gs = sqrt(g[3] * g[3] + g[4] * g[4] + g[5] * g[5]) + 1e-12;
+  for (int i = 0; i < 36; ++i) S.H[36 * p + i] = 0.0;
+  for (int i = 0; i < 6; ++i) S.H[36 * p + 7 * i] = i < 3 ? step_angle / gw : step_shift / gs;
+  for (int i = 0; i < 36; ++i) S.H0[36 * p + i] = S.H[36 * p + i];
+  S.t[p] = 1.0;
+  refine_candidate(S, p);
+}
+
+__global__ void refine_step_kernel(RefineState S, int P) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const double* o = S.out + 7 * p;
+  const double ft = o[0];
+  const double* gt = o + 1;
+  double* g = S.g + 6 * p;
+  const double* step = S.step + 6 * p;
+  const double* dir = S.dir + 6 * p;
+  const double t = S.t[p];
+  double dg = 0.0;
+  for (int i = 0; i < 6; ++i) dg += dir[i] * g[i];
+  const bool ok = ft > S.f[p] + 1e-4 * t * dg;
+  if (ok) {
+    // BFGS update of the inverse Hessian of -CC: y = (-gt) - (-g)
+    double y[6], sy = 0.0;
+    for (int i = 0; i < 6; ++i) {
+      y[i] = g[i] - gt[i];
+      sy += step[i] * y[i];
+    }
+    if (sy > 1e-18) {
+      const double rho = 1.0 / sy;
+      double* H = S.H + 36 * p;
+      double V[36], VH[36];
+      for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) V[6 * i + j] = (i == j ? 1.0 : 0.0) - rho * step[i] * y[j];
+      for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) {
+          double a = 0.0;
+          for (int l = 0; l < 6; ++l) a += V[6 * i + l] * H[6 * l + j];
+          VH[6 * i + j] = a;
+        }
+      for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) {
+          double a = 0.0;
+          for (int l = 0; l < 6; ++l) a += VH[6 * i + l] * V[6 * j + l];
+          H[6 * i + j] = a + rho * step[i] * step[j];
+        }
+    }
+    for (int i = 0; i < 9; ++i) S.R[9 * p + i] = S.Rt[9 * p + i];
+    for (int i = 0; i < 3; ++i) S.s[3 * p + i] = S.st[3 * p + i];
+    S.f[p] = ft;
+    for (int i = 0; i < 6; ++i) g[i] = gt[i];
+    S.t[p] = fmin(1.0, 2.0 * t);
+  } else {
+    S.t[p] = 0.3 * t;
+  }
+  refine_candidate(S, p);
+}
+
+__global__ void refine_end_kernel(RefineState S, int P, double* __restrict__ R, double* __restrict__ s,
+                                  double* __restrict__ cc) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  for (int i = 0; i < 9; ++i) R[9 * p + i] = S.R[9 * p + i];
+  for (int i = 0; i < 3; ++i) s[3 * p + i] = S.s[3 * p + i];
+  cc[p] = S.f[p];
+}
+
 }  // namespace emc
 
 using namespace emc;
@@ -958,26 +1175,36 @@ emc_status emc_align_eval_grad(emc_ctx* ctx, int n, int n_part, int table_format
     EMC_REQUIRE(table_format == EMC_TABLE_C64 || table_format == EMC_TABLE_F16, "unknown table format %d",
                 table_format);
     EMC_REQUIRE(support_radius > 0 && support_radius < 0.5 * n, "support radius must be in (0, n/2)");
-    int ns = 0;
-    const int2* supp = support_list(ctx, n, support_radius, &ns);
-    EMC_REQUIRE(ns > 0, "empty support");
-    // split every particle's support over enough workgroups to fill the
-    // device (~16 per CU over the whole batch), at least 256 frequencies each
-    int g = std::max(1, std::min((ns + 255) / 256, (4096 + n_part - 1) / n_part));
-    double* part = (double*)scratch(ctx, "grad_part", sizeof(double) * kGradTerms * (size_t)g * n_part);
-    {
-      KTimer kt(ctx, "align_eval_grad");
-      if (table_format == EMC_TABLE_C64)
-        hipLaunchKernelGGL(align_eval_grad_kernel<TabC64>, dim3(g, n_part), dim3(kBatchThreads), 0,
-                           ctx->stream, (const float2*)d_F2, n, (const float2*)d_Q, d_N, d_MdM, d_shift, supp,
-                           ns, part);
-      else
-        hipLaunchKernelGGL(align_eval_grad_kernel<TabF16>, dim3(g, n_part), dim3(kBatchThreads), 0,
-                           ctx->stream, (const half2_t*)d_F2, n, (const float2*)d_Q, d_N, d_MdM, d_shift,
-                           supp, ns, part);
+    eval_grad_impl(ctx, n, n_part, table_format, d_F2, d_Q, d_N, d_A, d_MdM, d_shift, support_radius, d_out);
+  });
+}
+
+emc_status emc_align_refine(emc_ctx* ctx, int n, int n_part, int table_format, const void* d_F2, const void* d_Q,
+                            const float* d_N, const double* d_A, double* d_R, double* d_s, int iters,
+                            double step_angle, double step_shift, double support_radius, double* d_cc) {
+  return guard([&] {
+    use_device(ctx);
+    EMC_REQUIRE(n >= 8 && n <= kMaxN, "box size must be 8..%d", kMaxN);
+    EMC_REQUIRE(n_part > 0, "empty batch");
+    EMC_REQUIRE(iters >= 0, "iters must be >= 0");
+    EMC_REQUIRE(step_angle > 0 && step_shift > 0, "initial steps must be positive");
+    EMC_REQUIRE(d_F2 && d_Q && d_N && d_A && d_R && d_s && d_cc, "null pointer");
+    EMC_REQUIRE(table_format == EMC_TABLE_C64 || table_format == EMC_TABLE_F16, "unknown table format %d",
+                table_format);
+    EMC_REQUIRE(support_radius > 0 && support_radius < 0.5 * n, "support radius must be in (0, n/2)");
+    RefineState st;
+    char* base = (char*)scratch(ctx, "refine_state", RefineState::bytes(n_part));
+    st.bind(base, n_part);
+    const dim3 grid(ceil_div(n_part, 64)), block(64);
+    hipLaunchKernelGGL(refine_begin_kernel, grid, block, 0, ctx->stream, st, (const double*)d_R,
+                       (const double*)d_s, n_part);
+    eval_grad_impl(ctx, n, n_part, table_format, d_F2, d_Q, d_N, d_A, st.MdM, st.sf, support_radius, st.out);
+    hipLaunchKernelGGL(refine_first_kernel, grid, block, 0, ctx->stream, st, n_part, step_angle, step_shift);
+    for (int it = 0; it < iters; ++it) {
+      eval_grad_impl(ctx, n, n_part, table_format, d_F2, d_Q, d_N, d_A, st.MdM, st.sf, support_radius, st.out);
+      hipLaunchKernelGGL(refine_step_kernel, grid, block, 0, ctx->stream, st, n_part);
     }
-    hipLaunchKernelGGL(adjoint_fold_kernel, dim3(n_part), dim3(kRedThreads), 0, ctx->stream,
-                       (const double*)part, g, d_A, d_out);
+    hipLaunchKernelGGL(refine_end_kernel, grid, block, 0, ctx->stream, st, n_part, d_R, d_s, d_cc);
     EMC_HIP(hipGetLastError());
   });
 }

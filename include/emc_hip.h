@@ -329,6 +329,21 @@ emc_status emc_align_eval_grad(emc_ctx* ctx, int n, int n_part, int table_format
                                const float* d_N, const double* d_A, const float* d_MdM,
                                const float* d_shift, double support_radius, double* d_out);
 
+/* Gradient refinement of one pose per particle, replacing the dense local
+ * refinement grid of alignment/BH_alignRaw3d_v2.m:1448-1644: iters steps of
+ * batched quasi-Newton ascent of the CC over a right-multiplied rotation
+ * increment and the shift, all on the device (iters + 1 adjoint
+ * evaluations, no host round trip). Step: (w, ds) = t H g, accepted when the
+ * CC rises by >= 1e-4 t (H g).g, then the BFGS update of H and t = min(1, 2t),
+ * else t *= 0.3; H starts as diag(step_angle/|g_w| x3, step_shift/|g_s| x3)
+ * and is reset to it when H g is not an ascent direction. d_R: 9 doubles per
+ * particle (row-major R, R maps particle to reference frame as in
+ * emc_align_batched), d_s: 3 doubles (shift, px); both updated in place;
+ * d_cc: the final CC per particle. Stream-ordered. */
+emc_status emc_align_refine(emc_ctx* ctx, int n, int n_part, int table_format, const void* d_F2, const void* d_Q,
+                            const float* d_N, const double* d_A, double* d_R, double* d_s, int iters,
+                            double step_angle, double step_shift, double support_radius, double* d_cc);
+
 /* ------------------------------------------------- template search (A16) */
 
 /* One tomogram chunk of alignment/BH_templateSearch3d_2.m:713-887 for the
