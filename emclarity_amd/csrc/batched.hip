@@ -588,7 +588,10 @@ __global__ void __launch_bounds__(kBatchThreads) align_eval_grad_kernel(
     const typename Tab::T* __restrict__ F2, int n, const float2* __restrict__ Q, const float* __restrict__ Nw,
     const float* __restrict__ MdM, const float* __restrict__ shifts, const int2* __restrict__ supp,
     int n_supp, double* __restrict__ part) {
-  const int p = blockIdx.y;
+  // blockIdx.x = particle, blockIdx.y = chunk of the support list (FFT
+  // order of the [kz][kx][ky] layout: lanes of a wave sample neighbouring
+  // frequencies)
+  const int p = blockIdx.x, chunk = blockIdx.y, nchunk = gridDim.y;
   const int cx = n / 2 + 1;
   const int64_t nh = (int64_t)cx * n * n;
   const float2* Qp = Q + (int64_t)p * nh;
@@ -601,8 +604,8 @@ __global__ void __launch_bounds__(kBatchThreads) align_eval_grad_kernel(
   float v[kGradTerms];
 #pragma unroll
   for (int i = 0; i < kGradTerms; ++i) v[i] = 0.0f;
-  const int per = (n_supp + gridDim.x - 1) / gridDim.x;
-  const int i0 = blockIdx.x * per, i1 = min(n_supp, i0 + per);
+  const int per = (n_supp + nchunk - 1) / nchunk;
+  const int i0 = chunk * per, i1 = min(n_supp, i0 + per);
   for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
     const int2 e = supp[i];
     const float fk[3] = {(float)((e.y & 255) - 128), (float)(((e.y >> 8) & 255) - 128),
@@ -645,7 +648,7 @@ __global__ void __launch_bounds__(kBatchThreads) align_eval_grad_kernel(
   for (int i = 0; i < kGradTerms; ++i) d[i] = (double)v[i];
   block_sum<kGradTerms>(d);
   if (threadIdx.x == 0) {
-    double* o = part + ((int64_t)p * gridDim.x + blockIdx.x) * kGradTerms;
+    double* o = part + ((int64_t)p * nchunk + chunk) * kGradTerms;
 #pragma unroll
     for (int i = 0; i < kGradTerms; ++i) o[i] = d[i];
   }
@@ -837,10 +840,10 @@ void eval_grad_impl(emc_ctx* ctx, int n, int n_part, int table_format, const voi
   {
     KTimer kt(ctx, "align_eval_grad");
     if (table_format == EMC_TABLE_C64)
-      hipLaunchKernelGGL(align_eval_grad_kernel<TabC64>, dim3(g, n_part), dim3(kBatchThreads), 0, ctx->stream,
+      hipLaunchKernelGGL(align_eval_grad_kernel<TabC64>, dim3(n_part, g), dim3(kBatchThreads), 0, ctx->stream,
                          (const float2*)d_F2, n, (const float2*)d_Q, d_N, d_MdM, d_shift, supp, ns, part);
     else
-      hipLaunchKernelGGL(align_eval_grad_kernel<TabF16>, dim3(g, n_part), dim3(kBatchThreads), 0, ctx->stream,
+      hipLaunchKernelGGL(align_eval_grad_kernel<TabF16>, dim3(n_part, g), dim3(kBatchThreads), 0, ctx->stream,
                          (const half2_t*)d_F2, n, (const float2*)d_Q, d_N, d_MdM, d_shift, supp, ns, part);
   }
   hipLaunchKernelGGL(adjoint_fold_kernel, dim3(n_part), dim3(kRedThreads), 0, ctx->stream, (const double*)part,
