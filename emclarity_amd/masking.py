@@ -110,14 +110,23 @@ def mexCTF(doHalfGrid, doSqCTF, nX, nY, pixelSize, waveLength, CS, defocus1, def
 
 
 def BH_mask3d(SHAPE, SIZE, RADIUS, CENTER, ctx=None) -> DeviceArray:
-    """masking/BH_mask3d.m for SHAPE 'rectangle' at CENTER [0,0,0] (the
-    valid-area mask of template search, BH_templateSearch3d_2.m:402): a
+    """masking/BH_mask3d.m at CENTER [0,0,0] for the shapes template search
+    uses. 'rectangle' (the valid-area mask, BH_templateSearch3d_2.m:402): a
     window of ones 2*RADIUS+13 per axis (:102-108), padded to SIZE with the
     7-voxel cosine taper and the half-voxel origin shift of even sizes
-    (:244-280), then tapered at the volume edge (:462-466). Other shapes are
-    not part of this path."""
-    if str(SHAPE).lower() != "rectangle" or any(float(c) != 0.0 for c in CENTER):
-        raise NotImplementedError("only BH_mask3d('rectangle', SIZE, RADIUS, [0,0,0]) is provided")
+    (:244-280), then tapered at the volume edge (:462-466). 'sphere' (the
+    peak erase mask, :1102): emc_mask3d_sphere. Other shapes are not part of
+    this path."""
+    shape = str(SHAPE).lower()
+    if shape not in ("rectangle", "sphere") or any(float(c) != 0.0 for c in CENTER):
+        raise NotImplementedError("BH_mask3d: only 'rectangle' and 'sphere' at [0,0,0] are provided")
+    if shape == "sphere":
+        from .device import default_context
+        c = ctx or default_context()
+        size = [int(v) for v in SIZE]
+        out = DeviceArray((size[2], size[1], size[0]), np.float32, c)
+        _lib.call("emc_mask3d_sphere", c.handle, _lib.i3(size), _lib.d_arr(np.asarray(RADIUS, np.float64)), out.vp)
+        return out
     from .sizing import BH_multi_padVal
     size = np.asarray(SIZE, dtype=np.int64)
     win = (2 * np.asarray(RADIUS, dtype=np.float64) + 13).astype(np.int64)

@@ -136,3 +136,98 @@ def template_search(tomogram, template, angles_pts, size_chunk, valid_area, n_it
     res_peak = res_peak[pre[2]:pre[2] + nz, pre[1]:pre[1] + ny, pre[0]:pre[0] + nx]
     res_ang = res_ang[pre[2]:pre[2] + nz, pre[1]:pre[1] + ny, pre[0]:pre[0] + nx]
     return res_peak, res_ang
+
+
+# ------------------------------------------------------------ peak extraction
+
+def high_threshold(peak_threshold, tomo_size, n_angles_first):
+    """highThr of BH_templateSearch3d_2.m:337: sqrt(2) erfcinv(ceil(0.1
+    peakThreshold) 2 / (numel(tomogram) nAngles(1)))."""
+    from scipy.special import erfcinv
+    return float(np.sqrt(2.0) * erfcinv(np.ceil(peak_threshold * 0.10) * 2.0
+                                        / (float(np.prod(tomo_size)) * float(n_angles_first))))
+
+
+def extract_peaks(mag, ang, angle_list, lattice_radius, erase_radius, peak_threshold, high_thr,
+                  sampling_rate=1, erase_type="sphere", max_tries=10000, convention="Bah", n_peaks=1, ctx=None):
+    """Greedy peak extraction of BH_templateSearch3d_2.m:1036-1275 on the
+    cropped score map ``mag`` and angle-id map ``ang`` ((z, y, x), ids 1-based
+    into ``angle_list`` rows (phi, theta, psi-phi), 0 = none).
+    lattice_radius (szK) and erase_radius are (x, y, z) in voxels. The map is
+    cropped by szK and padded by szK + rmDim, the ids by rmDim, rmDim =
+    max(erase, szK) + 7 (:1036-1080; the reference pads on the host, so do
+    these copies); the erase mask is BH_mask3d(erase_type, 2 rmDim + 1,
+    erase_radius) with its cut-off lowered by 1 % steps until it keeps the
+    sphere's volume (:1102-1122); then emc_tsearch_peaks. Returns (n, 7):
+    x, y, z (times sampling_rate), phi, theta, psi - phi, score. Only
+    nPeaks = 1 (the reference's extra peaks draw random numbers)."""
+    from .geometry import _CONV
+    from .masking import BH_mask3d, BH_padZeros3d
+    if n_peaks != 1:
+        raise NotImplementedError("nPeaks > 1 draws random numbers in the reference; only nPeaks = 1")
+    szK = np.asarray(lattice_radius, dtype=np.int64).reshape(3)
+    er = np.asarray(erase_radius, dtype=np.float64).reshape(3)
+    rm = int(max(er.max(), szK.max())) + 7
+    m = BH_padZeros3d(mag, -szK, -szK, ctx=ctx)
+    m = BH_padZeros3d(m, szK + rm, szK + rm, ctx=ctx)
+    ctx = m.ctx
+    a = np.pad(np.asarray(ang, dtype=np.int32), rm)
+    if a.shape != m.shape:
+        raise ValueError("score and angle maps differ in size")
+    d_ang = DeviceArray.from_numpy(a, ctx=ctx)
+    e = 2 * rm + 1
+    mask = BH_mask3d(erase_type, [e, e, e], er, [0, 0, 0], ctx=ctx)
+    mh = mask.get()
+    cutoff, tries = 0.98, 0
+    pre = 0.0 if str(erase_type).lower() == "rectangle" else 4.0 / 3.0 * np.pi
+    while np.sum(mh > cutoff) < pre * np.prod(er):
+        cutoff *= 0.99
+        tries += 1
+        if tries > 1000:
+            raise RuntimeError("Did not find an appropriate erase mask")
+    al = np.ascontiguousarray(np.asarray(angle_list, dtype=np.float64).reshape(-1, 3))
+    max_peaks = int(2 * peak_threshold)
+    out = np.zeros((max(max_peaks, 1), 7))
+    nf = _lib.C.c_int()
+    _lib.call("emc_tsearch_peaks", ctx.handle, m.vp, d_ang.vp, _lib.i3(m.dims), mask.vp, rm,
+              al.ctypes.data_as(_lib.C.POINTER(_lib.C.c_double)), al.shape[0], _CONV[convention],
+              float(cutoff), float(high_thr), max_peaks, int(max_tries),
+              out.ctypes.data_as(_lib.C.POINTER(_lib.C.c_double)), _lib.C.byref(nf))
+    peaks = out[:nf.value].copy()
+    peaks[:, :3] *= sampling_rate
+    return peaks
+
+
+def _matlab_d(v):
+    """MATLAB fprintf '%d' of a double: integers print as integers, other
+    values switch to '%e'."""
+    v = float(v)
+    return str(int(v)) if v == int(v) else f"{v:e}"
+
+
+def write_template_csv(csv_path, pos_path, peaks, sampling_rate, n_previous=0, convention="Bah",
+                       symmetry_matrices=None):
+    """The CSV and .pos files of BH_templateSearch3d_2.m:1283-1341 (consumed
+    by BH_geometryInitialize): per peak i one row of 26 fields
+    'score samplingRate 0 i+nPrevious 1 1 1 1 1 0 x y z phi theta psi r(9) 1'
+    with r = BH_defineMatrix([phi, theta, psi], convention, 'inv') (times the
+    in-plane symmetry matrix rem(n + S, S) + 1 when S > 1), column-major, and
+    the .pos line x y z / samplingRate. Formats as the reference's fprintf."""
+    from .geometry import BH_defineMatrix
+    peaks = np.asarray(peaks, dtype=np.float64).reshape(-1, 7)
+    S = len(symmetry_matrices) if symmetry_matrices is not None else 1
+    with open(csv_path, "w") as fc, open(pos_path, "w") as fp:
+        n = 1
+        for i, pk in enumerate(peaks, start=1):
+            if not np.all(pk[0:3]):
+                continue
+            r = BH_defineMatrix(pk[3:6], convention, "inv")
+            if S > 1:
+                r = r @ np.asarray(symmetry_matrices[(n + S) % S])
+            rv = r.reshape(-1, order="F")
+            fields = ([f"{pk[6]:1.2f}", _matlab_d(sampling_rate), "0", _matlab_d(i + n_previous)]
+                      + ["1"] * 5 + ["0"] + [f"{v:f}" for v in pk[0:3]] + [_matlab_d(v) for v in pk[3:6]]
+                      + [f"{v:f}" for v in rv] + ["1"])
+            fc.write(" ".join(fields) + " \n")
+            fp.write("%f %f %f\n" % tuple(pk[0:3] / sampling_rate))
+            n += 1

@@ -392,6 +392,30 @@ emc_status emc_tsearch_preprocess_chunk(emc_ctx* ctx, const float* d_in, const i
                                         int stats_radius, float sign, const float* d_valid_mask, float* d_out,
                                         double* h_masked_sum);
 
+/* BH_mask3d('sphere', dims, radius, [0,0,0]) (masking/BH_mask3d.m:126-160,
+ * :462-466): ellipsoid of the given radii (x, y, z) about floor(N/2), six
+ * cosine-weighted shells, convn with BH_multi_gaussian3d(5, 0.5), divided by
+ * its maximum, the core reset to 1, then the 7-voxel edge taper. Every
+ * dimension >= 14. Stream-ordered. */
+emc_status emc_mask3d_sphere(emc_ctx* ctx, const int dims[3], const double radius[3], float* d_out);
+
+/* Greedy peak extraction of alignment/BH_templateSearch3d_2.m:1100-1275 for
+ * nPeaks = 1, on the padded score map d_mag (modified: erased in place) and
+ * angle-id map d_ang (1-based ids into angle_list rows (phi, theta, psi-phi),
+ * <= 0 = none), both dims. Repeats while fewer than max_peaks peaks
+ * (2 * peakThreshold) were taken, fewer than max_tries maxima visited and
+ * the maximum > high_thr: take the first maximum (x fastest); if its id > 0,
+ * record (x, y, z) = 1-based voxel + centre of mass of the 3^3 box - rm_dim,
+ * the angles and the score, rotate d_removal_mask ((2 rm_dim + 1)^3) by the
+ * angles ('forward', convention as emc_define_matrix) and zero the map
+ * where it is >= mask_cutoff around the voxel; else zero that voxel.
+ * h_peaks: max_peaks x 7 doubles (x, y, z, phi, theta, psi-phi, score);
+ * *n_found peaks. Synchronous. */
+emc_status emc_tsearch_peaks(emc_ctx* ctx, float* d_mag, const int32_t* d_ang, const int dims[3],
+                             const float* d_removal_mask, int rm_dim, const double* angle_list, int n_angles,
+                             int convention, double mask_cutoff, double high_thr, int max_peaks, int max_tries,
+                             double* h_peaks, int* n_found);
+
 /* d[i] *= s (the global-variance scaling of the chunk stack, :605-611). */
 emc_status emc_scale(emc_ctx* ctx, float* d, int64_t count, double s);
 

@@ -1041,3 +1041,73 @@ def tsearch_preprocess(tomogram, size_chunk, valid_area, n_iters, tomo_pre, tomo
                 corners.append((cx, cy, cz))
     gvar = fx2 / fn - (fx / fn) ** 2                                  # :603
     return [c / math.sqrt(gvar) for c in chunks], corners, gvar, vmask
+
+
+# --------------------------------------------------------------------------
+# Template-search peak extraction (alignment/BH_templateSearch3d_2.m:1036-1275)
+# --------------------------------------------------------------------------
+
+
+def gaussian3d_taps(size=5, sd=0.5):
+    """1-D factor of BH_multi_gaussian3d(size, sd) (statistics/... :1-30):
+    exp(-0.5 (x/sd)^2) on x = -floor(size/2).., normalised (the 3-D kernel is
+    the outer product, normalised to sum 1)."""
+    x = np.arange(size) - size // 2
+    g = np.exp(-0.5 * (x / sd) ** 2)
+    return g / g.sum()
+
+
+def mask3d_sphere(size, radius):
+    """BH_mask3d('sphere', size, radius, [0,0,0]) (masking/BH_mask3d.m:80-160,
+    :462-466); size and radius (x, y, z); returns (nz, ny, nx)."""
+    nx, ny, nz = (int(v) for v in size)
+    rx, ry, rz = (float(v) for v in radius)
+    z, y, x = np.meshgrid(np.arange(nz) - nz // 2, np.arange(ny) - ny // 2, np.arange(nx) - nx // 2,
+                          indexing="ij")
+    ell = lambda s: (x / (rx + s)) ** 2 + (y / (ry + s)) ** 2 + (z / (rz + s)) ** 2
+    full = ell(0) <= 1
+    w = full.astype(np.float64)
+    taper = 0.5 + 0.5 * np.cos(np.arange(1, 7) * np.pi / 7)                 # :87
+    for s in range(1, 7):                                                     # :145-151
+        w = w + ((ell(s) <= 1).astype(np.float64) - w) * taper[s - 1]
+    k = gaussian3d_taps(5, 0.5)
+    for axis in (2, 1, 0):                                                    # convn 'same' (:152)
+        w = _conv_same_axis(w, k, axis)
+    w = w / w.max()                                                           # :153
+    w[full] = 1.0                                                             # :155
+    return pad_zeros3d(w, [0, 0, 0], [0, 0, 0], taper=True)                   # :462-466
+
+
+def tsearch_peaks(mag, ang, angle_list, removal_mask, rm_dim, mask_cutoff, high_thr, max_peaks, max_tries,
+                  convention="Bah"):
+    """The greedy loop of BH_templateSearch3d_2.m:1130-1272 for nPeaks = 1 on
+    padded maps (nz, ny, nx); mag is copied. Returns (n, 7): x, y, z (1-based
+    voxel + centre of mass - rm_dim), angles, score."""
+    mag = np.array(mag, dtype=np.float64, copy=True)
+    nz, ny, nx = mag.shape
+    flat = mag.reshape(-1)          # x fastest, as MATLAB's column-major (x, y, z)
+    peaks, tries = [], 0
+    coord = int(np.argmax(flat))
+    while len(peaks) < max_peaks and tries < max_tries and flat[coord] > high_thr:
+        tries += 1
+        z, r = divmod(coord, nx * ny)
+        y, x = divmod(r, nx)
+        a = int(ang[z, y, x])
+        if a > 0:
+            box = mag[z - 1:z + 2, y - 1:y + 2, x - 1:x + 2]
+            d = np.arange(-1, 2)
+            s = box.sum()
+            com = [np.sum(box * d[None, None, :]) / s, np.sum(box * d[None, :, None]) / s,
+                   np.sum(box * d[:, None, None]) / s]
+            angs = np.asarray(angle_list[a - 1], dtype=np.float64)
+            peaks.append([x + 1 + com[0] - rm_dim, y + 1 + com[1] - rm_dim, z + 1 + com[2] - rm_dim,
+                          angs[0], angs[1], angs[2], flat[coord]])
+            R = define_matrix(angs, convention, "forward")
+            rot = xform3d(removal_mask, R, np.zeros(3), True)
+            sl = (slice(z - rm_dim, z + rm_dim + 1), slice(y - rm_dim, y + rm_dim + 1),
+                  slice(x - rm_dim, x + rm_dim + 1))
+            mag[sl] = mag[sl] * (rot < mask_cutoff)
+        else:
+            flat[coord] = 0.0
+        coord = int(np.argmax(flat))
+    return np.array(peaks).reshape(-1, 7)

@@ -508,3 +508,80 @@ def test_tsearch_global_variance_squares_chunk_sums():
     N = len(sums) * 24 ** 3
     assert np.isclose(gvar, sum(s * s for s in sums) / N - (sum(sums) / N) ** 2, rtol=1e-12)
     assert len(chunks) == 4 and corners == [(0, 0, 0), (0, 16, 0), (16, 0, 0), (16, 16, 0)]
+
+
+# ------------------------------------------------ template-search peak picking
+
+def test_mask3d_sphere_pins():
+    """BH_mask3d('sphere'): the separable 5-tap convolution equals the full
+    3-D convn with the BH_multi_gaussian3d kernel (scipy, zero boundary)
+    applied to the shells; the core is 1; a voxel first inside at shell 1
+    carries the recurrence w = t1, then w + (1 - w) t_s; symmetric about
+    floor(N/2)."""
+    from scipy.ndimage import convolve
+    n, R = 31, 5.0
+    m = O.mask3d_sphere((n, n, n), (R, R, R))
+    c = n // 2
+    assert m.shape == (n, n, n) and m[c, c, c] == 1.0 and m[c, c, c + 5] == 1.0
+    z, y, x = np.meshgrid(*(np.arange(n) - c,) * 3, indexing="ij")
+    r2 = x ** 2 + y ** 2 + z ** 2
+    w = (r2 <= R * R).astype(float)
+    t = 0.5 + 0.5 * np.cos(np.arange(1, 7) * np.pi / 7)
+    for s in range(1, 7):
+        w = w + ((r2 <= (R + s) ** 2).astype(float) - w) * t[s - 1]
+    v = 0.0                      # r = 6: outside the core, inside from shell 1 on
+    for s in range(1, 7):
+        v = v + (1.0 - v) * t[s - 1]
+    assert w[c, c, c + 6] == pytest.approx(v, abs=1e-15)
+    g = O.gaussian3d_taps(5, 0.5)
+    k3 = g[:, None, None] * g[None, :, None] * g[None, None, :]
+    full = convolve(w, k3, mode="constant")
+    full = full / full.max()
+    full[r2 <= R * R] = 1.0
+    full = O.pad_zeros3d(full, [0, 0, 0], [0, 0, 0], taper=True)
+    assert np.allclose(m, full, atol=1e-12)
+    assert np.allclose(m, m[::-1, ::-1, ::-1], atol=1e-12)
+    assert np.all(m[:, :, 0] == 0) and m[c, c, c + 14] < 1e-6
+
+
+def _planted_map(rng, shape=(40, 44, 48), centres=((20, 20, 22), (30, 12, 15), (12, 30, 33)),
+                 amps=(5.0, 4.0, 3.0)):
+    nz, ny, nx = shape
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    mag = 0.05 * rng.standard_normal(shape)
+    for (cx, cy, cz), a in zip(centres, amps):
+        mag += a * np.exp(-((x - cx - 0.3) ** 2 + (y - cy + 0.2) ** 2 + (z - cz) ** 2) / 2.0)
+    ang = rng.integers(1, 20, size=shape).astype(np.int32)
+    return mag, ang
+
+
+def test_tsearch_peaks_recovers_planted_peaks():
+    rng = np.random.default_rng(3)
+    mag, ang = _planted_map(rng)
+    rm = 4 + 7                           # rmDim = erase radius + 7, as the reference sizes it
+    pad = lambda a: np.pad(a, rm)
+    grid = np.stack([np.arange(19) * 10.0, np.full(19, 5.0), np.zeros(19)], 1)
+    mask = O.mask3d_sphere((2 * rm + 1,) * 3, (4.0, 4.0, 4.0))
+    pk = O.tsearch_peaks(pad(mag), pad(ang), grid, mask, rm, 0.98, 1.0, 10, 1000)
+    assert len(pk) == 3                                  # the noise stays below the threshold
+    assert np.all(np.diff(pk[:, 6]) < 0)                 # taken in order of score
+    for p, (cx, cy, cz) in zip(pk, ((20, 20, 22), (30, 12, 15), (12, 30, 33))):
+        # 1-based coordinates; the centre of mass pulls toward the +0.3/-0.2 offsets
+        assert abs(p[0] - (cx + 1)) < 0.5 and abs(p[1] - (cy + 1)) < 0.5 and p[2] == pytest.approx(cz + 1, abs=0.05)
+        zz, yy, xx = cz, cy, cx
+        assert np.array_equal(p[3:6], grid[ang[zz, yy, xx] - 1])
+
+
+def test_template_csv_row_format(tmp_path):
+    from emclarity_amd.template_search import write_template_csv
+    from emclarity_amd.geometry import BH_defineMatrix
+    pk = np.array([[120.5, 64.0, 33.25, 57.2727272727, 12.0, -30.0, 7.891]])
+    write_template_csv(tmp_path / "a.csv", tmp_path / "a.pos", pk, 2, n_previous=4)
+    row = (tmp_path / "a.csv").read_text().split()
+    assert len(row) == 26
+    assert row[:10] == ["7.89", "2", "0", "5", "1", "1", "1", "1", "1", "0"]
+    assert row[10:13] == ["120.500000", "64.000000", "33.250000"]
+    assert row[13:16] == ["5.727273e+01", "12", "-30"]
+    r = BH_defineMatrix(pk[0, 3:6], "Bah", "inv").reshape(-1, order="F")
+    assert np.allclose([float(v) for v in row[16:25]], r, atol=1e-6) and row[25] == "1"
+    assert (tmp_path / "a.pos").read_text() == "60.250000 32.000000 16.625000\n"
