@@ -14,7 +14,7 @@ HEADER_PATH = Path(__file__).resolve().parent.parent / "include" / "emc_hip.h"
 
 EMC_OK = 0
 STATUS_NAMES = {0: "EMC_OK", 1: "EMC_ERR_INVALID", 2: "EMC_ERR_HIP", 3: "EMC_ERR_FFT",
-                4: "EMC_ERR_ALLOC", 5: "EMC_ERR_RANGE", 6: "EMC_ERR_NODEVICE"}
+                4: "EMC_ERR_ALLOC", 5: "EMC_ERR_RANGE", 6: "EMC_ERR_NODEVICE", 7: "EMC_ERR_IO"}
 
 _st = C.c_int
 _vp = C.c_void_p
@@ -73,6 +73,13 @@ SIGNATURES = {
     "emc_local_normalize": (_st, [_vp, _vp, _i3, C.c_int, _vp]),
     "emc_tsearch_preprocess_chunk": (_st, [_vp, _vp, _i3, _d, C.c_int, C.c_float, _vp, _vp, _d]),
     "emc_scale": (_st, [_vp, _vp, C.c_int64, C.c_double]),
+    "emc_mrc_read_header": (_st, [C.c_char_p, _vp]),
+    "emc_mrc_read_volume": (_st, [C.c_char_p, _i3, _f]),
+    "emc_mrc_upload": (_st, [_vp, C.c_char_p, C.c_int, C.c_int, _vp]),
+    "emc_mrc_to_float": (_st, [_vp, _vp, C.c_int, C.c_int64, _vp]),
+    "emc_mrc_write": (_st, [C.c_char_p, _f, _i3, _d, _d, C.c_int]),
+    "emc_extract_subtomos": (_st, [_vp, _vp, C.c_int, _i3, C.c_int, _d, _i3, _d, _vp,
+                                   C.POINTER(C.c_int32), _d]),
     "emc_batch_reference_spectrum": (_st, [_vp, C.c_int, _vp, C.c_int, _vp]),
     "emc_batch_prepare_particle": (_st, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "emc_batch_prepare_particles": (_st, [_vp, C.c_int, C.c_int, C.POINTER(C.c_void_p), _vp,

@@ -40,7 +40,8 @@ typedef enum {
   EMC_ERR_FFT = 3,       /* rocFFT error */
   EMC_ERR_ALLOC = 4,     /* device allocation failed */
   EMC_ERR_RANGE = 5,     /* data-dependent range error (e.g. COM box outside map) */
-  EMC_ERR_NODEVICE = 6   /* no HIP device */
+  EMC_ERR_NODEVICE = 6,  /* no HIP device */
+  EMC_ERR_IO = 7         /* a file could not be opened, read or written */
 } emc_status;
 
 typedef struct emc_ctx emc_ctx;
@@ -418,6 +419,88 @@ emc_status emc_tsearch_peaks(emc_ctx* ctx, float* d_mag, const int32_t* d_ang, c
 
 /* d[i] *= s (the global-variance scaling of the chunk stack, :605-611). */
 emc_status emc_scale(emc_ctx* ctx, float* d, int64_t count, double s);
+
+/* ------------------------------------------- MRC files and subtomograms */
+
+/* The 1024-byte MRC header in the field order and names of
+ * @MRCImage/private/readHeader.m:43-172 (IMOD/MRC2014 layout). `swapped` is
+ * 1 when the file's byte order is not the host's: readHeader.m:54-78
+ * reopens the file in the other order when nX, nY or nZ read negative, or
+ * all three exceed 65535. data_offset = 1024 + nBytesExtended. */
+typedef struct emc_mrc_header {
+  int32_t nx, ny, nz, mode;
+  int32_t nxstart, nystart, nzstart;
+  int32_t mx, my, mz;
+  float cell[3];        /* cellDimensionX/Y/Z */
+  float cell_angle[3];  /* cellAngleX/Y/Z */
+  int32_t mapc, mapr, maps;
+  float dmin, dmax, dmean;
+  int16_t space_group, n_symmetry_bytes;
+  int32_t n_bytes_extended;
+  int16_t creator_id, n_bytes_per_section, serialem_type;
+  int32_t imod_stamp, imod_flags;
+  int16_t idtype, lens, nd1, nd2, vd1, vd2;
+  float tilt_angles[6];
+  float origin[3];      /* xOrigin, yOrigin, zOrigin */
+  char map[4], machine_stamp[4];
+  float rms;            /* densityRMS */
+  int32_t n_labels;
+  char labels[10][80];
+  int32_t swapped;
+  int64_t data_offset;
+} emc_mrc_header;
+
+/* MRCImage(path, 0) + getHeader (readHeader.m:43-186). Host only. */
+emc_status emc_mrc_read_header(const char* path, emc_mrc_header* hdr);
+
+/* getVolume(MRCImage(path), [i0 i1], [j0 j1], [k0 k1]) (@MRCImage/getVolume.m:
+ * 40-298): range = {i0, i1, j0, j1, k0, k1}, 1-based and inclusive (x, y,
+ * z), NULL = the whole volume. Modes 0 (read as signed int8, as getVolume's
+ * 'int8' precision does), 1 (int16), 2 (float32) and 6 (uint16) convert to
+ * float32; other modes are rejected. h_out: (i1-i0+1) x (j1-j0+1) x
+ * (k1-k0+1) floats, x fastest. Host only. */
+emc_status emc_mrc_read_volume(const char* path, const int range[6], float* h_out);
+
+/* Sections k0..k1 (1-based, inclusive) of the file, as stored (nx*ny*mode
+ * bytes per section, modes 0/1/2/6), into device memory d_raw, in host byte
+ * order: read into two pinned staging buffers in turn while the previous one
+ * is copied, byte order fixed on the device. The tomogram then stays
+ * resident in its stored type (half the bytes of float32 for int16 maps).
+ * Replaces the whole-tomogram load of BH_multi_loadOrBuild.m:180-183.
+ * Synchronous. */
+emc_status emc_mrc_upload(emc_ctx* ctx, const char* path, int k0, int k1, void* d_raw);
+
+/* d_out[i] = float(d_raw[i]) for `count` values of MRC mode `mode`
+ * (0 int8, 1 int16, 2 float32, 6 uint16). Stream-ordered. */
+emc_status emc_mrc_to_float(emc_ctx* ctx, const void* d_raw, int mode, int64_t count, float* d_out);
+
+/* SAVE_IMG(MRCImage(vol), path[, pixel[, origin]]) for a float32 volume
+ * (@MRCImage/SAVE_IMG.m, private/setVolumeAndHeaderFromVolume.m,
+ * private/writeHeader.m): mode 2, cell = dims (x pixel when pixel is not
+ * NULL), angles 90, space group 1, IMOD stamp, 'MAP ', machine stamp
+ * 'DA\0\0', one empty label counted; min, max, mean and rms =
+ * std(std(std(vol))) (SAVE_IMG.m, the nested std of the reference). origin:
+ * NULL = 0; n_origin 1 = -dims/2 (set before the pixel scaling, as
+ * SAVE_IMG.m does); n_origin 3 = the given values. Host only. */
+emc_status emc_mrc_write(const char* path, const float* h_data, const int dims[3], const double* pixel,
+                         const double* origin, int n_origin);
+
+/* Batched subtomogram extraction. For each of n particles:
+ * BH_isWindowValid(tomo_dims, window, mask_radius, centre)
+ * (logicals/BH_isWindowValid.m:42-65), the window of the resident tomogram
+ * (alignment/BH_alignRaw3d_v2.m:1074-1076) and BH_padZeros3d(.., padLow,
+ * padTop, 'GPU', 'singleTaper') (:1085-1086), in one launch for all
+ * particles. d_tomo: the tomogram as stored (mode 0/1/2/6, see
+ * emc_mrc_upload), tomo_dims {nx, ny, nz}. h_centres: n x 3 (x, y, z) in the
+ * reference's 1-based CENTER convention (positionList(:, 11:13) /
+ * samplingRate + binShift, :1034). d_out: n windows of window[0] x
+ * window[1] x window[2] floats. h_valid[i] = 1, or 0 when the window keeps
+ * less than the mask needs ('noUse': that window is zero-filled);
+ * h_shifts: n x 3 (shiftVAL = centre - floor(centre)). Returns once the
+ * window table is on the device; the cut itself is stream-ordered. */
+emc_status emc_extract_subtomos(emc_ctx* ctx, const void* d_tomo, int mode, const int tomo_dims[3], int n,
+                                const double* h_centres, const int window[3], const double mask_radius[3],
+                                float* d_out, int32_t* h_valid, double* h_shifts);
 
 #ifdef __cplusplus
 }

@@ -1111,3 +1111,166 @@ def tsearch_peaks(mag, ang, angle_list, removal_mask, rm_dim, mask_cutoff, high_
             flat[coord] = 0.0
         coord = int(np.argmax(flat))
     return np.array(peaks).reshape(-1, 7)
+
+
+# --------------------------------------------------------------------------
+# (f)2 MRC files (@MRCImage/private/readHeader.m:43-186, getVolume.m:40-298,
+# SAVE_IMG.m, private/writeHeader.m, private/setVolumeAndHeaderFromVolume.m)
+# and subtomogram extraction (logicals/BH_isWindowValid.m:42-65,
+# alignment/BH_alignRaw3d_v2.m:1036-1086)
+# --------------------------------------------------------------------------
+
+# readHeader.m's read order: (name, numpy type code, count); "skip" fields
+# are the unnamed bytes it reads and drops
+_MRC_ORDER = [
+    ("nX", "i4", 1), ("nY", "i4", 1), ("nZ", "i4", 1), ("mode", "i4", 1),
+    ("nXStart", "i4", 1), ("nYStart", "i4", 1), ("nZStart", "i4", 1),
+    ("mX", "i4", 1), ("mY", "i4", 1), ("mZ", "i4", 1),
+    ("cellDimensionX", "f4", 1), ("cellDimensionY", "f4", 1), ("cellDimensionZ", "f4", 1),
+    ("cellAngleX", "f4", 1), ("cellAngleY", "f4", 1), ("cellAngleZ", "f4", 1),
+    ("mapColumns", "i4", 1), ("mapRows", "i4", 1), ("mapSections", "i4", 1),
+    ("minDensity", "f4", 1), ("maxDensity", "f4", 1), ("meanDensity", "f4", 1),
+    ("spaceGroup", "i2", 1), ("nSymmetryBytes", "i2", 1), ("nBytesExtended", "i4", 1),
+    ("creatorID", "i2", 1), ("skip", "u1", 30), ("nBytesPerSection", "i2", 1), ("serialEMType", "i2", 1),
+    ("skip", "u1", 20), ("imodStamp", "i4", 1), ("imodFlags", "i4", 1),
+    ("idtype", "i2", 1), ("lens", "i2", 1), ("ndl", "i2", 1), ("nd2", "i2", 1), ("vdl", "i2", 1),
+    ("vd2", "i2", 1), ("tiltAngles", "f4", 6), ("xOrigin", "f4", 1), ("yOrigin", "f4", 1),
+    ("zOrigin", "f4", 1), ("map", "u1", 4), ("machineStamp", "u1", 4), ("densityRMS", "f4", 1),
+    ("nLabels", "i4", 1),
+]
+_MRC_GETVOLUME_TYPES = {0: "i1", 1: "i2", 2: "f4", 6: "u2"}   # getVolume.m:150-178
+
+
+def mrc_read_header(path):
+    """Field by field in readHeader.m's order; the byte order flips when a
+    size reads negative or all three exceed 65535 (:54-78)."""
+    raw = open(path, "rb").read(1024)
+    first = np.frombuffer(raw[:12], "<i4")
+    swap = bool(np.any(first < 0) or np.all(first > 65535))
+    bo = ">" if swap else "<"
+    h, pos = {}, 0
+    for name, t, cnt in _MRC_ORDER:
+        dt = np.dtype(bo + t) if t != "u1" else np.dtype("u1")
+        v = np.frombuffer(raw[pos:pos + dt.itemsize * cnt], dt)
+        pos += dt.itemsize * cnt
+        if name == "skip":
+            continue
+        h[name] = bytes(v) if t == "u1" else (v.tolist() if cnt > 1 else v[0].item())
+    nl = max(0, min(10, h["nLabels"]))
+    h["labels"] = [raw[pos + 80 * i: pos + 80 * (i + 1)] for i in range(nl)]
+    h["byteOrder"] = bo
+    h["dataIndex"] = 1024 + max(h["nBytesExtended"], 0)
+    return h
+
+
+def mrc_get_volume(path, i_range=None, j_range=None, k_range=None):
+    """getVolume(MRCImage(path), [i0 i1], [j0 j1], [k0 k1]) -> float64 (k, j, i)
+    array; mode 0 reads as signed bytes (the 'int8' precision of :151-154)."""
+    h = mrc_read_header(path)
+    t = np.dtype(h["byteOrder"] + _MRC_GETVOLUME_TYPES[h["mode"]])
+    nx, ny, nz = h["nX"], h["nY"], h["nZ"]
+    data = np.fromfile(path, dtype=t, count=nx * ny * nz, offset=h["dataIndex"]).reshape(nz, ny, nx)
+    i0, i1 = i_range or (1, nx)
+    j0, j1 = j_range or (1, ny)
+    k0, k1 = k_range or (1, nz)
+    return data[k0 - 1:k1, j0 - 1:j1, i0 - 1:i1].astype(np.float64)
+
+
+def _matlab_std_default(v):
+    """MATLAB std(X) without a dimension: along the first non-singleton
+    dimension (a scalar gives 0), N-1 normalisation. ``v`` in MATLAB order."""
+    axes = [a for a in range(v.ndim) if v.shape[a] != 1]
+    if not axes:
+        return np.zeros_like(v)
+    return np.std(v, axis=axes[0], ddof=1, keepdims=True)
+
+
+def mrc_save(volume, path, pixel=None, origin=None):
+    """SAVE_IMG(MRCImage(single(volume)), path[, pixel[, origin]]): header of
+    setVolumeAndHeaderFromVolume.m, statistics of SAVE_IMG.m (min, max, mean,
+    and rms = std(std(std(vol))), each std along the first non-singleton
+    dimension, so 0 for a volume with a singleton axis), nLabels written
+    as 1 (writeHeader.m), the origin's one-value form = -size/2 set before
+    the cell is scaled by the pixel size. Field by field in the read order."""
+    v = np.asarray(volume, dtype=np.float32)
+    if v.ndim == 2:
+        v = v[None]
+    nz, ny, nx = v.shape
+    vd = v.astype(np.float64)
+    rms = vd.transpose(2, 1, 0)     # MATLAB (nx, ny, nz)
+    for _ in range(3):
+        rms = _matlab_std_default(rms)
+    rms = float(rms.reshape(-1)[0])
+    px = np.ones(3) if pixel is None else np.broadcast_to(np.asarray(pixel, float), (3,))
+    if origin is None:
+        org = [0.0, 0.0, 0.0]
+    elif np.size(origin) == 1:
+        org = [-nx / 2, -ny / 2, -nz / 2]
+    else:
+        org = list(np.asarray(origin, float))
+    h = {"nX": nx, "nY": ny, "nZ": nz, "mode": 2, "nXStart": 0, "nYStart": 0, "nZStart": 0,
+         "mX": nx, "mY": ny, "mZ": nz,
+         "cellDimensionX": nx * px[0], "cellDimensionY": ny * px[1], "cellDimensionZ": nz * px[2],
+         "cellAngleX": 90, "cellAngleY": 90, "cellAngleZ": 90,
+         "mapColumns": 1, "mapRows": 2, "mapSections": 3,
+         "minDensity": vd.min(), "maxDensity": vd.max(), "meanDensity": vd.mean(),
+         "spaceGroup": 1, "nSymmetryBytes": 0, "nBytesExtended": 0, "creatorID": 0,
+         "nBytesPerSection": 0, "serialEMType": 0, "imodStamp": 1146047817, "imodFlags": 0,
+         "idtype": 0, "lens": 0, "ndl": 0, "nd2": 0, "vdl": 0, "vd2": 0, "tiltAngles": [0.0] * 6,
+         "xOrigin": org[0], "yOrigin": org[1], "zOrigin": org[2],
+         "map": b"MAP ", "machineStamp": bytes([68, 65, 0, 0]), "densityRMS": rms, "nLabels": 1}
+    out = bytearray()
+    for name, t, cnt in _MRC_ORDER:
+        if name == "skip":
+            out += bytes(cnt)
+        elif t == "u1":
+            out += h[name]
+        else:
+            out += np.asarray(h[name], dtype="<" + t).reshape(cnt).tobytes()
+    out += bytes(1024 - len(out))
+    with open(path, "wb") as f:
+        f.write(bytes(out))
+        f.write(np.ascontiguousarray(v, dtype="<f4").tobytes())
+
+
+def window_valid(volume_size, window, mask_radius, center):
+    """BH_isWindowValid.m:42-65 -> (indices 2x3 1-based or None for 'noUse',
+    pads 2x3, shifts 3)."""
+    vs = np.asarray(volume_size, float)
+    ws = np.asarray(window, float)
+    c = np.asarray(center, float)
+    min_size = (np.max(mask_radius) + 6) * 2
+    lo_c = np.ceil((ws - 1) / 2)
+    hi_c = lo_c - np.mod(ws - 1, 2)
+    wc = np.floor(c)
+    LOW = wc - lo_c
+    TOP = vs - (wc + hi_c)
+    pad_lo = (np.abs(LOW) + 1) * (LOW <= 0)
+    pad_hi = np.abs(TOP) * (TOP <= 0)
+    ind = np.stack([LOW + pad_lo, wc + hi_c - pad_hi]).astype(np.int64)
+    pads = np.stack([pad_lo, pad_hi]).astype(np.int64)
+    avail = ws - pads[0] - pads[1]
+    if np.any(avail - min_size < -2):
+        return None, pads, c - wc
+    return ind, pads, c - wc
+
+
+def extract_subtomo(tomo, window, mask_radius, center):
+    """The per-particle cut of BH_alignRaw3d_v2.m:1036-1086 on a (z, y, x)
+    tomogram: window check, crop, BH_padZeros3d(.., 'singleTaper') with zero
+    fill. Returns (window (z, y, x) float64 or None, shift (3,))."""
+    nz, ny, nx = tomo.shape
+    ind, pads, shift = window_valid((nx, ny, nz), window, mask_radius, center)
+    if ind is None:
+        return None, shift
+    cut = np.asarray(tomo, np.float64)[ind[0, 2] - 1:ind[1, 2], ind[0, 1] - 1:ind[1, 1], ind[0, 0] - 1:ind[1, 0]]
+    return pad_zeros3d(cut, pads[0], pads[1], 0.0, taper=True), shift
+
+
+def calc_rg(stack_ny, r_coords):
+    """BH_multi_loadOrBuild.m:203-210."""
+    r = np.asarray(r_coords, float)
+    oY = math.ceil((stack_ny + 1) / 2)
+    nY = r[2] - r[1] + 1
+    dY = math.floor(r[1] + nY / 2) - oY
+    return np.array([[r[0], nY, r[3]], matlab_round(np.array([-r[4], dY, r[5]]))])
