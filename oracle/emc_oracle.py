@@ -1274,3 +1274,76 @@ def calc_rg(stack_ny, r_coords):
     nY = r[2] - r[1] + 1
     dY = math.floor(r[1] + nY / 2) - oY
     return np.array([[r[0], nY, r[3]], matlab_round(np.array([-r[4], dY, r[5]]))])
+
+
+# --------------------------------------------------------------------------
+# (f)3 alignment outputs: the per-tomogram rows of BH_alignRaw3d_v2.m:
+# 1862-1903 and metaData/BH_rawAlignmentsApply.m:40-89
+# --------------------------------------------------------------------------
+
+
+def align_resume_text(rows, n_peaks=1):
+    """alignResume/<prefix>/<tomo>.txt (:1894-1903): per row and peak
+    '%d %d %6.3f %6.3f %6.3f %6.6f %6.6f %6.3f %6.3f %6.3f ' then a newline
+    (MATLAB prints a non-integer under %d in %e form)."""
+    def d(v):
+        v = float(v)
+        return "%d" % int(v) if v == int(v) else ("%e" % v)
+    out = []
+    for r in np.asarray(rows, float).reshape(len(rows), -1):
+        line = ""
+        for p in range(n_peaks):
+            b = r[10 * p:10 * p + 10]
+            line += "%s %s %6.3f %6.3f %6.3f %6.6f %6.6f %6.3f %6.3f %6.3f " % (
+                d(b[0]), d(b[1]), b[2], b[3], b[4], b[5], b[6], b[7], b[8], b[9])
+        out.append(line + "\n")
+    return "".join(out)
+
+
+def unique_particle_rows(sort_ccc):
+    """:1866-1884: drop rows whose particle id (column 2) is 0, then keep the
+    first row of every particle id in order ([~,a] = unique(.., 'stable'))."""
+    s = np.asarray(sort_ccc, float)
+    s = s[s[:, 1] != 0]
+    seen, keep = set(), []
+    for i, pid in enumerate(s[:, 1]):
+        if pid not in seen:
+            seen.add(pid)
+            keep.append(i)
+    return s[keep]
+
+
+def raw_alignments_apply(geometry, best_angles, sampling, n_peaks, convention="Bah",
+                         update_weights=False, update_class=False):
+    """BH_rawAlignmentsApply.m:40-89, particle by particle: for every
+    particle whose class (column 26) is not -9999, the row of its id (column
+    4) in the tomogram's alignment rows; none -> class -9999; else rotation
+    columns 17:25 <- reshape(old * define_matrix(angles, conv, 'inv')),
+    position 11:13 += shifts * sampling, column 1 <- CCC, column 2 <- the
+    weight column if asked, column 26 <- the reference id if asked."""
+    out = {}
+    for tomo, pos in geometry.items():
+        pos = np.array(pos, dtype=float, copy=True)
+        new = np.asarray(best_angles.get(tomo, np.zeros((0, 10 * n_peaks))), float)
+        if new.size:
+            for p in range(n_peaks):
+                o, q = 26 * p, 10 * p
+                for i in range(pos.shape[0]):
+                    if pos[i, 25 + o] == -9999:
+                        continue
+                    hits = np.flatnonzero(new[:, 1] == pos[i, 3 + o])
+                    if hits.size == 0:
+                        pos[i, 25 + o] = -9999
+                        continue
+                    row = new[hits[0]]
+                    old = pos[i, 16 + o:25 + o].reshape(3, 3, order="F")
+                    rot = old @ define_matrix(row[2 + q:5 + q], convention, "inv")
+                    pos[i, 16 + o:25 + o] = rot.reshape(-1, order="F")
+                    pos[i, 10 + o:13 + o] += row[7 + q:10 + q] * sampling
+                    pos[i, 0 + o] = row[5 + q]
+                    if update_weights:
+                        pos[i, 1 + o] = row[6 + q]
+                    if update_class:
+                        pos[i, 25 + o] = row[0 + q]
+        out[tomo] = pos
+    return out
