@@ -78,7 +78,9 @@ SIGNATURES = {
     "emc_mrc_upload": (_st, [_vp, C.c_char_p, C.c_int, C.c_int, _vp]),
     "emc_mrc_to_float": (_st, [_vp, _vp, C.c_int, C.c_int64, _vp]),
     "emc_mrc_write": (_st, [C.c_char_p, _f, _i3, _d, _d, C.c_int]),
-    "emc_extract_subtomos": (_st, [_vp, _vp, C.c_int, _i3, C.c_int, _d, _i3, _d, _vp,
+    "emc_average_accumulate": (_st, [_vp, C.c_int, _i3, _vp, _d, _d, C.POINTER(C.c_int32), _d, C.c_int, _d, _vp,
+                                     _i3, _vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_int32)]),
+    "emc_extract_subtomos": (_st, [_vp, _vp, C.c_int, _i3, C.c_int, _d, _i3, _d, C.c_int, _vp,
                                    C.POINTER(C.c_int32), _d]),
     "emc_batch_reference_spectrum": (_st, [_vp, C.c_int, _vp, C.c_int, _vp]),
     "emc_batch_prepare_particle": (_st, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),

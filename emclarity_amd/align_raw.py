@@ -53,7 +53,12 @@ class ParticleAligner:
         self.vol_mask = dev(vol_mask)
         self.ref_wgt_rot = dev(ref_wgt_rot)
         self.ref_wgt = dev(ref_wgt)
-        self.angles = np.eye(3) if angles is None else np.asarray(angles, dtype=np.float64).reshape(3, 3)
+        if angles is None:
+            self.angles = np.eye(3)
+        else:
+            a = np.asarray(angles, dtype=np.float64)
+            # a geometry 9-vector (columns 17:25) is column-major, as reshape(angles, 3, 3)
+            self.angles = a.reshape(3, 3) if a.ndim == 2 else a.reshape(3, 3, order="F")
         self.shift_val = np.asarray(shift_val, dtype=np.float64)
         self.peak_com = [int(c) for c in peak_com]
         # ref_FT1 = conj(BH_bandLimitCenterNormalize(ref.*volMask, 1, ...)) (:607-608)

@@ -8,7 +8,8 @@
 // particle's window is cut from it in one launch: BH_isWindowValid
 // (logicals/BH_isWindowValid.m:42-65) on the host per particle, then the crop
 // + 'singleTaper' pad of BH_padZeros3d (masking/BH_padZeros3d.m:189-229) per
-// voxel. The cut is HBM-bound: mode bytes read + 4 B written per window voxel.
+// voxel ('single' without the taper, as the averaging uses). The cut is
+// HBM-bound: mode bytes read + 4 B written per window voxel.
 #include <cerrno>
 #include <cmath>
 #include <cstring>
@@ -239,7 +240,7 @@ __device__ __forceinline__ float edge_weight(int i, int n) {
 template <int MODE>
 __global__ void __launch_bounds__(256) extract_kernel(const void* __restrict__ tomo, int64_t tnx, int64_t tny,
                                                       const Window* __restrict__ wins, int wx, int wy, int wz,
-                                                      int vec4, float* __restrict__ out) {
+                                                      int taper, int vec4, float* __restrict__ out) {
   const int p = blockIdx.y;
   const int qx = (wx + 3) >> 2;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -251,14 +252,14 @@ __global__ void __launch_bounds__(256) extract_kernel(const void* __restrict__ t
   const int y = Y - w.pad[1], z = Z - w.pad[2];
   const bool row_in = w.valid && (unsigned)y < (unsigned)w.ext[1] && (unsigned)z < (unsigned)w.ext[2];
   float v[4];
-  const float wyz = row_in ? edge_weight(y, w.ext[1]) * edge_weight(z, w.ext[2]) : 0.0f;
+  const float wyz = row_in ? (taper ? edge_weight(y, w.ext[1]) * edge_weight(z, w.ext[2]) : 1.0f) : 0.0f;
   const int64_t row = ((int64_t)(w.src[2] + z) * tny + (w.src[1] + y)) * tnx + w.src[0];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int x = X0 + k - w.pad[0];
     v[k] = 0.0f;
     if (row_in && (unsigned)x < (unsigned)w.ext[0])
-      v[k] = load_mode<MODE>(tomo, row + x) * (edge_weight(x, w.ext[0]) * wyz);
+      v[k] = taper ? load_mode<MODE>(tomo, row + x) * (edge_weight(x, w.ext[0]) * wyz) : load_mode<MODE>(tomo, row + x);
   }
   float* o = out + ((int64_t)p * wz + Z) * ((int64_t)wy * wx) + (int64_t)Y * wx + X0;
   if (vec4) {
@@ -469,7 +470,7 @@ emc_status emc_mrc_write(const char* path, const float* h_data, const int dims[3
 
 emc_status emc_extract_subtomos(emc_ctx* ctx, const void* d_tomo, int mode, const int tomo_dims[3], int n,
                                 const double* h_centres, const int window[3], const double mask_radius[3],
-                                float* d_out, int32_t* h_valid, double* h_shifts) {
+                                int taper, float* d_out, int32_t* h_valid, double* h_shifts) {
   return guard([&] {
     use_device(ctx);
     mode_bytes(mode);
@@ -529,10 +530,10 @@ emc_status emc_extract_subtomos(emc_ctx* ctx, const void* d_tomo, int mode, cons
       const Window* w = d_wins + p0;
       float* o = d_out + (int64_t)p0 * wvox;
       switch (mode) {
-        case 0: hipLaunchKernelGGL(extract_kernel<0>, grid, dim3(256), 0, ctx->stream, d_tomo, tnx, tny, w, window[0], window[1], window[2], vec4, o); break;
-        case 1: hipLaunchKernelGGL(extract_kernel<1>, grid, dim3(256), 0, ctx->stream, d_tomo, tnx, tny, w, window[0], window[1], window[2], vec4, o); break;
-        case 2: hipLaunchKernelGGL(extract_kernel<2>, grid, dim3(256), 0, ctx->stream, d_tomo, tnx, tny, w, window[0], window[1], window[2], vec4, o); break;
-        default: hipLaunchKernelGGL(extract_kernel<6>, grid, dim3(256), 0, ctx->stream, d_tomo, tnx, tny, w, window[0], window[1], window[2], vec4, o); break;
+        case 0: hipLaunchKernelGGL(extract_kernel<0>, grid, dim3(256), 0, ctx->stream, d_tomo, tnx, tny, w, window[0], window[1], window[2], taper, vec4, o); break;
+        case 1: hipLaunchKernelGGL(extract_kernel<1>, grid, dim3(256), 0, ctx->stream, d_tomo, tnx, tny, w, window[0], window[1], window[2], taper, vec4, o); break;
+        case 2: hipLaunchKernelGGL(extract_kernel<2>, grid, dim3(256), 0, ctx->stream, d_tomo, tnx, tny, w, window[0], window[1], window[2], taper, vec4, o); break;
+        default: hipLaunchKernelGGL(extract_kernel<6>, grid, dim3(256), 0, ctx->stream, d_tomo, tnx, tny, w, window[0], window[1], window[2], taper, vec4, o); break;
       }
       EMC_HIP(hipGetLastError());
     }

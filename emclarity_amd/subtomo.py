@@ -76,9 +76,11 @@ class ResidentTomogram:
         self.raw.free()
 
 
-def extract_subtomos(tomo: ResidentTomogram, centres, window, mask_radius, out: DeviceArray | None = None):
+def extract_subtomos(tomo: ResidentTomogram, centres, window, mask_radius, out: DeviceArray | None = None,
+                     taper: bool = True):
     """Cut every particle's window: BH_isWindowValid(tomo size, window,
-    mask_radius, centre) + crop + BH_padZeros3d(.., 'singleTaper').
+    mask_radius, centre) + crop + BH_padZeros3d(.., 'singleTaper') (taper
+    False: 'single', plain zeros, as transformations/BH_average3d.m:1345).
 
     centres: (n, 3) (x, y, z) in the reference's 1-based CENTER convention
     (positionList(:, 11:13) ./ samplingRate + binShift). Returns
@@ -97,7 +99,7 @@ def extract_subtomos(tomo: ResidentTomogram, centres, window, mask_radius, out: 
     shifts = np.zeros((n, 3), np.float64)
     pd = C.POINTER(C.c_double)
     _lib.call("emc_extract_subtomos", tomo.raw.ctx.handle, tomo.raw.vp, tomo.mode, _lib.i3(tomo.dims), n,
-              c.ctypes.data_as(pd), _lib.i3(w), _lib.d_arr(mr), out.vp,
+              c.ctypes.data_as(pd), _lib.i3(w), _lib.d_arr(mr), int(bool(taper)), out.vp,
               valid.ctypes.data_as(C.POINTER(C.c_int32)), shifts.ctypes.data_as(pd))
     return out, valid.astype(bool), shifts
 

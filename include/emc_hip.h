@@ -490,7 +490,8 @@ emc_status emc_mrc_write(const char* path, const float* h_data, const int dims[3
  * (logicals/BH_isWindowValid.m:42-65), the window of the resident tomogram
  * (alignment/BH_alignRaw3d_v2.m:1074-1076) and BH_padZeros3d(.., padLow,
  * padTop, 'GPU', 'singleTaper') (:1085-1086), in one launch for all
- * particles. d_tomo: the tomogram as stored (mode 0/1/2/6, see
+ * particles; taper 0 pads with plain zeros ('single', the averaging's cut,
+ * transformations/BH_average3d.m:1328-1346). d_tomo: the tomogram as stored (mode 0/1/2/6, see
  * emc_mrc_upload), tomo_dims {nx, ny, nz}. h_centres: n x 3 (x, y, z) in the
  * reference's 1-based CENTER convention (positionList(:, 11:13) /
  * samplingRate + binShift, :1034). d_out: n windows of window[0] x
@@ -500,7 +501,29 @@ emc_status emc_mrc_write(const char* path, const float* h_data, const int dims[3
  * window table is on the device; the cut itself is stream-ordered. */
 emc_status emc_extract_subtomos(emc_ctx* ctx, const void* d_tomo, int mode, const int tomo_dims[3], int n,
                                 const double* h_centres, const int window[3], const double mask_radius[3],
-                                float* d_out, int32_t* h_valid, double* h_shifts);
+                                int taper, float* d_out, int32_t* h_valid, double* h_shifts);
+
+/* ------------------------------------------------------------ averaging */
+
+/* The per-particle body of transformations/BH_average3d.m:1417-1441 and its
+ * half-set accumulation (:1465-1490) for n particles of one window size, in
+ * the given order. Particle p: P = interpolator(window_p, M_p, t_p, 'inv',
+ * symmetry) (M_p = h_mats + 9p, the geometry's column-major 17:25; t_p =
+ * h_shifts + 3p, shiftVAL), then P = (P - mean(P(mask > 0.01))) ./
+ * rms(..(mask > 0.01)) .* mask (d_interp_mask, window-sized), and
+ * d_sum_odd (h_half[p] == 1) or d_sum_eve (== 2) += P * h_weight[p]. With
+ * d_wedges (n x wedge_dims, e.g. emc_weight_mask per particle) the rotated
+ * wedges interpolator(wedge_p, M_p, 0, 'inv', symmetry) * weight go to
+ * d_wedge_odd / d_wedge_eve. S_colmajor: nsym C_n mates (NULL with nsym 1).
+ * h_used[p] = 0 where the normalised particle would not be finite (the
+ * reference zeroes and ignores it, :1445-1462). The sums are read and
+ * updated in place: call again with the next batch. Synchronous. */
+emc_status emc_average_accumulate(emc_ctx* ctx, int n, const int window[3], const float* d_windows,
+                                  const double* h_mats, const double* h_shifts, const int32_t* h_half,
+                                  const double* h_weight, int nsym, const double* S_colmajor,
+                                  const float* d_interp_mask, const int wedge_dims[3], const float* d_wedges,
+                                  float* d_sum_odd, float* d_sum_eve, float* d_wedge_odd, float* d_wedge_eve,
+                                  int32_t* h_used);
 
 #ifdef __cplusplus
 }

@@ -236,11 +236,18 @@ def symmetry_matrices(symmetry, convention="Bah"):
     return [define_matrix([0.0, 0.0, i * 360.0 / n], convention, "forward") for i in range(n)]
 
 
+def _as_matrix(a):
+    """A 3x3 matrix as given, or a 9-vector read column-major, as MATLAB's
+    reshape(angles, 3, 3) (interpolator.m:163) reads geometry columns 17:25."""
+    a = np.asarray(a, dtype=np.float64)
+    return a.reshape(3, 3) if a.ndim == 2 else a.reshape(3, 3, order="F")
+
+
 def interp3d(vol, angles, shifts, convention="Bah", direction="forward", symmetry="C1"):
     """interpolator.interp3d (interpolator.m:99-154): Euler triple -> matrix
     (:159-161), sum over symmetry mates M*S_i (:108-143), / nSym (:145-147)."""
     a = np.asarray(angles, dtype=np.float64)
-    M = define_matrix(a, convention, direction) if a.size == 3 else a.reshape(3, 3)
+    M = define_matrix(a, convention, direction) if a.size == 3 else _as_matrix(a)
     fwd = direction.lower() == "forward"
     mats = symmetry_matrices(symmetry, convention)
     out = None
@@ -659,7 +666,7 @@ def align_orientation(inp: AlignInputs, rot_mat, est_peak0):
 
 def align_initial(inp: AlignInputs):
     """getInitialCCC (:1222-1232, :1355-1369): CCC at the current pose."""
-    A = np.asarray(inp.angles, dtype=np.float64).reshape(3, 3)
+    A = _as_matrix(inp.angles)
     part = interp3d(inp.particle, A, inp.shift_val, "Bah", "inv", inp.symmetry)
     wmask = interp3d(inp.wedge, A, [0, 0, 0], "Bah", "inv", inp.symmetry)
     rp = band_limit_center_normalize(part * inp.vol_mask, inp.bandpass, inp.pad_calc)
@@ -669,7 +676,7 @@ def align_initial(inp: AlignInputs):
 def align_grid(inp: AlignInputs, grid_angles):
     """Coarse grid (:1119-1419) over an explicit (phi, theta, psi) list with
     the azimuthal randomiser zeroed. Returns (ccc[n], shifts[n,3])."""
-    A = np.asarray(inp.angles, dtype=np.float64).reshape(3, 3)
+    A = _as_matrix(inp.angles)
     n = len(grid_angles)
     ccc = np.zeros(n)
     est = np.zeros((n, 3))
@@ -1255,16 +1262,17 @@ def window_valid(volume_size, window, mask_radius, center):
     return ind, pads, c - wc
 
 
-def extract_subtomo(tomo, window, mask_radius, center):
+def extract_subtomo(tomo, window, mask_radius, center, taper=True):
     """The per-particle cut of BH_alignRaw3d_v2.m:1036-1086 on a (z, y, x)
     tomogram: window check, crop, BH_padZeros3d(.., 'singleTaper') with zero
-    fill. Returns (window (z, y, x) float64 or None, shift (3,))."""
+    fill (taper False: 'single', BH_average3d.m:1345). Returns (window
+    (z, y, x) float64 or None, shift (3,))."""
     nz, ny, nx = tomo.shape
     ind, pads, shift = window_valid((nx, ny, nz), window, mask_radius, center)
     if ind is None:
         return None, shift
     cut = np.asarray(tomo, np.float64)[ind[0, 2] - 1:ind[1, 2], ind[0, 1] - 1:ind[1, 1], ind[0, 0] - 1:ind[1, 0]]
-    return pad_zeros3d(cut, pads[0], pads[1], 0.0, taper=True), shift
+    return pad_zeros3d(cut, pads[0], pads[1], 0.0, taper=taper), shift
 
 
 def calc_rg(stack_ny, r_coords):
@@ -1347,3 +1355,46 @@ def raw_alignments_apply(geometry, best_angles, sampling, n_peaks, convention="B
                         pos[i, 25 + o] = row[0 + q]
         out[tomo] = pos
     return out
+
+
+# --------------------------------------------------------------------------
+# (f)4 averaging: the per-particle body of transformations/BH_average3d.m:
+# 1417-1441 and the half-set sums of :1465-1490
+# --------------------------------------------------------------------------
+
+
+def average_particles(windows, mats, shifts, halves, weights, interp_mask, wedges=None, symmetry="C1"):
+    """Particle by particle in FP64: P = interpolator(window, M, shift, 'inv',
+    symmetry); P = (P - mean(P(mask > 0.01))) / rms(..) .* mask; a particle
+    whose normalised mean is not finite is skipped (:1445-1462); else added
+    times its weight to the odd (half 1) or even (2) sum, and its rotated
+    wedge likewise. Returns (odd, eve, wedge_odd, wedge_eve, used)."""
+    mask = np.asarray(interp_mask, np.float64)
+    binary = mask > 0.01
+    odd = np.zeros(mask.shape)
+    eve = np.zeros(mask.shape)
+    wshape = None if wedges is None else np.asarray(wedges[0]).shape
+    wodd = None if wedges is None else np.zeros(wshape)
+    weve = None if wedges is None else np.zeros(wshape)
+    used = []
+    for i, win in enumerate(windows):
+        P = interp3d(win, np.asarray(mats[i], float), shifts[i], "Bah", "inv", symmetry)
+        P = P - P[binary].mean()
+        with np.errstate(divide="ignore", invalid="ignore"):
+            P = P / math.sqrt(float(np.mean(P[binary] ** 2)))
+            P = P * mask
+        ok = bool(np.isfinite(P.mean()))
+        used.append(ok)
+        if not ok:
+            continue
+        if halves[i] == 1:
+            odd += P * weights[i]
+        else:
+            eve += P * weights[i]
+        if wedges is not None:
+            W = interp3d(wedges[i], np.asarray(mats[i], float), [0, 0, 0], "Bah", "inv", symmetry) * weights[i]
+            if halves[i] == 1:
+                wodd += W
+            else:
+                weve += W
+    return odd, eve, wodd, weve, np.array(used)

@@ -37,9 +37,9 @@ def _centres(rng, dims, n):
     return c
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode,taper", [(1, True), (2, True), (2, False)])
 @pytest.mark.parametrize("window", [(32, 32, 32), (27, 30, 25)])
-def test_extract_matches_oracle(ctx, mode, window):
+def test_extract_matches_oracle(ctx, mode, taper, window):
     rng = np.random.default_rng(sum(window) + mode)
     shape = (70, 80, 90)
     tomo = _data(mode, shape=shape, seed=mode).astype(np.int16 if mode == 1 else np.float32)
@@ -47,12 +47,12 @@ def test_extract_matches_oracle(ctx, mode, window):
     assert t.mode == mode
     mr = (4.0, 4.0, 4.0)
     c = _centres(rng, t.dims, 40)
-    out, valid, shifts = extract_subtomos(t, c, window, mr)
+    out, valid, shifts = extract_subtomos(t, c, window, mr, taper=taper)
     got = out.get()
     scale = float(np.max(np.abs(tomo)))
     assert not valid[3]
     for i in range(c.shape[0]):
-        ref, sh = O.extract_subtomo(tomo, window, mr, c[i])
+        ref, sh = O.extract_subtomo(tomo, window, mr, c[i], taper=taper)
         assert np.allclose(shifts[i], sh, atol=0, rtol=0)
         if ref is None:
             assert not valid[i] and np.all(got[i] == 0)

@@ -123,6 +123,14 @@ def test_xform_fwd_inv_sign_identity(rng):
     assert np.allclose(O.xform3d(v, M, t, True), O.xform3d(v, M, -M @ t, False), atol=1e-12)
 
 
+def test_interp3d_reads_a_geometry_vector_column_major(rng):
+    v = rng.standard_normal((8, 9, 10))
+    M = O.define_matrix([20, 35, -60], "Bah", "inv")
+    a = O.interp3d(v, M.reshape(-1, order="F"), [0.5, 0, -1], "Bah", "inv")
+    assert np.array_equal(a, O.interp3d(v, M, [0.5, 0, -1], "Bah", "inv"))
+    assert np.array_equal(a, O.interp3d(v, [20, 35, -60], [0.5, 0, -1], "Bah", "inv"))
+
+
 def test_interp3d_symmetry_average_invariant(rng):
     v = rng.standard_normal((9, 9, 9))
     M = O.define_matrix([15, 30, 45], "Bah", "forward")
