@@ -100,7 +100,7 @@ class BatchAligner:
         self.bandpass = as_device(bandpass, np.float32, self.ctx)
         n2 = 2 * self.n
         texel = np.complex64 if table == "c64" else np.float32   # f16: two halves per texel
-        self.F2 = DeviceArray((n2 + 2, n2 + 1, self.n + 2, 2), texel, self.ctx)
+        self.F2 = DeviceArray((n2 + 1, n2 + 1, self.n + 2, 4), texel, self.ctx)
         r = as_device(ref, np.float32, self.ctx)
         _lib.call("emc_batch_reference_spectrum", self.ctx.handle, self.n, r.vp, self._fmt, self.F2.vp)
         self.n_part = 0

@@ -190,36 +190,32 @@ typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 
 // A table entry holds the texels (x, y) and (x, y+1); entries x0 and x0+1
 // are adjacent, so one z-plane of a 2x2x2 footprint is one contiguous quad.
+// Table entry (x, y, z) holds the four texels (x, y, z), (x, y+1, z),
+// (x, y, z+1), (x, y+1, z+1); entries x and x+1 are adjacent, so a trilinear
+// footprint (x0..x0+1, y0..y0+1, z0..z0+1) is one contiguous octet: 64 bytes
+// (complex64) or 32 bytes (fp16) inside one 128-byte cache line for all but
+// the octets that start in a line's last quarter. Octet order: x0 (y0z0,
+// y1z0, y0z1, y1z1), then x1 likewise.
 struct TabC64 {
   using T = float2;
-  struct alignas(16) Quad {  // (x0,y0), (x0,y1), (x1,y0), (x1,y1): 32 bytes
-    float2 a, b, c, d;
+  struct alignas(16) Oct {
+    float2 t[8];
   };
   __device__ static void store(T* dst, float2 v, float) { *dst = v; }
-  __device__ static Quad load(const T* p) { return *reinterpret_cast<const Quad*>(p); }
-  __device__ static void unpack(const Quad& q, float2& x0y0, float2& x0y1, float2& x1y0, float2& x1y1) {
-    x0y0 = q.a;
-    x0y1 = q.b;
-    x1y0 = q.c;
-    x1y1 = q.d;
-  }
+  __device__ static Oct load(const T* p) { return *reinterpret_cast<const Oct*>(p); }
+  __device__ static float2 texel(const Oct& o, int i) { return o.t[i]; }
 };
 
 struct TabF16 {
   using T = half2_t;
-  struct alignas(8) Quad {  // 16 bytes, one load
-    half2_t a, b, c, d;
+  struct alignas(16) Oct {  // 32 bytes, two loads
+    half2_t t[8];
   };
   __device__ static void store(T* dst, float2 v, float s) {
     *dst = half2_t{(_Float16)(v.x * s), (_Float16)(v.y * s)};
   }
-  __device__ static Quad load(const T* p) { return *reinterpret_cast<const Quad*>(p); }
-  __device__ static void unpack(const Quad& q, float2& x0y0, float2& x0y1, float2& x1y0, float2& x1y1) {
-    x0y0 = make_float2((float)q.a.x, (float)q.a.y);
-    x0y1 = make_float2((float)q.b.x, (float)q.b.y);
-    x1y0 = make_float2((float)q.c.x, (float)q.c.y);
-    x1y1 = make_float2((float)q.d.x, (float)q.d.y);
-  }
+  __device__ static Oct load(const T* p) { return *reinterpret_cast<const Oct*>(p); }
+  __device__ static float2 texel(const Oct& o, int i) { return make_float2((float)o.t[i].x, (float)o.t[i].y); }
 };
 
 // Scale of the fp16 table: per-block maxima of max(|Re|, |Im|) (as
@@ -269,10 +265,10 @@ __device__ inline float2 raw_texel(const float2* __restrict__ raw, int n, int tx
   return cj ? make_float2(v.x, -v.y) : v;
 }
 
-// Centred reference table F2c: entries (x, y, z) for x in [0, n+1], y in
-// [-n, n], z in [-n, n+1], entry index [(z + n)(2n+1) + (y + n)](n+2) + x,
-// each the texel pair (x, y, z), (x, y+1, z). A folded sample with |j| <= n
-// reads its footprint without wrapping: two quads, planes z0 and z0+1.
+// Centred reference table F2c: entries (x, y, z) for x in [0, n+1], y and z
+// in [-n, n], entry index [(z + n)(2n+1) + (y + n)](n+2) + x, each the four
+// texels (x, y, z), (x, y+1, z), (x, y, z+1), (x, y+1, z+1). A folded sample
+// with |j| <= n reads its footprint, entries x0 and x0+1, without wrapping.
 template <class Tab>
 __global__ void f2_centre_kernel(const float2* __restrict__ raw, int n, const float* __restrict__ scale,
                                  typename Tab::T* __restrict__ F2c) {
@@ -282,9 +278,12 @@ __global__ void f2_centre_kernel(const float2* __restrict__ raw, int n, const fl
   const int zi = blockIdx.z;
   if (x >= X || yi >= Y) return;
   const float s = scale ? scale[0] : 1.0f;
-  typename Tab::T* e = F2c + 2 * (((int64_t)zi * Y + yi) * X + x);
-  Tab::store(e, raw_texel(raw, n, x, yi - n, zi - n), s);
-  Tab::store(e + 1, raw_texel(raw, n, x, yi - n + 1, zi - n), s);
+  typename Tab::T* e = F2c + 4 * (((int64_t)zi * Y + yi) * X + x);
+  const int y = yi - n, z = zi - n;
+  Tab::store(e, raw_texel(raw, n, x, y, z), s);
+  Tab::store(e + 1, raw_texel(raw, n, x, y + 1, z), s);
+  Tab::store(e + 2, raw_texel(raw, n, x, y, z + 1), s);
+  Tab::store(e + 3, raw_texel(raw, n, x, y + 1, z + 1), s);
 }
 
 struct Tri {
@@ -293,14 +292,14 @@ struct Tri {
 };
 
 // Trilinear sample of the centred table at j (index units of the padded
-// grid), split into fetch (the 2x2x2 footprint as two quads) and
+// grid), split into fetch (the 2x2x2 footprint, one octet) and
 // interpolation so a caller can keep several fetches in flight. The
 // trilinear interpolant of a Hermitian table is Hermitian, I(-j) = conj I(j),
 // so a sample with jx < 0 is folded as a whole. The clamp keeps every read
 // inside the table for any j; samples with |j| < n are never clamped.
 template <class Tab>
 struct Foot {
-  typename Tab::Quad t[2];  // planes z0, z0+1
+  typename Tab::Oct t;  // the 2x2x2 footprint
   float fx, fy, fz;
   bool cj;
 };
@@ -324,17 +323,16 @@ __device__ inline Foot<Tab> f2_fetch(const typename Tab::T* __restrict__ F2c, in
   q.fy = jy - yf;
   q.fz = jz - zf;
   const int X = n + 2, Y = 2 * n + 1;
-  const typename Tab::T* p = F2c + 2 * ((((int)zf + n) * Y + ((int)yf + n)) * X + (int)xf);
-  q.t[0] = Tab::load(p);
-  q.t[1] = Tab::load(p + 2 * X * Y);
+  q.t = Tab::load(F2c + 4 * ((((int)zf + n) * Y + ((int)yf + n)) * X + (int)xf));
   return q;
 }
 
 template <bool GRAD, class Tab>
 __device__ inline Tri f2_interp(const Foot<Tab>& q) {
-  float2 t000, t001, t010, t011, t100, t101, t110, t111;  // t[z][y][x]
-  Tab::unpack(q.t[0], t000, t010, t001, t011);
-  Tab::unpack(q.t[1], t100, t110, t101, t111);
+  // t[z][y][x]; octet order x0 (y0z0, y1z0, y0z1, y1z1), x1 likewise
+  const float2 t000 = Tab::texel(q.t, 0), t010 = Tab::texel(q.t, 1), t100 = Tab::texel(q.t, 2),
+               t110 = Tab::texel(q.t, 3), t001 = Tab::texel(q.t, 4), t011 = Tab::texel(q.t, 5),
+               t101 = Tab::texel(q.t, 6), t111 = Tab::texel(q.t, 7);
   const float fx = q.fx, fy = q.fy, fz = q.fz;
   auto lerp2 = [](float2 a, float2 b, float w) {
     return make_float2(a.x + w * (b.x - a.x), a.y + w * (b.y - a.y));
@@ -1064,7 +1062,7 @@ emc_status emc_batch_reference_spectrum(emc_ctx* ctx, int n, const float* d_ref,
                        ctx->stream, d_ref, pad, n);
     EMC_HIP(hipGetLastError());
     fft_r2c(cached_plan(ctx, m, m, m, 1), pad, raw);
-    const dim3 grid(ceil_div(n + 2, 64), ceil_div(m + 1, 4), m + 2), block(64, 4, 1);
+    const dim3 grid(ceil_div(n + 2, 64), ceil_div(m + 1, 4), m + 1), block(64, 4, 1);
     if (table_format == EMC_TABLE_C64) {
       hipLaunchKernelGGL(f2_centre_kernel<TabC64>, grid, block, 0, ctx->stream, (const float2*)raw, n,
                          (const float*)nullptr, (float2*)d_F2);

@@ -286,10 +286,12 @@ enum { EMC_TABLE_C64 = 0, EMC_TABLE_F16 = 1 };
 
 /* Reference spectrum for one box size: the half spectrum F of the reference
  * zero-padded to 2n with its centre n/2 at the origin, stored centred as
- * texel pairs: d_F2 holds (2n+2) x (2n+1) x (n+2) entries of two texels of
- * table_format, entry [(kz + n)(2n+1) + (ky + n)](n+2) + kx = (F(kx, ky, kz),
- * F(kx, ky+1, kz)) for kx in [0, n+1], ky in [-n, n], kz in [-n, n+1]
- * (indices taken mod 2n; kx = n+1 by Hermitian symmetry). */
+ * texel quadruples: d_F2 holds (2n+1) x (2n+1) x (n+2) entries of four
+ * texels of table_format, entry [(kz + n)(2n+1) + (ky + n)](n+2) + kx =
+ * (F(kx, ky, kz), F(kx, ky+1, kz), F(kx, ky, kz+1), F(kx, ky+1, kz+1)) for
+ * kx in [0, n+1], ky and kz in [-n, n] (indices taken mod 2n; kx = n+1 by
+ * Hermitian symmetry), so entries kx and kx+1 hold a trilinear footprint
+ * contiguously. */
 emc_status emc_batch_reference_spectrum(emc_ctx* ctx, int n, const float* d_ref, int table_format,
                                         void* d_F2);
 
