@@ -4,11 +4,9 @@
 //
 // The reference binds the input to a 3D texture on every call (cudaArray
 // allocation + copy, mexXform3d.cu:231-279). Here the input stays where it is
-// in HBM and each output voxel gathers its 8 neighbours directly: volumes of
-// the alignment path (<= 128^3 FP32 = 8 MiB) stay resident in the 4 MiB-per-
-// XCD L2 / 256 MiB MALL, so the gathers hit cache and HBM sees each input
-// byte about once. Threads run along x in 64-lane wavefronts so stores are
-// fully coalesced.
+// in HBM; each block stages the input brick its 8x8x8 output tile maps to in
+// LDS with coalesced row loads and interpolates from there, so HBM sees each
+// input byte about once and the 8 neighbour reads of a sample are LDS reads.
 //
 // Geometry (index units, u = x - w/2 on the output window grid):
 //   fwd: p = M (u - t)        inv: p = M u + t
@@ -22,73 +20,174 @@
 
 namespace emc {
 
-__device__ inline float xform_voxel(const float* __restrict__ in, const XformArgs& a, int nsym,
-                                    const float* __restrict__ mats, float tx, float ty, float tz,
-                                    int x, int y, int z) {
+// The sample point of window voxel (x, y, z) under one matrix (row-major
+// m) and shift t, in input index coordinates.
+__device__ inline void xform_point(const XformArgs& a, const float* __restrict__ m, float tx, float ty, float tz,
+                                   int x, int y, int z, float& px, float& py, float& pz) {
   const float u0 = (float)x - (float)(a.wx / 2);
   const float v0 = (float)y - (float)(a.wy / 2);
   const float w0 = (float)z - (float)(a.wz / 2);
   const float cx = (float)(a.nx / 2), cy = (float)(a.ny / 2), cz = (float)(a.nz / 2);
-  float acc = 0.0f;
-  for (int s = 0; s < nsym; ++s) {
-    const float* m = mats + 9 * s;
-    float px, py, pz;
-    if (a.fwd) {
-      const float u = u0 - tx, v = v0 - ty, w = w0 - tz;
-      px = u * m[0] + v * m[1] + w * m[2];
-      py = u * m[3] + v * m[4] + w * m[5];
-      pz = u * m[6] + v * m[7] + w * m[8];
-    } else {
-      px = u0 * m[0] + v0 * m[1] + w0 * m[2] + tx;
-      py = u0 * m[3] + v0 * m[4] + w0 * m[5] + ty;
-      pz = u0 * m[6] + v0 * m[7] + w0 * m[8] + tz;
+  if (a.fwd) {
+    const float u = u0 - tx, v = v0 - ty, w = w0 - tz;
+    px = u * m[0] + v * m[1] + w * m[2] + cx;
+    py = u * m[3] + v * m[4] + w * m[5] + cy;
+    pz = u * m[6] + v * m[7] + w * m[8] + cz;
+  } else {
+    px = u0 * m[0] + v0 * m[1] + w0 * m[2] + tx + cx;
+    py = u0 * m[3] + v0 * m[4] + w0 * m[5] + ty + cy;
+    pz = u0 * m[6] + v0 * m[7] + w0 * m[8] + tz + cz;
+  }
+}
+
+// LDS-staged trilinear rotation. A block owns an 8x8x8 tile of the output
+// grid (2 voxels per thread). Per symmetry mate it stages the input box
+// that the tile's window part maps to: the map is affine, so the sample
+// points of the tile lie inside the box spanned by its 8 corners, at most
+// 7 sqrt(3) + 2 < 16 voxels on a side for a rotation. The box is read with
+// coalesced row loads (zeros outside the volume: the border addressing of
+// the reference's texture) and the 8 neighbours of every sample come from
+// LDS, with the same arithmetic as a direct gather. A footprint outside the
+// staged box (a matrix that is not a rotation, or rounding at the box edge)
+// is gathered from global memory instead, so any matrix is exact.
+constexpr int kTile = 8;
+constexpr int kBox = 16;
+
+__device__ inline float lerp_corners(float c000, float c001, float c010, float c011, float c100, float c101,
+                                     float c110, float c111, float fx, float fy, float fz) {
+  const float c00 = c000 + fx * (c001 - c000);
+  const float c01 = c010 + fx * (c011 - c010);
+  const float c10 = c100 + fx * (c101 - c100);
+  const float c11 = c110 + fx * (c111 - c110);
+  const float c0 = c00 + fy * (c01 - c00);
+  const float c1 = c10 + fy * (c11 - c10);
+  return c0 + fz * (c1 - c0);
+}
+
+__global__ void __launch_bounds__(256) xform3d_tile_kernel(const float* __restrict__ in, float* __restrict__ out,
+                                                           XformArgs a, const float* __restrict__ d_mats,
+                                                           const float* __restrict__ d_shifts, int tiles_x,
+                                                           int tiles_y, int tiles_z) {
+  __shared__ float box[kBox * kBox * kBox];
+  const int b = blockIdx.y;  // batch entry (d_mats / d_shifts), 0 otherwise
+  // XCD-aware: workgroups are dealt round-robin over the 8 XCDs, so XCD i
+  // takes the i-th slab of consecutive tiles (z-major) and its L2 holds only
+  // the input region that slab maps to
+  const int ntiles = tiles_x * tiles_y * tiles_z;
+  const int per_xcd = (ntiles + 7) / 8;
+  const int tile = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (tile >= ntiles) return;  // uniform per block
+  const int X0 = (tile % tiles_x) * kTile, Y0 = ((tile / tiles_x) % tiles_y) * kTile,
+            Z0 = (tile / (tiles_x * tiles_y)) * kTile;
+  const int t = threadIdx.x;
+  const int lx = t % kTile, ly = (t / kTile) % kTile, lz = t / (kTile * kTile);  // lz in 0..3
+  float* ob = out + (int64_t)b * a.ox * a.oy * a.oz;
+  // the tile's part of the rotated window, in window coordinates
+  const int xlo = max(X0 - a.px, 0), xhi = min(X0 + kTile - 1 - a.px, a.wx - 1);
+  const int ylo = max(Y0 - a.py, 0), yhi = min(Y0 + kTile - 1 - a.py, a.wy - 1);
+  const int zlo = max(Z0 - a.pz, 0), zhi = min(Z0 + kTile - 1 - a.pz, a.wz - 1);
+  const bool any = xlo <= xhi && ylo <= yhi && zlo <= zhi;
+  float r[2] = {0.0f, 0.0f};
+  bool inwin[2];
+  int vx[2], vy[2], vz[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    vx[k] = X0 + lx - a.px;
+    vy[k] = Y0 + ly - a.py;
+    vz[k] = Z0 + lz + 4 * k - a.pz;
+    inwin[k] = (unsigned)vx[k] < (unsigned)a.wx && (unsigned)vy[k] < (unsigned)a.wy && (unsigned)vz[k] < (unsigned)a.wz;
+  }
+  if (any) {
+    float tx = a.t[0], ty = a.t[1], tz = a.t[2];
+    if (d_shifts) {
+      tx = d_shifts[3 * b];
+      ty = d_shifts[3 * b + 1];
+      tz = d_shifts[3 * b + 2];
     }
-    acc += sample_trilinear_zero(in, a.nx, a.ny, a.nz, px + cx, py + cy, pz + cz);
+    if (a.d_shift) {
+      tx += (float)a.d_shift[0];
+      ty += (float)a.d_shift[1];
+      tz += (float)a.d_shift[2];
+    }
+    const int nsym = d_mats ? 1 : a.nsym;
+    for (int s = 0; s < nsym; ++s) {
+      const float* m = d_mats ? d_mats + 9 * b : &a.m[s][0];
+      // box of the 8 corners (uniform over the block)
+      float mn[3] = {3.0e38f, 3.0e38f, 3.0e38f}, mx[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        float p[3];
+        xform_point(a, m, tx, ty, tz, (c & 1) ? xhi : xlo, (c & 2) ? yhi : ylo, (c & 4) ? zhi : zlo, p[0], p[1],
+                    p[2]);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          mn[i] = fminf(mn[i], p[i]);
+          mx[i] = fmaxf(mx[i], p[i]);
+        }
+      }
+      int lo[3], dim[3];
+      bool staged = true;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        staged = staged && mn[i] > -1.0e6f && mx[i] < 1.0e6f;
+        lo[i] = staged ? (int)floorf(mn[i]) : 0;
+        dim[i] = staged ? (int)floorf(mx[i]) + 2 - lo[i] : kBox + 1;
+        staged = staged && dim[i] <= kBox;
+      }
+      __syncthreads();  // the previous mate's box is no longer read
+      if (staged) {
+        // 16 lanes per box row, 16 rows per pass: every load of the thread is
+        // issued before the first LDS store, so the L2 round trips overlap
+        const int ix = t % kBox, gx = lo[0] + ix;
+        const bool xin = ix < dim[0] && (unsigned)gx < (unsigned)a.nx;
+        const int rows = dim[1] * dim[2];
+        float v[kBox];
+#pragma unroll
+        for (int j = 0; j < kBox; ++j) {
+          const int r = t / kBox + kBox * j;
+          const int iz = r / dim[1], iy = r - iz * dim[1];
+          const int gy = lo[1] + iy, gz = lo[2] + iz;
+          v[j] = 0.0f;
+          if (r < rows && xin && (unsigned)gy < (unsigned)a.ny && (unsigned)gz < (unsigned)a.nz)
+            v[j] = in[((int64_t)gz * a.ny + gy) * a.nx + gx];
+        }
+#pragma unroll
+        for (int j = 0; j < kBox; ++j) {
+          const int r = t / kBox + kBox * j;
+          const int iz = r / dim[1], iy = r - iz * dim[1];
+          if (r < rows) box[(iz * kBox + iy) * kBox + ix] = v[j];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        if (!inwin[k]) continue;
+        float px, py, pz;
+        xform_point(a, m, tx, ty, tz, vx[k], vy[k], vz[k], px, py, pz);
+        const float xf = floorf(px), yf = floorf(py), zf = floorf(pz);
+        const int ix = (int)xf - lo[0], iy = (int)yf - lo[1], iz = (int)zf - lo[2];
+        if (staged && xf > -1.0e6f && xf < 1.0e6f && yf > -1.0e6f && yf < 1.0e6f && zf > -1.0e6f && zf < 1.0e6f &&
+            ix >= 0 && iy >= 0 && iz >= 0 && ix + 1 < dim[0] && iy + 1 < dim[1] && iz + 1 < dim[2]) {
+          const float* q = box + (iz * kBox + iy) * kBox + ix;
+          r[k] += lerp_corners(q[0], q[1], q[kBox], q[kBox + 1], q[kBox * kBox], q[kBox * kBox + 1],
+                               q[kBox * kBox + kBox], q[kBox * kBox + kBox + 1], px - xf, py - yf, pz - zf);
+        } else {
+          r[k] += sample_trilinear_zero(in, a.nx, a.ny, a.nz, px, py, pz);
+        }
+      }
+    }
   }
-  return nsym > 1 ? acc * a.inv_nsym : acc;
-}
-
-__global__ void __launch_bounds__(256) xform3d_kernel(const float* __restrict__ in,
-                                                      float* __restrict__ out, XformArgs a) {
-  const int X = blockIdx.x * blockDim.x + threadIdx.x;
-  const int Y = blockIdx.y * blockDim.y + threadIdx.y;
-  const int Z = blockIdx.z;
-  if (X >= a.ox || Y >= a.oy) return;
-  float tx = a.t[0], ty = a.t[1], tz = a.t[2];
-  if (a.d_shift) {
-    tx += (float)a.d_shift[0];
-    ty += (float)a.d_shift[1];
-    tz += (float)a.d_shift[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int X = X0 + lx, Y = Y0 + ly, Z = Z0 + lz + 4 * k;
+    if (X >= a.ox || Y >= a.oy || Z >= a.oz) continue;
+    float v = a.pad_value;
+    if (inwin[k]) {
+      v = (!d_mats && a.nsym > 1) ? r[k] * a.inv_nsym : r[k];
+      if (a.d_mul) v *= a.d_mul[((int64_t)vz[k] * a.wy + vy[k]) * a.wx + vx[k]];
+    }
+    ob[((int64_t)Z * a.oy + Y) * a.ox + X] = v;
   }
-  const int x = X - a.px, y = Y - a.py, z = Z - a.pz;
-  float r = a.pad_value;
-  if ((unsigned)x < (unsigned)a.wx && (unsigned)y < (unsigned)a.wy && (unsigned)z < (unsigned)a.wz) {
-    r = xform_voxel(in, a, a.nsym, &a.m[0][0], tx, ty, tz, x, y, z);
-    if (a.d_mul) r *= a.d_mul[((int64_t)z * a.wy + y) * a.wx + x];
-  }
-  out[((int64_t)Z * a.oy + Y) * a.ox + X] = r;
-}
-
-// Batched form: orientation b uses the row-major matrix d_mats[9*b] and the
-// shift d_shifts[3*b] (nsym == 1); output volume b at out + b*ox*oy*oz.
-__global__ void __launch_bounds__(256) xform3d_batched_kernel(const float* __restrict__ in,
-                                                              float* __restrict__ out,
-                                                              const float* __restrict__ d_mats,
-                                                              const float* __restrict__ d_shifts,
-                                                              XformArgs a) {
-  const int X = blockIdx.x * blockDim.x + threadIdx.x;
-  const int Y = blockIdx.y * blockDim.y + threadIdx.y;
-  const int Z = blockIdx.z % a.oz;
-  const int b = blockIdx.z / a.oz;
-  if (X >= a.ox || Y >= a.oy) return;
-  const int x = X - a.px, y = Y - a.py, z = Z - a.pz;
-  float r = a.pad_value;
-  if ((unsigned)x < (unsigned)a.wx && (unsigned)y < (unsigned)a.wy && (unsigned)z < (unsigned)a.wz) {
-    r = xform_voxel(in, a, 1, d_mats + 9 * b, d_shifts[3 * b], d_shifts[3 * b + 1],
-                    d_shifts[3 * b + 2], x, y, z);
-    if (a.d_mul) r *= a.d_mul[((int64_t)z * a.wy + y) * a.wx + x];
-  }
-  out[(((int64_t)b * a.oz + Z) * a.oy + Y) * a.ox + X] = r;
 }
 
 XformArgs xform_args(const int in_dims[3], const float* Rrow, int nsym, const float t[3], int fwd) {
@@ -108,21 +207,24 @@ XformArgs xform_args(const int in_dims[3], const float* Rrow, int nsym, const fl
   return a;
 }
 
-void launch_xform(emc_ctx* ctx, const float* in, float* out, const XformArgs& a) {
-  dim3 block(64, 4, 1);
-  dim3 grid(ceil_div(a.ox, 64), ceil_div(a.oy, 4), a.oz);
-  KTimer kt(ctx, "xform3d");
-  hipLaunchKernelGGL(xform3d_kernel, grid, block, 0, ctx->stream, in, out, a);
+static void launch_tiles(emc_ctx* ctx, const float* in, float* out, const XformArgs& a, const float* d_mats,
+                         const float* d_shifts, int batch) {
+  const int tx = ceil_div(a.ox, kTile), ty = ceil_div(a.oy, kTile), tz = ceil_div(a.oz, kTile);
+  EMC_REQUIRE((int64_t)tx * ty * tz < ((int64_t)1 << 31) && batch <= 65535, "rotation grid too large");
+  const int64_t ntiles = (int64_t)tx * ty * tz;
+  hipLaunchKernelGGL(xform3d_tile_kernel, dim3((unsigned)(8 * ((ntiles + 7) / 8)), batch), dim3(256), 0, ctx->stream,
+                     in, out, a, d_mats, d_shifts, tx, ty, tz);
   EMC_HIP(hipGetLastError());
+}
+
+void launch_xform(emc_ctx* ctx, const float* in, float* out, const XformArgs& a) {
+  KTimer kt(ctx, "xform3d");
+  launch_tiles(ctx, in, out, a, nullptr, nullptr, 1);
 }
 
 void launch_xform_batched(emc_ctx* ctx, const float* in, float* out, const float* d_mats,
                           const float* d_shifts, int batch, const XformArgs& a) {
-  dim3 block(64, 4, 1);
-  dim3 grid(ceil_div(a.ox, 64), ceil_div(a.oy, 4), a.oz * batch);
-  hipLaunchKernelGGL(xform3d_batched_kernel, grid, block, 0, ctx->stream, in, out, d_mats, d_shifts,
-                     a);
-  EMC_HIP(hipGetLastError());
+  launch_tiles(ctx, in, out, a, d_mats, d_shifts, batch);
 }
 
 // column-major MATLAB 9-vector (double) -> row-major float rows, as the
