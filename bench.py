@@ -133,40 +133,37 @@ class BatchedRun:
 
 
 def cpu_baseline(budget_s, classes, particles, host, wedges):
-    """FP64 oracle (CPU restatement of the reference's two-stage body) on a
-    bounded sample: orientations per size class within the time budget,
-    extrapolated to the 1,045 + refinement orientations a particle needs."""
-    from oracle import emc_oracle as O
+    """The oracle's FP32 C restatement of the reference's two-stage plain
+    alignment body (oracle/emc_cpu.c, OpenMP over orientations, all the
+    threads OpenMP is given) on a bounded sample: batches of orientations per
+    size class within the time budget, extrapolated to the 1,045 + 45 + 1
+    orientations (local grid, refinement grid, final) a particle needs."""
+    from oracle import emc_cpu
+    threads = emc_cpu.max_threads()
     per_orient, counts = {}, {}
     t_all = time.perf_counter()
     for n in (64, 96, 128):
         i = next(k for k, q in enumerate(particles) if q.n == n)
         p, sc = particles[i], classes[n]
-        inp = O.AlignInputs(particle=host[i].astype(np.float64), ref=sc.ref.astype(np.float64),
-                            ref_wgt_rot=None, ref_wgt=1.0,
-                            ref_ft1=O.ref_ft1_of(sc.ref.astype(np.float64), sc.vol_mask, [[0] * 3, [0] * 3]),
-                            vol_mask=sc.vol_mask.astype(np.float64), peak_mask=sc.peak_mask.astype(np.float64),
-                            bandpass=sc.bandpass.astype(np.float64), wedge=wedges[i].get().astype(np.float64),
-                            angles=p.start, shift_val=np.zeros(3), pad_calc=[[0] * 3, [0] * 3])
-        k = 0
-        t0 = time.perf_counter()
         share = budget_s * {64: 0.2, 96: 0.3, 128: 0.5}[n]
-        while True:
-            R = p.start @ O.define_matrix([3.0 * k, 3.0, -3.0 * k], "Bah", "inv")
-            O.align_orientation(inp, R, np.zeros(3))
-            k += 1
-            if time.perf_counter() - t0 > share:
-                break
-        per_orient[n] = (time.perf_counter() - t0) / k
+        k, elapsed = 0, 0.0
+        while elapsed < share:
+            pts = np.array([[3.0 * (k + j), 3.0, -3.0 * (k + j)] for j in range(2 * threads)])
+            t0 = time.perf_counter()
+            emc_cpu.align_grid(host[i], sc.ref, sc.vol_mask, sc.peak_mask, sc.bandpass, wedges[i].get(), p.start,
+                               pts)
+            elapsed += time.perf_counter() - t0
+            k += len(pts)
+        per_orient[n] = elapsed / k
         counts[n] = k
     n_orient = 1045 + 45 + 1
     mix = {64: 0.5, 96: 0.3, 128: 0.2}
     sec = sum(mix[n] * per_orient[n] * n_orient for n in mix)
-    return {"value": round(1.0 / sec, 6), "unit": "alignments/s", "cores": 1, "kind": "port",
-            "sample": (f"FP64 numpy oracle of the reference's two-stage alignRaw body: "
-                       f"{counts[64]}/{counts[96]}/{counts[128]} orientations at 64/96/128^3 "
-                       f"({time.perf_counter() - t_all:.1f} s), extrapolated to {n_orient} orientations "
-                       f"per particle, 50/30/20 mix; single thread")}
+    return {"value": round(1.0 / sec, 6), "unit": "alignments/s", "cores": threads, "kind": "port",
+            "sample": (f"FP32 C restatement (oracle/emc_cpu.c) of the reference's two-stage alignRaw body, "
+                       f"OpenMP over orientations on {threads} threads: {counts[64]}/{counts[96]}/{counts[128]} "
+                       f"orientations at 64/96/128^3 ({time.perf_counter() - t_all:.1f} s), extrapolated to "
+                       f"{n_orient} orientations per particle, 50/30/20 mix")}
 
 
 COUNTERS = os.path.join(ROOT, "profiles", "align_fused_counters.md")
