@@ -33,6 +33,8 @@ SIGNATURES = {
     "emc_timing_enable": (_st, [_vp, C.c_int]),
     "emc_timing_reset": (_st, [_vp]),
     "emc_timing_read": (_st, [_vp, C.c_char_p, _d, C.POINTER(C.c_int64)]),
+    "emc_tune_set": (_st, [_vp, C.c_char_p, C.c_int, C.c_int]),
+    "emc_tune_get": (_st, [_vp, C.c_char_p, C.c_int, C.POINTER(C.c_int)]),
     "emc_malloc": (_st, [_vp, C.c_size_t, C.POINTER(_vp)]),
     "emc_free": (_st, [_vp, _vp]),
     "emc_memcpy_h2d": (_st, [_vp, _vp, _vp, C.c_size_t]),

@@ -1,0 +1,136 @@
+"""Autotuner of the batched search's launch shapes.
+
+Two parameters shape the batched path's hot launches (emc_tune_set):
+  search_tile  kx extent of the (kx, ky) column patches of the search
+               schedule (ky extent 256 / search_tile): the shape of the
+               reference-table region the units of one pass read together
+  grad_blocks  workgroups an adjoint evaluation aims for: how finely each
+               particle's support is split (fill against fold work)
+
+``python -m emclarity_amd.autotune [--out PATH]`` times every candidate on
+the device with HIP events on the library's stream, for each box size of the
+alignment workload (64/96/128^3 with 50/30/20 particles, 114 coarse
+orientations) and both table formats, and writes the fastest to a JSON
+table. The table measured on MI355X ships as ``tuning/mi355x.json``;
+BatchAligner applies its entries, and falls back to the library defaults
+(16 x 16 patches, 4096) for box sizes it does not list.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+from . import _lib
+
+CANDIDATES = {"search_tile": [4, 8, 16, 32, 64], "grad_blocks": [1024, 2048, 4096, 8192, 16384]}
+TABLE = Path(__file__).resolve().parent / "tuning" / "mi355x.json"
+_CACHE: dict | None = None
+
+
+def load_table(path=None) -> dict:
+    global _CACHE
+    if path is None and _CACHE is not None:
+        return _CACHE
+    p = Path(path) if path else TABLE
+    try:
+        with open(p) as f:
+            t = json.load(f)
+    except OSError:
+        t = {}
+    if path is None:
+        _CACHE = t
+    return t
+
+
+def params_for(n: int, table: str, tuning: dict | None = None) -> dict:
+    """The tuned parameters for box size n and a table format ({} if none)."""
+    t = tuning if tuning is not None else load_table()
+    return dict(t.get("params", {}).get(table, {}).get(str(int(n)), {}))
+
+
+def apply(ctx, n: int, params: dict):
+    for name, value in params.items():
+        _lib.call("emc_tune_set", ctx.handle, name.encode(), int(n), int(value))
+
+
+def _timer(ctx, names):
+    total = 0.0
+    for name in names:
+        ms, cnt = C.c_double(), C.c_int64()
+        _lib.call("emc_timing_read", ctx.handle, name.encode(), C.byref(ms), C.byref(cnt))
+        total += ms.value
+    return total
+
+
+def _timed(ctx, fn, names, reps):
+    fn()                                      # warm-up: schedules, scratch
+    ctx.synchronize()
+    _lib.call("emc_timing_reset", ctx.handle)
+    _lib.call("emc_timing_enable", ctx.handle, 1)
+    for _ in range(reps):
+        fn()
+    ctx.synchronize()
+    ms = _timer(ctx, names) / reps
+    _lib.call("emc_timing_enable", ctx.handle, 0)
+    return ms
+
+
+def tune(ctx, counts=((64, 50), (96, 30), (128, 20)), tables=("c64", "f16"), reps=3, seed=7, log=sys.stderr):
+    """Time every candidate; returns the table (params + all timings, ms)."""
+    from .batched import BatchAligner
+    from .geometry import eulerSearch
+    from .workload import make_particle, make_size_class
+    grid = eulerSearch("C1", 15, 6, 15, 6)
+    out = {"device": "gfx950", "workload": "batched alignment, 114 coarse orientations, SNR 0.1",
+           "candidates": CANDIDATES, "params": {}, "timings_ms": {}}
+    rng = np.random.default_rng(seed)
+    for n, count in counts:
+        sc, blobs = make_size_class(n)
+        parts = [make_particle(rng, sc, blobs) for _ in range(count)]
+        for table in tables:
+            al = BatchAligner(n, sc.ref, sc.bandpass, table=table, ctx=ctx, tuning={})
+            al.set_particles([p.data for p in parts], [None] * count, [p.start for p in parts])
+            R = np.stack([p.start for p in parts])
+            s = np.zeros((count, 3))
+            best, times = {}, {}
+            for name in ("search_tile", "grad_blocks"):
+                times[name] = {}
+                for v in CANDIDATES[name]:
+                    apply(ctx, n, {name: v})
+                    if name == "search_tile":
+                        ms = _timed(ctx, lambda: al.search(grid), ("align_fused", "align_final"), reps)
+                    else:
+                        ms = _timed(ctx, lambda: al._eval_local(R, s), ("align_eval_grad",), reps)
+                    times[name][str(v)] = round(ms, 4)
+                    print(f"[autotune] n={n} {table} {name}={v}: {ms:.3f} ms", file=log, flush=True)
+                best[name] = int(min(CANDIDATES[name], key=lambda v: times[name][str(v)]))
+                apply(ctx, n, {name: best[name]})
+            out["params"].setdefault(table, {})[str(n)] = best
+            out["timings_ms"].setdefault(table, {})[str(n)] = times
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description="time the batched path's launch shapes and write the table")
+    ap.add_argument("--out", default=str(TABLE))
+    ap.add_argument("--reps", type=int, default=3)
+    a = ap.parse_args(argv)
+    from .device import default_context
+    t0 = time.perf_counter()
+    res = tune(default_context(), reps=a.reps)
+    res["seconds"] = round(time.perf_counter() - t0, 1)
+    os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
+    with open(a.out, "w") as f:
+        json.dump(res, f, indent=1)
+    print(json.dumps(res["params"]))
+
+
+if __name__ == "__main__":
+    main()

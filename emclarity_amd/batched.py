@@ -84,9 +84,13 @@ class BatchAligner:
     reference table format: "c64" (complex64) or "f16" (half-precision
     parts, half the gather bytes)."""
 
-    def __init__(self, n, ref, bandpass, support_radius=None, peak_radius=6, ctx=None, table="c64"):
+    def __init__(self, n, ref, bandpass, support_radius=None, peak_radius=6, ctx=None, table="c64", tuning=None):
         self.ctx = ctx or default_context()
         self.n = int(n)
+        # launch shapes from the autotuner's table (tuning None: the shipped
+        # table; {}: the library defaults)
+        from .autotune import params_for
+        self._tune = params_for(self.n, table, tuning)
         self.peak_radius = int(peak_radius)
         if table not in TABLE_FORMATS:
             raise ValueError(f"table must be one of {sorted(TABLE_FORMATS)}")
@@ -134,8 +138,14 @@ class BatchAligner:
                   self.Q.vp, self.N.vp, self.A.vp)
 
     # ------------------------------------------------------------ search
+    def _apply_tuning(self):
+        if self._tune:
+            from .autotune import apply
+            apply(self.ctx, self.n, self._tune)
+
     def search(self, angles_pts):
         """Scores (P, O) and shifts (P, O, 3) of every (particle, orientation)."""
+        self._apply_tuning()
         pts = np.ascontiguousarray(np.asarray(angles_pts, dtype=np.float64).reshape(-1, 3))
         O = pts.shape[0]
         score = DeviceArray((self.n_part, O), np.float32, self.ctx)
@@ -149,6 +159,7 @@ class BatchAligner:
     def eval_grad(self, R, dR, s):
         """CC and gradient at per-particle poses: R (P,3,3), dR (P,3,3,3)
         derivatives w.r.t. three rotation parameters, s (P,3)."""
+        self._apply_tuning()
         P = self.n_part
         h = self._h_in
         hm = h[:36 * P].reshape(P, 36)
@@ -173,6 +184,7 @@ class BatchAligner:
         """Batched quasi-Newton ascent of the CC over (rotation increment,
         shift) for every particle, on the device (emc_align_refine: iters + 1
         adjoint evaluations, one result copy). Returns (R, s, cc)."""
+        self._apply_tuning()
         P = self.n_part
         R = DeviceArray.from_numpy(np.ascontiguousarray(np.asarray(R0, dtype=np.float64).reshape(P, 9)),
                                    ctx=self.ctx)

@@ -761,20 +761,22 @@ void launch_fused_r(emc_ctx* ctx, int peak_radius, const void* F2, int n, const 
 // column count, first piece, piece count), pieces (int4: kx, first slot, end
 // slot, 0), columns]. A column is a (kx, ky) with kx^2 + ky^2 <= r^2, packed
 // kx | (ky + 128) << 8 | h << 16, h = max |kz| with the frequency inside the
-// sphere (the same float test as the kernel). Columns are tiled in 16 x 16
-// (kx, ky) patches, consecutive patches merged while they fit one
+// sphere (the same float test as the kernel). Columns are tiled in (kx, ky)
+// patches of "search_tile" x (256 / "search_tile") columns (16 x 16 by
+// default), consecutive patches merged while they fit one
 // workgroup: the units running together on an XCD (nearby orientations)
 // then read a compact region of the reference table per pass.
 const int* support_schedule(emc_ctx* ctx, int n, float kmax2, int* ngroups) {
-  char key[64];
-  snprintf(key, sizeof(key), "sched_%d_%.6f", n, (double)kmax2);
+  // patches of tx (kx) x ty (ky) columns, tx * ty = one workgroup's columns
+  const int tx = tune_value(ctx, "search_tile", n, 16), ty = kBatchThreads / tx;
+  char key[80];
+  snprintf(key, sizeof(key), "sched_%d_%.6f_t%d", n, (double)kmax2, tx);
   auto it = ctx->scratch.find(key);
   auto mt = ctx->meta.find(key);
   if (it != ctx->scratch.end() && it->second.ptr && mt != ctx->meta.end()) {
     *ngroups = mt->second;
     return (const int*)it->second.ptr;
   }
-  constexpr int kTile = 16;
   std::map<std::pair<int, int>, std::vector<int>> tiles;  // ordered by (kx tile, ky tile)
   for (int kx = 0; kx <= n / 2; ++kx) {
     if ((float)(kx * kx) > kmax2) break;
@@ -782,7 +784,7 @@ const int* support_schedule(emc_ctx* ctx, int n, float kmax2, int* ngroups) {
       if ((float)(kx * kx + ky * ky) > kmax2) continue;
       int h = 0;
       while (h + 1 <= n / 2 - 1 && (float)(kx * kx + ky * ky + (h + 1) * (h + 1)) <= kmax2) ++h;
-      tiles[{kx / kTile, (ky + 128) / kTile}].push_back(kx | ((ky + 128) << 8) | (h << 16));
+      tiles[{kx / tx, (ky + 128) / ty}].push_back(kx | ((ky + 128) << 8) | (h << 16));
     }
   }
   std::vector<int> cols, groups, pieces;
@@ -832,8 +834,10 @@ void eval_grad_impl(emc_ctx* ctx, int n, int n_part, int table_format, const voi
   const int2* supp = support_list(ctx, n, support_radius, &ns);
   EMC_REQUIRE(ns > 0, "empty support");
   // split every particle's support over enough workgroups to fill the
-  // device (~16 per CU over the whole batch), at least 256 frequencies each
-  int g = std::max(1, std::min((ns + 255) / 256, (4096 + n_part - 1) / n_part));
+  // device ("grad_blocks" over the whole batch, 4096 = ~16 per CU by
+  // default), at least 256 frequencies each
+  const int target = tune_value(ctx, "grad_blocks", n, 4096);
+  int g = std::max(1, std::min((ns + 255) / 256, (target + n_part - 1) / n_part));
   double* part = (double*)scratch(ctx, "grad_part", sizeof(double) * kGradTerms * (size_t)g * n_part);
   {
     KTimer kt(ctx, "align_eval_grad");

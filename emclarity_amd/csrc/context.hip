@@ -253,6 +253,27 @@ void release_cache(emc_ctx* ctx) {
 
 using namespace emc;
 
+namespace {
+struct TuneRange {
+  const char* name;
+  int lo, hi, dflt;
+};
+constexpr TuneRange kTune[] = {{"search_tile", 2, 128, 16}, {"grad_blocks", 256, 65536, 4096}};
+
+const TuneRange* find_tune(const char* name) {
+  for (const auto& t : kTune)
+    if (name && std::strcmp(name, t.name) == 0) return &t;
+  return nullptr;
+}
+}  // namespace
+
+int emc::tune_value(emc_ctx* ctx, const char* name, int n, int dflt) {
+  auto it = ctx->meta.find(std::string("tune:") + name + ":" + std::to_string(n));
+  if (it != ctx->meta.end()) return it->second;
+  it = ctx->meta.find(std::string("tune:") + name + ":0");
+  return it != ctx->meta.end() ? it->second : dflt;
+}
+
 extern "C" {
 
 const char* emc_last_error(void) { return g_last_error.c_str(); }
@@ -310,6 +331,26 @@ emc_status emc_ctx_synchronize(emc_ctx* ctx) {
 }
 
 void* emc_ctx_stream(emc_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+emc_status emc_tune_set(emc_ctx* ctx, const char* name, int box, int value) {
+  return guard([&] {
+    EMC_REQUIRE(ctx, "null context");
+    const TuneRange* t = find_tune(name);
+    EMC_REQUIRE(t, "unknown launch parameter '%s'", name ? name : "(null)");
+    EMC_REQUIRE(box >= 0 && box <= 4096, "box size %d out of range", box);
+    EMC_REQUIRE(value >= t->lo && value <= t->hi, "%s must be %d..%d", t->name, t->lo, t->hi);
+    ctx->meta[std::string("tune:") + t->name + ":" + std::to_string(box)] = value;
+  });
+}
+
+emc_status emc_tune_get(emc_ctx* ctx, const char* name, int box, int* value) {
+  return guard([&] {
+    EMC_REQUIRE(ctx && value, "null pointer");
+    const TuneRange* t = find_tune(name);
+    EMC_REQUIRE(t, "unknown launch parameter '%s'", name ? name : "(null)");
+    *value = tune_value(ctx, t->name, box, t->dflt);
+  });
+}
 
 emc_status emc_timing_enable(emc_ctx* ctx, int on) {
   return guard([&] {

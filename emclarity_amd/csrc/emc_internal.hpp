@@ -119,6 +119,8 @@ namespace emc {
 
 // Make `ctx`'s device current for the calling thread.
 void use_device(emc_ctx* ctx);
+// Launch-shape parameter `name` for box size n (emc_tune_set), else `dflt`.
+int tune_value(emc_ctx* ctx, const char* name, int n, int dflt);
 // Scratch buffer of at least `bytes` under `name` (contents undefined).
 void* scratch(emc_ctx* ctx, const char* name, size_t bytes);
 // Pinned host staging of at least `bytes`.

@@ -70,6 +70,18 @@ emc_status emc_timing_enable(emc_ctx* ctx, int on);
 emc_status emc_timing_reset(emc_ctx* ctx);
 emc_status emc_timing_read(emc_ctx* ctx, const char* name, double* ms_total, int64_t* launches);
 
+/* Launch-shape parameters of the batched search, per context (no
+ * counterpart in the reference: they are what the autotuner sets).
+ *   "search_tile"  kx extent of the (kx, ky) column patches of the search
+ *                  schedule, whose ky extent is 256 / search_tile (one
+ *                  workgroup's columns), 2..128 (default 16: 16 x 16)
+ *   "grad_blocks"  workgroups an adjoint evaluation aims for, 256..65536
+ *                  (default 4096)
+ * box = 0 sets the value for every box size, else for that box size only
+ * (which wins). emc_tune_get returns the value in effect for `box`. */
+emc_status emc_tune_set(emc_ctx* ctx, const char* name, int box, int value);
+emc_status emc_tune_get(emc_ctx* ctx, const char* name, int box, int* value);
+
 /* Device memory helpers so a host without a GPU array type (C harness, MEX
  * gateway on AMD, ctypes) can drive the library. Copies are synchronous.
  * emc_free returns an emc_malloc block to the context's cache without
