@@ -5,6 +5,7 @@ batched-alignment and template-search workloads:
   average  100 particles of 128^3 with 128^3 wedges into the half-set sums
   peaks    greedy peak extraction on a 512 x 512 x 256 score map, 150 peaks
   tsprep   pre-processing of one 480^3 template-search chunk
+  tsearch  one 480^3 chunk against a 48^3 template, 32 orientations
 
     python -m emclarity_amd.bench_paths [--path all|extract|average|peaks|tsprep] [--reps N]
 
@@ -141,7 +142,34 @@ def bench_tsprep(ctx, reps):
     _line("tsprep", wall, {"chunk": s, "GB_in_out": round(2 * 4 * s ** 3 / 1e9, 3)})
 
 
-PATHS = {"extract": bench_extract, "average": bench_average, "peaks": bench_peaks, "tsprep": bench_tsprep}
+def bench_tsearch(ctx, reps):
+    from .geometry import eulerSearch
+    rng = np.random.default_rng(5)
+    s, t, n_orient = 480, 48, 32
+    chunk = DeviceArray.from_numpy(rng.standard_normal((s, s, s), dtype=np.float32), ctx=ctx)
+    tmpl = DeviceArray.from_numpy(rng.standard_normal((t, t, t), dtype=np.float32), ctx=ctx)
+    pts = np.ascontiguousarray(eulerSearch("C1", 180, 15, 180, 15)[:n_orient])
+    vlo, vd = 24, s - 48
+    peak = DeviceArray.zeros((vd, vd, vd), np.float32, ctx)
+    ang = DeviceArray.zeros((vd, vd, vd), np.int32, ctx)
+    pl = (s - t) // 2
+
+    def run():
+        _lib.call("emc_template_search_chunk", ctx.handle, chunk.vp, _lib.i3((s, s, s)), tmpl.vp, _lib.i3((t, t, t)),
+                  _lib.i3((pl, pl, pl)), _lib.i3((vlo, vlo, vlo)), _lib.i3((vd, vd, vd)),
+                  pts.ctypes.data_as(C.POINTER(C.c_double)), n_orient, 1, 1, peak.vp, ang.vp)
+
+    wall = _timed(ctx, run, reps)
+    hc = (s // 2 + 1) * s * s
+    b_ts = 8 * t ** 3 + 12 * s ** 3 + 40 * hc + 24 * vd ** 3      # per (chunk, orientation)
+    per = wall / n_orient
+    _line("tsearch", wall, {"chunk": s, "template": t, "orientations": n_orient,
+                            "ms_per_orientation": round(per * 1e3, 4), "bytes_per_orientation": b_ts,
+                            "GBps": round(b_ts / per / 1e9, 1)})
+
+
+PATHS = {"extract": bench_extract, "average": bench_average, "peaks": bench_peaks, "tsprep": bench_tsprep,
+         "tsearch": bench_tsearch}
 
 
 def main(argv=None):

@@ -58,22 +58,26 @@ __global__ void conj_mul_batch_kernel(const float2* __restrict__ T0, float2* __r
   }
 }
 
-// per-map sums over the valid area (partials[b][block][2])
+// per-map sums over the valid area (partials[b][block][2]): blocks stride
+// over the rows (y, z) of the valid area, lanes run along x (32-bit row
+// index, no per-element division); a fixed grid keeps the sums deterministic
+constexpr int kValidBlocks = 1024;
+
 __global__ void __launch_bounds__(kRedThreads) valid_sums_kernel(const float* __restrict__ ccf,
                                                                  int nx, int ny, int nz, int vx0,
                                                                  int vy0, int vz0, int vx, int vy,
                                                                  int vz, double* __restrict__ part) {
-  const int64_t nv = (int64_t)vx * vy * vz;
   const float* m = ccf + (int64_t)blockIdx.y * nx * ny * nz;
   double v[2] = {0.0, 0.0};
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int x = (int)(i % vx);
-    const int64_t yz = i / vx;
-    const int y = (int)(yz % vy), z = (int)(yz / vy);
-    const double a = m[((int64_t)(z + vz0) * ny + (y + vy0)) * nx + (x + vx0)];
-    v[0] += a;
-    v[1] += a * a;
+  const int rows = vy * vz;
+  for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+    const int z = r / vy, y = r - z * vy;
+    const float* row = m + ((int64_t)(z + vz0) * ny + (y + vy0)) * nx + vx0;
+    for (int x = threadIdx.x; x < vx; x += blockDim.x) {
+      const double a = row[x];
+      v[0] += a;
+      v[1] += a * a;
+    }
   }
   block_sum<2>(v);
   if (threadIdx.x == 0) {
@@ -102,7 +106,8 @@ __global__ void __launch_bounds__(kRedThreads) std_fold_kernel(const double* __r
   }
 }
 
-// score/angle update over the valid area for a block of maps, in order
+// score/angle update over the valid area for a block of maps, in order;
+// blocks stride over rows (y, z), lanes along x
 __global__ void __launch_bounds__(256) mip_update_kernel(const float* __restrict__ ccf, int nb_maps,
                                                          int nx, int ny, int nz, int vx0, int vy0,
                                                          int vz0, int vx, int vy, int vz,
@@ -110,28 +115,28 @@ __global__ void __launch_bounds__(256) mip_update_kernel(const float* __restrict
                                                          float* __restrict__ peak,
                                                          int32_t* __restrict__ angle, int first_id,
                                                          int init_first) {
-  const int64_t nv = (int64_t)vx * vy * vz;
   const int64_t nc = (int64_t)nx * ny * nz;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int x = (int)(i % vx);
-    const int64_t yz = i / vx;
-    const int y = (int)(yz % vy), z = (int)(yz / vy);
-    const int64_t src = ((int64_t)(z + vz0) * ny + (y + vy0)) * nx + (x + vx0);
-    float pk = peak[i];
-    int32_t an = angle[i];
-    for (int b = 0; b < nb_maps; ++b) {
-      const float v = ccf[b * nc + src] / stdv[b];
-      if (b == 0 && init_first) {  // magTmp = ccfmap; angTmp = 1 (:802-823)
-        pk = v;
-        an = 1;
-      } else if (pk < v) {  // strict < (:851-854)
-        pk = v;
-        an = first_id + b;
+  const int rows = vy * vz;
+  for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+    const int z = r / vy, y = r - z * vy;
+    const int64_t src0 = ((int64_t)(z + vz0) * ny + (y + vy0)) * nx + vx0;
+    const int64_t dst0 = (int64_t)r * vx;
+    for (int x = threadIdx.x; x < vx; x += blockDim.x) {
+      float pk = peak[dst0 + x];
+      int32_t an = angle[dst0 + x];
+      for (int b = 0; b < nb_maps; ++b) {
+        const float v = ccf[b * nc + src0 + x] / stdv[b];
+        if (b == 0 && init_first) {  // magTmp = ccfmap; angTmp = 1 (:802-823)
+          pk = v;
+          an = 1;
+        } else if (pk < v) {  // strict < (:851-854)
+          pk = v;
+          an = first_id + b;
+        }
       }
+      peak[dst0 + x] = pk;
+      angle[dst0 + x] = an;
     }
-    peak[i] = pk;
-    angle[i] = an;
   }
 }
 
@@ -171,7 +176,7 @@ emc_status emc_template_search_chunk(emc_ctx* ctx, const float* d_chunk, const i
     float* tpad = (float*)scratch(ctx, "ts_tpad", sizeof(float) * nc * blk);
     float2* T = (float2*)scratch(ctx, "ts_T", sizeof(float2) * nh * blk);
     float* mats = (float*)scratch(ctx, "ts_mats", sizeof(float) * 12 * blk);
-    double* part = (double*)scratch(ctx, "ts_part", sizeof(double) * 2 * 256 * blk);
+    double* part = (double*)scratch(ctx, "ts_part", sizeof(double) * 2 * kValidBlocks * blk);
     float* stdv = (float*)scratch(ctx, "ts_std", sizeof(float) * blk);
 
     // tomoFou = swapQuadrants .* R2C(chunk); copy first: the FFT input is not
@@ -214,12 +219,12 @@ emc_status emc_template_search_chunk(emc_ctx* ctx, const float* d_chunk, const i
                          (const float2*)tomo, T, nh);
       EMC_HIP(hipGetLastError());
       fft_c2r(p, T, tpad);  // ccfmap (:759-761), C2R reuses the rotate buffer
-      hipLaunchKernelGGL(valid_sums_kernel, dim3(256, nb), dim3(kRedThreads), 0, ctx->stream,
+      hipLaunchKernelGGL(valid_sums_kernel, dim3(kValidBlocks, nb), dim3(kRedThreads), 0, ctx->stream,
                          (const float*)tpad, cd[0], cd[1], cd[2], valid_low[0], valid_low[1],
                          valid_low[2], vd[0], vd[1], vd[2], part);
       hipLaunchKernelGGL(std_fold_kernel, dim3(nb), dim3(kRedThreads), 0, ctx->stream,
-                         (const double*)part, 256, (double)nvalid, stdv);
-      hipLaunchKernelGGL(mip_update_kernel, dim3(2048), dim3(256), 0, ctx->stream, (const float*)tpad,
+                         (const double*)part, kValidBlocks, (double)nvalid, stdv);
+      hipLaunchKernelGGL(mip_update_kernel, dim3(4096), dim3(256), 0, ctx->stream, (const float*)tpad,
                          nb, cd[0], cd[1], cd[2], valid_low[0], valid_low[1], valid_low[2], vd[0],
                          vd[1], vd[2], (const float*)stdv, d_peak, d_angle, first_angle_id + o0,
                          (init_first && o0 == 0) ? 1 : 0);
