@@ -41,9 +41,8 @@ __global__ void __launch_bounds__(256) xcorr_half_kernel(const float2* __restric
   const int64_t nh = (int64_t)cx * ny * nz;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nh;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const int x = (int)(i % cx);
-    const int64_t yz = i / cx;
-    const int y = (int)(yz % ny), z = (int)(yz / ny);
+    int x, y, z;
+    split3(i, cx, ny, x, y, z);
     const int64_t cp = ((int64_t)shifted_index(z, nz) * ny + shifted_index(y, ny)) * nx +
                        shifted_index(x, nx);
     const int mx = mirror_index(x, nx), my = mirror_index(y, ny), mz = mirror_index(z, nz);
@@ -78,9 +77,8 @@ __global__ void __launch_bounds__(kRedThreads) rot_sums_half_kernel(
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nh;
        i += (int64_t)gridDim.x * blockDim.x) {
     if (i == 0) continue;  // P(0) = 0
-    const int x = (int)(i % cx);
-    const int64_t yz = i / cx;
-    const int y = (int)(yz % ny), z = (int)(yz / ny);
+    int x, y, z;
+    split3(i, cx, ny, x, y, z);
     const bool self = (x == 0) || (nx % 2 == 0 && x == nx / 2);
     const int mx = mirror_index(x, nx), my = mirror_index(y, ny), mz = mirror_index(z, nz);
     const int64_t fp = ((int64_t)z * ny + y) * nx + x;      // full-grid FFT order

@@ -47,7 +47,8 @@ __global__ void __launch_bounds__(kAvgThreads) avg_rotate_kernel(
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   for (int64_t v = (int64_t)blockIdx.x * kAvgThreads + threadIdx.x; v < nv;
        v += (int64_t)gridDim.x * kAvgThreads) {
-    const int x = (int)(v % nx), y = (int)((v / nx) % ny), z = (int)(v / ((int64_t)nx * ny));
+    int x, y, z;
+    split3(v, nx, ny, x, y, z);
     const float u = (float)x - cx, w = (float)y - cy, q = (float)z - cz;
     float r = 0.0f;
     for (int s = 0; s < nsym; ++s) {

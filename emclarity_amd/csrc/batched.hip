@@ -105,9 +105,8 @@ __global__ void __launch_bounds__(kRedThreads) prep_particle_kernel(
   double v[1] = {0.0};
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nh;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const int x = (int)(i % cx);
-    const int64_t yz = i / cx;
-    const int y = (int)(yz % n), z = (int)(yz / n);
+    int x, y, z;
+    split3(i, cx, n, x, y, z);
     const float b = bp[((int64_t)z * n + y) * n + x];
     const int64_t cp = ((int64_t)shifted_index(z, n) * n + shifted_index(y, n)) * n + shifted_index(x, n);
     const int mx = mirror_index(x, n), my = mirror_index(y, n), mz = mirror_index(z, n);

@@ -260,6 +260,24 @@ __device__ __host__ inline uint32_t key_index(unsigned long long key) {
   return 0xFFFFFFFFu - (uint32_t)(key & 0xFFFFFFFFull);
 }
 
+// (x, y, z) of linear index i in an nx x ny x * grid (x fastest). The GPU has
+// no integer divider: a 64-bit division is a long emulated sequence, a
+// 32-bit one a short reciprocal-based one, so indices below 2^32 (every
+// volume here) take the 32-bit path.
+__device__ inline void split3(int64_t i, int nx, int ny, int& x, int& y, int& z) {
+  if ((uint64_t)i <= 0xFFFFFFFFull) {
+    const uint32_t u = (uint32_t)i, q = u / (uint32_t)nx, zz = q / (uint32_t)ny;
+    x = (int)(u - q * (uint32_t)nx);
+    y = (int)(q - zz * (uint32_t)ny);
+    z = (int)zz;
+  } else {
+    const int64_t q = i / nx;
+    x = (int)(i - q * nx);
+    z = (int)(q / ny);
+    y = (int)(q - (int64_t)z * ny);
+  }
+}
+
 // Trilinear sample at index coordinates (x,y,z), zero outside [0,n-1]
 // (texture border addressing of the reference's mexXform3d).
 __device__ inline float sample_trilinear_zero(const float* __restrict__ v, int nx, int ny, int nz,

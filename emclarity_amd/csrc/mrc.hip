@@ -245,9 +245,9 @@ __global__ void __launch_bounds__(256) extract_kernel(const void* __restrict__ t
   const int qx = (wx + 3) >> 2;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= (int64_t)qx * wy * wz) return;
-  const int X0 = (int)(q % qx) * 4;
-  const int Y = (int)((q / qx) % wy);
-  const int Z = (int)(q / ((int64_t)qx * wy));
+  int X0, Y, Z;
+  split3(q, qx, wy, X0, Y, Z);
+  X0 *= 4;
   const Window w = wins[p];
   const int y = Y - w.pad[1], z = Z - w.pad[2];
   const bool row_in = w.valid && (unsigned)y < (unsigned)w.ext[1] && (unsigned)z < (unsigned)w.ext[2];

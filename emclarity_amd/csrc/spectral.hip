@@ -135,8 +135,9 @@ __global__ void __launch_bounds__(kRedThreads) blcn_pass_kernel(float2* __restri
   double v[1] = {0.0};
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nh;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const int x = (int)(i % cx);
-    const int64_t yz = i / cx;
+    int x, y, z;
+    split3(i, cx, ny, x, y, z);
+    const int64_t yz = (int64_t)z * ny + y;
     float2 f = Fb[i];
     if (i == 0) f = make_float2(0.0f, 0.0f);
     if (bp) {

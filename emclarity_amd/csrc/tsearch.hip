@@ -29,9 +29,8 @@ __global__ void swapq_kernel(float2* __restrict__ F, int nx, int ny, int nz) {
   const int64_t nh = (int64_t)cx * ny * nz;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nh;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const int x = (int)(i % cx);
-    const int64_t yz = i / cx;
-    const int y = (int)(yz % ny), z = (int)(yz / ny);
+    int x, y, z;
+    split3(i, cx, ny, x, y, z);
     auto frac = [](int j, int n) {
       int64_t a = ((int64_t)(j - n / 2) * (n / 2 + 1)) % n;
       if (a < 0) a += n;
