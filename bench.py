@@ -110,23 +110,34 @@ class BatchedRun:
         # per size class: reference and bandpass resident in HBM, and the
         # bandpass's support radius (a property of the filter, made once)
         from emclarity_amd.batched import support_radius_of
-        from emclarity_amd.device import DeviceArray
+        from emclarity_amd.device import Context, DeviceArray
         self.resident = {n: (DeviceArray.from_numpy(classes[n].ref), DeviceArray.from_numpy(classes[n].bandpass),
                              support_radius_of(classes[n].bandpass, n)) for n in self.groups}
+        # one context (stream) per box size: the size classes' chains of
+        # table build, preparation, search, pick and refinement overlap
+        self.ctxs = {n: Context(0) for n in self.groups}
 
     def run(self):
-        from emclarity_amd.batched import BatchAligner, align_batch
+        from emclarity_amd.batched import BatchAligner, collect, refine_stage, search_stage
         R = [None] * len(self.particles)
         S = [None] * len(self.particles)
         self.units = {}
-        for n, idx in self.groups.items():
-            sc = self.classes[n]
+        aligners, handles = {}, {}
+        for n, idx in self.groups.items():      # tables and preparation, overlapped
             ref, bp, support = self.resident[n]
-            al = BatchAligner(n, ref, bp, support_radius=support, peak_radius=6, table=self.table)
+            al = BatchAligner(n, ref, bp, support_radius=support, peak_radius=6, table=self.table, ctx=self.ctxs[n])
             al.set_particles([self.particles[i].data for i in idx], [self.wedges[i] for i in idx],
                              [self.particles[i].start for i in idx])
-            r, s, _, _ = align_batch(al, self.grid, refine_iters=self.iters)
+            aligners[n] = al
             self.units[n] = (len(idx) * len(self.grid), al.support)
+        prev = None
+        for n in self.groups:                   # searches one after another (each fills the GPU)
+            handles[n] = search_stage(aligners[n], self.grid, after=prev)
+            prev = self.ctxs[n]
+        for n in self.groups:                   # refinements overlap the later searches and each other
+            refine_stage(handles[n], self.iters)
+        for n, idx in self.groups.items():
+            r, s, _, _ = collect(handles[n])
             for k, i in enumerate(idx):
                 R[i], S[i] = r[k], s[k]
         return R, S
@@ -185,27 +196,37 @@ def counter_traffic(table):
     return None
 
 
-def kernel_time(ctx, name):
+def kernel_time(ctxs, name):
+    """Total ms and launches of a timed kernel over the contexts' streams."""
     import ctypes as C
     from emclarity_amd import _lib
-    ms, cnt = C.c_double(), C.c_int64()
-    _lib.call("emc_timing_read", ctx.handle, name.encode(), C.byref(ms), C.byref(cnt))
-    return ms.value, cnt.value
+    total, n = 0.0, 0
+    for ctx in ctxs:
+        ms, cnt = C.c_double(), C.c_int64()
+        _lib.call("emc_timing_read", ctx.handle, name.encode(), C.byref(ms), C.byref(cnt))
+        total += ms.value
+        n += cnt.value
+    return total, n
 
 
-def timed(ctx, fn, steps, label):
+def timed(ctxs, fn, steps, label):
+    """K steps between synchronisations of every stream involved; HIP-event
+    kernel timing on each of them."""
     from emclarity_amd import _lib
-    _lib.call("emc_timing_reset", ctx.handle)
-    _lib.call("emc_timing_enable", ctx.handle, 1)
-    ctx.synchronize()
+    for ctx in ctxs:
+        _lib.call("emc_timing_reset", ctx.handle)
+        _lib.call("emc_timing_enable", ctx.handle, 1)
+        ctx.synchronize()
     t0 = time.perf_counter()
     out = None
     for s in range(steps):
         out = fn()
         print(f"[bench] {label} step {s + 1}/{steps} done", file=sys.stderr, flush=True)
-    ctx.synchronize()
+    for ctx in ctxs:
+        ctx.synchronize()
     dt = time.perf_counter() - t0
-    _lib.call("emc_timing_enable", ctx.handle, 0)
+    for ctx in ctxs:
+        _lib.call("emc_timing_enable", ctx.handle, 0)
     return out, dt
 
 
@@ -247,8 +268,9 @@ def main():
         run = BatchedRun(particles, classes, wedges, coarse, a.refine_iters, table)
         for w in range(a.warmup):
             run.run()
-        (Rb, Sb), dt = timed(ctx, run.run, a.steps, f"batched ({table} table)")
-        ms, cnt = kernel_time(ctx, "align_fused")
+        ctxs = [ctx] + list(run.ctxs.values())
+        (Rb, Sb), dt = timed(ctxs, run.run, a.steps, f"batched ({table} table)")
+        ms, cnt = kernel_time(ctxs, "align_fused")
         flops = sum(fused_flops_per_unit(n, sup, 6) * u for n, (u, sup) in run.units.items())
         samples = sum(fused_samples_per_unit(n, sup) * u for n, (u, sup) in run.units.items())
         return {"value": len(particles) * a.steps / dt, "dt": dt, "acc": accuracy(particles, Rb, Sb),
@@ -299,7 +321,7 @@ def main():
         pp = [particles[i] for i in sub]
         pr = PlainRun(pp, classes, [wedges[i] for i in sub], local_grid())
         pr.run()
-        rows, pdt = timed(ctx, pr.run, 1, "plain")
+        rows, pdt = timed([ctx], pr.run, 1, "plain")
         plain = {"value": round(len(pp) / pdt, 4), "unit": "alignments/s", "particles": len(pp),
                  "orientations": 1045, "accuracy": accuracy_rows(pp, rows)}
 

@@ -29,6 +29,7 @@ SIGNATURES = {
     "emc_ctx_create": (_st, [C.c_int, C.POINTER(_vp)]),
     "emc_ctx_destroy": (_st, [_vp]),
     "emc_ctx_synchronize": (_st, [_vp]),
+    "emc_ctx_wait": (_st, [_vp, _vp]),
     "emc_ctx_stream": (_vp, [_vp]),
     "emc_timing_enable": (_st, [_vp, C.c_int]),
     "emc_timing_reset": (_st, [_vp]),
@@ -66,6 +67,7 @@ SIGNATURES = {
                              _d, C.c_int, C.c_int, _d, _d]),
     "emc_template_search_chunk": (_st, [_vp, _vp, _i3, _vp, _i3, _i3, _i3, _i3, _d, C.c_int, C.c_int,
                                         C.c_int, _vp, _vp]),
+    "emc_align_pick": (_st, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "emc_align_refine": (_st, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int,
                                C.c_double, C.c_double, C.c_double, _vp]),
     "emc_mask3d_sphere": (_st, [_vp, _i3, _d, _vp]),

@@ -143,8 +143,9 @@ class BatchAligner:
             from .autotune import apply
             apply(self.ctx, self.n, self._tune)
 
-    def search(self, angles_pts):
-        """Scores (P, O) and shifts (P, O, 3) of every (particle, orientation)."""
+    def search_device(self, angles_pts):
+        """emc_align_batched, results left on the device: score (P, O) and
+        shift (P, O, 3) DeviceArrays (stream-ordered, no wait)."""
         self._apply_tuning()
         pts = np.ascontiguousarray(np.asarray(angles_pts, dtype=np.float64).reshape(-1, 3))
         O = pts.shape[0]
@@ -153,7 +154,24 @@ class BatchAligner:
         _lib.call("emc_align_batched", self.ctx.handle, self.n, self.n_part, self._fmt, self.F2.vp, self.Q.vp,
                   self.N.vp, self.A.vp, self.d_starts.vp, pts.ctypes.data_as(C.POINTER(C.c_double)), O,
                   self.peak_radius, self.support, score.vp, shift.vp)
+        return score, shift
+
+    def search(self, angles_pts):
+        """Scores (P, O) and shifts (P, O, 3) of every (particle, orientation)."""
+        score, shift = self.search_device(angles_pts)
         return score.get(), shift.get()
+
+    def pick(self, score, shift, d_pts):
+        """Best orientation per particle on the device (emc_align_pick):
+        (R0 (P, 9) row-major FP64, s0 (P, 3), best index (P,), best score (P,))."""
+        P, O = score.shape
+        R0 = DeviceArray((P, 9), np.float64, self.ctx)
+        s0 = DeviceArray((P, 3), np.float64, self.ctx)
+        best = DeviceArray((P,), np.int32, self.ctx)
+        bs = DeviceArray((P,), np.float32, self.ctx)
+        _lib.call("emc_align_pick", self.ctx.handle, P, O, score.vp, shift.vp, self.d_starts.vp, d_pts.vp, R0.vp,
+                  s0.vp, best.vp, bs.vp)
+        return R0, s0, best, bs
 
     # ------------------------------------------------------------ adjoint
     def eval_grad(self, R, dR, s):
@@ -180,19 +198,26 @@ class BatchAligner:
         dR = np.stack([R @ g for g in _G], axis=1)
         return self.eval_grad(R, dR, s)
 
+    def refine_device(self, R, s, iters=12, step_angle=0.02, step_shift=0.3):
+        """emc_align_refine on device arrays R (P, 9) row-major and s (P, 3),
+        FP64, updated in place; returns the final CC as a device array."""
+        self._apply_tuning()
+        cc = DeviceArray((self.n_part,), np.float64, self.ctx)
+        _lib.call("emc_align_refine", self.ctx.handle, self.n, self.n_part, self._fmt, self.F2.vp, self.Q.vp,
+                  self.N.vp, self.A.vp, R.vp, s.vp, int(iters), float(step_angle), float(step_shift), self.support,
+                  cc.vp)
+        return cc
+
     def refine(self, R0, s0, iters=12, step_angle=0.02, step_shift=0.3):
         """Batched quasi-Newton ascent of the CC over (rotation increment,
         shift) for every particle, on the device (emc_align_refine: iters + 1
         adjoint evaluations, one result copy). Returns (R, s, cc)."""
-        self._apply_tuning()
         P = self.n_part
         R = DeviceArray.from_numpy(np.ascontiguousarray(np.asarray(R0, dtype=np.float64).reshape(P, 9)),
                                    ctx=self.ctx)
         s = DeviceArray.from_numpy(np.ascontiguousarray(np.asarray(s0, dtype=np.float64).reshape(P, 3)),
                                    ctx=self.ctx)
-        cc = DeviceArray((P,), np.float64, self.ctx)
-        _lib.call("emc_align_refine", self.ctx.handle, self.n, P, self._fmt, self.F2.vp, self.Q.vp, self.N.vp,
-                  self.A.vp, R.vp, s.vp, int(iters), float(step_angle), float(step_shift), self.support, cc.vp)
+        cc = self.refine_device(R, s, iters, step_angle, step_shift)
         return R.get().reshape(P, 3, 3), s.get(), cc.get()
 
 
@@ -258,17 +283,44 @@ def fused_samples_per_unit(n, support):
     return sum(2 * h + 1 for g in passes for (_, _, h) in g)
 
 
+def search_stage(aligner: BatchAligner, grid_pts, after=None):
+    """Queue the coarse search over grid_pts and the pick of each particle's
+    best orientation (first maximum) on the aligner's stream; no host round
+    trip. ``after``: a Context whose queued work the search waits for, so
+    searches of several aligners on their own streams run one after another
+    while the rest of their work overlaps."""
+    pts = np.ascontiguousarray(np.asarray(grid_pts, dtype=np.float64).reshape(-1, 3))
+    d_pts = DeviceArray.from_numpy(pts, ctx=aligner.ctx)
+    if after is not None:
+        aligner.ctx.wait_for(after)
+    score, shift = aligner.search_device(pts)
+    R, s, best, bs = aligner.pick(score, shift, d_pts)
+    return {"aligner": aligner, "P": aligner.n_part, "R": R, "s": s, "cc": None, "best": best, "best_score": bs,
+            "keep": (score, shift, d_pts)}
+
+
+def refine_stage(h, refine_iters=12):
+    """Queue the adjoint refinement from a search_stage handle's picks."""
+    if refine_iters > 0:
+        h["cc"] = h["aligner"].refine_device(h["R"], h["s"], iters=refine_iters)
+    return h
+
+
+def align_batch_async(aligner: BatchAligner, grid_pts, refine_iters=12):
+    """search_stage + refine_stage; collect() waits for the results."""
+    return refine_stage(search_stage(aligner, grid_pts), refine_iters)
+
+
+def collect(h):
+    """(R (P,3,3), shift (P,3), cc (P,), grid best index (P,)) of an
+    align_batch_async handle."""
+    P = h["P"]
+    cc = h["cc"].get() if h["cc"] is not None else h["best_score"].get().astype(np.float64)
+    return h["R"].get().reshape(P, 3, 3), h["s"].get(), cc, h["best"].get().astype(np.int64)
+
+
 def align_batch(aligner: BatchAligner, grid_pts, refine_iters=12):
     """Coarse search over grid_pts, best per particle (first maximum), then
     adjoint refinement. Returns (R (P,3,3), shift (P,3), cc (P,), grid best
     index (P,))."""
-    from .workload import local_matrix
-    score, shift = aligner.search(grid_pts)
-    best = np.argmax(score, axis=1)
-    P = aligner.n_part
-    R0 = np.stack([aligner.starts[i] @ local_matrix(*grid_pts[best[i]]) for i in range(P)])
-    s0 = shift[np.arange(P), best].astype(np.float64)
-    if refine_iters <= 0:
-        return R0, s0, score[np.arange(P), best].astype(np.float64), best
-    R, s, cc = aligner.refine(R0, s0, iters=refine_iters)
-    return R, s, cc, best
+    return collect(align_batch_async(aligner, grid_pts, refine_iters))

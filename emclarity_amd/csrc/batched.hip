@@ -164,6 +164,38 @@ __device__ inline void local_mat(double phi, double theta, double psi, double L[
   mm_d(L, c, L);
 }
 
+// Best orientation of each particle (the first maximum of its scores, as
+// numpy's argmax; NaN lowest) and the refinement's start pose R0 = A *
+// local(pts[best]) (row-major, FP64), s0 = the shift found with it.
+__global__ void __launch_bounds__(kRedThreads) pick_kernel(const float* __restrict__ score,
+                                                          const float* __restrict__ shift,
+                                                          const double* __restrict__ start_cm,
+                                                          const double* __restrict__ pts, int n_orient,
+                                                          double* __restrict__ R0, double* __restrict__ s0,
+                                                          int* __restrict__ best, float* __restrict__ best_score) {
+  const int p = blockIdx.x;
+  const float* sc = score + (int64_t)p * n_orient;
+  unsigned long long key = 0ull;
+  for (int o = threadIdx.x; o < n_orient; o += blockDim.x) {
+    const unsigned long long k = argmax_key(sc[o], (uint32_t)o);
+    key = k > key ? k : key;
+  }
+  key = block_max_u64(key);  // valid in thread 0
+  if (threadIdx.x != 0) return;
+  int o = (int)key_index(key);
+  if (o < 0 || o >= n_orient) o = 0;
+  double A[3][3], L[3][3], R[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) A[r][c] = start_cm[9 * p + c * 3 + r];
+  local_mat(pts[3 * o], pts[3 * o + 1], pts[3 * o + 2], L);
+  mm_d(A, L, R);
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) R0[9 * p + 3 * r + c] = R[r][c];
+  for (int k = 0; k < 3; ++k) s0[3 * p + k] = (double)shift[((int64_t)p * n_orient + o) * 3 + k];
+  if (best) best[p] = o;
+  if (best_score) best_score[p] = sc[o];
+}
+
 __global__ void compose_kernel(const double* __restrict__ start_cm, const double* __restrict__ pts,
                                int n_part, int n_orient, float* __restrict__ M) {
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
@@ -187,8 +219,6 @@ __global__ void compose_kernel(const double* __restrict__ start_cm, const double
 // correlation and its gradient do not depend on s.
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 
-// A table entry holds the texels (x, y) and (x, y+1); entries x0 and x0+1
-// are adjacent, so one z-plane of a 2x2x2 footprint is one contiguous quad.
 // Table entry (x, y, z) holds the four texels (x, y, z), (x, y+1, z),
 // (x, y, z+1), (x, y+1, z+1); entries x and x+1 are adjacent, so a trilinear
 // footprint (x0..x0+1, y0..y0+1, z0..z0+1) is one contiguous octet: 64 bytes
@@ -1180,6 +1210,19 @@ emc_status emc_align_eval_grad(emc_ctx* ctx, int n, int n_part, int table_format
                 table_format);
     EMC_REQUIRE(support_radius > 0 && support_radius < 0.5 * n, "support radius must be in (0, n/2)");
     eval_grad_impl(ctx, n, n_part, table_format, d_F2, d_Q, d_N, d_A, d_MdM, d_shift, support_radius, d_out);
+  });
+}
+
+emc_status emc_align_pick(emc_ctx* ctx, int n_part, int n_orient, const float* d_score, const float* d_shift,
+                          const double* d_start_cm, const double* d_angles_pts, double* d_R0, double* d_s0,
+                          int32_t* d_best, float* d_best_score) {
+  return guard([&] {
+    use_device(ctx);
+    EMC_REQUIRE(n_part > 0 && n_orient > 0, "empty batch");
+    EMC_REQUIRE(d_score && d_shift && d_start_cm && d_angles_pts && d_R0 && d_s0, "null pointer");
+    hipLaunchKernelGGL(pick_kernel, dim3(n_part), dim3(kRedThreads), 0, ctx->stream, d_score, d_shift, d_start_cm,
+                       d_angles_pts, n_orient, d_R0, d_s0, (int*)d_best, d_best_score);
+    EMC_HIP(hipGetLastError());
   });
 }
 

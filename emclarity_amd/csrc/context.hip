@@ -323,6 +323,19 @@ emc_status emc_ctx_destroy(emc_ctx* ctx) {
   });
 }
 
+emc_status emc_ctx_wait(emc_ctx* waiter, emc_ctx* signaller) {
+  return guard([&] {
+    EMC_REQUIRE(waiter && signaller, "null context");
+    EMC_REQUIRE(waiter->device == signaller->device, "contexts on different devices");
+    use_device(signaller);
+    hipEvent_t e;
+    EMC_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    EMC_HIP(hipEventRecord(e, signaller->stream));
+    EMC_HIP(hipStreamWaitEvent(waiter->stream, e, 0));
+    EMC_HIP(hipEventDestroy(e));  // released once the recorded work completes
+  });
+}
+
 emc_status emc_ctx_synchronize(emc_ctx* ctx) {
   return guard([&] {
     use_device(ctx);

@@ -27,6 +27,11 @@ class Context:
     def synchronize(self):
         _lib.call("emc_ctx_synchronize", self.handle)
 
+    def wait_for(self, other: "Context"):
+        """Later work on this context's stream waits for the work queued on
+        ``other``'s stream so far (no host wait)."""
+        _lib.call("emc_ctx_wait", self.handle, other.handle)
+
     def close(self):
         if self.handle:
             _lib.call("emc_ctx_destroy", self.handle)

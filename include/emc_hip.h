@@ -58,6 +58,10 @@ const char* emc_version(void);
 emc_status emc_ctx_create(int device, emc_ctx** out);
 emc_status emc_ctx_destroy(emc_ctx* ctx);
 emc_status emc_ctx_synchronize(emc_ctx* ctx);
+/* Make later work on `waiter`'s stream wait for the work queued on
+ * `signaller`'s stream so far (an event recorded on one, waited on by the
+ * other; no host wait). Both contexts on the same device. */
+emc_status emc_ctx_wait(emc_ctx* waiter, emc_ctx* signaller);
 /* The hipStream_t of the context, as void*, for callers that interleave their
  * own work on the same stream. */
 void* emc_ctx_stream(emc_ctx* ctx);
@@ -355,6 +359,17 @@ emc_status emc_align_eval_grad(emc_ctx* ctx, int n, int n_part, int table_format
  * particle (row-major R, R maps particle to reference frame as in
  * emc_align_batched), d_s: 3 doubles (shift, px); both updated in place;
  * d_cc: the final CC per particle. Stream-ordered. */
+/* The refinement's start from emc_align_batched's scores without a host
+ * round trip: per particle the best orientation (first maximum, as the
+ * grid search takes it), d_R0 = A * BH_defineMatrix([phi, theta, psi - phi],
+ * 'Bah', 'inv') of that orientation (9 doubles, row-major, A the particle's
+ * d_start_cm), d_s0 = its shift (3 doubles); d_best / d_best_score (NULL =
+ * not wanted) the index and score. d_angles_pts: the n_orient (phi, theta,
+ * psi) triples in device memory. Stream-ordered. */
+emc_status emc_align_pick(emc_ctx* ctx, int n_part, int n_orient, const float* d_score, const float* d_shift,
+                          const double* d_start_cm, const double* d_angles_pts, double* d_R0, double* d_s0,
+                          int32_t* d_best, float* d_best_score);
+
 emc_status emc_align_refine(emc_ctx* ctx, int n, int n_part, int table_format, const void* d_F2, const void* d_Q,
                             const float* d_N, const double* d_A, double* d_R, double* d_s, int iters,
                             double step_angle, double step_shift, double support_radius, double* d_cc);
