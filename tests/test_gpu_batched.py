@@ -209,3 +209,34 @@ def test_device_refinement_matches_host_restatement(ctx):
             assert abs(cd[i] - ch[i]) < 1e-9
     f0, _ = al._eval_local(R0, s0)
     assert np.all(cd >= f0)   # only ascending steps are taken (deterministic adjoint)
+
+
+def test_pick_matches_host(ctx):
+    """emc_align_pick: first maximum (numpy argmax, ties included) and
+    R0 = A @ defineMatrix([phi, theta, psi - phi], 'Bah', 'inv') in FP64."""
+    import ctypes as C
+    from emclarity_amd import _lib
+    from emclarity_amd.device import DeviceArray
+    rng = np.random.default_rng(11)
+    P, NO = 7, 301
+    score = rng.standard_normal((P, NO)).astype(np.float32)
+    score[2, [5, 40]] = 9.0                       # a tie: the first wins
+    shift = rng.uniform(-3, 3, (P, NO, 3)).astype(np.float32)
+    starts = np.stack([O.define_matrix(rng.uniform(0, 360, 3), "Bah", "inv") for _ in range(P)])
+    pts = np.stack([rng.uniform(0, 360, NO), rng.uniform(0, 15, NO), rng.uniform(-15, 15, NO)], 1)
+    d_start = DeviceArray.from_numpy(np.ascontiguousarray(starts.transpose(0, 2, 1).reshape(P, 9)))
+    d_pts = DeviceArray.from_numpy(np.ascontiguousarray(pts))
+    R0 = DeviceArray((P, 9), np.float64)
+    s0 = DeviceArray((P, 3), np.float64)
+    best = DeviceArray((P,), np.int32)
+    bs = DeviceArray((P,), np.float32)
+    _lib.call("emc_align_pick", ctx.handle, P, NO, DeviceArray.from_numpy(score).vp, DeviceArray.from_numpy(shift).vp,
+              d_start.vp, d_pts.vp, R0.vp, s0.vp, best.vp, bs.vp)
+    b = np.argmax(score, axis=1)
+    assert np.array_equal(best.get(), b) and b[2] == 5
+    assert np.array_equal(bs.get(), score[np.arange(P), b])
+    assert np.array_equal(s0.get(), shift[np.arange(P), b].astype(np.float64))
+    for i in range(P):
+        ph, th, ps = pts[b[i]]
+        ref = starts[i] @ O.define_matrix([ph, th, ps - ph], "Bah", "inv")
+        assert np.max(np.abs(R0.get()[i].reshape(3, 3) - ref)) < 1e-12
