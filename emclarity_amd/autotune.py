@@ -1,9 +1,11 @@
 """Autotuner of the batched search's launch shapes.
 
-Two parameters shape the batched path's hot launches (emc_tune_set):
+Three parameters shape the batched path's hot launches (emc_tune_set):
   search_tile  kx extent of the (kx, ky) column patches of the search
                schedule (ky extent 256 / search_tile): the shape of the
                reference-table region the units of one pass read together
+  search_passes  passes of that schedule per launch: they share the launch's
+               last, partly filled round of workgroups
   grad_blocks  workgroups an adjoint evaluation aims for: how finely each
                particle's support is split (fill against fold work)
 
@@ -13,7 +15,8 @@ alignment workload (64/96/128^3 with 50/30/20 particles, 114 coarse
 orientations) and both table formats, and writes the fastest to a JSON
 table. The table measured on MI355X ships as ``tuning/mi355x.json``;
 BatchAligner applies its entries, and falls back to the library defaults
-(16 x 16 patches, 4096) for box sizes it does not list.
+(16 x 16 patches, 2 passes per launch, 4096) for box sizes it does not
+list.
 """
 from __future__ import annotations
 
@@ -29,7 +32,8 @@ import numpy as np
 
 from . import _lib
 
-CANDIDATES = {"search_tile": [4, 8, 16, 32, 64], "grad_blocks": [1024, 2048, 4096, 8192, 16384]}
+CANDIDATES = {"search_tile": [4, 8, 16, 32, 64], "search_passes": [1, 2, 3, 4],
+              "grad_blocks": [1024, 2048, 4096, 8192, 16384]}
 TABLE = Path(__file__).resolve().parent / "tuning" / "mi355x.json"
 _CACHE: dict | None = None
 
@@ -100,11 +104,11 @@ def tune(ctx, counts=((64, 50), (96, 30), (128, 20)), tables=("c64", "f16"), rep
             R = np.stack([p.start for p in parts])
             s = np.zeros((count, 3))
             best, times = {}, {}
-            for name in ("search_tile", "grad_blocks"):
+            for name in ("search_tile", "search_passes", "grad_blocks"):
                 times[name] = {}
                 for v in CANDIDATES[name]:
                     apply(ctx, n, {name: v})
-                    if name == "search_tile":
+                    if name != "grad_blocks":
                         ms = _timed(ctx, lambda: al.search(grid), ("align_fused", "align_final"), reps)
                     else:
                         ms = _timed(ctx, lambda: al._eval_local(R, s), ("align_eval_grad",), reps)

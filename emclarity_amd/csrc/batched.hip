@@ -425,7 +425,7 @@ template <int RR, class Tab>
 __global__ void __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(RR <= 6 ? 4 : 3))) align_pass_kernel(
     const typename Tab::T* __restrict__ F2, int n, const float2* __restrict__ Q, const float* __restrict__ Nw,
     const float* __restrict__ Mall, int n_orient, int unit0, int n_units, int units_per_xcd,
-    const int* __restrict__ sched, int g, const float2* __restrict__ twg, float* __restrict__ maps,
+    const int* __restrict__ sched, int g0, int npass, const float2* __restrict__ twg, float* __restrict__ maps,
     double* __restrict__ dsums) {
   constexpr int D = 2 * RR + 1;
   __shared__ float2 tw[kMaxN][D];              // e^{2 pi i k d / n}
@@ -434,11 +434,20 @@ __global__ void __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per
   __shared__ uint8_t kyslot[kBatchThreads];    // FFT-order ky of each column slot (< 128)
   __shared__ double red[kBatchThreads / kWave];
 
+  // Passes g0 .. g0 + npass - 1 of the schedule in one launch, pass-major
+  // (the blocks of a pass are dispatched together). Pass g accumulates into
+  // map buffer g % npass, so no two blocks of a launch update one map.
   // XCD-aware unit: workgroups are dealt round-robin over the 8 XCDs;
   // consecutive units (the orientations of one particle) land on one XCD.
   // lu indexes this launch's units (maps, dsums), unit all units.
-  const int lu = (blockIdx.x % 8) * units_per_xcd + blockIdx.x / 8;
-  if (lu >= n_units) return;  // uniform per block
+  const int per_pass = 8 * units_per_xcd;
+  const int g = g0 + (int)blockIdx.x / per_pass;
+  const int slot = (int)blockIdx.x % per_pass;
+  const int lu = (slot % 8) * units_per_xcd + slot / 8;
+  if (lu >= n_units || g >= sched[0]) return;  // uniform per block
+  const int buf = g % npass;
+  maps += (int64_t)buf * n_units * (D * D * D);
+  dsums += (int64_t)buf * n_units;
   const int unit = unit0 + lu;
   const int t = threadIdx.x;
   const int p = unit / n_orient;
@@ -528,12 +537,12 @@ __global__ void __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per
     }
     float* row = mp + (dz * D + dy) * D;
 #pragma unroll
-    for (int dx = 0; dx < D; ++dx) row[dx] = (g == 0 ? 0.0f : row[dx]) + vals[dx];
+    for (int dx = 0; dx < D; ++dx) row[dx] = (g < npass ? 0.0f : row[dx]) + vals[dx];
   }
   if (t == 0) {
     double v = 0.0;
     for (int w = 0; w < kBatchThreads / kWave; ++w) v += red[w];
-    dsums[lu] = (g == 0 ? 0.0 : dsums[lu]) + v;
+    dsums[lu] = (g < npass ? 0.0 : dsums[lu]) + v;
   }
 }
 
@@ -542,15 +551,20 @@ __global__ void __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per
 // normalised score.
 template <int RR>
 __global__ void __launch_bounds__(kBatchThreads) align_final_kernel(const float* __restrict__ maps,
-                                                                   const double* __restrict__ dsums,
-                                                                   const double* __restrict__ A, int n_orient,
-                                                                   int unit0, float* __restrict__ score,
+                                                                   const double* __restrict__ dsums, int nbuf,
+                                                                   int n_units, const double* __restrict__ A,
+                                                                   int n_orient, int unit0,
+                                                                   float* __restrict__ score,
                                                                    float* __restrict__ shift) {
   constexpr int D = 2 * RR + 1;
   __shared__ float map[D * D * D];
   const int lu = blockIdx.x, unit = unit0 + lu, t = threadIdx.x;
-  const float* mp = maps + (int64_t)lu * (D * D * D);
-  for (int i = t; i < D * D * D; i += kBatchThreads) map[i] = mp[i];
+  // the pass buffers added in a fixed order
+  for (int i = t; i < D * D * D; i += kBatchThreads) {
+    float v = maps[(int64_t)lu * (D * D * D) + i];
+    for (int b = 1; b < nbuf; ++b) v += maps[((int64_t)b * n_units + lu) * (D * D * D) + i];
+    map[i] = v;
+  }
   __syncthreads();
   unsigned long long key = 0ull;
   for (int i = t; i < D * D * D; i += kBatchThreads) {
@@ -575,7 +589,9 @@ __global__ void __launch_bounds__(kBatchThreads) align_final_kernel(const float*
     const float sx = (float)(bx - RR) + para(best, 1, bx);
     const float sy = (float)(by - RR) + para(best, D, by);
     const float sz = (float)(bz - RR) + para(best, D * D, bz);
-    const double norm = sqrt(A[unit / n_orient] * dsums[lu]);
+    double ds = dsums[lu];
+    for (int b = 1; b < nbuf; ++b) ds += dsums[(int64_t)b * n_units + lu];
+    const double norm = sqrt(A[unit / n_orient] * ds);
     score[unit] = norm > 0.0 ? (float)((double)bv / norm) : 0.0f;
     shift[3 * (int64_t)unit + 0] = sx;
     shift[3 * (int64_t)unit + 1] = sy;
@@ -752,24 +768,29 @@ void launch_fused(emc_ctx* ctx, const void* F2, int n, const float2* Q, const fl
     hipLaunchKernelGGL(twiddle_kernel, dim3(ceil_div(n * D, 256)), dim3(256), 0, ctx->stream, n, RR, tw);
     EMC_HIP(hipGetLastError());
   }
-  // units in chunks so the per-unit maps stay bounded (<= 1.3 GB at r = 8)
+  // npass passes per launch ("search_passes"): the last, partly filled
+  // round of workgroups of each launch is shared by npass passes; each pass
+  // of a launch has its own map buffer
+  const int npass = std::max(1, std::min(tune_value(ctx, "search_passes", n, 2), ngroups));
+  // units in chunks so the per-unit maps stay bounded (<= 1.3 GB per buffer at r = 8)
   constexpr int kChunkUnits = 65536;
   const int cu = std::min(n_units, kChunkUnits);
-  float* maps = (float*)scratch(ctx, "bat_maps", sizeof(float) * D * D * D * (size_t)cu);
-  double* dsums = (double*)scratch(ctx, "bat_dsums", sizeof(double) * (size_t)cu);
+  float* maps = (float*)scratch(ctx, "bat_maps", sizeof(float) * D * D * D * (size_t)cu * npass);
+  double* dsums = (double*)scratch(ctx, "bat_dsums", sizeof(double) * (size_t)cu * npass);
   for (int u0 = 0; u0 < n_units; u0 += kChunkUnits) {
     const int nu = std::min(kChunkUnits, n_units - u0);
     const int per = ceil_div(nu, 8);
-    for (int g = 0; g < ngroups; ++g) {
+    for (int g = 0; g < ngroups; g += npass) {
+      const int np = std::min(npass, ngroups - g);
       KTimer kt(ctx, "align_fused");
-      hipLaunchKernelGGL((align_pass_kernel<RR, Tab>), dim3(8 * per), dim3(kBatchThreads), 0, ctx->stream,
-                         (const typename Tab::T*)F2, n, Q, N, M, n_orient, u0, nu, per, sched, g,
+      hipLaunchKernelGGL((align_pass_kernel<RR, Tab>), dim3(8 * per * np), dim3(kBatchThreads), 0, ctx->stream,
+                         (const typename Tab::T*)F2, n, Q, N, M, n_orient, u0, nu, per, sched, g, npass,
                          (const float2*)tw, maps, dsums);
       EMC_HIP(hipGetLastError());
     }
     KTimer kt(ctx, "align_final");
     hipLaunchKernelGGL(align_final_kernel<RR>, dim3(nu), dim3(kBatchThreads), 0, ctx->stream,
-                       (const float*)maps, (const double*)dsums, A, n_orient, u0, score, shift);
+                       (const float*)maps, (const double*)dsums, npass, nu, A, n_orient, u0, score, shift);
     EMC_HIP(hipGetLastError());
   }
 }

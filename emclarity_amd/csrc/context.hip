@@ -258,7 +258,7 @@ struct TuneRange {
   const char* name;
   int lo, hi, dflt;
 };
-constexpr TuneRange kTune[] = {{"search_tile", 2, 128, 16}, {"grad_blocks", 256, 65536, 4096}};
+constexpr TuneRange kTune[] = {{"search_tile", 2, 128, 16}, {"grad_blocks", 256, 65536, 4096}, {"search_passes", 1, 4, 2}};
 
 const TuneRange* find_tune(const char* name) {
   for (const auto& t : kTune)

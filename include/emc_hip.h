@@ -81,6 +81,9 @@ emc_status emc_timing_read(emc_ctx* ctx, const char* name, double* ms_total, int
  *                  workgroup's columns), 2..128 (default 16: 16 x 16)
  *   "grad_blocks"  workgroups an adjoint evaluation aims for, 256..65536
  *                  (default 4096)
+ *   "search_passes" passes of the search schedule per launch, 1..4
+ *                  (default 2): they share the launch's last, partly
+ *                  filled round of workgroups
  * box = 0 sets the value for every box size, else for that box size only
  * (which wins). emc_tune_get returns the value in effect for `box`. */
 emc_status emc_tune_set(emc_ctx* ctx, const char* name, int box, int value);
