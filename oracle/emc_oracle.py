@@ -227,13 +227,52 @@ def xform3d(vol, M, t, fwd):
     return _trilinear_zero(vol, p[0] + nx // 2, p[1] + ny // 2, p[2] + nz // 2)
 
 
+def _frame(a, b):
+    e1 = a / np.linalg.norm(a)
+    e2 = b - (b @ e1) * e1
+    e2 /= np.linalg.norm(e2)
+    return np.stack([e1, e2, np.cross(e1, e2)], axis=1)
+
+
+def _vertex_group(vertices):
+    """Every rotation of a regular polyhedron, by enumeration: the rotation
+    taking a fixed vertex and one of its nearest neighbours onto each vertex
+    and each of its nearest neighbours."""
+    V = np.asarray(vertices, dtype=np.float64)
+    d = np.linalg.norm(V[:, None] - V[None], axis=2)
+    near = d[0][d[0] > 1e-9].min()
+    nb = lambda i: [j for j in range(len(V)) if abs(d[i, j] - near) < 1e-9]
+    F0 = _frame(V[0], V[nb(0)[0]])
+    return [_frame(V[a], V[b]) @ F0.T for a in range(len(V)) for b in nb(a)]
+
+
 def symmetry_matrices(symmetry, convention="Bah"):
-    """C_n operator list (interpolator.m:188-198). Other groups are tables in
-    the reference (:199-393); only C_n is restated here."""
-    if symmetry[0] != "C":
-        raise NotImplementedError(symmetry)
-    n = int(symmetry[1:])
-    return [define_matrix([0.0, 0.0, i * 360.0 / n], convention, "forward") for i in range(n)]
+    """Symmetry operators of interpolator.m:173-380. C<n>: BH_defineMatrix([0,
+    0, i 360/n]) (:188-198). D<n>: the reference makes 2n matrices but keeps
+    nSymMats = n (:199-210), so its interpolator uses the n C_n ones ('Bah').
+    O: the octahedron (+-x, +-y, +-z)'s 24 rotations. I / I2: the 60
+    rotations of the icosahedron with vertices at the cyclic permutations of
+    (0, +-1, +-phi) / (+-1, 0, +-phi), the two settings of :245-372."""
+    s = symmetry.strip()
+    if s[0] in "CD":
+        n = int(s[1:])
+        conv = convention if s[0] == "C" else "Bah"
+        return [define_matrix([0.0, 0.0, i * 360.0 / n], conv, "forward") for i in range(n)]
+    if s == "O":
+        return _vertex_group([v for a in range(3) for v in (np.eye(3)[a], -np.eye(3)[a])])
+    if s in ("I", "I1", "I2"):
+        phi = (1.0 + math.sqrt(5.0)) / 2.0
+        base = (1.0, 0.0, phi) if s == "I2" else (0.0, 1.0, phi)
+        nz = [i for i in range(3) if base[i] != 0]
+        V = []
+        for sa in (1, -1):
+            for sb in (1, -1):
+                v = np.array(base)
+                v[nz[0]] *= sa
+                v[nz[1]] *= sb
+                V += [np.roll(v, k) for k in range(3)]
+        return _vertex_group(V)
+    raise NotImplementedError(symmetry)
 
 
 def _as_matrix(a):

@@ -72,3 +72,15 @@ def test_sizing_helpers():
     ind, pad, sh = sizing.BH_isWindowValid([200, 200, 100], [64] * 3, [20] * 3, [100.3, 50.0, 40.75])
     assert np.allclose(sh, [0.3, 0.0, 0.75]) and np.array_equal(ind[:, 0], [68, 131])
     assert sizing.BH_isWindowValid([200, 200, 100], [64] * 3, [20] * 3, [3.0, 50.0, 40.0])[0] == "noUse"
+
+
+def test_symmetry_operators_match_the_oracle():
+    """Host point groups (closure of generators) against the oracle's
+    (enumeration of vertex frames), as sets; D<n> is the n C_n operators."""
+    from emclarity_amd.geometry import symmetry_matrices
+    for name in ("C5", "D3", "O", "I", "I2"):
+        a, b = symmetry_matrices(name), O.symmetry_matrices(name)
+        assert len(a) == len(b)
+        assert all(any(np.max(np.abs(x - y)) < 1e-9 for y in b) for x in a), name
+    with pytest.raises(ValueError):
+        symmetry_matrices("O", "Helical")

@@ -48,12 +48,13 @@ def test_xform3d_exact_cases(ctx, rng):
     assert np.array_equal(r, O.xform3d(v, Rz, [0, 0, 0], True).astype(np.float32))
 
 
-def test_interpolator_symmetry(ctx, rng):
+@pytest.mark.parametrize("symmetry", ["C3", "D4", "O", "I", "I2"])
+def test_interpolator_symmetry(ctx, rng, symmetry):
     v = rng.standard_normal((24, 24, 24)).astype(np.float32)
     ip = interpolator(v, [0, 0, 0], [0, 0, 0], "Bah", "forward", "C1")
     ang, sh = [12.0, 33.0, -70.0], [0.5, -1.0, 0.25]
-    out = ip.interp3d(ang, sh, "Bah", "inv", "C3").get()
-    ref = O.interp3d(v, ang, sh, "Bah", "inv", "C3")
+    out = ip.interp3d(ang, sh, "Bah", "inv", symmetry).get()
+    ref = O.interp3d(v, ang, sh, "Bah", "inv", symmetry)
     assert np.max(np.abs(out - ref)) < 2e-4
 
 

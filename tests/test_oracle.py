@@ -593,3 +593,51 @@ def test_template_csv_row_format(tmp_path):
     r = BH_defineMatrix(pk[0, 3:6], "Bah", "inv").reshape(-1, order="F")
     assert np.allclose([float(v) for v in row[16:25]], r, atol=1e-6) and row[25] == "1"
     assert (tmp_path / "a.pos").read_text() == "60.250000 32.000000 16.625000\n"
+
+
+# ------------------------------------------------------- point-group operators
+
+def _is_group(mats):
+    for a in mats:
+        assert abs(np.linalg.det(a) - 1.0) < 1e-12 and np.allclose(a @ a.T, np.eye(3), atol=1e-12)
+        for b in mats[:12]:
+            assert any(np.max(np.abs(a @ b - c)) < 1e-9 for c in mats)
+
+
+@pytest.mark.parametrize("name,order", [("O", 24), ("I", 60), ("I2", 60)])
+def test_point_groups(name, order):
+    G = O.symmetry_matrices(name)
+    assert len(G) == order
+    _is_group(G)
+    # 2-fold axes along x, y and z in every setting
+    for a in np.eye(3):
+        assert any(np.allclose(g, 2 * np.outer(a, a) - np.eye(3), atol=1e-12) for g in G)
+    if name == "O":
+        # 4-folds on the axes: the signed permutations with determinant +1
+        assert all(np.allclose(np.sort(np.abs(g), axis=1)[:, -1], 1) for g in G)
+    else:
+        phi = (1 + 5 ** 0.5) / 2
+        v = np.array([1.0, 0.0, phi]) if name == "I2" else np.array([0.0, 1.0, phi])
+        v /= np.linalg.norm(v)
+        # a 5-fold about the setting's vertex, and about none of the other setting's
+        rot72 = [g for g in G if abs(np.trace(g) - (1 + 2 * np.cos(np.radians(72)))) < 1e-9]
+        assert len(rot72) == 12
+        assert any(np.allclose(g @ v, v, atol=1e-12) for g in rot72)
+        w = np.array([0.0, 1.0, phi]) if name == "I2" else np.array([1.0, 0.0, phi])
+        w /= np.linalg.norm(w)
+        assert not any(np.allclose(g @ w, w, atol=1e-9) for g in rot72)
+
+
+def test_dihedral_uses_the_cyclic_operators():
+    """interpolator.m:199-210 keeps nSymMats = n for D<n>: the D4 average is
+    the C4 average."""
+    assert all(np.allclose(a, b) for a, b in zip(O.symmetry_matrices("D4"), O.symmetry_matrices("C4")))
+
+
+def test_symmetrised_volume_is_invariant():
+    rng = np.random.default_rng(5)
+    v = rng.standard_normal((9, 9, 9))
+    G = O.symmetry_matrices("O")
+    a = O.interp3d(v, np.eye(3), [0, 0, 0], "Bah", "inv", "O")
+    b = O.interp3d(v, G[7], [0, 0, 0], "Bah", "inv", "O")
+    assert np.allclose(a, b, atol=1e-10)
