@@ -165,11 +165,150 @@ __global__ void __launch_bounds__(kRedThreads) xcf_rot_full_kernel(
     for (int k = 0; k < 3; ++k) part[3 * blockIdx.x + k] = v[k];
 }
 
+// Shell-weighted form: block (b, s) reduces shell s's voxel list with
+// stride gridDim.x * blockDim.x (fixed partition: deterministic sums)
+constexpr int kShellBlocks = 64;
+
+__global__ void __launch_bounds__(kRedThreads) xcf_rot_shell_kernel(
+    const float2* __restrict__ P, const float2* __restrict__ R, const float* __restrict__ wp,
+    const float* __restrict__ wr, const int64_t* __restrict__ offsets, const uint32_t* __restrict__ idx,
+    double* __restrict__ part) {
+  const int s = blockIdx.y;
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int64_t j = offsets[s] + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < offsets[s + 1];
+       j += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = (int64_t)idx[j] - 1;  // MATLAB linear index
+    const float2 p = P[i], r = R[i];
+    const double a = wp ? (double)wp[i] : 1.0;
+    const double c = wr ? (double)wr[i] : 1.0;
+    const double re = (double)p.x * r.x - (double)p.y * r.y;
+    v[0] += c * a * re;
+    v[1] += ((double)r.x * r.x + (double)r.y * r.y) * a * a;
+    v[2] += ((double)p.x * p.x + (double)p.y * p.y) * c * c;
+  }
+  block_sum<3>(v);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 3; ++k) part[((int64_t)s * gridDim.x + blockIdx.x) * 3 + k] = v[k];
+}
+
+// Per shell: CCC_s = sum num / sqrt(sum d1 sum d2), then
+// sum over shells (in order) of (CCC_s^2 / (CCC_s + w^-1.5))^3 (abs if asked)
+// (statistics/BH_multi_xcf_Rotational.m:53-70)
+__global__ void __launch_bounds__(kWave) xcf_rot_shell_fold_kernel(const double* __restrict__ part, int n_shells,
+                                                                  int nb, int abs_flag, double w,
+                                                                  double* __restrict__ out) {
+  __shared__ double cc[64];
+  for (int s0 = 0; s0 < n_shells; s0 += 64) {
+    const int ns = min(64, n_shells - s0);
+    if ((int)threadIdx.x < ns) {
+      double v[3] = {0.0, 0.0, 0.0};
+      const double* p = part + (int64_t)(s0 + threadIdx.x) * nb * 3;
+      for (int b = 0; b < nb; ++b)
+        for (int k = 0; k < 3; ++k) v[k] += p[3 * b + k];
+      cc[threadIdx.x] = v[0] / sqrt(v[1] * v[2]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double acc = s0 == 0 ? 0.0 : out[0];
+      const double k = pow(w, -1.5);
+      for (int s = 0; s < ns; ++s) {
+        const double c = cc[s];
+        double t = c * c / (c + k);
+        t = t * t * t;
+        acc += abs_flag ? fabs(t) : t;
+      }
+      out[0] = acc;
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace emc
+
+struct emc_shell_plan {
+  int n_shells = 0;
+  int64_t n_idx = 0;
+  int64_t max_index = 0;
+  int64_t* d_offsets = nullptr;
+  uint32_t* d_idx = nullptr;
+};
 
 using namespace emc;
 
 extern "C" {
+
+emc_status emc_shell_plan_create(emc_ctx* ctx, int n_shells, const int64_t* h_offsets, const uint32_t* h_indices,
+                                 emc_shell_plan** out) {
+  return guard([&] {
+    use_device(ctx);
+    EMC_REQUIRE(out && h_offsets && n_shells > 0, "null pointer or no shells");
+    EMC_REQUIRE(h_offsets[0] == 0, "offsets must start at 0");
+    for (int s = 0; s < n_shells; ++s) EMC_REQUIRE(h_offsets[s + 1] >= h_offsets[s], "offsets must not decrease");
+    const int64_t total = h_offsets[n_shells];
+    EMC_REQUIRE(total == 0 || h_indices, "null index list");
+    int64_t mx = 0;
+    for (int64_t j = 0; j < total; ++j) {
+      EMC_REQUIRE(h_indices[j] >= 1, "shell indices are 1-based");
+      mx = std::max<int64_t>(mx, h_indices[j]);
+    }
+    auto* p = new emc_shell_plan;
+    p->n_shells = n_shells;
+    p->n_idx = total;
+    p->max_index = mx;
+    if (hipMalloc(&p->d_offsets, sizeof(int64_t) * (n_shells + 1)) != hipSuccess ||
+        hipMalloc(&p->d_idx, sizeof(uint32_t) * std::max<int64_t>(total, 1)) != hipSuccess) {
+      (void)hipGetLastError();
+      if (p->d_offsets) (void)hipFree(p->d_offsets);
+      delete p;
+      fail(EMC_ERR_ALLOC, "shell plan: cannot allocate");
+    }
+    EMC_HIP(hipMemcpy(p->d_offsets, h_offsets, sizeof(int64_t) * (n_shells + 1), hipMemcpyHostToDevice));
+    if (total) EMC_HIP(hipMemcpy(p->d_idx, h_indices, sizeof(uint32_t) * total, hipMemcpyHostToDevice));
+    *out = p;
+  });
+}
+
+emc_status emc_shell_plan_destroy(emc_ctx* ctx, emc_shell_plan* plan) {
+  return guard([&] {
+    if (!plan) return;
+    use_device(ctx);
+    EMC_HIP(hipStreamSynchronize(ctx->stream));
+    (void)hipFree(plan->d_offsets);
+    (void)hipFree(plan->d_idx);
+    delete plan;
+  });
+}
+
+emc_status emc_xcf_rotational_shells(emc_ctx* ctx, const emc_shell_plan* plan, const void* d_p, const void* d_r,
+                                     const float* d_wp, const float* d_wr, const int dims[3], int abs_flag,
+                                     double w, double* cc_out) {
+  return guard([&] {
+    use_device(ctx);
+    EMC_REQUIRE(plan && d_p && d_r && dims && cc_out, "null pointer");
+    const int64_t n = (int64_t)dims[0] * dims[1] * dims[2];
+    EMC_REQUIRE(plan->max_index <= n, "shell index %lld beyond the %lld-voxel volume", (long long)plan->max_index,
+                (long long)n);
+    const float* wp = d_wp;
+    const float* wr = d_wr;
+    float h0[2] = {0.0f, 0.0f};
+    if (wp) EMC_HIP(hipMemcpyAsync(&h0[0], wp, 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (wr) EMC_HIP(hipMemcpyAsync(&h0[1], wr, 4, hipMemcpyDeviceToHost, ctx->stream));
+    EMC_HIP(hipStreamSynchronize(ctx->stream));
+    if (wp && h0[0] != h0[0]) wp = nullptr;  // NaN first element: weight 1 (:34-40)
+    if (wr && h0[1] != h0[1]) wr = nullptr;
+    double* part =
+        (double*)scratch(ctx, "xrot_shell_part", sizeof(double) * 3 * kShellBlocks * (size_t)plan->n_shells);
+    double* out = (double*)scratch(ctx, "xrot_shell_out", sizeof(double));
+    hipLaunchKernelGGL(xcf_rot_shell_kernel, dim3(kShellBlocks, plan->n_shells), dim3(kRedThreads), 0, ctx->stream,
+                       (const float2*)d_p, (const float2*)d_r, wp, wr, (const int64_t*)plan->d_offsets,
+                       (const uint32_t*)plan->d_idx, part);
+    hipLaunchKernelGGL(xcf_rot_shell_fold_kernel, dim3(1), dim3(kWave), 0, ctx->stream, (const double*)part,
+                       plan->n_shells, kShellBlocks, abs_flag, w, out);
+    EMC_HIP(hipGetLastError());
+    EMC_HIP(hipMemcpyAsync(cc_out, out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    EMC_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
 
 emc_status emc_xcf_translational(emc_ctx* ctx, emc_fft_plan* plan, const void* d_a, const void* d_b,
                                  const int dims[3], const float* d_peak_mask, const int com[3],

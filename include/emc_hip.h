@@ -250,6 +250,22 @@ emc_status emc_xcf_rotational(emc_ctx* ctx, const void* d_p_c64, const void* d_r
                               const float* d_wp, const float* d_wr, const int dims[3],
                               double* cc_out);
 
+/* Shell-weighted form of BH_multi_xcf_Rotational (wCCC a cell,
+ * statistics/BH_multi_xcf_Rotational.m:42-70, the cell built by
+ * BH_multi_cRef's calc_shells): a plan holds the shells' voxel lists
+ * (wCCC{3..end}: 1-based linear indices into the full grid, concatenated,
+ * h_offsets[s]..h_offsets[s+1] for shell s), made once per set of shells.
+ * Per shell CCC_s = sum Re(Wr P R Wp) / sqrt(sum|R Wp|^2 sum|P Wr|^2) over
+ * its voxels; cc = sum_s f(CCC_s), f(c) = (c^2 / (c + w^-1.5))^3, |f| when
+ * abs_flag (wCCC{1}), w = wCCC{2}. Synchronous. */
+typedef struct emc_shell_plan emc_shell_plan;
+emc_status emc_shell_plan_create(emc_ctx* ctx, int n_shells, const int64_t* h_offsets, const uint32_t* h_indices,
+                                 emc_shell_plan** out);
+emc_status emc_shell_plan_destroy(emc_ctx* ctx, emc_shell_plan* plan);
+emc_status emc_xcf_rotational_shells(emc_ctx* ctx, const emc_shell_plan* plan, const void* d_p_c64,
+                                     const void* d_r_c64, const float* d_wp, const float* d_wr, const int dims[3],
+                                     int abs_flag, double w, double* cc_out);
+
 /* -------------------------------------------- alignment loop (A15, A17) */
 
 /* One particle's plain grid search, the two-stage body of

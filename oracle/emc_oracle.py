@@ -633,15 +633,40 @@ def xcf_translational(A, B, peak_mask, peak_com, return_map=False):
 # --------------------------------------------------------------------------
 
 
-def xcf_rotational(P, R, Wp, Wr):
+def xcf_rotational(P, R, Wp, Wr, wCCC=0):
     """CC = sum Re(Wr P R Wp) / sqrt(sum|R Wp|^2 * sum|P Wr|^2) (:73-78);
-    NaN first element of a weight -> weight 1 (:34-40)."""
+    NaN first element of a weight -> weight 1 (:34-40).
+
+    wCCC a list [abs flag, w, shell_1, shell_2, ...] (the reference's cell,
+    :50-70): each shell a list of 1-based MATLAB linear indices; volumes are
+    (nz, ny, nx), whose flat C order runs x fastest like MATLAB's linear
+    index, so index i names flat element i-1. Per shell the CC above over
+    its voxels only, c; the result is sum over shells of
+    (c^2 / (c + w^-1.5))^3, its absolute value when the flag is set (:65-69).
+    """
     Wr = np.asarray(Wr, dtype=np.float64)
     Wp = np.asarray(Wp, dtype=np.float64)
     if np.isnan(Wr.flat[0]):
         Wr = np.float64(1.0)
     if np.isnan(Wp.flat[0]):
         Wp = np.float64(1.0)
+    if isinstance(wCCC, (list, tuple)):
+        shape = np.shape(P)
+        P = np.asarray(P, np.complex128).ravel()
+        R = np.asarray(R, np.complex128).ravel()
+        Wp = np.broadcast_to(Wp, shape).ravel()
+        Wr = np.broadcast_to(Wr, shape).ravel()
+        take_abs = bool(np.asarray(wCCC[0]).ravel()[0])
+        k = float(np.asarray(wCCC[1]).ravel()[0]) ** -1.5
+        total = 0.0
+        for shell in wCCC[2:]:
+            j = np.asarray(shell, np.int64).ravel() - 1
+            num = np.sum(np.real(Wr[j] * P[j] * R[j] * Wp[j]))
+            den = np.sum(np.abs(R[j] * Wp[j]) ** 2) * np.sum(np.abs(P[j] * Wr[j]) ** 2)
+            c = float(num / math.sqrt(den)) if den > 0 else float("nan")
+            t = (c * c / (c + k)) ** 3
+            total += abs(t) if take_abs else t
+        return total
     num = np.sum(np.real(Wr * P * R * Wp))
     den = np.sum(np.abs(R * Wp) ** 2) * np.sum(np.abs(P * Wr) ** 2)
     return float(num / math.sqrt(den))

@@ -185,6 +185,35 @@ def test_xcf_rotational_matches_oracle(ctx, rng):
     assert abs(BH_multi_xcf_Rotational(P, np.conj(P), 1, 1) - 1.0) < 1e-6
 
 
+def test_xcf_rotational_shells_matches_oracle(ctx, rng):
+    """Shell-weighted CCC: radial shells (the layout calc_shells makes), NaN
+    weight, abs flag, the ShellPlan reused across calls; an index beyond the
+    volume is rejected."""
+    from emclarity_amd.xcf import ShellPlan
+    n = 24
+    P = np.fft.fftn(rng.standard_normal((n, n, n))).astype(np.complex64)
+    R = np.conj(np.fft.fftn(rng.standard_normal((n, n, n)) + 0.5 * np.fft.ifftn(P).real)).astype(np.complex64)
+    wp = rng.uniform(0, 1, (n, n, n)).astype(np.float32)
+    wr = rng.uniform(0, 1, (n, n, n)).astype(np.float32)
+    f = np.fft.fftfreq(n) * n
+    z, y, x = np.meshgrid(f, f, f, indexing="ij")
+    r = np.sqrt(x * x + y * y + z * z).ravel()
+    edges = [0, 2.5] + list(np.arange(3.5, 12.5, 1.5))
+    shells = [np.flatnonzero((lo <= r) & (r < hi)).astype(np.uint32) + 1 for lo, hi in zip(edges[:-1], edges[1:])]
+    w = float(sum(len(s) for s in shells[1:]))
+    P64, R64 = P.astype(np.complex128), R.astype(np.complex128)
+    for flag in (1, 0):
+        cell = [flag, w] + shells
+        plan = ShellPlan(cell)
+        for Wp, Wr in ((wp, wr), (wp, np.full((n, n, n), np.nan, np.float32)), (1, 1)):
+            ref = O.xcf_rotational(P64, R64, Wp, Wr, cell)
+            for c in (BH_multi_xcf_Rotational(P, R, Wp, Wr, plan), BH_multi_xcf_Rotational(P, R, Wp, Wr, cell)):
+                assert abs(c - ref) < 1e-5 * max(abs(ref), 1e-6), (flag, c, ref)
+        plan.close()
+    with pytest.raises(_lib.EmcError):
+        BH_multi_xcf_Rotational(P, R, 1, 1, [1, 1.0, np.array([n ** 3 + 1])])
+
+
 def test_device_block_cache(ctx):
     """emc_free keeps emc_malloc blocks for reuse (same size class, same
     address), data written after reuse is intact, and a second free of the

@@ -321,6 +321,51 @@ def test_xcf_rotational_invariants(rng):
     assert math.isclose(O.xcf_rotational(P, np.conj(Q), w, wr), c, rel_tol=1e-12)
 
 
+def _radial_shells(n, edges):
+    """1-based linear indices of the voxels with edges[i] <= r < edges[i+1]
+    (radius from the FFT origin of an (n, n, n) grid)."""
+    f = np.fft.fftfreq(n) * n
+    z, y, x = np.meshgrid(f, f, f, indexing="ij")
+    r = np.sqrt(x * x + y * y + z * z).ravel()
+    return [np.flatnonzero((lo <= r) & (r < hi)) + 1 for lo, hi in zip(edges[:-1], edges[1:])]
+
+
+def test_xcf_rotational_shells_closed_forms(rng):
+    """Shell-weighted CCC (BH_multi_xcf_Rotational.m:50-70) against values
+    the formula fixes: per-shell scale invariance (each shell's own
+    denominator), f(1) = (1/(1+w^-1.5))^3, f(-1) with and without abs, a
+    one-voxel shell at MATLAB index 1 and one full shell = f(plain CC)."""
+    n = 8
+    R = np.fft.fftn(rng.standard_normal((n, n, n)))
+    W = rng.uniform(0.2, 1, R.shape)
+    sh = _radial_shells(n, [0, 1.5, 3, 7])
+    # P a different positive multiple of conj(R) on each shell: every c = 1
+    P = np.conj(R).copy()
+    for s, t in zip(sh, (0.5, 3.0, 20.0)):
+        P.flat[s - 1] *= t
+    w = 4.0
+    f1 = (1.0 / (1.0 + 0.125)) ** 3
+    assert math.isclose(O.xcf_rotational(P, R, W, W, [0, w] + sh), 3 * f1, rel_tol=1e-12)
+    # the same P with one shared denominator would not give c = 1 per shell
+    assert O.xcf_rotational(P, R, W, W) < 0.99
+    # c = -1 on shell 2 only: f(-1) = (1/(-1+w^-1.5))^3 < 0; abs flips it
+    Pm = P.copy()
+    Pm.flat[sh[1] - 1] *= -1
+    fm1 = (1.0 / (-1.0 + 0.125)) ** 3
+    assert math.isclose(O.xcf_rotational(Pm, R, W, W, [0, w] + sh), 2 * f1 + fm1, rel_tol=1e-12)
+    assert math.isclose(O.xcf_rotational(Pm, R, W, W, [1, w] + sh), 2 * f1 - fm1, rel_tol=1e-12)
+    # one-voxel shell at MATLAB index 1 = flat element 0: c = cos(phase)
+    Q = np.fft.fftn(rng.standard_normal((n, n, n)))
+    Q.flat[0] = np.conj(R.flat[0]) * np.exp(0.3j)
+    c = math.cos(0.3)
+    assert math.isclose(O.xcf_rotational(Q, R, W, W, [0, w, [1]]), (c * c / (c + 0.125)) ** 3, rel_tol=1e-12)
+    # one shell holding every voxel: f of the plain CC
+    cc = O.xcf_rotational(Q, R, W, 1.0)
+    full = np.arange(1, n ** 3 + 1)
+    assert math.isclose(O.xcf_rotational(Q, R, W, 1.0, [0, w, full]), (cc * cc / (cc + 0.125)) ** 3,
+                        rel_tol=1e-12)
+
+
 # ------------------------------------------------------------- template search
 
 def test_swap_quadrants_is_shift_and_sign():
