@@ -59,6 +59,7 @@ SIGNATURES = {
     "emc_fft_c2c": (_st, [_vp, _vp, _vp, C.c_int]),
     "emc_bandpass3d": (_st, [_vp, _i3, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _vp]),
     "emc_pad_zeros3d": (_st, [_vp, _vp, _i3, _i3, _i3, C.c_float, C.c_int, _vp]),
+    "emc_ft_condition": (_st, [_vp, _vp, _i3, C.c_int, C.c_int, _vp, _vp]),
     "emc_band_limit_center_normalize": (_st, [_vp, _vp, _vp, _i3, _i3, _i3, _vp, _vp]),
     "emc_ref_ft1": (_st, [_vp, _vp, _vp, _vp, _i3, _i3, _i3, _vp]),
     "emc_xcf_translational": (_st, [_vp, _vp, _vp, _vp, _i3, _vp, _i3, _vp, _d, _d]),

@@ -190,6 +190,59 @@ void blcn_half(emc_ctx* ctx, float2* F, const int dims[3], int batch, const floa
   EMC_HIP(hipGetLastError());
 }
 
+// fourierTransformer.fwdFFT's options on an R2C half spectrum (cx = nx/2+1
+// x-planes): f' = f * ctf * norm * bp, then (centre) f'(0) = 0 and
+// f' / sqrt(2 * sum |f'|^2 over x < cx - invTrim). Pass 1 (centre only)
+// forms per-block partial sums; pass 2 writes f' scaled.
+__device__ inline float2 ftc_value(const float2* F, int64_t i, const float* ctf, const float* bp, float norm) {
+  float2 f = F[i];
+  if (ctf) {
+    f.x *= ctf[i];
+    f.y *= ctf[i];
+  }
+  f.x *= norm;
+  f.y *= norm;
+  if (bp) {
+    f.x *= bp[i];
+    f.y *= bp[i];
+  }
+  return f;
+}
+
+__global__ void __launch_bounds__(kRedThreads) ftc_sum_kernel(const float2* __restrict__ F, int64_t nh, int cx,
+                                                              int x_end, const float* __restrict__ ctf,
+                                                              const float* __restrict__ bp, float norm,
+                                                              double* __restrict__ part) {
+  double v[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nh; i += (int64_t)gridDim.x * blockDim.x) {
+    if (i == 0 || (int)(i % cx) >= x_end) continue;
+    const float2 f = ftc_value(F, i, ctf, bp, norm);
+    v[0] += (double)f.x * f.x + (double)f.y * f.y;
+  }
+  block_sum<1>(v);
+  if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+
+__global__ void __launch_bounds__(kRedThreads) ftc_fold_kernel(const double* __restrict__ part, int nb,
+                                                               double* __restrict__ scale) {
+  double v[1] = {0.0};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) v[0] += part[b];
+  block_sum<1>(v);
+  if (threadIdx.x == 0) scale[0] = 1.0 / sqrt(2.0 * v[0]);
+}
+
+__global__ void ftc_apply_kernel(float2* __restrict__ F, int64_t nh, const float* __restrict__ ctf,
+                                 const float* __restrict__ bp, float norm, const double* __restrict__ scale) {
+  const float s = scale ? (float)scale[0] : 1.0f;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nh; i += (int64_t)gridDim.x * blockDim.x) {
+    float2 f = ftc_value(F, i, ctf, bp, norm);
+    if (scale) {
+      f = i == 0 ? make_float2(0.0f, 0.0f) : make_float2(f.x * s, f.y * s);
+    }
+    F[i] = f;
+  }
+}
+
 static __global__ void mul_f32_kernel(const float* __restrict__ a, const float* __restrict__ b,
                                       float* __restrict__ o, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -252,6 +305,37 @@ emc_status emc_pad_zeros3d(emc_ctx* ctx, const float* d_in, const int dims[3], c
     EMC_REQUIRE(d_in != d_out, "d_out must not alias d_in");
     PadArgs a = pad_args(dims, pad_low, pad_top, extrap_val, taper);
     launch_pad(ctx, d_in, d_out, a, nullptr);
+  });
+}
+
+emc_status emc_ft_condition(emc_ctx* ctx, void* d_ft_c64, const int dims[3], int do_norm, int do_center,
+                            const float* d_bp_half, const float* d_ctf_half) {
+  return guard([&] {
+    use_device(ctx);
+    EMC_REQUIRE(d_ft_c64 && dims, "null pointer");
+    EMC_REQUIRE(dims[0] > 0 && dims[1] > 0 && dims[2] > 0, "bad dims");
+    EMC_REQUIRE(do_norm >= 0 && do_norm <= 2, "do_norm must be 0, 1 or 2");
+    const int cx = dims[0] / 2 + 1;
+    const int64_t nh = (int64_t)cx * dims[1] * dims[2];
+    const int inv_trim = dims[0] % 2 ? 1 : 2;
+    // (1/sqrt(numel))^doNorm, only without centring (fourierTransformer.m:130-132)
+    const float norm =
+        (do_norm && !do_center) ? (float)pow(1.0 / sqrt((double)dims[0] * dims[1] * dims[2]), do_norm) : 1.0f;
+    float2* F = (float2*)d_ft_c64;
+    const int grid = (int)std::min<int64_t>(1024, ceil_div(nh, (int64_t)256));
+    double* scale = nullptr;
+    if (do_center) {
+      const int nb = 256;
+      double* part = (double*)scratch(ctx, "ftc_part", sizeof(double) * (nb + 1));
+      scale = part + nb;
+      hipLaunchKernelGGL(ftc_sum_kernel, dim3(nb), dim3(kRedThreads), 0, ctx->stream, (const float2*)F, nh, cx,
+                         cx - inv_trim, d_ctf_half, d_bp_half, norm, part);
+      hipLaunchKernelGGL(ftc_fold_kernel, dim3(1), dim3(kRedThreads), 0, ctx->stream, (const double*)part, nb,
+                         scale);
+    }
+    hipLaunchKernelGGL(ftc_apply_kernel, dim3(grid), dim3(256), 0, ctx->stream, F, nh, d_ctf_half, d_bp_half, norm,
+                       (const double*)scale);
+    EMC_HIP(hipGetLastError());
   });
 }
 

@@ -87,20 +87,44 @@ class fourierTransformer:
         self.normalization_factor = 1.0 / np.sqrt(np.prod(self.inputSize))
         self.plan = FFTPlan(self.inputSize, 1, self.ctx)
         self.bandpass = None
-        self.bpVals = None
+        self.bpVals = [0.0, 0.0, 0.0, 0.0]
 
-    def fwdFFT(self, inputVol, doNorm=0, doCenter=False, bp_vals=None):
-        """fwdFFT(x) (:92-145): R2C half spectrum, unscaled. The normalise /
-        centre / bandpass options of the reference are used only by the
-        template-search preprocessing, which this package does not implement."""
-        if doNorm or doCenter or bp_vals is not None:
-            raise NotImplementedError("fwdFFT options (norm/center/bandpass) are not implemented")
+    def makeBandPass(self, sizeInput, bpValsNew):
+        """(:294-328) BH_bandpass3d(size, bp(1), bp(2), bp(3), 'GPU', bp(4))
+        cut to the first nx/2+1 x-planes; rebuilt only when
+        |sum(bpVals - bpValsNew)| > 1e-4, as the reference compares."""
+        bp = [float(v) for v in bpValsNew]
+        if self.bandpass is None or abs(sum(np.subtract(self.bpVals, bp))) > 1e-4:
+            from .masking import BH_bandpass3d
+            self.bandpass = BH_bandpass3d(sizeInput, bp[0], bp[1], bp[2], "GPU", bp[3], half=True, ctx=self.ctx)
+            self.bpVals = bp
+
+    def fwdFFT(self, inputVol, doNorm=0, doCenter=False, bp_vals=None, ctf=None):
+        """fwdFFT(x[, doNorm, doCenter[, bp[, ctf]]]) (:92-145): R2C half
+        spectrum; then x ctf (half grid), x (1/sqrt(numel))^doNorm unless
+        centring, x bandpass (bp_vals = [HIGH_THRESH, HIGH_CUT, LOW_CUT,
+        PIXEL_SIZE]), and with doCenter F(0) = 0 and F / sqrt(2 sum|F|^2)
+        over the x-planes below nx/2+1-invTrim."""
         v = as_device(inputVol, np.float32, self.ctx)
-        return self.plan.r2c(v)
+        ft = self.plan.r2c(v)
+        bp = None
+        if bp_vals is not None and not (np.isscalar(bp_vals) or len(bp_vals) == 0):
+            self.makeBandPass(v.dims, bp_vals)
+            bp = self.bandpass
+        c = None if ctf is None else as_device(ctf, np.float32, self.ctx)
+        if c is not None and c.shape != ft.shape:
+            raise ValueError(f"ctf must be the half grid {ft.shape}, not {c.shape}")
+        if doNorm or doCenter or bp is not None or c is not None:
+            _lib.call("emc_ft_condition", self.ctx.handle, ft.vp, _lib.i3(self.inputSize), int(doNorm),
+                      int(bool(doCenter)), bp.vp if bp is not None else None, c.vp if c is not None else None)
+        return ft
 
     def invFFT(self, inputVol, doNorm=0):
-        """invFFT(x) (:147-159), unscaled; the input is preserved."""
-        if doNorm:
-            raise NotImplementedError("invFFT normalisation is not implemented")
+        """invFFT(x[, doNorm]) (:147-159): C2R, then x (1/sqrt(numel))^doNorm;
+        the input is preserved."""
         v = as_device(inputVol, np.complex64, self.ctx).copy()
-        return self.plan.c2r(v)
+        out = self.plan.c2r(v)
+        if doNorm:
+            _lib.call("emc_scale", self.ctx.handle, out.vp, C.c_int64(int(np.prod(out.shape))),
+                      float(self.normalization_factor ** int(doNorm)))
+        return out

@@ -207,6 +207,16 @@ emc_status emc_pad_zeros3d(emc_ctx* ctx, const float* d_in, const int dims[3],
                            const int pad_low[3], const int pad_top[3], float extrap_val,
                            int taper, float* d_out);
 
+/* fourierTransformer.fwdFFT(vol, doNorm, doCenter, bp, ctf) options
+ * (testScripts/fourierTransformer.m:92-145) applied in place to the R2C half
+ * spectrum d_ft of a real volume of `dims` (nx/2+1 x-planes): f *= ctf
+ * (d_ctf_half, NULL = none); without centring f *= (1/sqrt(numel))^do_norm;
+ * f *= bandpass (d_bp_half: BH_bandpass3d's first nx/2+1 x-planes, NULL =
+ * none); with do_center f(0) = 0 and f /= sqrt(2 sum|f|^2 over the x-planes
+ * x < nx/2+1 - invTrim), invTrim = 2 for even nx, 1 for odd. Sums in FP64. */
+emc_status emc_ft_condition(emc_ctx* ctx, void* d_ft_c64, const int dims[3], int do_norm, int do_center,
+                            const float* d_bp_half, const float* d_ctf_half);
+
 /* BH_bandLimitCenterNormalize(IMG, BANDPASS, '', PAD, 'single')
  * (masking/BH_bandLimitCenterNormalize.m:27-61) on the half spectrum: pad
  * with the image mean, R2C, F(0)=0, x bandpass (NULL = none; full-grid FFT

@@ -628,6 +628,30 @@ def xcf_translational(A, B, peak_mask, peak_com, return_map=False):
     return peak
 
 
+def fwd_fft(vol, doNorm=0, doCenter=False, bp_half=None, ctf=None):
+    """fourierTransformer.fwdFFT (testScripts/fourierTransformer.m:92-145) on
+    a (nz, ny, nx) volume: R2C half spectrum along x (the last axis here,
+    MATLAB's first), x ctf, x (1/sqrt(numel))^doNorm when not centring
+    (:130-132), x bandpass (:133-136), and with centring F(1) = 0 then
+    F / sqrt(2 sum|F(1:end-invTrim,:,:)|^2) (:137-140), invTrim = 2 for
+    even nx, 1 for odd (:62-66)."""
+    vol = np.asarray(vol, np.float64)
+    nx = vol.shape[-1]
+    ft = np.fft.rfftn(vol)
+    if ctf is not None:
+        ft = ft * ctf
+    if doNorm and not doCenter:
+        ft = ft * (1.0 / math.sqrt(vol.size)) ** doNorm
+    if bp_half is not None:
+        ft = ft * bp_half
+    if doCenter:
+        ft = ft.copy()
+        ft.flat[0] = 0
+        inv_trim = 1 if nx % 2 else 2
+        ft = ft / math.sqrt(2.0 * np.sum(np.abs(ft[..., :ft.shape[-1] - inv_trim]) ** 2))
+    return ft
+
+
 # --------------------------------------------------------------------------
 # A14 BH_multi_xcf_Rotational (statistics/BH_multi_xcf_Rotational.m:29-80)
 # --------------------------------------------------------------------------

@@ -128,6 +128,27 @@ def test_fft_roundtrip_and_numpy(ctx, rng):
     assert np.max(np.abs(inv.real - v)) < 1e-4
 
 
+@pytest.mark.parametrize("shape", [(12, 20, 32), (9, 14, 25)])
+def test_fourier_transformer_options_match_oracle(ctx, rng, shape):
+    """fwdFFT(x, doNorm, doCenter, bp, ctf) and invFFT(x, doNorm), even and
+    odd nx; the bandpass is BH_bandpass3d's first nx/2+1 x-planes."""
+    v = rng.standard_normal(shape).astype(np.float32) + 1.5
+    nz, ny, nx = shape
+    bpv = [1e-3, 40.0, 8.0, 2.5]
+    bp_half = O.bandpass3d((nx, ny, nz), *bpv)[..., :nx // 2 + 1]
+    ctf = rng.uniform(-1, 1, (nz, ny, nx // 2 + 1)).astype(np.float32)
+    ft = fourierTransformer(v)
+    for norm, centre, bp, c in ((1, False, None, None), (2, False, bpv, None), (0, True, bpv, None),
+                                (2, True, None, ctf), (1, False, bpv, ctf)):
+        out = ft.fwdFFT(v, norm, centre, bp, c).get()
+        ref = O.fwd_fft(v, norm, centre, bp_half if bp else None, c)
+        assert rel(out, ref) < 2e-5, (norm, centre, bp is not None, c is not None)
+    F = ft.fwdFFT(v)
+    for norm in (1, 2):
+        back = ft.invFFT(F, norm).get()
+        assert rel(back, v * v.size / np.sqrt(v.size) ** norm) < 1e-5
+
+
 def test_band_limit_center_normalize_matches_oracle(ctx, rng):
     img = (rng.standard_normal((24, 24, 24)) + 2.0).astype(np.float32)
     bp = BH_bandpass3d((28, 28, 28), 1e-3, 40.0, 8.0, "GPU", 2.5)

@@ -237,6 +237,34 @@ def test_bandpass_closed_form():
 
 # ----------------------------------------------- band-limit / normalise / pad
 
+def test_fwd_fft_options_closed_forms(rng):
+    """fourierTransformer.fwdFFT options: doNorm 1/2 scale by N^-1/2 and N^-1
+    (MATLAB's fftn is the unnormalised DFT, pinned on a delta); centring on a
+    volume whose x=0 and Nyquist planes hold only the DC term (an offset plus
+    a cosine along x) leaves F(0) = 0 and, by Parseval, a full-grid
+    sum |F|^2 of exactly 1; centring ignores doNorm."""
+    nz, ny, nx = 6, 10, 16
+    d = np.zeros((nz, ny, nx))
+    d[0, 0, 0] = 1.0
+    assert np.allclose(O.fwd_fft(d), 1.0)
+    n = d.size
+    v = rng.standard_normal((nz, ny, nx))
+    assert np.allclose(O.fwd_fft(v, 1), np.fft.rfftn(v) / math.sqrt(n), rtol=0, atol=1e-12)
+    assert np.allclose(O.fwd_fft(v, 2), np.fft.rfftn(v) / n, rtol=0, atol=1e-12)
+    g = rng.standard_normal((nz, ny, 1))
+    x = np.arange(nx)
+    vol = 3.0 + g * np.cos(2 * np.pi * 3 * x / nx)
+    for norm in (0, 1, 2):
+        ft = O.fwd_fft(vol, norm, True)
+        assert ft.flat[0] == 0
+        full = np.sum(np.abs(ft[..., 0]) ** 2) + 2 * np.sum(np.abs(ft[..., 1:nx // 2]) ** 2) + \
+            np.sum(np.abs(ft[..., nx // 2]) ** 2)
+        assert math.isclose(full, 1.0, rel_tol=1e-12)
+    # half bandpass multiplies plane-wise; ctf multiplies before the norm
+    bp = rng.uniform(0, 1, (nz, ny, nx // 2 + 1))
+    assert np.allclose(O.fwd_fft(v, 1, False, bp, 2 * bp), 2 * bp * bp * np.fft.rfftn(v) / math.sqrt(n))
+
+
 def test_blcn_invariants(rng):
     img = rng.standard_normal((8, 8, 8)) + 3.0
     F = O.band_limit_center_normalize(img, None, [[0, 0, 0], [0, 0, 0]])
