@@ -38,13 +38,23 @@ class ParticleAligner:
     (the per-particle state of BH_alignRaw3d_v2.m:1084-1110)."""
 
     def __init__(self, particle, ref, wedge, bandpass, peak_mask, vol_mask=None, ref_wgt_rot=None,
-                 ref_wgt=None, angles=None, shift_val=(0.0, 0.0, 0.0), peak_com=(3, 3, 3), ctx=None):
+                 ref_wgt=None, angles=None, shift_val=(0.0, 0.0, 0.0), peak_com=(3, 3, 3), pad_calc=None,
+                 ctx=None):
+        """particle, ref, vol_mask: window W^3; wedge, bandpass, peak_mask,
+        ref_wgt_rot, ref_wgt: calc C^3. pad_calc = [[low x,y,z], [top x,y,z]]
+        (BH_multi_validArea's padCalc) with C = W + low + top; None: C = W."""
         self.particle = as_device(particle, np.float32, ctx)
         self.ctx = self.particle.ctx
         dims = self.particle.dims
         if not (dims[0] == dims[1] == dims[2]):
             raise ValueError("cubic boxes only")
-        self.n = dims[0]
+        self.w = dims[0]
+        pc = np.zeros((2, 3), int) if pad_calc is None else np.asarray(pad_calc, int).reshape(2, 3)
+        calc = self.w + pc[0] + pc[1]
+        if not (calc[0] == calc[1] == calc[2]) or pc.min() < 0:
+            raise ValueError(f"padCalc {pc.tolist()} must pad the window to a cube")
+        self.pad_calc = pc
+        self.n = int(calc[0])
         dev = lambda a: None if a is None else as_device(a, np.float32, self.ctx)
         self.ref = dev(ref)
         self.wedge = dev(wedge)
@@ -62,11 +72,19 @@ class ParticleAligner:
         self.shift_val = np.asarray(shift_val, dtype=np.float64)
         self.peak_com = [int(c) for c in peak_com]
         # ref_FT1 = conj(BH_bandLimitCenterNormalize(ref.*volMask, 1, ...)) (:607-608)
-        self.plan = FFTPlan(dims, 1, self.ctx)
+        for name in ("wedge", "bandpass", "peak_mask", "ref_wgt_rot", "ref_wgt"):
+            a = getattr(self, name)
+            if a is not None and a.dims != (self.n,) * 3:
+                raise ValueError(f"{name} must be the calc size {self.n}^3, not {a.dims}")
+        for name in ("ref", "vol_mask"):
+            a = getattr(self, name)
+            if a is not None and a.dims != dims:
+                raise ValueError(f"{name} must be the window size {self.w}^3, not {a.dims}")
+        self.plan = FFTPlan((self.n,) * 3, 1, self.ctx)
         self.ref_ft1 = DeviceArray((self.n, self.n, self.n // 2 + 1), np.complex64, self.ctx)
         _lib.call("emc_ref_ft1", self.ctx.handle, self.plan.handle, self.ref.vp,
                   self.vol_mask.vp if self.vol_mask is not None else None, _lib.i3(dims),
-                  _lib.i3((0, 0, 0)), _lib.i3((0, 0, 0)), self.ref_ft1.vp)
+                  _lib.i3(pc[0]), _lib.i3(pc[1]), self.ref_ft1.vp)
 
     def grid(self, angles_pts, est0, mode=3):
         """emc_align_grid over (phi, theta, psi) rows -> (ccc[n], estPeak[n,3])."""
@@ -76,7 +94,8 @@ class ParticleAligner:
         est = np.zeros((n, 3))
         dp = C.POINTER(C.c_double)
         vp = lambda a: a.vp if a is not None else None
-        _lib.call("emc_align_grid", self.ctx.handle, self.n, self.particle.vp, self.ref.vp,
+        _lib.call("emc_align_grid_padded", self.ctx.handle, self.w, self.n, _lib.i3(self.pad_calc[0]),
+                  self.particle.vp, self.ref.vp,
                   vp(self.ref_wgt_rot), vp(self.ref_wgt), self.ref_ft1.vp, vp(self.vol_mask),
                   self.peak_mask.vp, self.bandpass.vp, self.wedge.vp,
                   _lib.d_arr(self.angles.reshape(-1, order="F")), _lib.d_arr(est0),

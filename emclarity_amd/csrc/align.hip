@@ -125,11 +125,16 @@ __global__ void copy_peak_kernel(const PeakResult* __restrict__ r, double* __res
 }
 
 struct AlignBuffers {
-  float *rref, *rwdg, *rmask, *map, *rpart, *rwedge;
+  float *rref, *rwdg, *rmask, *map, *rpart, *rwedge, *cpad;
   float2 *fref, *P1, *X, *F2;
   PeakResult* res;
-  double *ccc, *est, *est0, *part;
+  double *ccc, *est, *est0, *part, *stats;
 };
+
+// mean = sum / n for the pad value (stats[0] = sum)
+__global__ void mean_of_sum_kernel(const double* __restrict__ stats, double n, double* __restrict__ mean) {
+  if (threadIdx.x == 0) mean[0] = stats[0] / n;
+}
 
 }  // namespace emc
 
@@ -137,35 +142,46 @@ using namespace emc;
 
 extern "C" {
 
-emc_status emc_align_grid(emc_ctx* ctx, int n, const float* d_particle, const float* d_ref,
-                          const float* d_ref_wgt_rot, const float* d_ref_wgt, const void* d_ref_ft1,
-                          const float* d_vol_mask, const float* d_peak_mask, const float* d_bandpass,
-                          const float* d_wedge, const double angles_colmajor[9],
-                          const double est0[3], const int peak_com[3], const double* angles_pts,
-                          int n_orient, int mode, double* ccc_out, double* shift_out) {
+emc_status emc_align_grid_padded(emc_ctx* ctx, int w, int n, const int pad_low[3], const float* d_particle,
+                                 const float* d_ref, const float* d_ref_wgt_rot, const float* d_ref_wgt,
+                                 const void* d_ref_ft1, const float* d_vol_mask, const float* d_peak_mask,
+                                 const float* d_bandpass, const float* d_wedge, const double angles_colmajor[9],
+                                 const double est0[3], const int peak_com[3], const double* angles_pts,
+                                 int n_orient, int mode, double* ccc_out, double* shift_out) {
   return guard([&] {
     use_device(ctx);
-    EMC_REQUIRE(n > 0 && n <= 1024, "bad box size %d", n);
+    EMC_REQUIRE(n > 0 && n <= 1024, "bad calc size %d", n);
+    EMC_REQUIRE(w > 0 && w <= n, "window size %d must be in 1..calc size %d", w, n);
     EMC_REQUIRE(d_particle && d_ref && d_ref_ft1 && d_bandpass && d_wedge && angles_colmajor &&
-                    est0 && peak_com && angles_pts && ccc_out && shift_out,
+                    est0 && peak_com && angles_pts && ccc_out && shift_out && pad_low,
                 "null pointer");
+    int pad_top[3];
+    for (int k = 0; k < 3; ++k) {
+      pad_top[k] = n - w - pad_low[k];
+      EMC_REQUIRE(pad_low[k] >= 0 && pad_top[k] >= 0, "padCalc low %d does not fit window %d in calc %d",
+                  pad_low[k], w, n);
+    }
     EMC_REQUIRE(n_orient > 0, "n_orient must be > 0");
     EMC_REQUIRE(mode >= 1 && mode <= 3, "mode must be 1 (translation), 2 (score) or 3 (both)");
     const int dims[3] = {n, n, n};
+    const int wdims[3] = {w, w, w};
+    const bool padded = w < n;
     const int64_t nv = (int64_t)n * n * n;
+    const int64_t wv = (int64_t)w * w * w;
     const int64_t nh = (int64_t)(n / 2 + 1) * n * n;
     emc_fft_plan* plan = cached_plan(ctx, n, n, n, 1);
 
     AlignBuffers B;
     char* base = (char*)scratch(ctx, "align_buf",
-                                sizeof(float) * nv * 6 + sizeof(float2) * nh * 4 + 256 * 8);
-    B.rref = (float*)base;
-    B.rwdg = B.rref + nv;
+                                sizeof(float) * (nv * 5 + wv * 2) + sizeof(float2) * nh * 4 + 256 * 8);
+    B.rwdg = (float*)base;
     B.rmask = B.rwdg + nv;
     B.map = B.rmask + nv;
-    B.rpart = B.map + nv;
-    B.rwedge = B.rpart + nv;
-    B.fref = (float2*)(B.rwedge + nv);
+    B.rwedge = B.map + nv;
+    B.cpad = B.rwedge + nv;
+    B.rref = B.cpad + nv;
+    B.rpart = B.rref + wv;
+    B.fref = (float2*)(B.rpart + wv);
     B.P1 = B.fref + nh;
     B.X = B.P1 + nh;
     B.F2 = B.X + nh;
@@ -174,18 +190,31 @@ emc_status emc_align_grid(emc_ctx* ctx, int n, const float* d_particle, const fl
     B.est = (double*)scratch(ctx, "align_est", sizeof(double) * 3 * n_orient);
     B.est0 = (double*)scratch(ctx, "align_est0", sizeof(double) * 3);
     B.part = (double*)scratch(ctx, "align_part", sizeof(double) * 3 * kRedBlocks);
+    B.stats = (double*)scratch(ctx, "align_stats", sizeof(double) * 4);
     double* h_stage = (double*)pinned(ctx, sizeof(double) * 4 * (size_t)n_orient + 64);
     std::memcpy(h_stage, est0, sizeof(double) * 3);
     EMC_HIP(hipMemcpyAsync(B.est0, h_stage, sizeof(double) * 3, hipMemcpyHostToDevice, ctx->stream));
     EMC_HIP(hipStreamSynchronize(ctx->stream));
+
+    // A window volume on its way to the calc-size FFT: padded to C with its
+    // own mean (BH_bandLimitCenterNormalize.m:48, padCalc), or used as is.
+    const PadArgs pa = pad_args(wdims, pad_low, pad_top, 0.0f, 0);
+    auto to_calc = [&](const float* vol) -> const float* {
+      if (!padded) return vol;
+      reduce_sum_sumsq(ctx, vol, wv, B.stats);
+      hipLaunchKernelGGL(mean_of_sum_kernel, dim3(1), dim3(1), 0, ctx->stream, (const double*)B.stats,
+                         (double)wv, B.stats + 2);
+      launch_pad(ctx, vol, B.cpad, pa, B.stats + 2);
+      return B.cpad;
+    };
 
     const bool stage1 = mode & 1, stage2 = mode & 2;
     const float zero3[3] = {0.0f, 0.0f, 0.0f};
     const float est0f[3] = {(float)est0[0], (float)est0[1], (float)est0[2]};
     if (stage1) {
       // rotPart_FT of stage 1 is orientation independent (:1316-1318):
-      // R2C(particle) with DC = 0 and the bandpass, computed once.
-      fft_r2c(plan, d_particle, B.P1);
+      // R2C(particle padded to C) with DC = 0 and the bandpass, computed once.
+      fft_r2c(plan, to_calc(d_particle), B.P1);
       double* d_scale = B.part;  // unused scale
       blcn_half(ctx, B.P1, dims, 1, d_bandpass, d_scale, false);
     }
@@ -208,8 +237,8 @@ emc_status emc_align_grid(emc_ctx* ctx, int n, const float* d_particle, const fl
         for (int c = 0; c < 3; ++c) rowsT[r * 3 + c] = rows[c * 3 + r];  // RotMat'
       const double* d_shift2 = B.est0;
       if (stage1) {
-        XformArgs a1 = xform_args(dims, rowsT, 1, est0f, 1);
-        launch_xform(ctx, d_ref, B.rref, a1);  // :1287-1290
+        XformArgs a1 = xform_args(wdims, rowsT, 1, est0f, 1);
+        launch_xform(ctx, d_ref, B.rref, a1);  // :1287-1290 (window size)
         XformArgs a0 = xform_args(dims, rowsT, 1, zero3, 1);
         const float* wa = nullptr;
         if (d_ref_wgt_rot) {
@@ -217,7 +246,7 @@ emc_status emc_align_grid(emc_ctx* ctx, int n, const float* d_particle, const fl
           wa = B.rwdg;
         }
         launch_xform(ctx, d_peak_mask, B.rmask, a0);  // :1300-1303
-        fft_r2c(plan, B.rref, B.fref);
+        fft_r2c(plan, to_calc(B.rref), B.fref);        // :1311-1314
         hipLaunchKernelGGL(xcorr_half_kernel, dim3(g1), dim3(256), 0, ctx->stream, (const float2*)B.P1,
                            (const float2*)B.fref, wa, d_wedge, B.X, n, n, n,
                            (float)(1.0 / (double)nv));
@@ -230,13 +259,13 @@ emc_status emc_align_grid(emc_ctx* ctx, int n, const float* d_particle, const fl
         d_shift2 = B.est + 3 * o;
       }
       if (stage2) {
-        XformArgs a2 = xform_args(dims, rows, 1, zero3, 0);
+        XformArgs a2 = xform_args(wdims, rows, 1, zero3, 0);
         a2.d_shift = d_shift2;
         a2.d_mul = d_vol_mask;
         launch_xform(ctx, d_particle, B.rpart, a2);  // :1213-1216, x volMask (:1375)
         XformArgs a3 = xform_args(dims, rows, 1, zero3, 0);
         launch_xform(ctx, d_wedge, B.rwedge, a3);  // :1251-1254
-        fft_r2c(plan, B.rpart, B.F2);
+        fft_r2c(plan, to_calc(B.rpart), B.F2);     // :1375-1377
         hipLaunchKernelGGL(rot_sums_half_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, ctx->stream,
                            (const float2*)B.F2, d_bandpass, (const float2*)d_ref_ft1,
                            (const float*)B.rwedge, d_ref_wgt, n, n, n, B.part);
@@ -247,7 +276,6 @@ emc_status emc_align_grid(emc_ctx* ctx, int n, const float* d_particle, const fl
     }
     // one synchronisation for the whole list
     double* h_ccc = h_stage;
-    double* h_est = h_stage + n_orient;
     PeakResult* h_res = nullptr;
     std::vector<PeakResult> res;
     if (stage2)
@@ -260,7 +288,6 @@ emc_status emc_align_grid(emc_ctx* ctx, int n, const float* d_particle, const fl
       h_res = res.data();
     }
     EMC_HIP(hipStreamSynchronize(ctx->stream));
-    (void)h_est;
     for (int o = 0; o < n_orient; ++o) {
       if (stage1 && h_res[o].status != 0)
         fail(EMC_ERR_RANGE, "orientation %d: failed to box out COM calc", o);
@@ -268,6 +295,18 @@ emc_status emc_align_grid(emc_ctx* ctx, int n, const float* d_particle, const fl
       for (int k = 0; k < 3; ++k) shift_out[3 * o + k] = stage1 ? h_res[o].peak[k] : est0[k];
     }
   });
+}
+
+emc_status emc_align_grid(emc_ctx* ctx, int n, const float* d_particle, const float* d_ref,
+                          const float* d_ref_wgt_rot, const float* d_ref_wgt, const void* d_ref_ft1,
+                          const float* d_vol_mask, const float* d_peak_mask, const float* d_bandpass,
+                          const float* d_wedge, const double angles_colmajor[9],
+                          const double est0[3], const int peak_com[3], const double* angles_pts,
+                          int n_orient, int mode, double* ccc_out, double* shift_out) {
+  const int zero[3] = {0, 0, 0};
+  return emc_align_grid_padded(ctx, n, n, zero, d_particle, d_ref, d_ref_wgt_rot, d_ref_wgt, d_ref_ft1,
+                               d_vol_mask, d_peak_mask, d_bandpass, d_wedge, angles_colmajor, est0, peak_com,
+                               angles_pts, n_orient, mode, ccc_out, shift_out);
 }
 
 }  // extern "C"

@@ -287,8 +287,9 @@ emc_status emc_xcf_rotational_shells(emc_ctx* ctx, const emc_shell_plan* plan, c
  *   stage 2: rotate the particle (inv, shift estPeak) and its SF3D wedge,
  *            mask, band-limit-normalise, rotational CC (:1199-1256, :1371-1400).
  * RotMat = angles_colmajor * BH_defineMatrix([phi,theta,psi-phi],'Bah','inv')
- * (:1175-1176). Window W^3 = calc C^3 (padCalc zero), C1 symmetry, one
- * reference. Inputs (device, FP32): particle W^3, ref W^3, ref_wgt_rot C^3
+ * (:1175-1176). Window W^3 = calc C^3 (padCalc zero; emc_align_grid_padded
+ * takes W < C), C1 symmetry, one reference. Inputs (device, FP32):
+ * particle W^3, ref W^3, ref_wgt_rot C^3
  * centred, ref_wgt C^3 FFT order (NULL = 1), ref_ft1 half spectrum of C^3
  * (conj(BH_bandLimitCenterNormalize(ref.*volMask,1,...)), :607), vol_mask W^3,
  * peak_mask C^3, bandpass C^3 FFT order, wedge = SF3D C^3 centred.
@@ -308,6 +309,20 @@ emc_status emc_align_grid(emc_ctx* ctx, int n, const float* d_particle, const fl
                           const double angles_colmajor[9], const double est0[3],
                           const int peak_com[3], const double* angles_pts, int n_orient, int mode,
                           double* ccc_out, double* shift_out);
+/* The same with window W^3 < calc C^3 (BH_multi_validArea's sizeWindow and
+ * sizeCalc, alignment/BH_alignRaw3d_v2.m:304): particle, ref and vol_mask
+ * are W^3, the other volumes C^3. The rotated reference (stage 1), the
+ * particle (stage 1, once) and the rotated, masked particle (stage 2) are
+ * padded to C^3 with their own mean, pad_low[k] planes below and
+ * C - W - pad_low[k] above (padCalc, BH_bandLimitCenterNormalize.m:48)
+ * before their FFT; every rotation runs at its volume's own size.
+ * emc_align_grid is this call with W = C. */
+emc_status emc_align_grid_padded(emc_ctx* ctx, int w, int c, const int pad_low[3], const float* d_particle,
+                                 const float* d_ref, const float* d_ref_wgt_rot, const float* d_ref_wgt,
+                                 const void* d_ref_ft1_c64, const float* d_vol_mask, const float* d_peak_mask,
+                                 const float* d_bandpass, const float* d_wedge, const double angles_colmajor[9],
+                                 const double est0[3], const int peak_com[3], const double* angles_pts,
+                                 int n_orient, int mode, double* ccc_out, double* shift_out);
 
 /* ------------------------------- batched alignment and its adjoint (new) */
 

@@ -734,7 +734,7 @@ def align_orientation(inp: AlignInputs, rot_mat, est_peak0):
     trim = inp.particle                                    # padWindow == 0
     i_rot_ref = xform3d(inp.ref, R.T, est_peak0, True)
     if inp.ref_wgt_rot is None:                            # no reference weight: 1
-        i_rot_wdg = np.ones(inp.ref.shape)
+        i_rot_wdg = np.ones(inp.peak_mask.shape)
     else:
         i_rot_wdg = xform3d(inp.ref_wgt_rot, R.T, [0, 0, 0], True)
     i_rot_mask = xform3d(inp.peak_mask, R.T, [0, 0, 0], True)

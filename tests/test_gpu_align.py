@@ -55,6 +55,37 @@ def test_align_grid_matches_oracle(ctx, n):
     assert abs(c0[0] - O.align_initial(inp)) < 1e-4 * abs(O.align_initial(inp))
 
 
+@pytest.mark.parametrize("w,c", [(30, 32), (29, 32)])
+def test_align_grid_padded_matches_oracle(ctx, w, c):
+    """Window W^3 smaller than the calc size C^3 (padCalc low floor((C-W)/2),
+    top the rest, BH_multi_padVal): the window volumes are padded with their
+    own mean before every FFT."""
+    truth = (40.0, 30.0, 20.0)
+    case, _, _, _, vm, _ = _inputs(w, 13, truth, snr=2.0)
+    _, wedge, bp, pm, _, rwr = _inputs(c, 13, truth, snr=2.0)
+    lo = (c - w) // 2
+    pad = [[lo] * 3, [c - w - lo] * 3]
+    pts = np.array([truth, (0.0, 0.0, 0.0), (40.0, 30.0, 30.0), (80.0, 30.0, 20.0), (200.0, 60.0, -40.0)])
+    shift_val = np.array([0.25, -0.5, 0.5])
+    al = ParticleAligner(case.particle, case.ref, wedge, bp, pm, vol_mask=vm, ref_wgt_rot=rwr,
+                         shift_val=shift_val, peak_com=(3, 3, 3), pad_calc=pad)
+    ccc, est = al.grid(pts, shift_val, mode=3)
+    inp = O.AlignInputs(particle=case.particle.astype(np.float64), ref=case.ref.astype(np.float64),
+                        ref_wgt_rot=rwr.astype(np.float64), ref_wgt=1.0,
+                        ref_ft1=O.ref_ft1_of(case.ref.astype(np.float64), vm, pad),
+                        vol_mask=vm.astype(np.float64), peak_mask=pm.astype(np.float64),
+                        bandpass=bp.astype(np.float64), wedge=wedge.astype(np.float64),
+                        angles=np.eye(3), shift_val=shift_val, pad_calc=pad)
+    rc, re = O.align_grid(inp, pts)
+    assert np.max(np.abs(ccc - rc) / np.abs(rc)) < 1e-4
+    assert np.max(np.abs(est - re)) < 1e-3
+    assert int(np.argmax(ccc)) == int(np.argmax(rc))
+    c0, _ = al.grid([[0.0, 0.0, 0.0]], shift_val, mode=2)
+    assert abs(c0[0] - O.align_initial(inp)) < 1e-4 * abs(O.align_initial(inp))
+    with pytest.raises(ValueError):
+        ParticleAligner(case.particle, case.ref, wedge, bp, pm, vol_mask=vm, pad_calc=[[1, 1, 1], [0, 0, 0]])
+
+
 def test_align_particle_recovers_ground_truth(ctx):
     """Config-0 style (64^3, 2.5 A/px, +-60 wedge, CTF df 3 um, SNR 0.1):
     the grid search + refinement recovers the orientation within 1 degree

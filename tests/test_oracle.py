@@ -714,3 +714,36 @@ def test_symmetrised_volume_is_invariant():
     a = O.interp3d(v, np.eye(3), [0, 0, 0], "Bah", "inv", "O")
     b = O.interp3d(v, G[7], [0, 0, 0], "Bah", "inv", "O")
     assert np.allclose(a, b, atol=1e-10)
+
+
+# ------------------------------------------------------------- alignment loop
+
+def test_align_grid_padded_agrees_with_unpadded():
+    """The oracle's two-stage body with a window smaller than the calc size
+    (the 63 -> 64 case of BH_multi_validArea, here 23 and 24 -> 26): the
+    same particle seen through a smaller window gives the stage-1 shift of
+    the W = C run to 0.05 px at the true orientation and at most others (a
+    wrong orientation's peak is broad and may move), mean padding adds no
+    edge, and stage 2 puts the truth first."""
+    from emclarity_amd import synthetic as S
+    truth = (40.0, 30.0, 20.0)
+    pts = np.array([truth, (0.0, 0.0, 0.0), (60.0, 30.0, 20.0), (40.0, 45.0, 20.0), (40.0, 30.0, 40.0)])
+    c = 26
+    wedge = np.fft.fftshift(S.wedge_mask(c, 80.0))
+    bp = O.bandpass3d((c, c, c), 1e-3, 40.0, 6.0, 2.5)
+    pm = S.sphere_mask(c, c // 4).astype(np.float64)
+    out = {}
+    for w, pad in ((26, [[0] * 3, [0] * 3]), (23, [[1] * 3, [2] * 3]), (24, [[1] * 3, [1] * 3])):
+        case = S.make_align_case(4, n=w, euler=truth, snr=50.0, n_blobs=10, blob_radius=5.0,
+                                 shift=(1.0, -0.5, 0.25))
+        vm = S.sphere_mask(w, 0.45 * w).astype(np.float64)
+        ref = case.ref.astype(np.float64)
+        inp = O.AlignInputs(particle=case.particle.astype(np.float64), ref=ref, ref_wgt_rot=None, ref_wgt=1.0,
+                            ref_ft1=O.ref_ft1_of(ref, vm, pad), vol_mask=vm, peak_mask=pm, bandpass=bp,
+                            wedge=wedge, angles=np.eye(3), shift_val=np.zeros(3), pad_calc=pad)
+        out[w] = O.align_grid(inp, pts)
+    for w in (23, 24):
+        cc, est = out[w]
+        assert int(np.argmax(cc)) == 0
+        d = np.max(np.abs(est - out[26][1]), axis=1)
+        assert d[0] < 0.05 and np.median(d) < 0.05, (w, d)
