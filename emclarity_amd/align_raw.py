@@ -39,10 +39,17 @@ class ParticleAligner:
 
     def __init__(self, particle, ref, wedge, bandpass, peak_mask, vol_mask=None, ref_wgt_rot=None,
                  ref_wgt=None, angles=None, shift_val=(0.0, 0.0, 0.0), peak_com=(3, 3, 3), pad_calc=None,
-                 ctx=None):
+                 symmetry="C1", ctx=None):
         """particle, ref, vol_mask: window W^3; wedge, bandpass, peak_mask,
         ref_wgt_rot, ref_wgt: calc C^3. pad_calc = [[low x,y,z], [top x,y,z]]
-        (BH_multi_validArea's padCalc) with C = W + low + top; None: C = W."""
+        (BH_multi_validArea's padCalc) with C = W + low + top; None: C = W.
+        symmetry: the point group the particle and wedge rotations average
+        over (C<n>, D<n>, O, I, I2; interpolator.m's table)."""
+        from .geometry import symmetry_matrices
+        mats = symmetry_matrices(symmetry)
+        self.symmetry = symmetry
+        self.nsym = len(mats)
+        self.sym = np.ascontiguousarray(np.stack([m.reshape(-1, order="F") for m in mats]).ravel())
         self.particle = as_device(particle, np.float32, ctx)
         self.ctx = self.particle.ctx
         dims = self.particle.dims
@@ -95,7 +102,7 @@ class ParticleAligner:
         dp = C.POINTER(C.c_double)
         vp = lambda a: a.vp if a is not None else None
         _lib.call("emc_align_grid_padded", self.ctx.handle, self.w, self.n, _lib.i3(self.pad_calc[0]),
-                  self.particle.vp, self.ref.vp,
+                  self.nsym, _lib.d_arr(self.sym), self.particle.vp, self.ref.vp,
                   vp(self.ref_wgt_rot), vp(self.ref_wgt), self.ref_ft1.vp, vp(self.vol_mask),
                   self.peak_mask.vp, self.bandpass.vp, self.wedge.vp,
                   _lib.d_arr(self.angles.reshape(-1, order="F")), _lib.d_arr(est0),

@@ -142,7 +142,8 @@ using namespace emc;
 
 extern "C" {
 
-emc_status emc_align_grid_padded(emc_ctx* ctx, int w, int n, const int pad_low[3], const float* d_particle,
+emc_status emc_align_grid_padded(emc_ctx* ctx, int w, int n, const int pad_low[3], int nsym,
+                                 const double* S_colmajor, const float* d_particle,
                                  const float* d_ref, const float* d_ref_wgt_rot, const float* d_ref_wgt,
                                  const void* d_ref_ft1, const float* d_vol_mask, const float* d_peak_mask,
                                  const float* d_bandpass, const float* d_wedge, const double angles_colmajor[9],
@@ -163,6 +164,8 @@ emc_status emc_align_grid_padded(emc_ctx* ctx, int w, int n, const int pad_low[3
     }
     EMC_REQUIRE(n_orient > 0, "n_orient must be > 0");
     EMC_REQUIRE(mode >= 1 && mode <= 3, "mode must be 1 (translation), 2 (score) or 3 (both)");
+    EMC_REQUIRE(nsym >= 1 && nsym <= kMaxSym, "nsym must be 1..%d", kMaxSym);
+    EMC_REQUIRE(nsym == 1 || S_colmajor, "symmetry matrices required for nsym > 1");
     const int dims[3] = {n, n, n};
     const int wdims[3] = {w, w, w};
     const bool padded = w < n;
@@ -233,6 +236,24 @@ emc_status emc_align_grid_padded(emc_ctx* ctx, int w, int n, const int pad_low[3
         }
       float rows[9], rowsT[9];
       colmajor_to_rows(R, rows);
+      // stage 2 rotates with the symmetry mates R * S_i and averages them
+      // (the particle and wedge interp3d calls pass `symmetry`, :1213-1254)
+      float srows[9 * kMaxSym];
+      for (int sy = 0; sy < nsym; ++sy) {
+        if (!S_colmajor) {
+          std::memcpy(srows, rows, sizeof(rows));
+          break;
+        }
+        const double* S = S_colmajor + 9 * sy;
+        double P[9];
+        for (int c = 0; c < 3; ++c)
+          for (int r = 0; r < 3; ++r) {
+            double acc = 0.0;
+            for (int k = 0; k < 3; ++k) acc += R[k * 3 + r] * S[c * 3 + k];
+            P[c * 3 + r] = acc;
+          }
+        colmajor_to_rows(P, srows + 9 * sy);
+      }
       for (int r = 0; r < 3; ++r)
         for (int c = 0; c < 3; ++c) rowsT[r * 3 + c] = rows[c * 3 + r];  // RotMat'
       const double* d_shift2 = B.est0;
@@ -259,11 +280,11 @@ emc_status emc_align_grid_padded(emc_ctx* ctx, int w, int n, const int pad_low[3
         d_shift2 = B.est + 3 * o;
       }
       if (stage2) {
-        XformArgs a2 = xform_args(wdims, rows, 1, zero3, 0);
+        XformArgs a2 = xform_args(wdims, srows, nsym, zero3, 0);
         a2.d_shift = d_shift2;
         a2.d_mul = d_vol_mask;
         launch_xform(ctx, d_particle, B.rpart, a2);  // :1213-1216, x volMask (:1375)
-        XformArgs a3 = xform_args(dims, rows, 1, zero3, 0);
+        XformArgs a3 = xform_args(dims, srows, nsym, zero3, 0);
         launch_xform(ctx, d_wedge, B.rwedge, a3);  // :1251-1254
         fft_r2c(plan, to_calc(B.rpart), B.F2);     // :1375-1377
         hipLaunchKernelGGL(rot_sums_half_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, ctx->stream,
@@ -304,7 +325,7 @@ emc_status emc_align_grid(emc_ctx* ctx, int n, const float* d_particle, const fl
                           const double est0[3], const int peak_com[3], const double* angles_pts,
                           int n_orient, int mode, double* ccc_out, double* shift_out) {
   const int zero[3] = {0, 0, 0};
-  return emc_align_grid_padded(ctx, n, n, zero, d_particle, d_ref, d_ref_wgt_rot, d_ref_wgt, d_ref_ft1,
+  return emc_align_grid_padded(ctx, n, n, zero, 1, nullptr, d_particle, d_ref, d_ref_wgt_rot, d_ref_wgt, d_ref_ft1,
                                d_vol_mask, d_peak_mask, d_bandpass, d_wedge, angles_colmajor, est0, peak_com,
                                angles_pts, n_orient, mode, ccc_out, shift_out);
 }

@@ -315,9 +315,14 @@ emc_status emc_align_grid(emc_ctx* ctx, int n, const float* d_particle, const fl
  * particle (stage 1, once) and the rotated, masked particle (stage 2) are
  * padded to C^3 with their own mean, pad_low[k] planes below and
  * C - W - pad_low[k] above (padCalc, BH_bandLimitCenterNormalize.m:48)
- * before their FFT; every rotation runs at its volume's own size.
- * emc_align_grid is this call with W = C. */
-emc_status emc_align_grid_padded(emc_ctx* ctx, int w, int c, const int pad_low[3], const float* d_particle,
+ * before their FFT; every rotation runs at its volume's own size. Stage 2
+ * rotates the particle and its wedge with the nsym symmetry mates R * S_i
+ * (S_colmajor: nsym column-major 3x3, NULL for nsym = 1) and averages them,
+ * as the reference's interp3d(..., 'inv', symmetry) calls (:1213-1254);
+ * stage 1 is C1 as in the reference. emc_align_grid is this call with
+ * W = C and nsym = 1. */
+emc_status emc_align_grid_padded(emc_ctx* ctx, int w, int c, const int pad_low[3], int nsym,
+                                 const double* S_colmajor, const float* d_particle,
                                  const float* d_ref, const float* d_ref_wgt_rot, const float* d_ref_wgt,
                                  const void* d_ref_ft1_c64, const float* d_vol_mask, const float* d_peak_mask,
                                  const float* d_bandpass, const float* d_wedge, const double angles_colmajor[9],

@@ -55,11 +55,12 @@ def test_align_grid_matches_oracle(ctx, n):
     assert abs(c0[0] - O.align_initial(inp)) < 1e-4 * abs(O.align_initial(inp))
 
 
-@pytest.mark.parametrize("w,c", [(30, 32), (29, 32)])
-def test_align_grid_padded_matches_oracle(ctx, w, c):
+@pytest.mark.parametrize("w,c,sym", [(30, 32, "C1"), (29, 32, "C1"), (32, 32, "C3"), (29, 32, "D2"), (24, 24, "O")])
+def test_align_grid_padded_matches_oracle(ctx, w, c, sym):
     """Window W^3 smaller than the calc size C^3 (padCalc low floor((C-W)/2),
     top the rest, BH_multi_padVal): the window volumes are padded with their
-    own mean before every FFT."""
+    own mean before every FFT; a point group averages the stage-2 rotations
+    of the particle and its wedge over the symmetry mates."""
     truth = (40.0, 30.0, 20.0)
     case, _, _, _, vm, _ = _inputs(w, 13, truth, snr=2.0)
     _, wedge, bp, pm, _, rwr = _inputs(c, 13, truth, snr=2.0)
@@ -68,14 +69,14 @@ def test_align_grid_padded_matches_oracle(ctx, w, c):
     pts = np.array([truth, (0.0, 0.0, 0.0), (40.0, 30.0, 30.0), (80.0, 30.0, 20.0), (200.0, 60.0, -40.0)])
     shift_val = np.array([0.25, -0.5, 0.5])
     al = ParticleAligner(case.particle, case.ref, wedge, bp, pm, vol_mask=vm, ref_wgt_rot=rwr,
-                         shift_val=shift_val, peak_com=(3, 3, 3), pad_calc=pad)
+                         shift_val=shift_val, peak_com=(3, 3, 3), pad_calc=pad, symmetry=sym)
     ccc, est = al.grid(pts, shift_val, mode=3)
     inp = O.AlignInputs(particle=case.particle.astype(np.float64), ref=case.ref.astype(np.float64),
                         ref_wgt_rot=rwr.astype(np.float64), ref_wgt=1.0,
                         ref_ft1=O.ref_ft1_of(case.ref.astype(np.float64), vm, pad),
                         vol_mask=vm.astype(np.float64), peak_mask=pm.astype(np.float64),
                         bandpass=bp.astype(np.float64), wedge=wedge.astype(np.float64),
-                        angles=np.eye(3), shift_val=shift_val, pad_calc=pad)
+                        angles=np.eye(3), shift_val=shift_val, pad_calc=pad, symmetry=sym)
     rc, re = O.align_grid(inp, pts)
     assert np.max(np.abs(ccc - rc) / np.abs(rc)) < 1e-4
     assert np.max(np.abs(est - re)) < 1e-3
