@@ -69,8 +69,8 @@ SIGNATURES = {
     "emc_xcf_rotational": (_st, [_vp, _vp, _vp, _vp, _vp, _i3, _d]),
     "emc_align_grid": (_st, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _i3,
                              _d, C.c_int, C.c_int, _d, _d]),
-    "emc_align_grid_padded": (_st, [_vp, C.c_int, C.c_int, _i3, C.c_int, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-                                    _d, _d, _i3, _d, C.c_int, C.c_int, _d, _d]),
+    "emc_align_grid_padded": (_st, [_vp, C.c_int, C.c_int, _i3, C.c_int, _d, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp,
+                                    _vp, _vp, _vp, _d, _d, _i3, _d, C.c_int, C.c_int, _d, _d]),
     "emc_template_search_chunk": (_st, [_vp, _vp, _i3, _vp, _i3, _i3, _i3, _i3, _d, C.c_int, C.c_int,
                                         C.c_int, _vp, _vp]),
     "emc_align_pick": (_st, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -143,19 +143,24 @@ using namespace emc;
 extern "C" {
 
 emc_status emc_align_grid_padded(emc_ctx* ctx, int w, int n, const int pad_low[3], int nsym,
-                                 const double* S_colmajor, const float* d_particle,
-                                 const float* d_ref, const float* d_ref_wgt_rot, const float* d_ref_wgt,
-                                 const void* d_ref_ft1, const float* d_vol_mask, const float* d_peak_mask,
-                                 const float* d_bandpass, const float* d_wedge, const double angles_colmajor[9],
-                                 const double est0[3], const int peak_com[3], const double* angles_pts,
-                                 int n_orient, int mode, double* ccc_out, double* shift_out) {
+                                 const double* S_colmajor, int n_ref, const float* d_particle,
+                                 const float* d_ref, const float* d_ref_wgt_rot, const float* const* d_ref_wgts,
+                                 const void* const* d_ref_ft1s, const float* d_vol_mask, const float* d_peak_mask,
+                                 const float* const* d_bandpasses, const float* d_wedge,
+                                 const double angles_colmajor[9], const double est0[3], const int peak_com[3],
+                                 const double* angles_pts, int n_orient, int mode, double* ccc_out,
+                                 double* shift_out) {
   return guard([&] {
     use_device(ctx);
     EMC_REQUIRE(n > 0 && n <= 1024, "bad calc size %d", n);
     EMC_REQUIRE(w > 0 && w <= n, "window size %d must be in 1..calc size %d", w, n);
-    EMC_REQUIRE(d_particle && d_ref && d_ref_ft1 && d_bandpass && d_wedge && angles_colmajor &&
-                    est0 && peak_com && angles_pts && ccc_out && shift_out && pad_low,
+    EMC_REQUIRE(n_ref >= 1 && n_ref <= 1024, "n_ref must be 1..1024");
+    EMC_REQUIRE(d_particle && d_ref && d_ref_ft1s && d_bandpasses && d_wedge && angles_colmajor && est0 &&
+                    peak_com && angles_pts && ccc_out && shift_out && pad_low,
                 "null pointer");
+    for (int r = 0; r < n_ref; ++r)
+      EMC_REQUIRE(d_ref_ft1s[r] && d_bandpasses[r], "reference %d: null ref_ft1 or bandpass", r);
+    const float* d_bandpass = d_bandpasses[0];  // stage 1 (the first reference of the list)
     int pad_top[3];
     for (int k = 0; k < 3; ++k) {
       pad_top[k] = n - w - pad_low[k];
@@ -189,12 +194,12 @@ emc_status emc_align_grid_padded(emc_ctx* ctx, int w, int n, const int pad_low[3
     B.X = B.P1 + nh;
     B.F2 = B.X + nh;
     B.res = (PeakResult*)scratch(ctx, "align_res", sizeof(PeakResult) * n_orient);
-    B.ccc = (double*)scratch(ctx, "align_ccc", sizeof(double) * n_orient);
+    B.ccc = (double*)scratch(ctx, "align_ccc", sizeof(double) * n_orient * (size_t)n_ref);
     B.est = (double*)scratch(ctx, "align_est", sizeof(double) * 3 * n_orient);
     B.est0 = (double*)scratch(ctx, "align_est0", sizeof(double) * 3);
     B.part = (double*)scratch(ctx, "align_part", sizeof(double) * 3 * kRedBlocks);
     B.stats = (double*)scratch(ctx, "align_stats", sizeof(double) * 4);
-    double* h_stage = (double*)pinned(ctx, sizeof(double) * 4 * (size_t)n_orient + 64);
+    double* h_stage = (double*)pinned(ctx, sizeof(double) * (3 + (size_t)n_ref) * n_orient + 64);
     std::memcpy(h_stage, est0, sizeof(double) * 3);
     EMC_HIP(hipMemcpyAsync(B.est0, h_stage, sizeof(double) * 3, hipMemcpyHostToDevice, ctx->stream));
     EMC_HIP(hipStreamSynchronize(ctx->stream));
@@ -287,11 +292,15 @@ emc_status emc_align_grid_padded(emc_ctx* ctx, int w, int n, const int pad_low[3
         XformArgs a3 = xform_args(dims, srows, nsym, zero3, 0);
         launch_xform(ctx, d_wedge, B.rwedge, a3);  // :1251-1254
         fft_r2c(plan, to_calc(B.rpart), B.F2);     // :1375-1377
-        hipLaunchKernelGGL(rot_sums_half_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, ctx->stream,
-                           (const float2*)B.F2, d_bandpass, (const float2*)d_ref_ft1,
-                           (const float*)B.rwedge, d_ref_wgt, n, n, n, B.part);
-        hipLaunchKernelGGL(rot_cc_fold_kernel, dim3(1), dim3(kRedThreads), 0, ctx->stream,
-                           (const double*)B.part, kRedBlocks, B.ccc + o);
+        // one rotated particle, scored against every reference of the list
+        // with that reference's bandpass, ref_FT1 and weight (:1278, :1375-1387)
+        for (int r = 0; r < n_ref; ++r) {
+          hipLaunchKernelGGL(rot_sums_half_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, ctx->stream,
+                             (const float2*)B.F2, d_bandpasses[r], (const float2*)d_ref_ft1s[r],
+                             (const float*)B.rwedge, d_ref_wgts ? d_ref_wgts[r] : nullptr, n, n, n, B.part);
+          hipLaunchKernelGGL(rot_cc_fold_kernel, dim3(1), dim3(kRedThreads), 0, ctx->stream,
+                             (const double*)B.part, kRedBlocks, B.ccc + (int64_t)o * n_ref + r);
+        }
         EMC_HIP(hipGetLastError());
       }
     }
@@ -300,7 +309,7 @@ emc_status emc_align_grid_padded(emc_ctx* ctx, int w, int n, const int pad_low[3
     PeakResult* h_res = nullptr;
     std::vector<PeakResult> res;
     if (stage2)
-      EMC_HIP(hipMemcpyAsync(h_ccc, B.ccc, sizeof(double) * n_orient, hipMemcpyDeviceToHost,
+      EMC_HIP(hipMemcpyAsync(h_ccc, B.ccc, sizeof(double) * n_orient * n_ref, hipMemcpyDeviceToHost,
                              ctx->stream));
     if (stage1) {
       res.resize(n_orient);
@@ -312,7 +321,7 @@ emc_status emc_align_grid_padded(emc_ctx* ctx, int w, int n, const int pad_low[3
     for (int o = 0; o < n_orient; ++o) {
       if (stage1 && h_res[o].status != 0)
         fail(EMC_ERR_RANGE, "orientation %d: failed to box out COM calc", o);
-      ccc_out[o] = stage2 ? h_ccc[o] : 0.0;
+      for (int r = 0; r < n_ref; ++r) ccc_out[(int64_t)o * n_ref + r] = stage2 ? h_ccc[(int64_t)o * n_ref + r] : 0.0;
       for (int k = 0; k < 3; ++k) shift_out[3 * o + k] = stage1 ? h_res[o].peak[k] : est0[k];
     }
   });
@@ -325,9 +334,12 @@ emc_status emc_align_grid(emc_ctx* ctx, int n, const float* d_particle, const fl
                           const double est0[3], const int peak_com[3], const double* angles_pts,
                           int n_orient, int mode, double* ccc_out, double* shift_out) {
   const int zero[3] = {0, 0, 0};
-  return emc_align_grid_padded(ctx, n, n, zero, 1, nullptr, d_particle, d_ref, d_ref_wgt_rot, d_ref_wgt, d_ref_ft1,
-                               d_vol_mask, d_peak_mask, d_bandpass, d_wedge, angles_colmajor, est0, peak_com,
-                               angles_pts, n_orient, mode, ccc_out, shift_out);
+  const float* wgts[1] = {d_ref_wgt};
+  const void* ft1s[1] = {d_ref_ft1};
+  const float* bps[1] = {d_bandpass};
+  return emc_align_grid_padded(ctx, n, n, zero, 1, nullptr, 1, d_particle, d_ref, d_ref_wgt_rot, wgts, ft1s,
+                               d_vol_mask, d_peak_mask, bps, d_wedge, angles_colmajor, est0, peak_com, angles_pts,
+                               n_orient, mode, ccc_out, shift_out);
 }
 
 }  // extern "C"

@@ -319,15 +319,23 @@ emc_status emc_align_grid(emc_ctx* ctx, int n, const float* d_particle, const fl
  * rotates the particle and its wedge with the nsym symmetry mates R * S_i
  * (S_colmajor: nsym column-major 3x3, NULL for nsym = 1) and averages them,
  * as the reference's interp3d(..., 'inv', symmetry) calls (:1213-1254);
- * stage 1 is C1 as in the reference. emc_align_grid is this call with
- * W = C and nsym = 1. */
+ * stage 1 is C1 as in the reference. Several references (refToAlign,
+ * :1267-1278): host arrays of n_ref device pointers d_ref_wgts (the array or
+ * its entries may be NULL), d_ref_ft1s and d_bandpasses; stage 1 rotates
+ * d_ref (the reference code always rotates ref_FT2_tmp{1}{1}, :1108) with
+ * the first reference's bandpass on the particle, whose shift the reference
+ * code keeps (its sortrows over all-zero CCC columns keeps the first row,
+ * :1198-1201); stage 2 scores the one rotated particle against every
+ * reference: ccc_out[o * n_ref + r]. emc_align_grid is this call with
+ * W = C, nsym = 1 and one reference. */
 emc_status emc_align_grid_padded(emc_ctx* ctx, int w, int c, const int pad_low[3], int nsym,
-                                 const double* S_colmajor, const float* d_particle,
-                                 const float* d_ref, const float* d_ref_wgt_rot, const float* d_ref_wgt,
-                                 const void* d_ref_ft1_c64, const float* d_vol_mask, const float* d_peak_mask,
-                                 const float* d_bandpass, const float* d_wedge, const double angles_colmajor[9],
-                                 const double est0[3], const int peak_com[3], const double* angles_pts,
-                                 int n_orient, int mode, double* ccc_out, double* shift_out);
+                                 const double* S_colmajor, int n_ref, const float* d_particle,
+                                 const float* d_ref, const float* d_ref_wgt_rot, const float* const* d_ref_wgts,
+                                 const void* const* d_ref_ft1s_c64, const float* d_vol_mask,
+                                 const float* d_peak_mask, const float* const* d_bandpasses, const float* d_wedge,
+                                 const double angles_colmajor[9], const double est0[3], const int peak_com[3],
+                                 const double* angles_pts, int n_orient, int mode, double* ccc_out,
+                                 double* shift_out);
 
 /* ------------------------------- batched alignment and its adjoint (new) */
 

@@ -719,6 +719,9 @@ class AlignInputs:
     pad_calc: np.ndarray            # [[low],[top]] W->C
     peak_com: tuple = (3, 3, 3)
     symmetry: str = "C1"
+    # stage-1 particle bandpass when it is another reference's (several
+    # references: stage 1 runs with the first of refToAlign, :1278-1318)
+    bandpass_stage1: object = None
 
 
 def ref_ft1_of(ref, vol_mask, pad_calc):
@@ -739,7 +742,8 @@ def align_orientation(inp: AlignInputs, rot_mat, est_peak0):
         i_rot_wdg = xform3d(inp.ref_wgt_rot, R.T, [0, 0, 0], True)
     i_rot_mask = xform3d(inp.peak_mask, R.T, [0, 0, 0], True)
     i_rot_ref = band_limit_center_normalize(i_rot_ref, None, inp.pad_calc)
-    rot_part = band_limit_center_normalize(trim, inp.bandpass, inp.pad_calc)
+    bp1 = inp.bandpass if inp.bandpass_stage1 is None else inp.bandpass_stage1
+    rot_part = band_limit_center_normalize(trim, bp1, inp.pad_calc)
     wedge_ifft = np.fft.ifftshift(inp.wedge)
     peak = xcf_translational(rot_part * np.fft.ifftshift(i_rot_wdg), np.conj(i_rot_ref) * wedge_ifft,
                              i_rot_mask, inp.peak_com)

@@ -87,6 +87,44 @@ def test_align_grid_padded_matches_oracle(ctx, w, c, sym):
         ParticleAligner(case.particle, case.ref, wedge, bp, pm, vol_mask=vm, pad_calc=[[1, 1, 1], [0, 0, 0]])
 
 
+def test_align_grid_multi_reference_matches_oracle(ctx):
+    """Several references (refToAlign): stage 1 rotates the first reference
+    volume with the first listed reference's bandpass; stage 2 scores the
+    rotated particle against each listed reference with its own ref_FT1 and
+    bandpass. Each column against the oracle run for that reference; a
+    reordered list changes the stage-1 bandpass; align_particle reports the
+    winning reference (1-based)."""
+    n = 32
+    truth = (40.0, 30.0, 20.0)
+    case, wedge, bp, pm, vm, rwr = _inputs(n, 11, truth, snr=2.0)
+    decoy = S.make_align_case(29, n=n, euler=truth, snr=2.0, n_blobs=16, blob_radius=0.22 * n).ref
+    bp2 = O.bandpass3d((n, n, n), 1e-3, 40.0, 10.0, 2.5).astype(np.float32)
+    refs, bps = [case.ref, decoy], [bp, bp2]
+    pts = np.array([truth, (0.0, 0.0, 0.0), (40.0, 30.0, 30.0), (200.0, 60.0, -40.0)])
+    sv = np.array([0.25, -0.5, 0.0])
+    al = ParticleAligner(case.particle, refs, wedge, bps, pm, vol_mask=vm, ref_wgt_rot=rwr, shift_val=sv)
+    pad0 = [[0] * 3, [0] * 3]
+
+    def oracle(r, first):
+        inp = O.AlignInputs(particle=case.particle.astype(np.float64), ref=case.ref.astype(np.float64),
+                            ref_wgt_rot=rwr.astype(np.float64), ref_wgt=1.0,
+                            ref_ft1=O.ref_ft1_of(refs[r].astype(np.float64), vm, pad0),
+                            vol_mask=vm.astype(np.float64), peak_mask=pm.astype(np.float64),
+                            bandpass=bps[r].astype(np.float64), wedge=wedge.astype(np.float64),
+                            angles=np.eye(3), shift_val=sv, pad_calc=pad0,
+                            bandpass_stage1=bps[first].astype(np.float64))
+        return O.align_grid(inp, pts)
+
+    for order in ([0, 1], [1, 0]):
+        ccc, est = al.grid_refs(pts, sv, mode=3, refs=order)
+        for j, r in enumerate(order):
+            rc, re = oracle(r, order[0])
+            assert np.max(np.abs(ccc[:, j] - rc) / np.abs(rc)) < 1e-4, (order, r)
+            assert np.max(np.abs(est - re)) < 1e-3, (order, r)
+    res = align_particle(al, pts, refine=False)
+    assert res.row[0] == 1 and not res.reverted
+
+
 def test_align_particle_recovers_ground_truth(ctx):
     """Config-0 style (64^3, 2.5 A/px, +-60 wedge, CTF df 3 um, SNR 0.1):
     the grid search + refinement recovers the orientation within 1 degree
