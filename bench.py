@@ -102,9 +102,14 @@ class BatchedRun:
 
     def __init__(self, particles, classes, wedges, grid, refine_iters, table="c64"):
         self.table = table
+        # largest box first: its refinement, the longest, then overlaps the
+        # smaller sizes' searches, and the smallest size's short refinement
+        # is the step's tail
         self.groups = {}
-        for i, p in enumerate(particles):
+        for i, p in sorted(enumerate(particles), key=lambda ip: -ip[1].n):
             self.groups.setdefault(p.n, []).append(i)
+        for n in self.groups:
+            self.groups[n].sort()
         self.particles, self.classes, self.wedges = particles, classes, wedges
         self.grid, self.iters = grid, refine_iters
         # per size class: reference and bandpass resident in HBM, and the
