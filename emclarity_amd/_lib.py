@@ -81,7 +81,8 @@ SIGNATURES = {
                                 C.c_int, C.c_int, _d, C.POINTER(C.c_int)]),
     "emc_moving_average": (_st, [_vp, _vp, _i3, C.c_int, _vp]),
     "emc_local_normalize": (_st, [_vp, _vp, _i3, C.c_int, _vp]),
-    "emc_tsearch_preprocess_chunk": (_st, [_vp, _vp, _i3, _d, C.c_int, C.c_float, _vp, _vp, _d]),
+    "emc_tsearch_preprocess_chunk": (_st, [_vp, _vp, _i3, _d, C.c_int, C.c_int, C.c_float, _vp, _vp, _d]),
+    "emc_median_filter3d": (_st, [_vp, _vp, _i3, C.c_int, _vp]),
     "emc_scale": (_st, [_vp, _vp, C.c_int64, C.c_double]),
     "emc_mrc_read_header": (_st, [C.c_char_p, _vp]),
     "emc_mrc_read_volume": (_st, [C.c_char_p, _i3, _f]),

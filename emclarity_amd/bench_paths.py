@@ -137,7 +137,7 @@ def bench_tsprep(ctx, reps):
     bp = np.array([1e-3, 600.0, 28.0, 10.0])
     total = C.c_double()
     wall = _timed(ctx, lambda: _lib.call("emc_tsearch_preprocess_chunk", ctx.handle, chunk.vp, _lib.i3((s, s, s)),
-                                         bp.ctypes.data_as(C.POINTER(C.c_double)), 8, -1.0, None, out.vp,
+                                         bp.ctypes.data_as(C.POINTER(C.c_double)), 0, 8, -1.0, None, out.vp,
                                          C.byref(total)), reps)
     _line("tsprep", wall, {"chunk": s, "GB_in_out": round(2 * 4 * s ** 3 / 1e9, 3)})
 

@@ -166,6 +166,76 @@ __global__ void scale_kernel(float* __restrict__ d, int64_t n, float s) {
 
 int grid_for(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 8192); }
 
+// medfilt3(x, [K K K]) with its default 'symmetric' padding: index -1 reads
+// 0, index n reads n - 1 (every dimension >= K/2 + 1, so one reflection
+// stays inside). The K^3 window goes to a private array and the median (the
+// element of rank K^3/2) is selected by Hoare-partition quickselect; every
+// partition step moves both ends by at least one, and both scans are
+// bounded by the current range.
+__device__ inline int reflect_index(int i, int n) { return i < 0 ? -i - 1 : (i >= n ? 2 * n - 1 - i : i); }
+
+template <int K>
+__global__ void __launch_bounds__(256) median3_kernel(const float* __restrict__ in, float* __restrict__ out, int nx,
+                                                      int ny, int nz) {
+  constexpr int M = K * K * K, H = K / 2;
+  const int64_t nv = (int64_t)nx * ny * nz;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += (int64_t)gridDim.x * blockDim.x) {
+    int x, y, z;
+    split3(i, nx, ny, x, y, z);
+    float v[M];
+    int c = 0;
+    for (int dz = -H; dz <= H; ++dz) {
+      const int zz = reflect_index(z + dz, nz);
+      for (int dy = -H; dy <= H; ++dy) {
+        const int64_t row = ((int64_t)zz * ny + reflect_index(y + dy, ny)) * nx;
+        for (int dx = -H; dx <= H; ++dx) v[c++] = in[row + reflect_index(x + dx, nx)];
+      }
+    }
+    constexpr int k = M / 2;
+    int lo = 0, hi = M - 1;
+    for (int guard = 0; lo < hi && guard < M; ++guard) {
+      const float pivot = v[(lo + hi) / 2];
+      int a = lo, b = hi;
+      while (a <= b) {
+        while (a <= hi && v[a] < pivot) ++a;
+        while (b >= lo && v[b] > pivot) --b;
+        if (a <= b) {
+          const float t = v[a];
+          v[a] = v[b];
+          v[b] = t;
+          ++a;
+          --b;
+        }
+      }
+      if (k <= b)
+        hi = b;
+      else if (k >= a)
+        lo = a;
+      else
+        break;
+    }
+    out[i] = v[k];
+  }
+}
+
+void median3(emc_ctx* ctx, const float* in, float* out, const int dims[3], int k) {
+  const int64_t nv = (int64_t)dims[0] * dims[1] * dims[2];
+  const int grid = grid_for(nv);
+  if (k == 3)
+    hipLaunchKernelGGL(median3_kernel<3>, dim3(grid), dim3(256), 0, ctx->stream, in, out, dims[0], dims[1], dims[2]);
+  else if (k == 5)
+    hipLaunchKernelGGL(median3_kernel<5>, dim3(grid), dim3(256), 0, ctx->stream, in, out, dims[0], dims[1], dims[2]);
+  else
+    hipLaunchKernelGGL(median3_kernel<7>, dim3(grid), dim3(256), 0, ctx->stream, in, out, dims[0], dims[1], dims[2]);
+  EMC_HIP(hipGetLastError());
+}
+
+void check_median(const int dims[3], int k) {
+  EMC_REQUIRE(k == 3 || k == 5 || k == 7, "median filter size can only be 3, 5 or 7, not %d", k);
+  for (int a = 0; a < 3; ++a)
+    EMC_REQUIRE(dims[a] > k / 2, "dimension %d (%d) too small for a %d^3 median", a, dims[a], k);
+}
+
 }  // namespace emc
 
 using namespace emc;
@@ -209,13 +279,25 @@ emc_status emc_local_normalize(emc_ctx* ctx, const float* d_in, const int dims[3
   });
 }
 
+emc_status emc_median_filter3d(emc_ctx* ctx, const float* d_in, const int dims[3], int k, float* d_out) {
+  return guard([&] {
+    use_device(ctx);
+    EMC_REQUIRE(d_in && d_out && dims, "null pointer");
+    EMC_REQUIRE(d_in != d_out, "d_out must not alias d_in");
+    EMC_REQUIRE(dims[0] > 0 && dims[1] > 0 && dims[2] > 0, "bad dims");
+    check_median(dims, k);
+    median3(ctx, d_in, d_out, dims, k);
+  });
+}
+
 emc_status emc_tsearch_preprocess_chunk(emc_ctx* ctx, const float* d_in, const int dims[3], const double bp[4],
-                                        int stats_radius, float sign, const float* d_valid_mask, float* d_out,
-                                        double* h_masked_sum) {
+                                        int med_filt, int stats_radius, float sign, const float* d_valid_mask,
+                                        float* d_out, double* h_masked_sum) {
   return guard([&] {
     use_device(ctx);
     EMC_REQUIRE(d_in && d_out && bp, "null pointer");
     check_ma(dims, stats_radius);
+    if (med_filt) check_median(dims, med_filt);
     const int nx = dims[0], ny = dims[1], nz = dims[2];
     const int64_t nv = (int64_t)nx * ny * nz, nh = (int64_t)(nx / 2 + 1) * ny * nz;
     // (x - mean) / rms, two FP64 passes
@@ -238,6 +320,11 @@ emc_status emc_tsearch_preprocess_chunk(emc_ctx* ctx, const float* d_in, const i
     hipLaunchKernelGGL(mul_half_real_kernel, dim3(grid_for(nh)), dim3(256), 0, ctx->stream, F,
                        (const float*)bph, nh, (float)(1.0 / (double)nv));
     fft_c2r(plan, F, y);
+    if (med_filt) {  // medfilt3 after the bandpass (:543-548)
+      float* m = (float*)scratch(ctx, "tsp_med", sizeof(float) * nv);
+      median3(ctx, y, m, dims, med_filt);
+      y = m;
+    }
     // local normalisation, sign, mask, masked sum
     float2* a = (float2*)scratch(ctx, "ma_out2", sizeof(float2) * nv);
     moving_average_sq(ctx, y, dims, stats_radius, a);

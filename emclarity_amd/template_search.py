@@ -40,11 +40,13 @@ def symmetric_pad(vol, pre, post):
 
 
 def preprocess_chunks(padded, corners, size_chunk, valid_calc, bp_vals, pixel, stats_radius,
-                      should_be_ctf=1, ctx=None):
+                      should_be_ctf=1, ctx=None, med_filt=0):
     """BH_templateSearch3d_2.m:480-611 on the padded tomogram (numpy, z,y,x):
     each chunk (x,y,z corner in ``corners``) is standardised, band-passed
     (bp_vals = HIGH_THRESH, HIGH_CUT, LOW_CUT at ``pixel``), locally
-    normalised over ``stats_radius`` voxels, multiplied by -shouldBeCTF and
+    median-filtered when ``med_filt`` is 3, 5 or 7 (Tmp_medianFilter,
+    :492-499, :543-548), locally normalised over ``stats_radius`` voxels,
+    multiplied by -shouldBeCTF and
     the valid-area mask BH_mask3d('rectangle', sizeChunk, validCalc/2)
     (:402). The reference accumulates the square of each chunk's masked sum
     (:590), so globalVariance = sum(s_i^2)/N - (sum(s_i)/N)^2 with N = number
@@ -65,7 +67,7 @@ def preprocess_chunks(padded, corners, size_chunk, valid_calc, bp_vals, pixel, s
         res = DeviceArray(raw.shape, np.float32, ctx)
         tmp = _lib.C.c_double()
         _lib.call("emc_tsearch_preprocess_chunk", ctx.handle, raw.vp, _lib.i3(sc), bp.ctypes.data_as(dp),
-                  int(stats_radius), float(-should_be_ctf), mask.vp, res.vp, _lib.C.byref(tmp))
+                  int(med_filt), int(stats_radius), float(-should_be_ctf), mask.vp, res.vp, _lib.C.byref(tmp))
         fx += tmp.value
         fx2 += tmp.value ** 2
         fn += float(np.prod(sc))

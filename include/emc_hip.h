@@ -468,16 +468,23 @@ emc_status emc_moving_average(emc_ctx* ctx, const float* d_in, const int dims[3]
 emc_status emc_local_normalize(emc_ctx* ctx, const float* d_in, const int dims[3], int feature_size,
                                float* d_out);
 
+/* medfilt3(x, [k k k]) (MATLAB's default 'symmetric' padding), k = 3, 5 or
+ * 7 (Tmp_medianFilter, alignment/BH_templateSearch3d_2.m:492-499, :543-548).
+ * Every dimension > k/2; d_out must not alias d_in. Stream-ordered. */
+emc_status emc_median_filter3d(emc_ctx* ctx, const float* d_in, const int dims[3], int k, float* d_out);
+
 /* One chunk of alignment/BH_templateSearch3d_2.m:532-589: x = (x - mean) / rms;
  * bandpass x = C2R(R2C(x) .* BH_bandpass3d half grid) / N with bp = {HIGH_THRESH,
  * HIGH_CUT, LOW_CUT, pixel} (fourierTransformer fwdFFT(x,0,0,bp) then
- * invFFT(.., 2)); local normalisation over stats_radius; d_out = sign * x .*
+ * invFFT(.., 2)); med_filt 3, 5 or 7: medfilt3 of the band-passed chunk
+ * (0 = none, the default Tmp_medianFilter); local normalisation over
+ * stats_radius; d_out = sign * x .*
  * d_valid_mask (sign = -shouldBeCTF; mask NULL = 1). *h_masked_sum (if not
  * NULL) = sum(d_out(mask > 0.1)), the chunk's term of the global variance
  * (:589-592). Synchronous when h_masked_sum is given. */
 emc_status emc_tsearch_preprocess_chunk(emc_ctx* ctx, const float* d_in, const int dims[3], const double bp[4],
-                                        int stats_radius, float sign, const float* d_valid_mask, float* d_out,
-                                        double* h_masked_sum);
+                                        int med_filt, int stats_radius, float sign, const float* d_valid_mask,
+                                        float* d_out, double* h_masked_sum);
 
 /* BH_mask3d('sphere', dims, radius, [0,0,0]) (masking/BH_mask3d.m:126-160,
  * :462-466): ellipsoid of the given radii (x, y, z) about floor(N/2), six

@@ -824,7 +824,7 @@ def template_search_chunk(tomo_chunk, template, grid_angles, pad_bin, trim_valid
         trot = interp3d(template, [phi, theta, psi - phi], [1, 1, 1], "Bah", "forward")
         tpad = pad_zeros3d(trot, pad_bin[0], pad_bin[1])
         temp_fou = np.conj(np.fft.rfftn(tpad))
-        ccf = np.fft.irfftn(tomo_fou * temp_fou, s=(nz, ny, nx)) * (nx * ny * nz)   # unscaled C2R
+        ccf = np.fft.irfftn(tomo_fou * temp_fou, s=(nz, ny, nx), axes=(0, 1, 2)) * (nx * ny * nz)   # unscaled C2R
         ccf = pad_zeros3d(ccf, trim_valid[0], trim_valid[1])          # padVal rows: negative = trim
         ccf = ccf / matlab_std(ccf)
         if mag is None and init_from_first:
@@ -1095,16 +1095,26 @@ def mask3d_rectangle(size, radius):
     return pad_zeros3d(win, [0, 0, 0], [0, 0, 0], taper=True)
 
 
-def tsearch_preprocess_chunk(chunk, bp_vals, pixel, stats_radius, sign, valid_mask=None):
+def median_filter3(x, k):
+    """medfilt3(x, [k k k]) with MATLAB's default 'symmetric' padding (the
+    border sample repeated in the mirror, scipy.ndimage's 'reflect' mode)."""
+    from scipy import ndimage
+    return ndimage.median_filter(np.asarray(x), size=int(k), mode="reflect")
+
+
+def tsearch_preprocess_chunk(chunk, bp_vals, pixel, stats_radius, sign, valid_mask=None, med_filt=0):
     """One chunk of BH_templateSearch3d_2.m:532-589: returns (out, tmp_sum).
-    bp_vals = (HIGH_THRESH, HIGH_CUT, LOW_CUT); sign = -shouldBeCTF."""
+    bp_vals = (HIGH_THRESH, HIGH_CUT, LOW_CUT); sign = -shouldBeCTF;
+    med_filt 3/5/7: medfilt3 after the bandpass (:543-548)."""
     x = np.asarray(chunk, dtype=np.float64)
     x = x - x.mean()                                                  # :535
     x = x / np.sqrt(np.mean(x * x))                                   # :536 (rms)
     nz, ny, nx = x.shape
     bp = bandpass3d((nx, ny, nz), bp_vals[0], bp_vals[1], bp_vals[2], pixel)
     # fwdFFT(x,0,0,[bp,pixel]) then invFFT(.., 2) (fourierTransformer.m:92-158, :294-328)
-    x = np.fft.irfftn(np.fft.rfftn(x) * bp[:, :, :nx // 2 + 1], s=x.shape)
+    x = np.fft.irfftn(np.fft.rfftn(x) * bp[:, :, :nx // 2 + 1], s=x.shape, axes=(0, 1, 2))
+    if med_filt:
+        x = median_filter3(x, med_filt)                               # :543-548
     x = local_normalize(x, stats_radius)                              # :558-561
     m = np.ones_like(x) if valid_mask is None else np.asarray(valid_mask, dtype=np.float64)
     out = (sign * x) * m                                              # :586
@@ -1112,7 +1122,7 @@ def tsearch_preprocess_chunk(chunk, bp_vals, pixel, stats_radius, sign, valid_ma
 
 
 def tsearch_preprocess(tomogram, size_chunk, valid_area, n_iters, tomo_pre, tomo_post, valid_calc,
-                       bp_vals, pixel, stats_radius, should_be_ctf=1):
+                       bp_vals, pixel, stats_radius, should_be_ctf=1, med_filt=0):
     """BH_templateSearch3d_2.m:397-398 (symmetric padding) and :480-611: the
     stack of pre-processed chunks (chunk corners in the padded tomogram) and
     the global variance. The reference accumulates the square of each chunk's
@@ -1132,7 +1142,8 @@ def tsearch_preprocess(tomogram, size_chunk, valid_area, n_iters, tomo_pre, tomo
             for iz in range(int(n_iters[2])):
                 cx, cy, cz = ix * va[0], iy * va[1], iz * va[2]
                 sub = t[cz:cz + sc[2], cy:cy + sc[1], cx:cx + sc[0]]
-                out, s = tsearch_preprocess_chunk(sub, bp_vals, pixel, stats_radius, -float(should_be_ctf), vmask)
+                out, s = tsearch_preprocess_chunk(sub, bp_vals, pixel, stats_radius, -float(should_be_ctf), vmask,
+                                                  med_filt)
                 fx += s
                 fx2 += s * s
                 fn += float(np.prod(sc))

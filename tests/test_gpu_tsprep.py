@@ -61,9 +61,38 @@ def test_preprocess_chunk(ctx):
     bp = np.array(list(BP) + [PIXEL])
     s = _lib.C.c_double()
     _lib.call("emc_tsearch_preprocess_chunk", ctx.handle, d.vp, _lib.i3(d.dims),
-              bp.ctypes.data_as(_lib.C.POINTER(_lib.C.c_double)), 8, -1.0, dm.vp, out.vp, _lib.C.byref(s))
+              bp.ctypes.data_as(_lib.C.POINTER(_lib.C.c_double)), 0, 8, -1.0, dm.vp, out.vp, _lib.C.byref(s))
     _close(out.get(), ref, 1e-4)
     assert abs(s.value - rsum) <= 1e-5 * float(np.sum(np.abs(ref))), (s.value, rsum)
+
+
+@pytest.mark.parametrize("k,shape", [(3, (9, 12, 17)), (5, (8, 10, 13)), (7, (6, 9, 11))])
+def test_median_filter_matches_oracle(ctx, k, shape):
+    """medfilt3 with symmetric padding: the median is one of the inputs, so
+    the result is bit-exact; ties and repeated values included."""
+    rng = np.random.default_rng(k)
+    x = rng.standard_normal(shape).astype(np.float32)
+    x[0, :3, :3] = 1.0                       # repeated values at a corner
+    x = np.round(x * 8) / 8                  # many ties
+    d = DeviceArray.from_numpy(x)
+    out = DeviceArray(x.shape, np.float32)
+    _lib.call("emc_median_filter3d", ctx.handle, d.vp, _lib.i3(d.dims), k, out.vp)
+    assert np.array_equal(out.get(), O.median_filter3(x, k))
+    with pytest.raises(_lib.EmcError):
+        _lib.call("emc_median_filter3d", ctx.handle, d.vp, _lib.i3(d.dims), 4, out.vp)
+
+
+def test_preprocess_chunk_median(ctx):
+    rng = np.random.default_rng(9)
+    chunk = (1.0 + rng.standard_normal((20, 24, 26))).astype(np.float32)
+    ref, rsum = O.tsearch_preprocess_chunk(chunk, BP, PIXEL, 6, 1.0, None, med_filt=3)
+    d = DeviceArray.from_numpy(chunk)
+    out = DeviceArray(chunk.shape, np.float32)
+    bp = np.array(list(BP) + [PIXEL])
+    s = _lib.C.c_double()
+    _lib.call("emc_tsearch_preprocess_chunk", ctx.handle, d.vp, _lib.i3(d.dims),
+              bp.ctypes.data_as(_lib.C.POINTER(_lib.C.c_double)), 3, 6, 1.0, None, out.vp, _lib.C.byref(s))
+    _close(out.get(), ref, 1e-4)
 
 
 def test_preprocess_driver_and_global_variance(ctx):

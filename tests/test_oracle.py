@@ -401,7 +401,7 @@ def test_swap_quadrants_is_shift_and_sign():
     n = 8
     sq = O.swap_quadrants_half((n, n, n))
     v = np.random.default_rng(3).standard_normal((n, n, n))
-    a = np.fft.irfftn(sq * np.fft.rfftn(v), s=(n, n, n))
+    a = np.fft.irfftn(sq * np.fft.rfftn(v), s=(n, n, n), axes=(0, 1, 2))
     sh = n // 2 + 1
     b = np.roll(v, (sh, sh, sh), axis=(0, 1, 2)) * (-1) ** (3 * (n // 2 + 1))
     assert np.allclose(a, b, atol=1e-12)
@@ -556,7 +556,7 @@ def test_preprocess_bandpass_half_grid_equals_full():
     rng = np.random.default_rng(4)
     x = rng.standard_normal((12, 10, 16))
     bp = O.bandpass3d((16, 10, 12), 1e-3, 40.0, 8.0, 2.5)
-    half = np.fft.irfftn(np.fft.rfftn(x) * bp[:, :, :9], s=x.shape)
+    half = np.fft.irfftn(np.fft.rfftn(x) * bp[:, :, :9], s=x.shape, axes=(0, 1, 2))
     full = np.real(np.fft.ifftn(np.fft.fftn(x) * bp))
     assert np.allclose(half, full, atol=1e-12)
 
@@ -747,3 +747,26 @@ def test_align_grid_padded_agrees_with_unpadded():
         assert int(np.argmax(cc)) == 0
         d = np.max(np.abs(est - out[26][1]), axis=1)
         assert d[0] < 0.05 and np.median(d) < 0.05, (w, d)
+
+
+# ------------------------------------------------------------- median filter
+
+@pytest.mark.parametrize("k", [3, 5, 7])
+def test_median_filter3_brute_force(k, rng):
+    """The oracle's medfilt3 (scipy 'reflect') against a direct statement:
+    numpy's 'symmetric' pad (MATLAB's 'symmetric': the border sample
+    repeated) and the median of every k^3 window; an isolated spike is
+    removed and a constant volume is unchanged."""
+    x = rng.standard_normal((7, 8, 9))
+    h = k // 2
+    p = np.pad(x, h, mode="symmetric")
+    want = np.empty_like(x)
+    for z in range(x.shape[0]):
+        for y in range(x.shape[1]):
+            for xx in range(x.shape[2]):
+                want[z, y, xx] = np.median(p[z:z + k, y:y + k, xx:xx + k])
+    assert np.array_equal(O.median_filter3(x, k), want)
+    spike = np.zeros((7, 8, 9))
+    spike[3, 4, 4] = 100.0
+    assert not O.median_filter3(spike, k).any()
+    assert np.array_equal(O.median_filter3(np.full((7, 8, 9), 2.5), k), np.full((7, 8, 9), 2.5))
