@@ -63,14 +63,15 @@ SIGNATURES = {
     "emc_band_limit_center_normalize": (_st, [_vp, _vp, _vp, _i3, _i3, _i3, _vp, _vp]),
     "emc_ref_ft1": (_st, [_vp, _vp, _vp, _vp, _i3, _i3, _i3, _vp]),
     "emc_xcf_translational": (_st, [_vp, _vp, _vp, _vp, _i3, _vp, _i3, _vp, _d, _d]),
-    "emc_shell_plan_create": (_st, [_vp, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_uint32), C.POINTER(_vp)]),
+    "emc_shell_plan_create": (_st, [_vp, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int64),
+                                    C.POINTER(C.c_uint32), C.POINTER(_vp)]),
     "emc_shell_plan_destroy": (_st, [_vp, _vp]),
-    "emc_xcf_rotational_shells": (_st, [_vp, _vp, _vp, _vp, _vp, _vp, _i3, C.c_int, C.c_double, _d]),
+    "emc_xcf_rotational_shells": (_st, [_vp, _vp, _vp, _vp, _vp, _vp, _i3, _d]),
     "emc_xcf_rotational": (_st, [_vp, _vp, _vp, _vp, _vp, _i3, _d]),
     "emc_align_grid": (_st, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _i3,
                              _d, C.c_int, C.c_int, _d, _d]),
     "emc_align_grid_padded": (_st, [_vp, C.c_int, C.c_int, _i3, C.c_int, _d, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp,
-                                    _vp, _vp, _vp, _d, _d, _i3, _d, C.c_int, C.c_int, _d, _d]),
+                                    _vp, _vp, _vp, _vp, _d, _d, _i3, _d, C.c_int, C.c_int, _d, _d]),
     "emc_template_search_chunk": (_st, [_vp, _vp, _i3, _vp, _i3, _i3, _i3, _i3, _d, C.c_int, C.c_int,
                                         C.c_int, _vp, _vp]),
     "emc_align_pick": (_st, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -39,7 +39,7 @@ class ParticleAligner:
 
     def __init__(self, particle, ref, wedge, bandpass, peak_mask, vol_mask=None, ref_wgt_rot=None,
                  ref_wgt=None, angles=None, shift_val=(0.0, 0.0, 0.0), peak_com=(3, 3, 3), pad_calc=None,
-                 symmetry="C1", ctx=None):
+                 symmetry="C1", wCCC=None, ctx=None):
         """particle, ref, vol_mask: window W^3; wedge, bandpass, peak_mask,
         ref_wgt_rot, ref_wgt: calc C^3. pad_calc = [[low x,y,z], [top x,y,z]]
         (BH_multi_validArea's padCalc) with C = W + low + top; None: C = W.
@@ -51,7 +51,12 @@ class ParticleAligner:
         a list, one per reference. Stage 1 rotates the first reference for
         every reference, as the reference code's refInterpolator holds
         ref_FT2_tmp{1}{1} (:1108); each reference gets its own ref_FT1,
-        bandpass and weight in stage 2."""
+        bandpass and weight in stage 2.
+
+        wCCC: None (plain CCC, flgWeightCCC 0), a wCCC cell or ShellPlan for
+        every reference, or a dict {reference index: cell or ShellPlan}: the
+        shell-weighted CCC of BH_multi_xcf_Rotational in stage 2, shells over
+        the calc grid."""
         from .geometry import symmetry_matrices
         mats = symmetry_matrices(symmetry)
         self.symmetry = symmetry
@@ -121,6 +126,14 @@ class ParticleAligner:
                       _lib.i3(pc[0]), _lib.i3(pc[1]), f.vp)
             self.ref_ft1s.append(f)
         self.ref_ft1 = self.ref_ft1s[0]
+        from .xcf import ShellPlan
+        plan = lambda c: None if c is None or (np.isscalar(c) and not c) else (
+            c if isinstance(c, ShellPlan) else ShellPlan(c, self.ctx))
+        if isinstance(wCCC, dict):
+            self.shells = [plan(wCCC.get(r)) for r in range(self.n_ref)]
+        else:
+            one = plan(wCCC)
+            self.shells = [one] * self.n_ref
 
     def grid_refs(self, angles_pts, est0, mode=3, refs=None):
         """emc_align_grid_padded over (phi, theta, psi) rows for the
@@ -140,7 +153,10 @@ class ParticleAligner:
         _lib.call("emc_align_grid_padded", self.ctx.handle, self.w, self.n, _lib.i3(self.pad_calc[0]),
                   self.nsym, _lib.d_arr(self.sym), nr, self.particle.vp, self.ref.vp,
                   vp(self.ref_wgt_rot), ptrs([self.ref_wgts[r] for r in refs]),
-                  ptrs([self.ref_ft1s[r] for r in refs]), vp(self.vol_mask), self.peak_mask.vp,
+                  ptrs([self.ref_ft1s[r] for r in refs]),
+                  (C.c_void_p * nr)(*[self.shells[r].handle.value if self.shells[r] is not None else None
+                                      for r in refs]),
+                  vp(self.vol_mask), self.peak_mask.vp,
                   ptrs([self.bandpasses[r] for r in refs]), self.wedge.vp,
                   _lib.d_arr(self.angles.reshape(-1, order="F")), _lib.d_arr(est0),
                   _lib.i3(self.peak_com), pts.ctypes.data_as(dp), n, int(mode),

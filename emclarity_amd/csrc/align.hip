@@ -111,6 +111,49 @@ __global__ void __launch_bounds__(kRedThreads) rot_sums_half_kernel(
     for (int k = 0; k < 3; ++k) part[3 * blockIdx.x + k] = v[k];
 }
 
+// Shell sums of the rotational CC (BH_multi_xcf_Rotational.m:50-70) from the
+// half spectrum: shell voxel j is the full-grid FFT-order index of k; for
+// x > n/2 the half grid holds -k, whose conjugate gives P(k) and R(k). The
+// bandpass and the weights are read at k itself. Block (b, s) writes the
+// partial (num, sum|R Wp|^2, sum|P Wr|^2) of shell s.
+__global__ void __launch_bounds__(kRedThreads) rot_shell_sums_half_kernel(
+    const float2* __restrict__ F2, const float* __restrict__ bp, const float2* __restrict__ R,
+    const float* __restrict__ wp_c, const float* __restrict__ wr, int n, const int64_t* __restrict__ offsets,
+    const uint32_t* __restrict__ idx, double* __restrict__ part) {
+  const int s = blockIdx.y;
+  const int cx = n / 2 + 1;
+  const int64_t c0 = ((int64_t)(n / 2) * n + n / 2) * n + n / 2;
+  const bool use_wp = wp_c && !(wp_c[c0] != wp_c[c0]);  // NaN first element -> 1 (:34-40)
+  const bool use_wr = wr && !(wr[0] != wr[0]);
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int64_t j = offsets[s] + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < offsets[s + 1];
+       j += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t f = (int64_t)idx[j] - 1;  // MATLAB linear index, x fastest
+    int x, y, z;
+    split3(f, n, n, x, y, z);
+    if (x == 0 && y == 0 && z == 0) continue;  // P(0) = 0
+    const bool mirror = x > n / 2;
+    const int hx = mirror ? n - x : x, hy = mirror ? (n - y) % n : y, hz = mirror ? (n - z) % n : z;
+    const int64_t h = ((int64_t)hz * n + hy) * cx + hx;
+    const float2 fp = F2[h], fr = R[h];
+    const double b = bp ? (double)bp[f] : 1.0;
+    // Re(P R) is the same for k and -k (conj(a) conj(b) = conj(ab))
+    const double px = fp.x * b, py = fp.y * b;
+    const double re = px * fr.x - py * fr.y;
+    const double r2 = (double)fr.x * fr.x + (double)fr.y * fr.y;
+    const double p2 = px * px + py * py;
+    const int64_t cp = ((int64_t)shifted_index(z, n) * n + shifted_index(y, n)) * n + shifted_index(x, n);
+    const double a = use_wp ? (double)wp_c[cp] : 1.0;
+    const double c = use_wr ? (double)wr[f] : 1.0;
+    v[0] += re * c * a;
+    v[1] += r2 * a * a;
+    v[2] += p2 * c * c;
+  }
+  block_sum<3>(v);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 3; ++k) part[((int64_t)s * gridDim.x + blockIdx.x) * 3 + k] = v[k];
+}
+
 __global__ void __launch_bounds__(kRedThreads) rot_cc_fold_kernel(const double* __restrict__ part,
                                                                   int nb, double* __restrict__ cc) {
   double v[3] = {0.0, 0.0, 0.0};
@@ -145,7 +188,8 @@ extern "C" {
 emc_status emc_align_grid_padded(emc_ctx* ctx, int w, int n, const int pad_low[3], int nsym,
                                  const double* S_colmajor, int n_ref, const float* d_particle,
                                  const float* d_ref, const float* d_ref_wgt_rot, const float* const* d_ref_wgts,
-                                 const void* const* d_ref_ft1s, const float* d_vol_mask, const float* d_peak_mask,
+                                 const void* const* d_ref_ft1s, const emc_shell_plan* const* shells,
+                                 const float* d_vol_mask, const float* d_peak_mask,
                                  const float* const* d_bandpasses, const float* d_wedge,
                                  const double angles_colmajor[9], const double est0[3], const int peak_com[3],
                                  const double* angles_pts, int n_orient, int mode, double* ccc_out,
@@ -158,8 +202,15 @@ emc_status emc_align_grid_padded(emc_ctx* ctx, int w, int n, const int pad_low[3
     EMC_REQUIRE(d_particle && d_ref && d_ref_ft1s && d_bandpasses && d_wedge && angles_colmajor && est0 &&
                     peak_com && angles_pts && ccc_out && shift_out && pad_low,
                 "null pointer");
-    for (int r = 0; r < n_ref; ++r)
+    int max_shells = 0;
+    for (int r = 0; r < n_ref; ++r) {
       EMC_REQUIRE(d_ref_ft1s[r] && d_bandpasses[r], "reference %d: null ref_ft1 or bandpass", r);
+      const emc_shell_plan* sp = shells ? shells[r] : nullptr;
+      if (sp) {
+        EMC_REQUIRE(sp->max_index <= (int64_t)n * n * n, "reference %d: shell index beyond the %d^3 calc grid", r, n);
+        max_shells = std::max(max_shells, sp->n_shells);
+      }
+    }
     const float* d_bandpass = d_bandpasses[0];  // stage 1 (the first reference of the list)
     int pad_top[3];
     for (int k = 0; k < 3; ++k) {
@@ -199,6 +250,9 @@ emc_status emc_align_grid_padded(emc_ctx* ctx, int w, int n, const int pad_low[3
     B.est0 = (double*)scratch(ctx, "align_est0", sizeof(double) * 3);
     B.part = (double*)scratch(ctx, "align_part", sizeof(double) * 3 * kRedBlocks);
     B.stats = (double*)scratch(ctx, "align_stats", sizeof(double) * 4);
+    double* shell_part =
+        max_shells ? (double*)scratch(ctx, "align_shell_part", sizeof(double) * 3 * kShellBlocks * (size_t)max_shells)
+                   : nullptr;
     double* h_stage = (double*)pinned(ctx, sizeof(double) * (3 + (size_t)n_ref) * n_orient + 64);
     std::memcpy(h_stage, est0, sizeof(double) * 3);
     EMC_HIP(hipMemcpyAsync(B.est0, h_stage, sizeof(double) * 3, hipMemcpyHostToDevice, ctx->stream));
@@ -295,11 +349,23 @@ emc_status emc_align_grid_padded(emc_ctx* ctx, int w, int n, const int pad_low[3
         // one rotated particle, scored against every reference of the list
         // with that reference's bandpass, ref_FT1 and weight (:1278, :1375-1387)
         for (int r = 0; r < n_ref; ++r) {
-          hipLaunchKernelGGL(rot_sums_half_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, ctx->stream,
-                             (const float2*)B.F2, d_bandpasses[r], (const float2*)d_ref_ft1s[r],
-                             (const float*)B.rwedge, d_ref_wgts ? d_ref_wgts[r] : nullptr, n, n, n, B.part);
-          hipLaunchKernelGGL(rot_cc_fold_kernel, dim3(1), dim3(kRedThreads), 0, ctx->stream,
-                             (const double*)B.part, kRedBlocks, B.ccc + (int64_t)o * n_ref + r);
+          const float* wr = d_ref_wgts ? d_ref_wgts[r] : nullptr;
+          double* cc = B.ccc + (int64_t)o * n_ref + r;
+          const emc_shell_plan* sp = shells ? shells[r] : nullptr;
+          if (sp) {  // wCCC a cell: shell-weighted score (:50-70)
+            hipLaunchKernelGGL(rot_shell_sums_half_kernel, dim3(kShellBlocks, sp->n_shells), dim3(kRedThreads), 0,
+                               ctx->stream, (const float2*)B.F2, d_bandpasses[r], (const float2*)d_ref_ft1s[r],
+                               (const float*)B.rwedge, wr, n, (const int64_t*)sp->d_offsets,
+                               (const uint32_t*)sp->d_idx, shell_part);
+            hipLaunchKernelGGL(xcf_rot_shell_fold_kernel, dim3(1), dim3(kWave), 0, ctx->stream,
+                               (const double*)shell_part, sp->n_shells, kShellBlocks, sp->abs_flag, sp->w, cc);
+          } else {
+            hipLaunchKernelGGL(rot_sums_half_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, ctx->stream,
+                               (const float2*)B.F2, d_bandpasses[r], (const float2*)d_ref_ft1s[r],
+                               (const float*)B.rwedge, wr, n, n, n, B.part);
+            hipLaunchKernelGGL(rot_cc_fold_kernel, dim3(1), dim3(kRedThreads), 0, ctx->stream,
+                               (const double*)B.part, kRedBlocks, cc);
+          }
         }
         EMC_HIP(hipGetLastError());
       }
@@ -338,7 +404,7 @@ emc_status emc_align_grid(emc_ctx* ctx, int n, const float* d_particle, const fl
   const void* ft1s[1] = {d_ref_ft1};
   const float* bps[1] = {d_bandpass};
   return emc_align_grid_padded(ctx, n, n, zero, 1, nullptr, 1, d_particle, d_ref, d_ref_wgt_rot, wgts, ft1s,
-                               d_vol_mask, d_peak_mask, bps, d_wedge, angles_colmajor, est0, peak_com, angles_pts,
+                               nullptr, d_vol_mask, d_peak_mask, bps, d_wedge, angles_colmajor, est0, peak_com, angles_pts,
                                n_orient, mode, ccc_out, shift_out);
 }
 

@@ -76,4 +76,21 @@ struct PeakResult {
 void peak_search(emc_ctx* ctx, const float* map, int stride, const int dims[3], const float* mask,
                  const int com[3], PeakResult* d_res, const double* d_add);
 
+// Shells of the shell-weighted CCC (emc_shell_plan_create): concatenated
+// 1-based full-grid voxel lists, shell s at d_idx[d_offsets[s] .. d_offsets[s+1]).
+constexpr int kShellBlocks = 64;
+__global__ void __launch_bounds__(kWave) xcf_rot_shell_fold_kernel(const double* __restrict__ part, int n_shells,
+                                                                  int nb, int abs_flag, double w,
+                                                                  double* __restrict__ out);
+
 }  // namespace emc
+
+struct emc_shell_plan {
+  int abs_flag = 0;  // wCCC{1}
+  double w = 0.0;    // wCCC{2}
+  int n_shells = 0;
+  int64_t n_idx = 0;
+  int64_t max_index = 0;
+  int64_t* d_offsets = nullptr;
+  uint32_t* d_idx = nullptr;
+};

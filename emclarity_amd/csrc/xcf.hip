@@ -167,7 +167,6 @@ __global__ void __launch_bounds__(kRedThreads) xcf_rot_full_kernel(
 
 // Shell-weighted form: block (b, s) reduces shell s's voxel list with
 // stride gridDim.x * blockDim.x (fixed partition: deterministic sums)
-constexpr int kShellBlocks = 64;
 
 __global__ void __launch_bounds__(kRedThreads) xcf_rot_shell_kernel(
     const float2* __restrict__ P, const float2* __restrict__ R, const float* __restrict__ wp,
@@ -225,20 +224,12 @@ __global__ void __launch_bounds__(kWave) xcf_rot_shell_fold_kernel(const double*
 
 }  // namespace emc
 
-struct emc_shell_plan {
-  int n_shells = 0;
-  int64_t n_idx = 0;
-  int64_t max_index = 0;
-  int64_t* d_offsets = nullptr;
-  uint32_t* d_idx = nullptr;
-};
-
 using namespace emc;
 
 extern "C" {
 
-emc_status emc_shell_plan_create(emc_ctx* ctx, int n_shells, const int64_t* h_offsets, const uint32_t* h_indices,
-                                 emc_shell_plan** out) {
+emc_status emc_shell_plan_create(emc_ctx* ctx, int abs_flag, double w, int n_shells, const int64_t* h_offsets,
+                                 const uint32_t* h_indices, emc_shell_plan** out) {
   return guard([&] {
     use_device(ctx);
     EMC_REQUIRE(out && h_offsets && n_shells > 0, "null pointer or no shells");
@@ -252,6 +243,8 @@ emc_status emc_shell_plan_create(emc_ctx* ctx, int n_shells, const int64_t* h_of
       mx = std::max<int64_t>(mx, h_indices[j]);
     }
     auto* p = new emc_shell_plan;
+    p->abs_flag = abs_flag ? 1 : 0;
+    p->w = w;
     p->n_shells = n_shells;
     p->n_idx = total;
     p->max_index = mx;
@@ -280,8 +273,7 @@ emc_status emc_shell_plan_destroy(emc_ctx* ctx, emc_shell_plan* plan) {
 }
 
 emc_status emc_xcf_rotational_shells(emc_ctx* ctx, const emc_shell_plan* plan, const void* d_p, const void* d_r,
-                                     const float* d_wp, const float* d_wr, const int dims[3], int abs_flag,
-                                     double w, double* cc_out) {
+                                     const float* d_wp, const float* d_wr, const int dims[3], double* cc_out) {
   return guard([&] {
     use_device(ctx);
     EMC_REQUIRE(plan && d_p && d_r && dims && cc_out, "null pointer");
@@ -303,7 +295,7 @@ emc_status emc_xcf_rotational_shells(emc_ctx* ctx, const emc_shell_plan* plan, c
                        (const float2*)d_p, (const float2*)d_r, wp, wr, (const int64_t*)plan->d_offsets,
                        (const uint32_t*)plan->d_idx, part);
     hipLaunchKernelGGL(xcf_rot_shell_fold_kernel, dim3(1), dim3(kWave), 0, ctx->stream, (const double*)part,
-                       plan->n_shells, kShellBlocks, abs_flag, w, out);
+                       plan->n_shells, kShellBlocks, plan->abs_flag, plan->w, out);
     EMC_HIP(hipGetLastError());
     EMC_HIP(hipMemcpyAsync(cc_out, out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     EMC_HIP(hipStreamSynchronize(ctx->stream));

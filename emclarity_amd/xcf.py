@@ -59,8 +59,8 @@ class ShellPlan:
         off[1:] = np.cumsum([sh.size for sh in shells])
         idx = np.ascontiguousarray(np.concatenate(shells).astype(np.uint32)) if off[-1] else np.zeros(1, np.uint32)
         h = C.c_void_p()
-        _lib.call("emc_shell_plan_create", self.ctx.handle, self.n_shells, off.ctypes.data_as(C.POINTER(C.c_int64)),
-                  idx.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(h))
+        _lib.call("emc_shell_plan_create", self.ctx.handle, self.abs_flag, self.w, self.n_shells,
+                  off.ctypes.data_as(C.POINTER(C.c_int64)), idx.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(h))
         self.handle = h
 
     def close(self):
@@ -98,7 +98,7 @@ def BH_multi_xcf_Rotational(rotPART_FT, REF_FT, wdgMask, refWdg, wCCC=0):
         plan = wCCC if isinstance(wCCC, ShellPlan) else ShellPlan(wCCC, P.ctx)
         _lib.call("emc_xcf_rotational_shells", P.ctx.handle, plan.handle, P.vp, R.vp,
                   wp.vp if wp is not None else None, wr.vp if wr is not None else None, _lib.i3(P.dims),
-                  plan.abs_flag, plan.w, C.byref(cc))
+                  C.byref(cc))
         return cc.value
     _lib.call("emc_xcf_rotational", P.ctx.handle, P.vp, R.vp, wp.vp if wp is not None else None,
               wr.vp if wr is not None else None, _lib.i3(P.dims), C.byref(cc))

@@ -262,19 +262,20 @@ emc_status emc_xcf_rotational(emc_ctx* ctx, const void* d_p_c64, const void* d_r
 
 /* Shell-weighted form of BH_multi_xcf_Rotational (wCCC a cell,
  * statistics/BH_multi_xcf_Rotational.m:42-70, the cell built by
- * BH_multi_cRef's calc_shells): a plan holds the shells' voxel lists
- * (wCCC{3..end}: 1-based linear indices into the full grid, concatenated,
- * h_offsets[s]..h_offsets[s+1] for shell s), made once per set of shells.
- * Per shell CCC_s = sum Re(Wr P R Wp) / sqrt(sum|R Wp|^2 sum|P Wr|^2) over
- * its voxels; cc = sum_s f(CCC_s), f(c) = (c^2 / (c + w^-1.5))^3, |f| when
- * abs_flag (wCCC{1}), w = wCCC{2}. Synchronous. */
+ * BH_multi_cRef's calc_shells): a plan holds the whole cell, abs_flag =
+ * wCCC{1}, w = wCCC{2} and the shells' voxel lists (wCCC{3..end}: 1-based
+ * linear indices into the full grid, concatenated, h_offsets[s] ..
+ * h_offsets[s+1] for shell s), made once per cell. Per shell CCC_s =
+ * sum Re(Wr P R Wp) / sqrt(sum|R Wp|^2 sum|P Wr|^2) over its voxels;
+ * cc = sum_s f(CCC_s), f(c) = (c^2 / (c + w^-1.5))^3, |f| when abs_flag.
+ * Synchronous. */
 typedef struct emc_shell_plan emc_shell_plan;
-emc_status emc_shell_plan_create(emc_ctx* ctx, int n_shells, const int64_t* h_offsets, const uint32_t* h_indices,
-                                 emc_shell_plan** out);
+emc_status emc_shell_plan_create(emc_ctx* ctx, int abs_flag, double w, int n_shells, const int64_t* h_offsets,
+                                 const uint32_t* h_indices, emc_shell_plan** out);
 emc_status emc_shell_plan_destroy(emc_ctx* ctx, emc_shell_plan* plan);
 emc_status emc_xcf_rotational_shells(emc_ctx* ctx, const emc_shell_plan* plan, const void* d_p_c64,
                                      const void* d_r_c64, const float* d_wp, const float* d_wr, const int dims[3],
-                                     int abs_flag, double w, double* cc_out);
+                                     double* cc_out);
 
 /* -------------------------------------------- alignment loop (A15, A17) */
 
@@ -326,12 +327,16 @@ emc_status emc_align_grid(emc_ctx* ctx, int n, const float* d_particle, const fl
  * the first reference's bandpass on the particle, whose shift the reference
  * code keeps (its sortrows over all-zero CCC columns keeps the first row,
  * :1198-1201); stage 2 scores the one rotated particle against every
- * reference: ccc_out[o * n_ref + r]. emc_align_grid is this call with
+ * reference: ccc_out[o * n_ref + r]; shells (NULL, or n_ref plans with
+ * NULL entries) scores reference r with the shell-weighted CCC of its wCCC
+ * cell (flgWeightCCC, :1355-1359, shells over the C^3 grid) instead of the
+ * plain CCC. emc_align_grid is this call with
  * W = C, nsym = 1 and one reference. */
 emc_status emc_align_grid_padded(emc_ctx* ctx, int w, int c, const int pad_low[3], int nsym,
                                  const double* S_colmajor, int n_ref, const float* d_particle,
                                  const float* d_ref, const float* d_ref_wgt_rot, const float* const* d_ref_wgts,
-                                 const void* const* d_ref_ft1s_c64, const float* d_vol_mask,
+                                 const void* const* d_ref_ft1s_c64, const emc_shell_plan* const* shells,
+                                 const float* d_vol_mask,
                                  const float* d_peak_mask, const float* const* d_bandpasses, const float* d_wedge,
                                  const double angles_colmajor[9], const double est0[3], const int peak_com[3],
                                  const double* angles_pts, int n_orient, int mode, double* ccc_out,

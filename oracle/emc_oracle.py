@@ -722,6 +722,9 @@ class AlignInputs:
     # stage-1 particle bandpass when it is another reference's (several
     # references: stage 1 runs with the first of refToAlign, :1278-1318)
     bandpass_stage1: object = None
+    # wCCC: 0 (plain CCC) or the shell cell [abs flag, w, shells...] of
+    # BH_multi_xcf_Rotational (:1355-1359)
+    wccc: object = 0
 
 
 def ref_ft1_of(ref, vol_mask, pad_calc):
@@ -752,7 +755,7 @@ def align_orientation(inp: AlignInputs, rot_mat, est_peak0):
     part = interp3d(inp.particle, R, est, "Bah", "inv", inp.symmetry)
     wmask = interp3d(inp.wedge, R, [0, 0, 0], "Bah", "inv", inp.symmetry)
     rot_part2 = band_limit_center_normalize(part * inp.vol_mask, inp.bandpass, inp.pad_calc)
-    ccc = xcf_rotational(rot_part2, inp.ref_ft1, np.fft.ifftshift(wmask), inp.ref_wgt)
+    ccc = xcf_rotational(rot_part2, inp.ref_ft1, np.fft.ifftshift(wmask), inp.ref_wgt, inp.wccc)
     return ccc, est
 
 
@@ -762,7 +765,7 @@ def align_initial(inp: AlignInputs):
     part = interp3d(inp.particle, A, inp.shift_val, "Bah", "inv", inp.symmetry)
     wmask = interp3d(inp.wedge, A, [0, 0, 0], "Bah", "inv", inp.symmetry)
     rp = band_limit_center_normalize(part * inp.vol_mask, inp.bandpass, inp.pad_calc)
-    return xcf_rotational(rp, inp.ref_ft1, np.fft.ifftshift(wmask), inp.ref_wgt)
+    return xcf_rotational(rp, inp.ref_ft1, np.fft.ifftshift(wmask), inp.ref_wgt, inp.wccc)
 
 
 def align_grid(inp: AlignInputs, grid_angles):

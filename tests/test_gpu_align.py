@@ -125,6 +125,48 @@ def test_align_grid_multi_reference_matches_oracle(ctx):
     assert res.row[0] == 1 and not res.reverted
 
 
+def _radial_cell(n, abs_flag):
+    """A wCCC cell the way BH_multi_cRef's calc_shells builds one: a low
+    resolution lump, then radial bands (1-based indices into the n^3 grid,
+    radius from the FFT origin); w = the voxel count beyond the first."""
+    f = np.fft.fftfreq(n) * n
+    z, y, x = np.meshgrid(f, f, f, indexing="ij")
+    r = np.sqrt(x * x + y * y + z * z).ravel()
+    edges = [0.0, 3.0] + list(np.arange(4.5, n / 2.0 - 1.0, 1.5))
+    shells = [np.flatnonzero((lo <= r) & (r < hi)).astype(np.uint32) + 1 for lo, hi in zip(edges[:-1], edges[1:])]
+    return [abs_flag, float(sum(len(s) for s in shells[1:]))] + shells
+
+
+@pytest.mark.parametrize("w,c,abs_flag", [(32, 32, 1), (29, 32, 0)])
+def test_align_grid_shell_weighted_matches_oracle(ctx, w, c, abs_flag):
+    """flgWeightCCC: stage 2 scores with the shell-weighted CCC of the wCCC
+    cell (shells over the calc grid), reading the half spectrum's mirror for
+    x > n/2; against the oracle's full-grid shell sums."""
+    truth = (40.0, 30.0, 20.0)
+    case, _, _, _, vm, _ = _inputs(w, 17, truth, snr=2.0)
+    _, wedge, bp, pm, _, rwr = _inputs(c, 17, truth, snr=2.0)
+    lo = (c - w) // 2
+    pad = [[lo] * 3, [c - w - lo] * 3]
+    cell = _radial_cell(c, abs_flag)
+    pts = np.array([truth, (0.0, 0.0, 0.0), (40.0, 30.0, 30.0), (200.0, 60.0, -40.0)])
+    sv = np.array([0.25, -0.5, 0.5])
+    al = ParticleAligner(case.particle, case.ref, wedge, bp, pm, vol_mask=vm, ref_wgt_rot=rwr, shift_val=sv,
+                         pad_calc=pad, wCCC=cell)
+    ccc, est = al.grid(pts, sv, mode=3)
+    inp = O.AlignInputs(particle=case.particle.astype(np.float64), ref=case.ref.astype(np.float64),
+                        ref_wgt_rot=rwr.astype(np.float64), ref_wgt=1.0,
+                        ref_ft1=O.ref_ft1_of(case.ref.astype(np.float64), vm, pad),
+                        vol_mask=vm.astype(np.float64), peak_mask=pm.astype(np.float64),
+                        bandpass=bp.astype(np.float64), wedge=wedge.astype(np.float64),
+                        angles=np.eye(3), shift_val=sv, pad_calc=pad, wccc=cell)
+    rc, re = O.align_grid(inp, pts)
+    assert np.max(np.abs(ccc - rc) / np.maximum(np.abs(rc), 1e-3)) < 2e-4, (ccc, rc)
+    assert np.max(np.abs(est - re)) < 1e-3
+    assert int(np.argmax(ccc)) == int(np.argmax(rc))
+    c0, _ = al.grid([[0.0, 0.0, 0.0]], sv, mode=2)
+    assert abs(c0[0] - O.align_initial(inp)) < 2e-4 * max(abs(O.align_initial(inp)), 1e-3)
+
+
 def test_align_particle_recovers_ground_truth(ctx):
     """Config-0 style (64^3, 2.5 A/px, +-60 wedge, CTF df 3 um, SNR 0.1):
     the grid search + refinement recovers the orientation within 1 degree
