@@ -876,31 +876,50 @@ const int* support_schedule(emc_ctx* ctx, int n, float kmax2, int* ngroups) {
   return d;
 }
 
-// Score + gradient launch shared by emc_align_eval_grad and the refinement.
-void eval_grad_impl(emc_ctx* ctx, int n, int n_part, int table_format, const void* d_F2, const void* d_Q,
-                    const float* d_N, const double* d_A, const float* d_MdM, const float* d_shift,
-                    double support_radius, double* d_out) {
-  int ns = 0;
-  const int2* supp = support_list(ctx, n, support_radius, &ns);
-  EMC_REQUIRE(ns > 0, "empty support");
+// Score + gradient launch shared by emc_align_eval_grad and the refinement:
+// the support list and the partial-sum buffer are set up first (they may
+// allocate or copy), the launches alone can then be captured into a graph.
+struct EvalSetup {
+  const int2* supp;
+  int ns, g;
+  double* part;
+};
+
+EvalSetup eval_grad_setup(emc_ctx* ctx, int n, int n_part, double support_radius) {
+  EvalSetup e;
+  e.supp = support_list(ctx, n, support_radius, &e.ns);
+  EMC_REQUIRE(e.ns > 0, "empty support");
   // split every particle's support over enough workgroups to fill the
   // device ("grad_blocks" over the whole batch, 4096 = ~16 per CU by
   // default), at least 256 frequencies each
   const int target = tune_value(ctx, "grad_blocks", n, 4096);
-  int g = std::max(1, std::min((ns + 255) / 256, (target + n_part - 1) / n_part));
-  double* part = (double*)scratch(ctx, "grad_part", sizeof(double) * kGradTerms * (size_t)g * n_part);
+  e.g = std::max(1, std::min((e.ns + 255) / 256, (target + n_part - 1) / n_part));
+  e.part = (double*)scratch(ctx, "grad_part", sizeof(double) * kGradTerms * (size_t)e.g * n_part);
+  return e;
+}
+
+void eval_grad_launch(emc_ctx* ctx, const EvalSetup& e, int n, int n_part, int table_format, const void* d_F2,
+                      const void* d_Q, const float* d_N, const double* d_A, const float* d_MdM, const float* d_shift,
+                      double* d_out, bool timed) {
   {
-    KTimer kt(ctx, "align_eval_grad");
+    KTimer kt(timed ? ctx : nullptr, "align_eval_grad");
     if (table_format == EMC_TABLE_C64)
-      hipLaunchKernelGGL(align_eval_grad_kernel<TabC64>, dim3(n_part, g), dim3(kBatchThreads), 0, ctx->stream,
-                         (const float2*)d_F2, n, (const float2*)d_Q, d_N, d_MdM, d_shift, supp, ns, part);
+      hipLaunchKernelGGL(align_eval_grad_kernel<TabC64>, dim3(n_part, e.g), dim3(kBatchThreads), 0, ctx->stream,
+                         (const float2*)d_F2, n, (const float2*)d_Q, d_N, d_MdM, d_shift, e.supp, e.ns, e.part);
     else
-      hipLaunchKernelGGL(align_eval_grad_kernel<TabF16>, dim3(n_part, g), dim3(kBatchThreads), 0, ctx->stream,
-                         (const half2_t*)d_F2, n, (const float2*)d_Q, d_N, d_MdM, d_shift, supp, ns, part);
+      hipLaunchKernelGGL(align_eval_grad_kernel<TabF16>, dim3(n_part, e.g), dim3(kBatchThreads), 0, ctx->stream,
+                         (const half2_t*)d_F2, n, (const float2*)d_Q, d_N, d_MdM, d_shift, e.supp, e.ns, e.part);
   }
-  hipLaunchKernelGGL(adjoint_fold_kernel, dim3(n_part), dim3(kRedThreads), 0, ctx->stream, (const double*)part,
-                     g, d_A, d_out);
+  hipLaunchKernelGGL(adjoint_fold_kernel, dim3(n_part), dim3(kRedThreads), 0, ctx->stream, (const double*)e.part,
+                     e.g, d_A, d_out);
   EMC_HIP(hipGetLastError());
+}
+
+void eval_grad_impl(emc_ctx* ctx, int n, int n_part, int table_format, const void* d_F2, const void* d_Q,
+                    const float* d_N, const double* d_A, const float* d_MdM, const float* d_shift,
+                    double support_radius, double* d_out) {
+  const EvalSetup e = eval_grad_setup(ctx, n, n_part, support_radius);
+  eval_grad_launch(ctx, e, n, n_part, table_format, d_F2, d_Q, d_N, d_A, d_MdM, d_shift, d_out, true);
 }
 
 // ------------------------------------------------------------ refinement
@@ -1263,17 +1282,75 @@ emc_status emc_align_refine(emc_ctx* ctx, int n, int n_part, int table_format, c
     RefineState st;
     char* base = (char*)scratch(ctx, "refine_state", RefineState::bytes(n_part));
     st.bind(base, n_part);
+    const EvalSetup e = eval_grad_setup(ctx, n, n_part, support_radius);
     const dim3 grid(ceil_div(n_part, 64)), block(64);
-    hipLaunchKernelGGL(refine_begin_kernel, grid, block, 0, ctx->stream, st, (const double*)d_R,
-                       (const double*)d_s, n_part);
-    eval_grad_impl(ctx, n, n_part, table_format, d_F2, d_Q, d_N, d_A, st.MdM, st.sf, support_radius, st.out);
-    hipLaunchKernelGGL(refine_first_kernel, grid, block, 0, ctx->stream, st, n_part, step_angle, step_shift);
-    for (int it = 0; it < iters; ++it) {
-      eval_grad_impl(ctx, n, n_part, table_format, d_F2, d_Q, d_N, d_A, st.MdM, st.sf, support_radius, st.out);
-      hipLaunchKernelGGL(refine_step_kernel, grid, block, 0, ctx->stream, st, n_part);
+    // kernels only (no allocation, copy, memset or host wait): capturable
+    auto body = [&](bool timed) {
+      hipLaunchKernelGGL(refine_begin_kernel, grid, block, 0, ctx->stream, st, (const double*)d_R,
+                         (const double*)d_s, n_part);
+      eval_grad_launch(ctx, e, n, n_part, table_format, d_F2, d_Q, d_N, d_A, st.MdM, st.sf, st.out, timed);
+      hipLaunchKernelGGL(refine_first_kernel, grid, block, 0, ctx->stream, st, n_part, step_angle, step_shift);
+      for (int it = 0; it < iters; ++it) {
+        eval_grad_launch(ctx, e, n, n_part, table_format, d_F2, d_Q, d_N, d_A, st.MdM, st.sf, st.out, timed);
+        hipLaunchKernelGGL(refine_step_kernel, grid, block, 0, ctx->stream, st, n_part);
+      }
+      hipLaunchKernelGGL(refine_end_kernel, grid, block, 0, ctx->stream, st, n_part, d_R, d_s, d_cc);
+      EMC_HIP(hipGetLastError());
+    };
+    // Replay the whole iteration chain as one HIP graph, captured again
+    // whenever an argument it bakes in changes; the "refine_graph"
+    // launch-shape value 0 launches eagerly instead. Per-kernel timing
+    // (host-side event bookkeeping) brackets the evaluations of eager
+    // launches only.
+    if (tune_value(ctx, "refine_graph", n, 1) == 0) {
+      body(true);
+      return;
     }
-    hipLaunchKernelGGL(refine_end_kernel, grid, block, 0, ctx->stream, st, n_part, d_R, d_s, d_cc);
-    EMC_HIP(hipGetLastError());
+    const int ki[6] = {n, n_part, table_format, iters, e.ns, e.g};
+    const double kd[2] = {step_angle, step_shift};
+    const void* kp[12] = {d_F2, d_Q, d_N, d_A, d_R, d_s, d_cc, base, e.supp, e.part, nullptr, nullptr};
+    std::vector<unsigned char> kb(sizeof(ki) + sizeof(kd) + sizeof(kp));
+    std::memcpy(kb.data(), ki, sizeof(ki));
+    std::memcpy(kb.data() + sizeof(ki), kd, sizeof(kd));
+    std::memcpy(kb.data() + sizeof(ki) + sizeof(kd), kp, sizeof(kp));
+    auto& gr = ctx->graphs["refine"];
+    if (!gr.exec || gr.key != kb) {
+      hipGraph_t graph = nullptr;
+      EMC_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      try {
+        body(false);
+      } catch (...) {
+        (void)hipStreamEndCapture(ctx->stream, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+      }
+      EMC_HIP(hipStreamEndCapture(ctx->stream, &graph));
+      bool updated = false;
+      if (gr.exec) {
+        hipGraphNode_t err_node = nullptr;
+        hipGraphExecUpdateResult res;
+        updated = hipGraphExecUpdate(gr.exec, graph, &err_node, &res) == hipSuccess &&
+                  res == hipGraphExecUpdateSuccess;
+        if (!updated) {
+          (void)hipGetLastError();
+          (void)hipGraphExecDestroy(gr.exec);
+          gr.exec = nullptr;
+        }
+      }
+      if (!updated) {
+        const hipError_t ie = hipGraphInstantiate(&gr.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) {
+          gr.exec = nullptr;
+          gr.key.clear();
+          EMC_HIP(ie);
+        }
+      } else {
+        (void)hipGraphDestroy(graph);
+      }
+      gr.key = kb;
+    }
+    EMC_HIP(hipGraphLaunch(gr.exec, ctx->stream));
   });
 }
 

@@ -258,7 +258,10 @@ struct TuneRange {
   const char* name;
   int lo, hi, dflt;
 };
-constexpr TuneRange kTune[] = {{"search_tile", 2, 128, 16}, {"grad_blocks", 256, 65536, 4096}, {"search_passes", 1, 4, 2}};
+constexpr TuneRange kTune[] = {{"search_tile", 2, 128, 16},
+                               {"grad_blocks", 256, 65536, 4096},
+                               {"search_passes", 1, 4, 2},
+                               {"refine_graph", 0, 1, 1}};
 
 const TuneRange* find_tune(const char* name) {
   for (const auto& t : kTune)
@@ -314,6 +317,8 @@ emc_status emc_ctx_destroy(emc_ctx* ctx) {
       (void)hipEventDestroy(p.b);
     }
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+    for (auto& kv : ctx->graphs)
+      if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
     for (auto& kv : ctx->scratch)
       if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     release_cache(ctx);

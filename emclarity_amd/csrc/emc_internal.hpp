@@ -100,6 +100,14 @@ struct emc_ctx {
   std::vector<Pending> pending;
   std::vector<hipEvent_t> event_pool;
   std::map<std::string, std::pair<double, int64_t>> ktimes;  // name -> (ms, launches)
+  // captured launch sequences replayed as HIP graphs, keyed by name; `key`
+  // holds every argument the capture baked in (pointers included), so a
+  // changed argument means a new capture
+  struct Graph {
+    hipGraphExec_t exec = nullptr;
+    std::vector<unsigned char> key;
+  };
+  std::map<std::string, Graph> graphs;
 };
 
 namespace emc {

@@ -240,3 +240,47 @@ def test_pick_matches_host(ctx):
         ph, th, ps = pts[b[i]]
         ref = starts[i] @ O.define_matrix([ph, th, ps - ph], "Bah", "inv")
         assert np.max(np.abs(R0.get()[i].reshape(3, 3) - ref)) < 1e-12
+
+
+def test_refinement_graph_replays_match_eager():
+    """emc_align_refine replays its iteration chain as a HIP graph: every
+    replay (the same buffers: the cached graph; new buffers: a new capture)
+    with eager work and host waits in between equals the eager launches
+    exactly (the same kernels in the same order, deterministic sums)."""
+    from emclarity_amd import autotune
+    from emclarity_amd.device import Context, DeviceArray
+    from emclarity_amd.workload import make_particle, make_size_class
+    c = Context(0)
+    n = 32
+    rng = np.random.default_rng(23)
+    sc, blobs = make_size_class(n)
+    parts = [make_particle(rng, sc, blobs) for _ in range(5)]
+    wedges = [BH_weightMaskMex((n, n, n), 1, p.tlt, [n // 2 + 1] * 3, [[n, n, n], [0, 0, 0]], ctx=c).get()
+              for p in parts]
+    al = BatchAligner(n, sc.ref, sc.bandpass, peak_radius=6, ctx=c)
+    al.set_particles([p.data for p in parts], wedges, [p.start for p in parts])
+    grid = eulerSearch("C1", 15, 6, 15, 6)
+    score, shift = al.search(grid)
+    best = np.argmax(score, axis=1)
+    R0 = np.stack([parts[i].start @ local_matrix(*grid[best[i]]) for i in range(5)])
+    s0 = shift[np.arange(5), best].astype(np.float64)
+    autotune.apply(c, n, {"refine_graph": 0})
+    Re, se, ce = al.refine(R0, s0, iters=8)
+    autotune.apply(c, n, {"refine_graph": 1})
+    for rep in range(3):                                  # new buffers each call
+        Rg, sg, cg = al.refine(R0, s0, iters=8)
+        assert np.array_equal(Rg, Re) and np.array_equal(sg, se) and np.array_equal(cg, ce), rep
+        al.search(grid[:20])                              # eager work between replays
+        c.synchronize()
+    R = DeviceArray((5, 9), np.float64, c)
+    s = DeviceArray((5, 3), np.float64, c)
+    import ctypes as C
+    from emclarity_amd import _lib
+    for rep in range(3):                                  # the same buffers: the cached graph
+        for d, h in ((R, np.ascontiguousarray(R0.reshape(5, 9))), (s, np.ascontiguousarray(s0))):
+            _lib.call("emc_memcpy_h2d", c.handle, d.vp, h.ctypes.data_as(C.c_void_p), C.c_size_t(h.nbytes))
+        cc = al.refine_device(R, s, iters=8)
+        assert np.array_equal(R.get().reshape(5, 3, 3), Re) and np.array_equal(s.get(), se), rep
+        assert np.array_equal(cc.get(), ce), rep
+        al._eval_local(R0, s0)                            # eager adjoint between replays
+    c.close()
