@@ -38,7 +38,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=100, help="particles per step (batched path)")
+    ap.add_argument("--batch", type=int, default=2000, help="particles per step (batched path)")
+    ap.add_argument("--pool", type=int, default=100,
+                    help="distinct rendered particles; the rest of the batch are copies of them with fresh "
+                         "noise and new start orientations")
     ap.add_argument("--plain", type=int, default=10, help="particles timed on the plain path (0 = skip)")
     ap.add_argument("--refine-iters", type=int, default=12)
     ap.add_argument("--cpu-sample", type=float, default=20.0,
@@ -245,7 +248,7 @@ def main():
     from emclarity_amd.device import DeviceArray, default_context
     from emclarity_amd.geometry import eulerSearch
     from emclarity_amd.masking import BH_weightMaskMex
-    from emclarity_amd.workload import local_grid, make_particle, make_size_class, mixed_sizes
+    from emclarity_amd.workload import copy_particle, local_grid, make_particle, make_size_class, mixed_sizes
 
     ctx = default_context()
     if a.copy_mib > 0:
@@ -258,14 +261,29 @@ def main():
     for n in (64, 96, 128):
         classes[n], blobs[n] = make_size_class(n)
     sizes = mixed_sizes(a.batch)
-    particles = [make_particle(rng, classes[n], blobs[n]) for n in sizes]
+    pool_sizes = mixed_sizes(min(a.pool, a.batch))
+    particles = [make_particle(rng, classes[n], blobs[n], keep_signal=True) for n in pool_sizes]
     wedges = [BH_weightMaskMex((p.n,) * 3, 1, p.tlt, [p.n // 2 + 1] * 3, [[p.n] * 3, [0, 0, 0]])
               for p in particles]
     host = [p.data for p in particles]
     for p in particles:
         p.data = DeviceArray.from_numpy(p.data)   # resident before timing
+    # the rest of the batch: each pool particle seen again with a fresh noise
+    # realisation and a new start (same object, pose, shift, wedge, defocus)
+    for n in (64, 96, 128):
+        src = [i for i, p in enumerate(particles[:len(pool_sizes)]) if p.n == n]
+        for k in range(sizes.count(n) - len(src)):
+            i = src[k % len(src)]
+            q = copy_particle(rng, particles[i])
+            q.data = DeviceArray.from_numpy(q.data)
+            particles.append(q)
+            wedges.append(wedges[i])
+            host.append(None)
+    for p in particles:
+        p.signal = None
     coarse = eulerSearch("C1", 15, 6, 15, 6)
-    print(f"[bench] {len(particles)} particles {dict(zip(*[x.tolist() for x in np.unique(sizes, return_counts=True)]))}, "
+    print(f"[bench] {len(particles)} particles {dict(zip(*[x.tolist() for x in np.unique(sizes, return_counts=True)]))} "
+          f"({len(pool_sizes)} rendered, the rest fresh-noise copies with new starts), "
           f"coarse grid {len(coarse)} orientations + {a.refine_iters} adjoint iterations",
           file=sys.stderr, flush=True)
 
@@ -337,7 +355,8 @@ def main():
             "data": "synthetic",
             "config": {"workload": "batched alignment: mixed 64/96/128^3 subtomograms (50/30/20), own wedge "
                                    "U[50,70] deg + defocus U[1.5,4.0] um, SNR 0.1, +-15 deg local search",
-                       "particles_per_step": len(particles), "path": "batched+adjoint",
+                       "particles_per_step": len(particles), "distinct_rendered": len(pool_sizes),
+                       "path": "batched+adjoint",
                        "reference_table": a.table,
                        "coarse_orientations": int(len(coarse)), "refine_iterations": a.refine_iters},
             "accuracy": acc, "variants": variants, "plain_gpu": plain,

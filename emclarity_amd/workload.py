@@ -31,6 +31,8 @@ class Particle:
     max_tilt: float
     defocus_um: float
     tlt: np.ndarray = field(repr=False, default=None)
+    signal: np.ndarray = field(repr=False, default=None)   # noise-free wedge x CTF volume (kept on request)
+    noise_sd: float = 0.0
 
 
 @dataclass
@@ -64,21 +66,42 @@ def make_size_class(n, seed=100, pixel_A=2.5):
                      S.sphere_mask(n, n // 8 + 4), S.sphere_mask(n, 0.45 * n)), blobs
 
 
-def make_particle(rng, sc: SizeClass, blobs, pixel_A=2.5, snr=0.1):
-    n = sc.n
-    A = random_rotation(rng)
-    # off-grid local truth inside the +-15 deg search
+def _local_truth(rng):
+    """Off-grid local truth inside the +-15 deg search: (phi, theta, psi)."""
     theta = rng.uniform(0.5, 12.0)
     phi = rng.uniform(0.0, 360.0)
     psi = rng.uniform(-12.0, 12.0)
+    return phi, theta, psi
+
+
+def make_particle(rng, sc: SizeClass, blobs, pixel_A=2.5, snr=0.1, keep_signal=False):
+    n = sc.n
+    A = random_rotation(rng)
+    phi, theta, psi = _local_truth(rng)
     Rt = A @ local_matrix(phi, theta, psi)
     shift = rng.uniform(-3.0, 3.0, 3)
     max_tilt = rng.uniform(50.0, 70.0)
     df = rng.uniform(1.5, 4.0)
     clean = S.render(blobs, n, Rt, shift)
-    data = S.degrade(clean, rng, max_tilt, pixel_A, df, snr, ctf=True).astype(np.float32)
+    sig = S.degrade(clean, rng, max_tilt, pixel_A, df, None, ctf=True)
+    sd = math.sqrt(sig.var() / snr)
+    data = (sig + rng.standard_normal(sig.shape) * sd).astype(np.float32)
     T = S.tilt_table(max_tilt, 3.0, df, pixel_A)
-    return Particle(n, data, A, Rt, (phi, theta, psi), shift, max_tilt, df, T)
+    return Particle(n, data, A, Rt, (phi, theta, psi), shift, max_tilt, df, T,
+                    sig.astype(np.float32) if keep_signal else None, sd)
+
+
+def copy_particle(rng, p: Particle):
+    """Another subtomogram of the same object, pose, shift, wedge and
+    defocus as ``p`` (made with keep_signal=True): a fresh noise realisation
+    at the same SNR and a new start orientation, the truth again an off-grid
+    point of the +-15 deg local search around it."""
+    if p.signal is None:
+        raise ValueError("the particle was made without keep_signal")
+    phi, theta, psi = _local_truth(rng)
+    A = p.truth_R @ local_matrix(phi, theta, psi).T
+    data = p.signal + rng.standard_normal(p.signal.shape, dtype=np.float32) * np.float32(p.noise_sd)
+    return Particle(p.n, data, A, p.truth_R, (phi, theta, psi), p.shift, p.max_tilt, p.defocus_um, p.tlt)
 
 
 def mixed_sizes(count, fractions=((64, 0.5), (96, 0.3), (128, 0.2))):
