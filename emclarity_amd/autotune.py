@@ -11,7 +11,7 @@ Three parameters shape the batched path's hot launches (emc_tune_set):
 
 ``python -m emclarity_amd.autotune [--out PATH]`` times every candidate on
 the device with HIP events on the library's stream, for each box size of the
-alignment workload (64/96/128^3 with 50/30/20 particles, 114 coarse
+alignment workload (64/96/128^3 with 1,000/600/400 particles, 114 coarse
 orientations) and both table formats, and writes the fastest to a JSON
 table. The table measured on MI355X ships as ``tuning/mi355x.json``;
 BatchAligner applies its entries, and falls back to the library defaults
@@ -33,7 +33,7 @@ import numpy as np
 from . import _lib
 
 CANDIDATES = {"search_tile": [4, 8, 16, 32, 64], "search_passes": [1, 2, 3, 4],
-              "grad_blocks": [1024, 2048, 4096, 8192, 16384]}
+              "grad_blocks": [2048, 4096, 8192, 16384, 32768, 65536]}
 TABLE = Path(__file__).resolve().parent / "tuning" / "mi355x.json"
 _CACHE: dict | None = None
 
@@ -86,18 +86,25 @@ def _timed(ctx, fn, names, reps):
     return ms
 
 
-def tune(ctx, counts=((64, 50), (96, 30), (128, 20)), tables=("c64", "f16"), reps=3, seed=7, log=sys.stderr):
-    """Time every candidate; returns the table (params + all timings, ms)."""
+def tune(ctx, counts=((64, 1000), (96, 600), (128, 400)), tables=("c64", "f16"), reps=3, seed=7, pool=(50, 30, 20),
+         log=sys.stderr):
+    """Time every candidate; returns the table (params + all timings, ms).
+    Per box size, ``pool`` particles are rendered and the rest of ``count``
+    are copies of them with fresh noise and new starts (as the bench's
+    2,000-particle step)."""
     from .batched import BatchAligner
     from .geometry import eulerSearch
-    from .workload import make_particle, make_size_class
+    from .workload import copy_particle, make_particle, make_size_class
     grid = eulerSearch("C1", 15, 6, 15, 6)
-    out = {"device": "gfx950", "workload": "batched alignment, 114 coarse orientations, SNR 0.1",
+    out = {"device": "gfx950",
+           "workload": "batched alignment, 114 coarse orientations, SNR 0.1, "
+                       + ", ".join(f"{c} x {n}^3" for n, c in counts),
            "candidates": CANDIDATES, "params": {}, "timings_ms": {}}
     rng = np.random.default_rng(seed)
-    for n, count in counts:
+    for (n, count), k in zip(counts, pool):
         sc, blobs = make_size_class(n)
-        parts = [make_particle(rng, sc, blobs) for _ in range(count)]
+        parts = [make_particle(rng, sc, blobs, keep_signal=True) for _ in range(min(k, count))]
+        parts += [copy_particle(rng, parts[i % len(parts)]) for i in range(count - len(parts))]
         for table in tables:
             al = BatchAligner(n, sc.ref, sc.bandpass, table=table, ctx=ctx, tuning={})
             al.set_particles([p.data for p in parts], [None] * count, [p.start for p in parts])
