@@ -1,9 +1,12 @@
 """Autotuner of the batched search's launch shapes.
 
-Three parameters shape the batched path's hot launches (emc_tune_set):
+Four parameters shape the batched path's hot launches (emc_tune_set):
   search_tile  kx extent of the (kx, ky) column patches of the search
                schedule (ky extent 256 / search_tile): the shape of the
                reference-table region the units of one pass read together
+  search_zsplit  kz segments per column, each its own pass: a pass reads
+               1/search_zsplit of its patch's table region (more passes,
+               each with its own y/x transform of the partial sums)
   search_passes  passes of that schedule per launch: they share the launch's
                last, partly filled round of workgroups
   grad_blocks  workgroups an adjoint evaluation aims for: how finely each
@@ -32,7 +35,7 @@ import numpy as np
 
 from . import _lib
 
-CANDIDATES = {"search_tile": [4, 8, 16, 32, 64], "search_passes": [1, 2, 3, 4],
+CANDIDATES = {"search_tile": [4, 8, 16, 32, 64], "search_zsplit": [1, 2, 3, 4], "search_passes": [1, 2, 3, 4],
               "grad_blocks": [2048, 4096, 8192, 16384, 32768, 65536]}
 TABLE = Path(__file__).resolve().parent / "tuning" / "mi355x.json"
 _CACHE: dict | None = None
@@ -111,7 +114,7 @@ def tune(ctx, counts=((64, 1000), (96, 600), (128, 400)), tables=("c64", "f16"),
             R = np.stack([p.start for p in parts])
             s = np.zeros((count, 3))
             best, times = {}, {}
-            for name in ("search_tile", "search_passes", "grad_blocks"):
+            for name in ("search_tile", "search_zsplit", "search_passes", "grad_blocks"):
                 times[name] = {}
                 for v in CANDIDATES[name]:
                     apply(ctx, n, {name: v})

@@ -40,6 +40,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <tuple>
 
 #include "ops.hpp"
 
@@ -459,24 +460,29 @@ __global__ void __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per
   if (t < 9) Ms[t] = Mall[9 * (int64_t)unit + t];
 
   // Column schedule (support_schedule): per pass at most kBatchThreads
-  // support columns (kx, ky, h), h the largest |kz| inside the support
-  // sphere; a piece is a run of slots of one kx inside a pass.
+  // support columns (kx, ky, [lo, hi]), [lo, hi] a segment of the column's
+  // kz range [-h, h] inside the support sphere; a piece is a run of slots of
+  // one kx inside a pass.
   const int ngroups = sched[0], npieces = sched[1];
   const int4* groups = reinterpret_cast<const int4*>(sched + 4);
   const int4* pieces = groups + ngroups;
   const int* cols = reinterpret_cast<const int*>(pieces + npieces);
   const int4 G = groups[g];  // (first column, column count, first piece, piece count)
   const bool active = t < G.y;
-  const int packed = active ? cols[G.x + t] : (128 << 8);  // idle slot: k = (0, 0), h = -1
+  const int packed = active ? cols[G.x + t] : (128 << 8);  // idle slot: k = (0, 0), empty kz range
   const int kx = packed & 255, ky = ((packed >> 8) & 255) - 128;
-  const int h = active ? ((packed >> 16) & 255) : -1;
+  const int lo = active ? ((packed >> 16) & 127) - 64 : 1;   // the column's kz segment [lo, hi]
+  const int hi = active ? ((packed >> 23) & 127) - 64 : 0;
   const int yy = ky < 0 ? ky + n : ky;
   kyslot[t] = (uint8_t)yy;
   __syncthreads();
   const float m00 = Ms[0], m01 = Ms[1], m02 = Ms[2], m10 = Ms[3], m11 = Ms[4], m12 = Ms[5],
               m20 = Ms[6], m21 = Ms[7], m22 = Ms[8];
-  int hw = h;  // kz bound uniform over the wavefront (twiddle reads stay broadcast)
-  for (int o = kWave / 2; o > 0; o >>= 1) hw = max(hw, __shfl_xor(hw, o, kWave));
+  int lw = lo, hw = hi;  // kz bounds uniform over the wavefront (twiddle reads stay broadcast)
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    lw = min(lw, __shfl_xor(lw, o, kWave));
+    hw = max(hw, __shfl_xor(hw, o, kWave));
+  }
   float2 acc[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) acc[d] = make_float2(0.0f, 0.0f);
@@ -485,11 +491,12 @@ __global__ void __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per
   const float bx = m00 * fkx + m01 * fky, by = m10 * fkx + m11 * fky, bz = m20 * fkx + m21 * fky;
   // two kz per iteration: both samples' loads are issued in one block
   // before either is consumed (accumulation order stays kz, kz + 1). A lane
-  // at its column end also fetches kz + 1 = h + 1: its Q index is valid
-  // (kz + 1 <= n/2) and the table read is clamped; it is not summed.
-  for (int kz = -hw; kz <= hw; kz += 2) {
+  // outside its segment also fetches (kz >= lw >= -(n/2 - 1), kz + 1 <=
+  // hw + 1 <= n/2: its Q index is valid and the table read is clamped); it
+  // is not summed.
+  for (int kz = lw; kz <= hw; kz += 2) {
     const int kz1 = kz + 1;
-    const bool pa = kz >= -h && kz <= h, pb = kz1 >= -h && kz1 <= h;
+    const bool pa = kz >= lo && kz <= hi, pb = kz1 >= lo && kz1 <= hi;
     if (pa || pb) {
       const int za = kz < 0 ? kz + n : kz, zb = kz1 < 0 ? kz1 + n : kz1;
       const int oa = (za * cx + kx) * n + yy, ob = (zb * cx + kx) * n + yy;
@@ -817,24 +824,36 @@ void launch_fused_r(emc_ctx* ctx, int peak_radius, const void* F2, int n, const 
 // workgroup: the units running together on an XCD (nearby orientations)
 // then read a compact region of the reference table per pass.
 const int* support_schedule(emc_ctx* ctx, int n, float kmax2, int* ngroups) {
-  // patches of tx (kx) x ty (ky) columns, tx * ty = one workgroup's columns
+  // patches of tx (kx) x ty (ky) columns, tx * ty = one workgroup's columns;
+  // each column's kz range [-h, h] cut into zs consecutive segments, one
+  // pass per (patch, segment): a pass then reads 1/zs of the patch's table
+  // region (the z-DFT sums and the norm are additive over segments)
   const int tx = tune_value(ctx, "search_tile", n, 16), ty = kBatchThreads / tx;
-  char key[80];
-  snprintf(key, sizeof(key), "sched_%d_%.6f_t%d", n, (double)kmax2, tx);
+  const int zs = tune_value(ctx, "search_zsplit", n, 1);
+  char key[96];
+  snprintf(key, sizeof(key), "sched_%d_%.6f_t%d_z%d", n, (double)kmax2, tx, zs);
   auto it = ctx->scratch.find(key);
   auto mt = ctx->meta.find(key);
   if (it != ctx->scratch.end() && it->second.ptr && mt != ctx->meta.end()) {
     *ngroups = mt->second;
     return (const int*)it->second.ptr;
   }
-  std::map<std::pair<int, int>, std::vector<int>> tiles;  // ordered by (kx tile, ky tile)
+  // column entry: kx | (ky + 128) << 8 | (lo + 64) << 16 | (hi + 64) << 23,
+  // [lo, hi] the column's kz segment (|kz| <= n/2 - 1 <= 63)
+  std::map<std::tuple<int, int, int>, std::vector<int>> tiles;  // ordered by (kx tile, ky tile, segment)
   for (int kx = 0; kx <= n / 2; ++kx) {
     if ((float)(kx * kx) > kmax2) break;
     for (int ky = -(n / 2 - 1); ky <= n / 2 - 1; ++ky) {
       if ((float)(kx * kx + ky * ky) > kmax2) continue;
       int h = 0;
       while (h + 1 <= n / 2 - 1 && (float)(kx * kx + ky * ky + (h + 1) * (h + 1)) <= kmax2) ++h;
-      tiles[{kx / tx, (ky + 128) / ty}].push_back(kx | ((ky + 128) << 8) | (h << 16));
+      const int len = 2 * h + 1;
+      for (int sg = 0; sg < zs; ++sg) {
+        const int lo = -h + sg * len / zs, hi = -h + (sg + 1) * len / zs - 1;
+        if (lo > hi) continue;
+        tiles[{kx / tx, (ky + 128) / ty, sg}].push_back(kx | ((ky + 128) << 8) | ((lo + 64) << 16) |
+                                                        ((hi + 64) << 23));
+      }
     }
   }
   std::vector<int> cols, groups, pieces;

@@ -84,6 +84,8 @@ emc_status emc_timing_read(emc_ctx* ctx, const char* name, double* ms_total, int
  *   "search_passes" passes of the search schedule per launch, 1..4
  *                  (default 2): they share the launch's last, partly
  *                  filled round of workgroups
+ *   "search_zsplit" kz segments per support column, each in its own pass
+ *                  of the search schedule, 1..4 (default 1)
  *   "refine_graph" 1 (default): emc_align_refine replays its iteration
  *                  chain as a captured HIP graph; 0: eager launches
  * box = 0 sets the value for every box size, else for that box size only

@@ -1,6 +1,7 @@
 """Launch-shape parameters on the device: names, ranges, the per-box
-override, and that the search's results do not depend on the patch tile
-beyond FP32 summation order (same best orientation, scores within 1e-5)."""
+override, and that the search's results do not depend on the patch tile or
+the kz segments per column beyond FP32 summation order (same best
+orientation, scores within 1e-5)."""
 import ctypes as C
 
 import numpy as np
@@ -40,11 +41,15 @@ def test_search_independent_of_tile(ctx):
     parts = [make_particle(rng, sc, blobs) for _ in range(4)]
     grid = eulerSearch("C1", 15, 6, 15, 6)
     out = []
-    for tile in (16, 4, 64):
-        al = BatchAligner(n, sc.ref, sc.bandpass, ctx=ctx, tuning={"params": {"c64": {"64": {"search_tile": tile}}}})
+    # patch tiles, and kz segments per column (search_zsplit: a column's kz
+    # range split over several passes)
+    for tile, zs in ((16, 1), (4, 1), (64, 1), (16, 2), (16, 4), (8, 3)):
+        al = BatchAligner(n, sc.ref, sc.bandpass, ctx=ctx,
+                          tuning={"params": {"c64": {"64": {"search_tile": tile, "search_zsplit": zs}}}})
         al.set_particles([p.data for p in parts], [None] * 4, [p.start for p in parts])
         out.append(al.search(grid))
     _lib.call("emc_tune_set", ctx.handle, b"search_tile", 64, 16)
+    _lib.call("emc_tune_set", ctx.handle, b"search_zsplit", 64, 1)
     for score, shift in out[1:]:
         assert np.array_equal(np.argmax(score, 1), np.argmax(out[0][0], 1))
         assert np.max(np.abs(score - out[0][0])) < 1e-5
