@@ -518,13 +518,17 @@ __global__ void __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per
   }
   __syncthreads();
   // y-pruned sums over each plane piece and the Hermitian x expansion; each
-  // thread owns (dy, dz) and its row of D map entries
-  float* mp = maps + (int64_t)lu * (D * D * D);
-  for (int o = t; o < D * D; o += kBatchThreads) {
-    const int dy = o / D, dz = o % D;
-    float vals[D];
+  // thread owns (dy, dz) and its row of D map entries (at most kOut per
+  // thread), kept in registers until every thread is done with T1
+  constexpr int kOut = (D * D + kBatchThreads - 1) / kBatchThreads;
+  float vals[kOut][D];
 #pragma unroll
-    for (int d = 0; d < D; ++d) vals[d] = 0.0f;
+  for (int q = 0; q < kOut; ++q) {
+    const int o = t + q * kBatchThreads;
+#pragma unroll
+    for (int d = 0; d < D; ++d) vals[q][d] = 0.0f;
+    if (o >= D * D) continue;
+    const int dy = o / D, dz = o % D;
     for (int pc = G.z; pc < G.z + G.w; ++pc) {
       const int4 pz = pieces[pc];  // (kx, first slot, end slot, 0)
       const int kxx = pz.x, s0 = pz.y, s1 = pz.z;
@@ -539,13 +543,27 @@ __global__ void __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per
 #pragma unroll
       for (int dx = 0; dx < D; ++dx) {
         const float2 e = tw[kxx][dx];
-        vals[dx] = fmaf(mk * sr, e.x, fmaf(-mk * si, e.y, vals[dx]));
+        vals[q][dx] = fmaf(mk * sr, e.x, fmaf(-mk * si, e.y, vals[q][dx]));
       }
     }
-    float* row = mp + (dz * D + dy) * D;
-#pragma unroll
-    for (int dx = 0; dx < D; ++dx) row[dx] = (g < npass ? 0.0f : row[dx]) + vals[dx];
   }
+  // The unit's D^3 map goes through LDS (T1's storage, D^3 floats <= 256 D
+  // float2) so its read-modify-write in global memory is coalesced: lane i
+  // of a wave touches float i of a run of 64.
+  static_assert(D * D * D <= 2 * kBatchThreads * D, "map staging must fit in T1");
+  __syncthreads();
+  float* stage = &T1[0][0].x;
+#pragma unroll
+  for (int q = 0; q < kOut; ++q) {
+    const int o = t + q * kBatchThreads;
+    if (o >= D * D) continue;
+    const int dy = o / D, dz = o % D;
+#pragma unroll
+    for (int dx = 0; dx < D; ++dx) stage[(dz * D + dy) * D + dx] = vals[q][dx];
+  }
+  __syncthreads();
+  float* mp = maps + (int64_t)lu * (D * D * D);
+  for (int i = t; i < D * D * D; i += kBatchThreads) mp[i] = (g < npass ? 0.0f : mp[i]) + stage[i];
   if (t == 0) {
     double v = 0.0;
     for (int w = 0; w < kBatchThreads / kWave; ++w) v += red[w];
