@@ -262,7 +262,8 @@ constexpr TuneRange kTune[] = {{"search_tile", 2, 128, 16},
                                {"grad_blocks", 256, 65536, 4096},
                                {"search_passes", 1, 4, 2},
                                {"refine_graph", 0, 1, 1},
-                               {"search_zsplit", 1, 4, 1}};
+                               {"search_zsplit", 1, 4, 1},
+                               {"tsearch_pruned", 0, 1, 1}};
 
 const TuneRange* find_tune(const char* name) {
   for (const auto& t : kTune)

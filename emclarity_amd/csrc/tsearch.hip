@@ -3,10 +3,12 @@
 // swapQuadrants, :713-764 rotate/pad/correlate/normalise, :851-887 max/argmax
 // update).
 //
-// Per orientation: rotate the template and write it straight into the
-// zero-padded chunk buffer (no separate pad pass), R2C, multiply by the
-// pre-centred tomogram spectrum, C2R, std over the valid area (FP64 sums),
-// then a fused per-voxel max/argmax update of the score and angle maps.
+// Per orientation: rotate the template into a template-sized block, form
+// the padded template's half spectrum by direct DFTs over that block fused
+// with the product by the pre-centred tomogram spectrum (or, for templates
+// over 64 voxels, rotate straight into the zero-padded chunk and R2C it),
+// C2R, std over the valid area (FP64 sums), then a fused per-voxel
+// max/argmax update of the score and angle maps.
 // Orientations are processed in blocks: the rotate, FFTs and multiply are
 // batched over the block; the updates are applied in orientation order, so
 // the strict '<' tie rule of the reference is preserved.
@@ -55,6 +57,162 @@ __global__ void conj_mul_batch_kernel(const float2* __restrict__ T0, float2* __r
     const float2 a = T0[i], b = Tb[i];
     Tb[i] = make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
   }
+}
+
+// ------------------------------------------------ pruned template spectrum
+// The zero-padded template's R2C has non-zero samples only in the
+// t_x x t_y x t_z block at pad_low, so its half spectrum is the separable
+// sum over that block,
+//   F(k) = sum_z e_z(kz, z) sum_y e_y(ky, y) sum_x e_x(kx, x) t(x, y, z),
+//   e_a(k, j) = exp(-2 pi i k (p_a + j) / N_a),
+// taken as three direct DFTs over the block: along x for the block's rows,
+// along y for the block's planes, and along z fused with the product by the
+// tomogram spectrum (tomoFou .* conj(tempFou), :756-759). No padded volume
+// is written and no 3D R2C runs per orientation.
+constexpr int kTMax = 64;  // template edge limit of the pruned path
+
+// e(k, j) = exp(-2 pi i k (p + j) / n) for k < nk, j < kTMax (zero for j >= jt);
+// the phase is reduced exactly in integers before the FP64 sincos
+__global__ void dft_twiddle_kernel(float2* __restrict__ e, int nk, int jt, int p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nk * kTMax) return;
+  const int k = i / kTMax, j = i % kTMax;
+  if (j >= jt) {
+    e[i] = make_float2(0.0f, 0.0f);
+    return;
+  }
+  const int64_t m = ((int64_t)k * (p + j)) % n;
+  double sn, cs;
+  sincospi(-2.0 * (double)m / (double)n, &sn, &cs);
+  e[i] = make_float2((float)cs, (float)sn);
+}
+
+// X1[b][z][y][kx] = sum_x t[b][z][y][x] e_x(kx, x) for z < tz, y < YP (rows
+// y >= ty are zero), kx < cx
+template <int YP>
+__global__ void tdft_x_kernel(const float* __restrict__ t, int tx, int ty, int tz, int cx,
+                              const float2* __restrict__ ex, float2* __restrict__ X1) {
+  const int b = blockIdx.y;
+  const int64_t per = (int64_t)tz * YP * cx;
+  const float* tb = t + (int64_t)b * tx * ty * tz;
+  float2* xb = X1 + (int64_t)b * per;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < per; i += (int64_t)gridDim.x * blockDim.x) {
+    const int kx = (int)(i % cx);
+    const int r = (int)(i / cx);
+    const int y = r % YP, z = r / YP;
+    float sr = 0.0f, si = 0.0f;
+    if (y < ty) {
+      const float* row = tb + ((int64_t)z * ty + y) * tx;
+      const float2* e = ex + (int64_t)kx * kTMax;
+      for (int x = 0; x < tx; ++x) {
+        const float v = row[x];
+        const float2 w = e[x];
+        sr = fmaf(v, w.x, sr);
+        si = fmaf(v, w.y, si);
+      }
+    }
+    xb[i] = make_float2(sr, si);
+  }
+}
+
+// X2[b][z][ky][kx] = sum_y X1[b][z][y][kx] e_y(ky, y) for z < ZP (planes
+// z >= tz are zero), ky in this block's chunk, kx < cx; one thread per (z, kx)
+template <int YP>
+__global__ void __launch_bounds__(256) tdft_y_kernel(const float2* __restrict__ X1, int tz, int zp, int ny, int cx,
+                                                     const float2* __restrict__ ey, int ky_chunk,
+                                                     float2* __restrict__ X2) {
+  const int b = blockIdx.y;
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= zp * cx) return;
+  const int z = col / cx, kx = col % cx;
+  const int ky0 = blockIdx.z * ky_chunk, ky1 = min(ny, ky0 + ky_chunk);
+  float2* ob = X2 + (int64_t)b * zp * ny * cx + (int64_t)z * ny * cx + kx;
+  if (z >= tz) {
+    for (int ky = ky0; ky < ky1; ++ky) ob[(int64_t)ky * cx] = make_float2(0.0f, 0.0f);
+    return;
+  }
+  const float2* ib = X1 + (int64_t)b * tz * YP * cx + (int64_t)z * YP * cx + kx;
+  float2 a[YP];
+#pragma unroll
+  for (int y = 0; y < YP; ++y) a[y] = ib[(int64_t)y * cx];
+  for (int ky = ky0; ky < ky1; ++ky) {
+    const float2* e = ey + (int64_t)ky * kTMax;
+    float sr = 0.0f, si = 0.0f;
+#pragma unroll
+    for (int y = 0; y < YP; ++y) {
+      const float2 w = e[y];
+      sr = fmaf(a[y].x, w.x, fmaf(-a[y].y, w.y, sr));
+      si = fmaf(a[y].x, w.y, fmaf(a[y].y, w.x, si));
+    }
+    ob[(int64_t)ky * cx] = make_float2(sr, si);
+  }
+}
+
+// out[b][kz][ky][kx] = tomo[kz][ky][kx] * conj(sum_z X2[b][z][ky][kx] e_z(kz, z))
+// for kz in this block's chunk; one thread per (ky, kx) column
+template <int ZP>
+__global__ void __launch_bounds__(256) tdft_z_mul_kernel(const float2* __restrict__ X2,
+                                                         const float2* __restrict__ tomo, int nz, int ny, int cx,
+                                                         const float2* __restrict__ ez, int kz_chunk,
+                                                         float2* __restrict__ out) {
+  const int b = blockIdx.y;
+  const int64_t ncol = (int64_t)ny * cx;
+  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= ncol) return;
+  const float2* xb = X2 + (int64_t)b * ZP * ncol + col;
+  float2 a[ZP];
+#pragma unroll
+  for (int z = 0; z < ZP; ++z) a[z] = xb[(int64_t)z * ncol];
+  const int kz0 = blockIdx.z * kz_chunk, kz1 = min(nz, kz0 + kz_chunk);
+  float2* ob = out + (int64_t)b * nz * ncol + col;
+  for (int kz = kz0; kz < kz1; ++kz) {
+    const float2* e = ez + (int64_t)kz * kTMax;
+    float fr = 0.0f, fi = 0.0f;
+#pragma unroll
+    for (int z = 0; z < ZP; ++z) {
+      const float2 w = e[z];
+      fr = fmaf(a[z].x, w.x, fmaf(-a[z].y, w.y, fr));
+      fi = fmaf(a[z].x, w.y, fmaf(a[z].y, w.x, fi));
+    }
+    const float2 tv = tomo[(int64_t)kz * ncol + col];
+    ob[(int64_t)kz * ncol] = make_float2(tv.x * fr + tv.y * fi, tv.y * fr - tv.x * fi);
+  }
+}
+
+int pruned_pad(int t) { return t <= 16 ? 16 : t <= 32 ? 32 : t <= 48 ? 48 : 64; }
+
+// the separable pruned template spectrum times the tomogram spectrum for a
+// block of nb rotated templates tb[nb][tz][ty][tx] (pad_low = p)
+void pruned_spectrum_mul(emc_ctx* ctx, const float* tb, const int td[3], const int cd[3], const float2* tomo,
+                         const float2* ex, const float2* ey, const float2* ez, int nb, float2* X1, float2* X2,
+                         float2* out) {
+  const int cx = cd[0] / 2 + 1, ny = cd[1], nz = cd[2];
+  const int yp = pruned_pad(td[1]), zp = pruned_pad(td[2]);
+  const int64_t per1 = (int64_t)td[2] * yp * cx;
+  const int g1 = (int)std::min<int64_t>((per1 + 255) / 256, 4096);
+  switch (yp) {
+    case 16: hipLaunchKernelGGL(tdft_x_kernel<16>, dim3(g1, nb), dim3(256), 0, ctx->stream, tb, td[0], td[1], td[2], cx, ex, X1); break;
+    case 32: hipLaunchKernelGGL(tdft_x_kernel<32>, dim3(g1, nb), dim3(256), 0, ctx->stream, tb, td[0], td[1], td[2], cx, ex, X1); break;
+    case 48: hipLaunchKernelGGL(tdft_x_kernel<48>, dim3(g1, nb), dim3(256), 0, ctx->stream, tb, td[0], td[1], td[2], cx, ex, X1); break;
+    default: hipLaunchKernelGGL(tdft_x_kernel<64>, dim3(g1, nb), dim3(256), 0, ctx->stream, tb, td[0], td[1], td[2], cx, ex, X1); break;
+  }
+  const int ky_chunk = 64;
+  const dim3 g2(ceil_div((int64_t)zp * cx, 256), nb, ceil_div(ny, ky_chunk));
+  switch (yp) {
+    case 16: hipLaunchKernelGGL(tdft_y_kernel<16>, g2, dim3(256), 0, ctx->stream, (const float2*)X1, td[2], zp, ny, cx, ey, ky_chunk, X2); break;
+    case 32: hipLaunchKernelGGL(tdft_y_kernel<32>, g2, dim3(256), 0, ctx->stream, (const float2*)X1, td[2], zp, ny, cx, ey, ky_chunk, X2); break;
+    case 48: hipLaunchKernelGGL(tdft_y_kernel<48>, g2, dim3(256), 0, ctx->stream, (const float2*)X1, td[2], zp, ny, cx, ey, ky_chunk, X2); break;
+    default: hipLaunchKernelGGL(tdft_y_kernel<64>, g2, dim3(256), 0, ctx->stream, (const float2*)X1, td[2], zp, ny, cx, ey, ky_chunk, X2); break;
+  }
+  const int kz_chunk = 128;
+  const dim3 g3(ceil_div((int64_t)ny * cx, 256), nb, ceil_div(nz, kz_chunk));
+  switch (zp) {
+    case 16: hipLaunchKernelGGL(tdft_z_mul_kernel<16>, g3, dim3(256), 0, ctx->stream, (const float2*)X2, tomo, nz, ny, cx, ez, kz_chunk, out); break;
+    case 32: hipLaunchKernelGGL(tdft_z_mul_kernel<32>, g3, dim3(256), 0, ctx->stream, (const float2*)X2, tomo, nz, ny, cx, ez, kz_chunk, out); break;
+    case 48: hipLaunchKernelGGL(tdft_z_mul_kernel<48>, g3, dim3(256), 0, ctx->stream, (const float2*)X2, tomo, nz, ny, cx, ez, kz_chunk, out); break;
+    default: hipLaunchKernelGGL(tdft_z_mul_kernel<64>, g3, dim3(256), 0, ctx->stream, (const float2*)X2, tomo, nz, ny, cx, ez, kz_chunk, out); break;
+  }
+  EMC_HIP(hipGetLastError());
 }
 
 // per-map sums over the valid area (partials[b][block][2]): blocks stride
@@ -186,6 +344,31 @@ emc_status emc_template_search_chunk(emc_ctx* ctx, const float* d_chunk, const i
     hipLaunchKernelGGL(swapq_kernel, dim3(2048), dim3(256), 0, ctx->stream, tomo, cd[0], cd[1], cd[2]);
     EMC_HIP(hipGetLastError());
 
+    // pruned template spectrum (every template edge <= kTMax; "tsearch_pruned"
+    // 0 selects the padded-volume R2C instead)
+    const bool pruned = tune_value(ctx, "tsearch_pruned", 0, 1) != 0 && td[0] <= kTMax && td[1] <= kTMax &&
+                        td[2] <= kTMax;
+    const int cx = cd[0] / 2 + 1;
+    float* tblk = nullptr;
+    float2 *X1 = nullptr, *X2 = nullptr, *ex = nullptr, *ey = nullptr, *ez = nullptr;
+    if (pruned) {
+      const int yp = pruned_pad(td[1]), zp = pruned_pad(td[2]);
+      tblk = (float*)scratch(ctx, "ts_tblk", sizeof(float) * (size_t)td[0] * td[1] * td[2] * blk);
+      X1 = (float2*)scratch(ctx, "ts_X1", sizeof(float2) * (size_t)td[2] * yp * cx * blk);
+      X2 = (float2*)scratch(ctx, "ts_X2", sizeof(float2) * (size_t)zp * cd[1] * cx * blk);
+      float2* tw = (float2*)scratch(ctx, "ts_tw", sizeof(float2) * (size_t)kTMax * (cx + cd[1] + cd[2]));
+      ex = tw;
+      ey = ex + (size_t)cx * kTMax;
+      ez = ey + (size_t)cd[1] * kTMax;
+      hipLaunchKernelGGL(dft_twiddle_kernel, dim3(ceil_div((int64_t)cx * kTMax, 256)), dim3(256), 0, ctx->stream,
+                         ex, cx, td[0], pad_low[0], cd[0]);
+      hipLaunchKernelGGL(dft_twiddle_kernel, dim3(ceil_div((int64_t)cd[1] * kTMax, 256)), dim3(256), 0,
+                         ctx->stream, ey, cd[1], td[1], pad_low[1], cd[1]);
+      hipLaunchKernelGGL(dft_twiddle_kernel, dim3(ceil_div((int64_t)cd[2] * kTMax, 256)), dim3(256), 0,
+                         ctx->stream, ez, cd[2], td[2], pad_low[2], cd[2]);
+      EMC_HIP(hipGetLastError());
+    }
+
     float* h_par = (float*)pinned(ctx, sizeof(float) * 12 * blk);
     const float one3[3] = {1.0f, 1.0f, 1.0f};
     for (int o0 = 0; o0 < n_orient; o0 += blk) {
@@ -202,21 +385,29 @@ emc_status emc_template_search_chunk(emc_ctx* ctx, const float* d_chunk, const i
       }
       EMC_HIP(hipMemcpyAsync(mats, h_par, sizeof(float) * 12 * blk, hipMemcpyHostToDevice,
                              ctx->stream));
-      // rotate into the zero-padded chunk (:744-754)
-      XformArgs xa = xform_args(td, nullptr, 1, one3, 1);
-      xa.ox = cd[0];
-      xa.oy = cd[1];
-      xa.oz = cd[2];
-      xa.px = pad_low[0];
-      xa.py = pad_low[1];
-      xa.pz = pad_low[2];
-      xa.pad_value = 0.0f;
-      launch_xform_batched(ctx, d_template, tpad, mats, mats + 9 * blk, nb, xa);
       emc_fft_plan* p = nb == blk ? planb : cached_plan(ctx, cd[0], cd[1], cd[2], nb);
-      fft_r2c(p, tpad, T);  // tempFou (:756)
-      hipLaunchKernelGGL(conj_mul_batch_kernel, dim3(1024, nb), dim3(256), 0, ctx->stream,
-                         (const float2*)tomo, T, nh);
-      EMC_HIP(hipGetLastError());
+      if (pruned) {
+        // rotate into a template-sized block (:744-747); its spectrum at the
+        // padded position (:751-756) times the tomogram spectrum
+        XformArgs xa = xform_args(td, nullptr, 1, one3, 1);
+        launch_xform_batched(ctx, d_template, tblk, mats, mats + 9 * blk, nb, xa);
+        pruned_spectrum_mul(ctx, tblk, td, cd, (const float2*)tomo, ex, ey, ez, nb, X1, X2, T);
+      } else {
+        // rotate into the zero-padded chunk (:744-754)
+        XformArgs xa = xform_args(td, nullptr, 1, one3, 1);
+        xa.ox = cd[0];
+        xa.oy = cd[1];
+        xa.oz = cd[2];
+        xa.px = pad_low[0];
+        xa.py = pad_low[1];
+        xa.pz = pad_low[2];
+        xa.pad_value = 0.0f;
+        launch_xform_batched(ctx, d_template, tpad, mats, mats + 9 * blk, nb, xa);
+        fft_r2c(p, tpad, T);  // tempFou (:756)
+        hipLaunchKernelGGL(conj_mul_batch_kernel, dim3(1024, nb), dim3(256), 0, ctx->stream,
+                           (const float2*)tomo, T, nh);
+        EMC_HIP(hipGetLastError());
+      }
       fft_c2r(p, T, tpad);  // ccfmap (:759-761), C2R reuses the rotate buffer
       hipLaunchKernelGGL(valid_sums_kernel, dim3(kValidBlocks, nb), dim3(kRedThreads), 0, ctx->stream,
                          (const float*)tpad, cd[0], cd[1], cd[2], valid_low[0], valid_low[1],

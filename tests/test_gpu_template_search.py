@@ -27,7 +27,12 @@ def _check(peak, ang, rpeak, rang):
     return True
 
 
-def test_template_search_chunk_matches_oracle(ctx):
+@pytest.mark.parametrize("pruned", [1, 0])
+def test_template_search_chunk_matches_oracle(ctx, pruned):
+    """pruned 1: the template's spectrum as direct DFTs over its block at
+    the padded position, fused with the product by the tomogram spectrum;
+    0: the zero-padded volume through rocFFT's R2C."""
+    _lib.call("emc_tune_set", ctx.handle, b"tsearch_pruned", 0, pruned)
     rng = np.random.default_rng(21)
     t = _template(rng)
     grid = eulerSearch("C1", 60, 30, 60, 30)[:14]
@@ -51,8 +56,36 @@ def test_template_search_chunk_matches_oracle(ctx):
         _lib.call("emc_template_search_chunk", ctx.handle, d_chunk.vp, _lib.i3((32, 32, 32)), d_t.vp,
                   _lib.i3((8, 8, 8)), _lib.i3(pad[0]), _lib.i3(vlow), _lib.i3((20, 20, 20)),
                   pts[a0:a1].ctypes.data_as(dp), a1 - a0, a0 + 1, first, peak.vp, ang.vp)
+    _lib.call("emc_tune_set", ctx.handle, b"tsearch_pruned", 0, 1)
     rpeak, rang = O.template_search_chunk(chunk.astype(np.float64), t, grid, pad[0:2], valid[0:2])
     _check(peak.get(), ang.get(), rpeak, rang)
+
+
+def test_template_search_pruned_matches_padded_odd_sizes(ctx):
+    """The pruned spectrum against the padded-volume R2C on an odd-sized,
+    non-cubic chunk and template with an off-centre block (every DFT length
+    and offset different), over several orientation blocks."""
+    rng = np.random.default_rng(5)
+    t = rng.standard_normal((9, 7, 11)).astype(np.float32)        # (z, y, x): 11 x 7 x 9
+    chunk = rng.standard_normal((29, 34, 37)).astype(np.float32)  # 37 x 34 x 29
+    grid = np.ascontiguousarray(eulerSearch("C1", 60, 30, 60, 30)[:21])
+    cd, td, pad, vlow, vd = (37, 34, 29), (11, 7, 9), (5, 20, 3), (4, 5, 6), (25, 22, 19)
+    d_chunk = DeviceArray.from_numpy(chunk)
+    d_t = DeviceArray.from_numpy(t)
+    dp = _lib.C.POINTER(_lib.C.c_double)
+    out = []
+    for pruned in (1, 0):
+        _lib.call("emc_tune_set", ctx.handle, b"tsearch_pruned", 0, pruned)
+        peak = DeviceArray.zeros((vd[2], vd[1], vd[0]), np.float32)
+        ang = DeviceArray.zeros((vd[2], vd[1], vd[0]), np.int32)
+        _lib.call("emc_template_search_chunk", ctx.handle, d_chunk.vp, _lib.i3(cd), d_t.vp, _lib.i3(td),
+                  _lib.i3(pad), _lib.i3(vlow), _lib.i3(vd), grid.ctypes.data_as(dp), len(grid), 1, 1,
+                  peak.vp, ang.vp)
+        out.append((peak.get(), ang.get()))
+    _lib.call("emc_tune_set", ctx.handle, b"tsearch_pruned", 0, 1)
+    (p1, a1), (p0, a0) = out
+    assert np.max(np.abs(p1 - p0)) < 1e-4 * np.max(np.abs(p0))
+    assert np.mean(a1 == a0) > 0.999
 
 
 def test_template_search_driver_chunks(ctx):
