@@ -227,13 +227,29 @@ __global__ void __launch_bounds__(kRedThreads) valid_sums_kernel(const float* __
   const float* m = ccf + (int64_t)blockIdx.y * nx * ny * nz;
   double v[2] = {0.0, 0.0};
   const int rows = vy * vz;
-  for (int r = blockIdx.x; r < rows; r += gridDim.x) {
-    const int z = r / vy, y = r - z * vy;
-    const float* row = m + ((int64_t)(z + vz0) * ny + (y + vy0)) * nx + vx0;
+  // four rows per step, their loads issued before any is summed
+  constexpr int kRows = 4;
+  for (int r0 = blockIdx.x; r0 < rows; r0 += kRows * gridDim.x) {
+    const float* row[kRows];
+    bool ok[kRows];
+#pragma unroll
+    for (int q = 0; q < kRows; ++q) {
+      const int r = r0 + q * gridDim.x;
+      ok[q] = r < rows;
+      const int rr = ok[q] ? r : r0;
+      const int z = rr / vy, y = rr - z * vy;
+      row[q] = m + ((int64_t)(z + vz0) * ny + (y + vy0)) * nx + vx0;
+    }
     for (int x = threadIdx.x; x < vx; x += blockDim.x) {
-      const double a = row[x];
-      v[0] += a;
-      v[1] += a * a;
+      float a[kRows];
+#pragma unroll
+      for (int q = 0; q < kRows; ++q) a[q] = row[q][x];
+#pragma unroll
+      for (int q = 0; q < kRows; ++q)
+        if (ok[q]) {
+          v[0] += (double)a[q];
+          v[1] += (double)a[q] * (double)a[q];
+        }
     }
   }
   block_sum<2>(v);
@@ -323,8 +339,12 @@ emc_status emc_template_search_chunk(emc_ctx* ctx, const float* d_chunk, const i
     const int64_t nh = (int64_t)(cd[0] / 2 + 1) * cd[1] * cd[2];
     const int64_t nvalid = (int64_t)vd[0] * vd[1] * vd[2];
     EMC_REQUIRE(nvalid > 1, "valid area must hold more than one voxel");
-    // orientation block: bounded by ~1.5 GiB of batch buffers
-    int blk = (int)std::max<int64_t>(1, std::min<int64_t>(16, ((int64_t)3 << 29) / (nc * 4 + nh * 8)));
+    // orientation block: up to 16 orientations, bounded by 16 GiB of batch
+    // buffers (a 480^3 chunk takes ~0.93 GB per orientation: the C2R map, the
+    // product spectrum and the template's partial DFT); one pass of the
+    // max/argmax update then serves the whole block
+    const int64_t per_orient = nc * 4 + nh * 8 + (int64_t)64 * cd[1] * (cd[0] / 2 + 1) * 8;
+    int blk = (int)std::max<int64_t>(1, std::min<int64_t>(16, ((int64_t)16 << 30) / per_orient));
     blk = std::min(blk, n_orient);
     emc_fft_plan* plan1 = cached_plan(ctx, cd[0], cd[1], cd[2], 1);
     emc_fft_plan* planb = cached_plan(ctx, cd[0], cd[1], cd[2], blk);
