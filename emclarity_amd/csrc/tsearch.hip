@@ -87,35 +87,42 @@ __global__ void dft_twiddle_kernel(float2* __restrict__ e, int nk, int jt, int p
   e[i] = make_float2((float)cs, (float)sn);
 }
 
-// X1[b][z][y][kx] = sum_x t[b][z][y][x] e_x(kx, x) for z < tz, y < YP (rows
-// y >= ty are zero), kx < cx
-template <int YP>
-__global__ void tdft_x_kernel(const float* __restrict__ t, int tx, int ty, int tz, int cx,
-                              const float2* __restrict__ ex, float2* __restrict__ X1) {
+// X1[b][z][kx][y] = sum_x t[b][z][y][x] e_x(kx, x) for z < tz, kx in this
+// block's chunk, y < YP (rows y >= ty are zero). One thread per (z, y) row,
+// the row in registers; the twiddle e_x(kx, .) is the same for the whole
+// wave and consecutive lanes write consecutive y.
+template <int XP, int YP>
+__global__ void __launch_bounds__(256) tdft_x_kernel(const float* __restrict__ t, int tx, int ty, int tz, int cx,
+                                                     const float2* __restrict__ ex, int kx_chunk,
+                                                     float2* __restrict__ X1) {
   const int b = blockIdx.y;
-  const int64_t per = (int64_t)tz * YP * cx;
-  const float* tb = t + (int64_t)b * tx * ty * tz;
-  float2* xb = X1 + (int64_t)b * per;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < per; i += (int64_t)gridDim.x * blockDim.x) {
-    const int kx = (int)(i % cx);
-    const int r = (int)(i / cx);
-    const int y = r % YP, z = r / YP;
+  const int rowi = blockIdx.x * blockDim.x + threadIdx.x;  // z * YP + y
+  if (rowi >= tz * YP) return;
+  const int z = rowi / YP, y = rowi % YP;
+  float v[XP];
+  const int yy = y < ty ? y : 0;  // padding rows read row 0 and keep zeros
+  const float* row = t + (int64_t)b * tx * ty * tz + ((int64_t)z * ty + yy) * tx;
+#pragma unroll
+  for (int x = 0; x < XP; ++x) {
+    const float r = x < tx ? row[x < tx ? x : 0] : 0.0f;
+    v[x] = y < ty ? r : 0.0f;
+  }
+  const int kx0 = blockIdx.z * kx_chunk, kx1 = min(cx, kx0 + kx_chunk);
+  float2* ob = X1 + (int64_t)b * tz * cx * YP + (int64_t)z * cx * YP + y;
+  for (int kx = kx0; kx < kx1; ++kx) {
+    const float2* e = ex + (int64_t)kx * kTMax;
     float sr = 0.0f, si = 0.0f;
-    if (y < ty) {
-      const float* row = tb + ((int64_t)z * ty + y) * tx;
-      const float2* e = ex + (int64_t)kx * kTMax;
-      for (int x = 0; x < tx; ++x) {
-        const float v = row[x];
-        const float2 w = e[x];
-        sr = fmaf(v, w.x, sr);
-        si = fmaf(v, w.y, si);
-      }
+#pragma unroll
+    for (int x = 0; x < XP; ++x) {
+      const float2 w = e[x];
+      sr = fmaf(v[x], w.x, sr);
+      si = fmaf(v[x], w.y, si);
     }
-    xb[i] = make_float2(sr, si);
+    ob[(int64_t)kx * YP] = make_float2(sr, si);
   }
 }
 
-// X2[b][z][ky][kx] = sum_y X1[b][z][y][kx] e_y(ky, y) for z < ZP (planes
+// X2[b][z][ky][kx] = sum_y X1[b][z][kx][y] e_y(ky, y) for z < ZP (planes
 // z >= tz are zero), ky in this block's chunk, kx < cx; one thread per (z, kx)
 template <int YP>
 __global__ void __launch_bounds__(256) tdft_y_kernel(const float2* __restrict__ X1, int tz, int zp, int ny, int cx,
@@ -131,10 +138,10 @@ __global__ void __launch_bounds__(256) tdft_y_kernel(const float2* __restrict__ 
     for (int ky = ky0; ky < ky1; ++ky) ob[(int64_t)ky * cx] = make_float2(0.0f, 0.0f);
     return;
   }
-  const float2* ib = X1 + (int64_t)b * tz * YP * cx + (int64_t)z * YP * cx + kx;
+  const float2* ib = X1 + (int64_t)b * tz * cx * YP + ((int64_t)z * cx + kx) * YP;
   float2 a[YP];
 #pragma unroll
-  for (int y = 0; y < YP; ++y) a[y] = ib[(int64_t)y * cx];
+  for (int y = 0; y < YP; ++y) a[y] = ib[y];
   for (int ky = ky0; ky < ky1; ++ky) {
     const float2* e = ey + (int64_t)ky * kTMax;
     float sr = 0.0f, si = 0.0f;
@@ -187,15 +194,27 @@ void pruned_spectrum_mul(emc_ctx* ctx, const float* tb, const int td[3], const i
                          const float2* ex, const float2* ey, const float2* ez, int nb, float2* X1, float2* X2,
                          float2* out) {
   const int cx = cd[0] / 2 + 1, ny = cd[1], nz = cd[2];
-  const int yp = pruned_pad(td[1]), zp = pruned_pad(td[2]);
-  const int64_t per1 = (int64_t)td[2] * yp * cx;
-  const int g1 = (int)std::min<int64_t>((per1 + 255) / 256, 4096);
-  switch (yp) {
-    case 16: hipLaunchKernelGGL(tdft_x_kernel<16>, dim3(g1, nb), dim3(256), 0, ctx->stream, tb, td[0], td[1], td[2], cx, ex, X1); break;
-    case 32: hipLaunchKernelGGL(tdft_x_kernel<32>, dim3(g1, nb), dim3(256), 0, ctx->stream, tb, td[0], td[1], td[2], cx, ex, X1); break;
-    case 48: hipLaunchKernelGGL(tdft_x_kernel<48>, dim3(g1, nb), dim3(256), 0, ctx->stream, tb, td[0], td[1], td[2], cx, ex, X1); break;
-    default: hipLaunchKernelGGL(tdft_x_kernel<64>, dim3(g1, nb), dim3(256), 0, ctx->stream, tb, td[0], td[1], td[2], cx, ex, X1); break;
+  const int xp = pruned_pad(td[0]), yp = pruned_pad(td[1]), zp = pruned_pad(td[2]);
+  const int kx_chunk = 32;
+  const dim3 g1(ceil_div((int64_t)td[2] * yp, 256), nb, ceil_div(cx, kx_chunk));
+#define EMC_TDFT_X(XP, YP)                                                                                    \
+  hipLaunchKernelGGL((tdft_x_kernel<XP, YP>), g1, dim3(256), 0, ctx->stream, tb, td[0], td[1], td[2], cx, ex, \
+                     kx_chunk, X1)
+#define EMC_TDFT_X_Y(XP)          \
+  switch (yp) {                   \
+    case 16: EMC_TDFT_X(XP, 16); break; \
+    case 32: EMC_TDFT_X(XP, 32); break; \
+    case 48: EMC_TDFT_X(XP, 48); break; \
+    default: EMC_TDFT_X(XP, 64); break; \
   }
+  switch (xp) {
+    case 16: EMC_TDFT_X_Y(16); break;
+    case 32: EMC_TDFT_X_Y(32); break;
+    case 48: EMC_TDFT_X_Y(48); break;
+    default: EMC_TDFT_X_Y(64); break;
+  }
+#undef EMC_TDFT_X_Y
+#undef EMC_TDFT_X
   const int ky_chunk = 64;
   const dim3 g2(ceil_div((int64_t)zp * cx, 256), nb, ceil_div(ny, ky_chunk));
   switch (yp) {
@@ -374,7 +393,7 @@ emc_status emc_template_search_chunk(emc_ctx* ctx, const float* d_chunk, const i
     if (pruned) {
       const int yp = pruned_pad(td[1]), zp = pruned_pad(td[2]);
       tblk = (float*)scratch(ctx, "ts_tblk", sizeof(float) * (size_t)td[0] * td[1] * td[2] * blk);
-      X1 = (float2*)scratch(ctx, "ts_X1", sizeof(float2) * (size_t)td[2] * yp * cx * blk);
+      X1 = (float2*)scratch(ctx, "ts_X1", sizeof(float2) * (size_t)td[2] * cx * yp * blk);
       X2 = (float2*)scratch(ctx, "ts_X2", sizeof(float2) * (size_t)zp * cd[1] * cx * blk);
       float2* tw = (float2*)scratch(ctx, "ts_tw", sizeof(float2) * (size_t)kTMax * (cx + cd[1] + cd[2]));
       ex = tw;
