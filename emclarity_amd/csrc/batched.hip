@@ -91,10 +91,17 @@ __global__ void ifftshift_many_kernel(PrepPtrs pp, float* __restrict__ out, int 
 // Q, N in the fused kernel's layout [kz][kx][ky] (ky fastest, coalesced for
 // threads running along ky); per-block partials of A = sum_full |P b|^2.
 // blockIdx.y = particle of the chunk; spectra, Q, N and partials are
-// stacked per particle.
+// stacked per particle. The spectrum ([kz][ky][kx]) is read and Q, N are
+// written through a 16 x 16 (kx, ky) tile in LDS, so both sides are
+// coalesced.
+constexpr int kPrepTile = 16;
+
 __global__ void __launch_bounds__(kRedThreads) prep_particle_kernel(
     const float2* __restrict__ P, const float* __restrict__ bp, PrepPtrs pp, int n,
     float2* __restrict__ Q, float* __restrict__ N, double* __restrict__ part) {
+  static_assert(kPrepTile * kPrepTile == kRedThreads, "one tile per workgroup");
+  __shared__ float2 sq[kPrepTile][kPrepTile + 1];
+  __shared__ float sn[kPrepTile][kPrepTile + 1];
   const int cx = n / 2 + 1;
   const int64_t nh = (int64_t)cx * n * n;
   const int b = blockIdx.y;
@@ -103,24 +110,38 @@ __global__ void __launch_bounds__(kRedThreads) prep_particle_kernel(
   Q += b * nh;
   N += b * nh;
   part += (int64_t)b * gridDim.x;
+  const int tx = threadIdx.x % kPrepTile, ty = threadIdx.x / kPrepTile;
+  const int nxt = (cx + kPrepTile - 1) / kPrepTile, nyt = (n + kPrepTile - 1) / kPrepTile;
+  const int ntiles = n * nyt * nxt;
   double v[1] = {0.0};
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nh;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    int x, y, z;
-    split3(i, cx, n, x, y, z);
-    const float b = bp[((int64_t)z * n + y) * n + x];
-    const int64_t cp = ((int64_t)shifted_index(z, n) * n + shifted_index(y, n)) * n + shifted_index(x, n);
-    const int mx = mirror_index(x, n), my = mirror_index(y, n), mz = mirror_index(z, n);
-    const int64_t cm = ((int64_t)shifted_index(mz, n) * n + shifted_index(my, n)) * n + shifted_index(mx, n);
-    const float S = wedge_c ? 0.5f * (wedge_c[cp] + wedge_c[cm]) : 1.0f;
-    float2 p = P[i];
-    if (i == 0) p = make_float2(0.0f, 0.0f);  // mean removed
-    const float w = b * b * S;
-    const int64_t o = ((int64_t)z * cx + x) * n + y;
-    Q[o] = make_float2(p.x * w, p.y * w);
-    N[o] = (i == 0) ? 0.0f : S * S * b * b;
-    const double m = (x == 0 || (n % 2 == 0 && x == n / 2)) ? 1.0 : 2.0;
-    v[0] += m * ((double)p.x * p.x + (double)p.y * p.y) * (double)b * b;
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {  // uniform per workgroup
+    const int xt = tile % nxt, r = tile / nxt;
+    const int yt = r % nyt, z = r / nyt;
+    const int x = xt * kPrepTile + tx, y = yt * kPrepTile + ty;
+    if (x < cx && y < n) {
+      const int64_t i = ((int64_t)z * n + y) * cx + x;
+      const float bb = bp[((int64_t)z * n + y) * n + x];
+      const int64_t cp = ((int64_t)shifted_index(z, n) * n + shifted_index(y, n)) * n + shifted_index(x, n);
+      const int mx = mirror_index(x, n), my = mirror_index(y, n), mz = mirror_index(z, n);
+      const int64_t cm = ((int64_t)shifted_index(mz, n) * n + shifted_index(my, n)) * n + shifted_index(mx, n);
+      const float S = wedge_c ? 0.5f * (wedge_c[cp] + wedge_c[cm]) : 1.0f;
+      float2 p = P[i];
+      if (i == 0) p = make_float2(0.0f, 0.0f);  // mean removed
+      const float w = bb * bb * S;
+      sq[ty][tx] = make_float2(p.x * w, p.y * w);
+      sn[ty][tx] = (i == 0) ? 0.0f : S * S * bb * bb;
+      const double m = (x == 0 || (n % 2 == 0 && x == n / 2)) ? 1.0 : 2.0;
+      v[0] += m * ((double)p.x * p.x + (double)p.y * p.y) * (double)bb * bb;
+    }
+    __syncthreads();
+    // transposed write: this thread stores (kx = xt*16 + ty, ky = yt*16 + tx)
+    const int xo = xt * kPrepTile + ty, yo = yt * kPrepTile + tx;
+    if (xo < cx && yo < n) {
+      const int64_t o = ((int64_t)z * cx + xo) * n + yo;
+      Q[o] = sq[tx][ty];
+      N[o] = sn[tx][ty];
+    }
+    __syncthreads();
   }
   block_sum<1>(v);
   if (threadIdx.x == 0) part[blockIdx.x] = v[0];
