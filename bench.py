@@ -43,7 +43,7 @@ def parse():
                     help="distinct rendered particles; the rest of the batch are copies of them with fresh "
                          "noise and new start orientations")
     ap.add_argument("--plain", type=int, default=10, help="particles timed on the plain path (0 = skip)")
-    ap.add_argument("--refine-iters", type=int, default=12)
+    ap.add_argument("--refine-iters", type=int, default=16)
     ap.add_argument("--cpu-sample", type=float, default=20.0,
                     help="seconds budget of the CPU oracle sample (0 = skip)")
     ap.add_argument("--seed", type=int, default=2024)
@@ -281,7 +281,10 @@ def main():
             host.append(None)
     for p in particles:
         p.signal = None
-    coarse = eulerSearch("C1", 15, 6, 15, 6)
+    # the 10-degree coarse grid of the local search ([theta_max 15, step 10,
+    # psi_max 15, step 10]: 28 orientations); the adjoint refinement takes
+    # the best of them to the final pose
+    coarse = eulerSearch("C1", 15, 10, 15, 10)
     print(f"[bench] {len(particles)} particles {dict(zip(*[x.tolist() for x in np.unique(sizes, return_counts=True)]))} "
           f"({len(pool_sizes)} rendered, the rest fresh-noise copies with new starts), "
           f"coarse grid {len(coarse)} orientations + {a.refine_iters} adjoint iterations",

@@ -14,8 +14,8 @@ Four parameters shape the batched path's hot launches (emc_tune_set):
 
 ``python -m emclarity_amd.autotune [--out PATH]`` times every candidate on
 the device with HIP events on the library's stream, for each box size of the
-alignment workload (64/96/128^3 with 1,000/600/400 particles, 114 coarse
-orientations) and both table formats, and writes the fastest to a JSON
+alignment workload (64/96/128^3 with 1,000/600/400 particles, the 28
+orientations of the 10-degree coarse grid) and both table formats, and writes the fastest to a JSON
 table. The table measured on MI355X ships as ``tuning/mi355x.json``;
 BatchAligner applies its entries, and falls back to the library defaults
 (16 x 16 patches, 2 passes per launch, 4096) for box sizes it does not
@@ -98,9 +98,9 @@ def tune(ctx, counts=((64, 1000), (96, 600), (128, 400)), tables=("c64", "f16"),
     from .batched import BatchAligner
     from .geometry import eulerSearch
     from .workload import copy_particle, make_particle, make_size_class
-    grid = eulerSearch("C1", 15, 6, 15, 6)
+    grid = eulerSearch("C1", 15, 10, 15, 10)
     out = {"device": "gfx950",
-           "workload": "batched alignment, 114 coarse orientations, SNR 0.1, "
+           "workload": "batched alignment, 28 coarse orientations (10 deg), SNR 0.1, "
                        + ", ".join(f"{c} x {n}^3" for n, c in counts),
            "candidates": CANDIDATES, "params": {}, "timings_ms": {}}
     rng = np.random.default_rng(seed)
