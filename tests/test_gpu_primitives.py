@@ -235,6 +235,38 @@ def test_xcf_rotational_shells_matches_oracle(ctx, rng):
         BH_multi_xcf_Rotational(P, R, 1, 1, [1, 1.0, np.array([n ** 3 + 1])])
 
 
+def test_new_entry_points_reject_bad_arguments(ctx):
+    """Error paths: fwdFFT option values, 0-based shell indices, a padCalc
+    that does not fit the calc box; each raises EmcError with the library
+    state intact (a good call follows)."""
+    import ctypes as C
+    from emclarity_amd.xcf import ShellPlan
+    F = DeviceArray((8, 8, 5), np.complex64)
+    for bad in ((3, 0), (-1, 0)):
+        with pytest.raises(_lib.EmcError):
+            _lib.call("emc_ft_condition", ctx.handle, F.vp, _lib.i3((8, 8, 8)), bad[0], bad[1], None, None)
+    with pytest.raises(ValueError):           # the host mirror checks first
+        ShellPlan([1, 1.0, np.array([0, 1, 2])])
+    off = np.array([0, 3], np.int64)
+    idx = np.array([0, 1, 2], np.uint32)
+    h = C.c_void_p()
+    with pytest.raises(_lib.EmcError):        # and the C ABI on its own
+        _lib.call("emc_shell_plan_create", ctx.handle, 1, 1.0, 1, off.ctypes.data_as(C.POINTER(C.c_int64)),
+                  idx.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(h))
+    v = DeviceArray.from_numpy(np.ones((8, 8, 8), np.float32))
+    big = DeviceArray.from_numpy(np.ones((10, 10, 10), np.float32))
+    z = (C.c_void_p * 1)(big.ptr)
+    ccc = np.zeros(1)
+    est = np.zeros(3)
+    dp = C.POINTER(C.c_double)
+    with pytest.raises(_lib.EmcError):   # window 8 with 3 planes below leaves -1 above in 10
+        _lib.call("emc_align_grid_padded", ctx.handle, 8, 10, _lib.i3((3, 3, 3)), 1, None, 1, v.vp, v.vp, None,
+                  None, z, None, None, big.vp, z, big.vp, _lib.d_arr(np.eye(3).ravel()), _lib.d_arr([0, 0, 0]),
+                  _lib.i3((1, 1, 1)), np.zeros(3).ctypes.data_as(dp), 1, 3, ccc.ctypes.data_as(dp),
+                  est.ctypes.data_as(dp))
+    _lib.call("emc_ft_condition", ctx.handle, F.vp, _lib.i3((8, 8, 8)), 1, 0, None, None)
+
+
 def test_device_block_cache(ctx):
     """emc_free keeps emc_malloc blocks for reuse (same size class, same
     address), data written after reuse is intact, and a second free of the
