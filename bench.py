@@ -44,6 +44,9 @@ def parse():
                          "noise and new start orientations")
     ap.add_argument("--plain", type=int, default=10, help="particles timed on the plain path (0 = skip)")
     ap.add_argument("--refine-iters", type=int, default=16)
+    ap.add_argument("--coarse-frac", type=float, default=0.6,
+                    help="support fraction of the first refinement stage (0 = one full-support stage)")
+    ap.add_argument("--coarse-iters", type=int, default=8, help="iterations of the first refinement stage")
     ap.add_argument("--cpu-sample", type=float, default=20.0,
                     help="seconds budget of the CPU oracle sample (0 = skip)")
     ap.add_argument("--seed", type=int, default=2024)
@@ -103,7 +106,7 @@ class BatchedRun:
     """Batched Fourier-space search over a coarse local grid + adjoint
     refinement, one BatchAligner per box size (particles, wedges resident)."""
 
-    def __init__(self, particles, classes, wedges, grid, refine_iters, table="c64"):
+    def __init__(self, particles, classes, wedges, grid, refine_iters, table="c64", coarse=None):
         self.table = table
         # largest box first: its refinement, the longest, then overlaps the
         # smaller sizes' searches, and the smallest size's short refinement
@@ -114,7 +117,7 @@ class BatchedRun:
         for n in self.groups:
             self.groups[n].sort()
         self.particles, self.classes, self.wedges = particles, classes, wedges
-        self.grid, self.iters = grid, refine_iters
+        self.grid, self.iters, self.coarse = grid, refine_iters, coarse
         # per size class: reference and bandpass resident in HBM, and the
         # bandpass's support radius (a property of the filter, made once)
         from emclarity_amd.batched import support_radius_of
@@ -143,7 +146,7 @@ class BatchedRun:
             handles[n] = search_stage(aligners[n], self.grid, after=prev)
             prev = self.ctxs[n]
         for n in self.groups:                   # refinements overlap the later searches and each other
-            refine_stage(handles[n], self.iters)
+            refine_stage(handles[n], self.iters, self.coarse)
         for n, idx in self.groups.items():
             r, s, _, _ = collect(handles[n])
             for k, i in enumerate(idx):
@@ -285,13 +288,16 @@ def main():
     # psi_max 15, step 10]: 28 orientations); the adjoint refinement takes
     # the best of them to the final pose
     coarse = eulerSearch("C1", 15, 10, 15, 10)
+    # refinement in two stages: the first iterations over a smaller support
+    # (fewer frequencies, a smoother score), the rest over the full support
+    c2f = (a.coarse_frac, a.coarse_iters) if a.coarse_frac > 0 and a.coarse_iters > 0 else None
     print(f"[bench] {len(particles)} particles {dict(zip(*[x.tolist() for x in np.unique(sizes, return_counts=True)]))} "
           f"({len(pool_sizes)} rendered, the rest fresh-noise copies with new starts), "
           f"coarse grid {len(coarse)} orientations + {a.refine_iters} adjoint iterations",
           file=sys.stderr, flush=True)
 
     def batched(table):
-        run = BatchedRun(particles, classes, wedges, coarse, a.refine_iters, table)
+        run = BatchedRun(particles, classes, wedges, coarse, a.refine_iters, table, c2f)
         for w in range(a.warmup):
             run.run()
         ctxs = [ctx] + list(run.ctxs.values())
@@ -361,7 +367,9 @@ def main():
                        "particles_per_step": len(particles), "distinct_rendered": len(pool_sizes),
                        "path": "batched+adjoint",
                        "reference_table": a.table,
-                       "coarse_orientations": int(len(coarse)), "refine_iterations": a.refine_iters},
+                       "coarse_orientations": int(len(coarse)), "refine_iterations": a.refine_iters,
+                       "refine_schedule": (f"{c2f[1]} at {c2f[0]:g} x the support radius, then {a.refine_iters - c2f[1]} "
+                                           f"over the full support") if c2f else "full support"},
             "accuracy": acc, "variants": variants, "plain_gpu": plain,
             "speedup_vs_plain": round(value / plain["value"], 2) if plain else None,
             "roofline": roof, "cpu_baseline": cpu}

@@ -198,14 +198,16 @@ class BatchAligner:
         dR = np.stack([R @ g for g in _G], axis=1)
         return self.eval_grad(R, dR, s)
 
-    def refine_device(self, R, s, iters=12, step_angle=0.02, step_shift=0.3):
+    def refine_device(self, R, s, iters=12, step_angle=0.02, step_shift=0.3, support_radius=None):
         """emc_align_refine on device arrays R (P, 9) row-major and s (P, 3),
-        FP64, updated in place; returns the final CC as a device array."""
+        FP64, updated in place; returns the final CC as a device array.
+        support_radius: the frequencies the CC is evaluated over (default:
+        the bandpass support)."""
         self._apply_tuning()
         cc = DeviceArray((self.n_part,), np.float64, self.ctx)
+        r = self.support if support_radius is None else float(support_radius)
         _lib.call("emc_align_refine", self.ctx.handle, self.n, self.n_part, self._fmt, self.F2.vp, self.Q.vp,
-                  self.N.vp, self.A.vp, R.vp, s.vp, int(iters), float(step_angle), float(step_shift), self.support,
-                  cc.vp)
+                  self.N.vp, self.A.vp, R.vp, s.vp, int(iters), float(step_angle), float(step_shift), r, cc.vp)
         return cc
 
     def refine(self, R0, s0, iters=12, step_angle=0.02, step_shift=0.3):
@@ -299,10 +301,20 @@ def search_stage(aligner: BatchAligner, grid_pts, after=None):
             "keep": (score, shift, d_pts)}
 
 
-def refine_stage(h, refine_iters=12):
-    """Queue the adjoint refinement from a search_stage handle's picks."""
-    if refine_iters > 0:
-        h["cc"] = h["aligner"].refine_device(h["R"], h["s"], iters=refine_iters)
+def refine_stage(h, refine_iters=12, coarse=None):
+    """Queue the adjoint refinement from a search_stage handle's picks.
+    coarse = (fraction, iterations): first that many iterations with the CC
+    evaluated inside fraction x the support radius (fewer frequencies, a
+    smoother score), then the rest over the full support, each stage from a
+    fresh quasi-Newton state at the previous stage's pose."""
+    al = h["aligner"]
+    n_full = refine_iters
+    if coarse is not None and coarse[1] > 0 and refine_iters > 0:
+        frac, k = float(coarse[0]), min(int(coarse[1]), refine_iters)
+        h["cc"] = al.refine_device(h["R"], h["s"], iters=k, support_radius=frac * al.support)
+        n_full = refine_iters - k
+    if n_full > 0:
+        h["cc"] = al.refine_device(h["R"], h["s"], iters=n_full)
     return h
 
 

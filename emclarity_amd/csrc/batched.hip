@@ -1371,7 +1371,8 @@ emc_status emc_align_refine(emc_ctx* ctx, int n, int n_part, int table_format, c
     std::memcpy(kb.data(), ki, sizeof(ki));
     std::memcpy(kb.data() + sizeof(ki), kd, sizeof(kd));
     std::memcpy(kb.data() + sizeof(ki) + sizeof(kd), kp, sizeof(kp));
-    auto& gr = ctx->graphs["refine"];
+    // one graph per support list (a coarse-to-fine refinement alternates two)
+    auto& gr = ctx->graphs["refine:" + std::to_string(e.ns)];
     if (!gr.exec || gr.key != kb) {
       hipGraph_t graph = nullptr;
       EMC_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));

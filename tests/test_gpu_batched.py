@@ -111,6 +111,32 @@ def test_batched_refinement_accuracy(ctx, table):
     assert ok >= 3, [(angle_error_deg(R[i], p.truth_R), s[i] - p.shift) for i, p in enumerate(parts)]
 
 
+def test_coarse_to_fine_refinement_accuracy(ctx):
+    """The bench's schedule: the 10-degree coarse grid, then 8 iterations
+    over 0.6 x the support radius and 8 over the full support
+    (refine_stage(coarse=...)) reaches <=1 deg and <=0.5 px; the first stage
+    alone moves every particle off its grid pick."""
+    from emclarity_amd.batched import collect, refine_stage, search_stage
+    from emclarity_amd.workload import make_particle, make_size_class
+    n = 64
+    rng = np.random.default_rng(19)
+    sc, blobs = make_size_class(n)
+    parts = [make_particle(rng, sc, blobs) for _ in range(6)]
+    wedges = [BH_weightMaskMex((n, n, n), 1, p.tlt, [n // 2 + 1] * 3, [[n, n, n], [0, 0, 0]]).get()
+              for p in parts]
+    al = BatchAligner(n, sc.ref, sc.bandpass, peak_radius=6)
+    al.set_particles([p.data for p in parts], wedges, [p.start for p in parts])
+    grid = eulerSearch("C1", 15, 10, 15, 10)
+    h0 = search_stage(al, grid)
+    R0 = h0["R"].get().reshape(-1, 3, 3)
+    R, s, cc, _ = collect(refine_stage(search_stage(al, grid), 16, coarse=(0.6, 8)))
+    ok = sum(int(angle_error_deg(R[i], p.truth_R) <= 1.0 and np.max(np.abs(s[i] - p.shift)) <= 0.5)
+             for i, p in enumerate(parts))
+    assert ok >= 5, [(angle_error_deg(R[i], p.truth_R), s[i] - p.shift) for i, p in enumerate(parts)]
+    Rc, _, _, _ = collect(refine_stage(search_stage(al, grid), 8, coarse=(0.6, 8)))
+    assert all(angle_error_deg(Rc[i], R0[i]) > 1e-3 for i in range(len(parts)))
+
+
 def test_batched_preparation_matches_single(ctx):
     """emc_batch_prepare_particles (chunks 16 + 2 + 1, some without wedge)
     against emc_batch_prepare_particle per particle, and A against the
