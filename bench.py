@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--coarse-frac", type=float, default=0.6,
                     help="support fraction of the first refinement stage (0 = one full-support stage)")
     ap.add_argument("--coarse-iters", type=int, default=8, help="iterations of the first refinement stage")
+    ap.add_argument("--search-frac", type=float, default=0.6,
+                    help="support fraction the coarse grid is scored over")
     ap.add_argument("--cpu-sample", type=float, default=20.0,
                     help="seconds budget of the CPU oracle sample (0 = skip)")
     ap.add_argument("--seed", type=int, default=2024)
@@ -106,7 +108,7 @@ class BatchedRun:
     """Batched Fourier-space search over a coarse local grid + adjoint
     refinement, one BatchAligner per box size (particles, wedges resident)."""
 
-    def __init__(self, particles, classes, wedges, grid, refine_iters, table="c64", coarse=None):
+    def __init__(self, particles, classes, wedges, grid, refine_iters, table="c64", coarse=None, search_frac=1.0):
         self.table = table
         # largest box first: its refinement, the longest, then overlaps the
         # smaller sizes' searches, and the smallest size's short refinement
@@ -117,7 +119,7 @@ class BatchedRun:
         for n in self.groups:
             self.groups[n].sort()
         self.particles, self.classes, self.wedges = particles, classes, wedges
-        self.grid, self.iters, self.coarse = grid, refine_iters, coarse
+        self.grid, self.iters, self.coarse, self.search_frac = grid, refine_iters, coarse, search_frac
         # per size class: reference and bandpass resident in HBM, and the
         # bandpass's support radius (a property of the filter, made once)
         from emclarity_amd.batched import support_radius_of
@@ -140,10 +142,12 @@ class BatchedRun:
             al.set_particles([self.particles[i].data for i in idx], [self.wedges[i] for i in idx],
                              [self.particles[i].start for i in idx])
             aligners[n] = al
-            self.units[n] = (len(idx) * len(self.grid), al.support)
+            # units and the support radius the search scores them over (the
+            # flop and sample counts of the roofline follow that radius)
+            self.units[n] = (len(idx) * len(self.grid), al.support * self.search_frac)
         prev = None
         for n in self.groups:                   # searches one after another (each fills the GPU)
-            handles[n] = search_stage(aligners[n], self.grid, after=prev)
+            handles[n] = search_stage(aligners[n], self.grid, after=prev, support_frac=self.search_frac)
             prev = self.ctxs[n]
         for n in self.groups:                   # refinements overlap the later searches and each other
             refine_stage(handles[n], self.iters, self.coarse)
@@ -297,7 +301,7 @@ def main():
           file=sys.stderr, flush=True)
 
     def batched(table):
-        run = BatchedRun(particles, classes, wedges, coarse, a.refine_iters, table, c2f)
+        run = BatchedRun(particles, classes, wedges, coarse, a.refine_iters, table, c2f, a.search_frac)
         for w in range(a.warmup):
             run.run()
         ctxs = [ctx] + list(run.ctxs.values())
@@ -367,7 +371,8 @@ def main():
                        "particles_per_step": len(particles), "distinct_rendered": len(pool_sizes),
                        "path": "batched+adjoint",
                        "reference_table": a.table,
-                       "coarse_orientations": int(len(coarse)), "refine_iterations": a.refine_iters,
+                       "coarse_orientations": int(len(coarse)), "search_support_fraction": a.search_frac,
+                       "refine_iterations": a.refine_iters,
                        "refine_schedule": (f"{c2f[1]} at {c2f[0]:g} x the support radius, then {a.refine_iters - c2f[1]} "
                                            f"over the full support") if c2f else "full support"},
             "accuracy": acc, "variants": variants, "plain_gpu": plain,

@@ -15,7 +15,8 @@ Four parameters shape the batched path's hot launches (emc_tune_set):
 ``python -m emclarity_amd.autotune [--out PATH]`` times every candidate on
 the device with HIP events on the library's stream, for each box size of the
 alignment workload (64/96/128^3 with 1,000/600/400 particles, the 28
-orientations of the 10-degree coarse grid) and both table formats, and writes the fastest to a JSON
+orientations of the 10-degree coarse grid, scored over 0.6 x the support
+radius as the bench's search is) and both table formats, and writes the fastest to a JSON
 table. The table measured on MI355X ships as ``tuning/mi355x.json``;
 BatchAligner applies its entries, and falls back to the library defaults
 (16 x 16 patches, 2 passes per launch, 4096) for box sizes it does not
@@ -90,7 +91,7 @@ def _timed(ctx, fn, names, reps):
 
 
 def tune(ctx, counts=((64, 1000), (96, 600), (128, 400)), tables=("c64", "f16"), reps=3, seed=7, pool=(50, 30, 20),
-         log=sys.stderr):
+         search_frac=0.6, log=sys.stderr):
     """Time every candidate; returns the table (params + all timings, ms).
     Per box size, ``pool`` particles are rendered and the rest of ``count``
     are copies of them with fresh noise and new starts (as the bench's
@@ -119,7 +120,8 @@ def tune(ctx, counts=((64, 1000), (96, 600), (128, 400)), tables=("c64", "f16"),
                 for v in CANDIDATES[name]:
                     apply(ctx, n, {name: v})
                     if name != "grad_blocks":
-                        ms = _timed(ctx, lambda: al.search(grid), ("align_fused", "align_final"), reps)
+                        ms = _timed(ctx, lambda: al.search(grid, search_frac * al.support),
+                                    ("align_fused", "align_final"), reps)
                     else:
                         ms = _timed(ctx, lambda: al._eval_local(R, s), ("align_eval_grad",), reps)
                     times[name][str(v)] = round(ms, 4)

@@ -143,22 +143,25 @@ class BatchAligner:
             from .autotune import apply
             apply(self.ctx, self.n, self._tune)
 
-    def search_device(self, angles_pts):
+    def search_device(self, angles_pts, support_radius=None):
         """emc_align_batched, results left on the device: score (P, O) and
-        shift (P, O, 3) DeviceArrays (stream-ordered, no wait)."""
+        shift (P, O, 3) DeviceArrays (stream-ordered, no wait).
+        support_radius: the frequencies scored (default: the bandpass
+        support)."""
         self._apply_tuning()
+        r = self.support if support_radius is None else float(support_radius)
         pts = np.ascontiguousarray(np.asarray(angles_pts, dtype=np.float64).reshape(-1, 3))
         O = pts.shape[0]
         score = DeviceArray((self.n_part, O), np.float32, self.ctx)
         shift = DeviceArray((self.n_part, O, 3), np.float32, self.ctx)
         _lib.call("emc_align_batched", self.ctx.handle, self.n, self.n_part, self._fmt, self.F2.vp, self.Q.vp,
                   self.N.vp, self.A.vp, self.d_starts.vp, pts.ctypes.data_as(C.POINTER(C.c_double)), O,
-                  self.peak_radius, self.support, score.vp, shift.vp)
+                  self.peak_radius, r, score.vp, shift.vp)
         return score, shift
 
-    def search(self, angles_pts):
+    def search(self, angles_pts, support_radius=None):
         """Scores (P, O) and shifts (P, O, 3) of every (particle, orientation)."""
-        score, shift = self.search_device(angles_pts)
+        score, shift = self.search_device(angles_pts, support_radius)
         return score.get(), shift.get()
 
     def pick(self, score, shift, d_pts):
@@ -285,17 +288,19 @@ def fused_samples_per_unit(n, support):
     return sum(2 * h + 1 for g in passes for (_, _, h) in g)
 
 
-def search_stage(aligner: BatchAligner, grid_pts, after=None):
+def search_stage(aligner: BatchAligner, grid_pts, after=None, support_frac=1.0):
     """Queue the coarse search over grid_pts and the pick of each particle's
     best orientation (first maximum) on the aligner's stream; no host round
     trip. ``after``: a Context whose queued work the search waits for, so
     searches of several aligners on their own streams run one after another
-    while the rest of their work overlaps."""
+    while the rest of their work overlaps. ``support_frac``: score the grid
+    over that fraction of the support radius (a coarse grid only has to find
+    the refinement's basin; the refinement uses the full support)."""
     pts = np.ascontiguousarray(np.asarray(grid_pts, dtype=np.float64).reshape(-1, 3))
     d_pts = DeviceArray.from_numpy(pts, ctx=aligner.ctx)
     if after is not None:
         aligner.ctx.wait_for(after)
-    score, shift = aligner.search_device(pts)
+    score, shift = aligner.search_device(pts, support_radius=support_frac * aligner.support)
     R, s, best, bs = aligner.pick(score, shift, d_pts)
     return {"aligner": aligner, "P": aligner.n_part, "R": R, "s": s, "cc": None, "best": best, "best_score": bs,
             "keep": (score, shift, d_pts)}
