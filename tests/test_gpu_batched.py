@@ -310,3 +310,47 @@ def test_refinement_graph_replays_match_eager():
         assert np.array_equal(cc.get(), ce), rep
         al._eval_local(R0, s0)                            # eager adjoint between replays
     c.close()
+
+
+def test_batched_entry_points_reject_bad_arguments(ctx):
+    """Error paths of the batched and adjoint entry points: a box size
+    outside 8..128, an empty batch, a support radius outside (0, n/2), a
+    peak radius other than 4/6/8, an unknown table format, null pointers and
+    refinement steps <= 0 each raise EmcError before any launch; a good call
+    follows on the same context."""
+    from emclarity_amd import _lib
+    from emclarity_amd.device import DeviceArray
+    buf = DeviceArray((4096,), np.float64)      # any non-null device pointer: no call reaches a launch
+    b = buf.vp
+    pts = _lib.d_arr([0.0, 0.0, 0.0])
+    bad_batched = [  # n, n_part, table, n_orient, peak_radius, support_radius
+        (6, 2, 0, 1, 6, 2.0), (300, 2, 0, 1, 6, 2.0), (16, 0, 0, 1, 6, 2.0), (16, 2, 0, 0, 6, 2.0),
+        (16, 2, 0, 1, 5, 2.0), (16, 2, 0, 1, 6, 8.0), (16, 2, 0, 1, 6, 0.0), (16, 2, 7, 1, 6, 2.0)]
+    for n, npart, tab, no, pr, sr in bad_batched:
+        with pytest.raises(_lib.EmcError):
+            _lib.call("emc_align_batched", ctx.handle, n, npart, tab, b, b, b, b, b, pts, no, pr, sr, b, b)
+    with pytest.raises(_lib.EmcError):
+        _lib.call("emc_align_batched", ctx.handle, 16, 2, 0, b, None, b, b, b, pts, 1, 6, 2.0, b, b)
+    for n, npart, tab, sr in ((6, 2, 0, 2.0), (16, 0, 0, 2.0), (16, 2, 3, 2.0), (16, 2, 0, 8.5)):
+        with pytest.raises(_lib.EmcError):
+            _lib.call("emc_align_eval_grad", ctx.handle, n, npart, tab, b, b, b, b, b, b, sr, b)
+    with pytest.raises(_lib.EmcError):
+        _lib.call("emc_align_eval_grad", ctx.handle, 16, 2, 0, b, b, b, b, None, b, 2.0, b)
+    for iters, sa, ss, sr in ((-1, 0.01, 0.1, 2.0), (4, 0.0, 0.1, 2.0), (4, 0.01, -0.1, 2.0), (4, 0.01, 0.1, 9.0)):
+        with pytest.raises(_lib.EmcError):
+            _lib.call("emc_align_refine", ctx.handle, 16, 2, 0, b, b, b, b, b, b, iters, sa, ss, sr, b)
+    for n, tab in ((6, 0), (16, 9)):
+        with pytest.raises(_lib.EmcError):
+            _lib.call("emc_batch_reference_spectrum", ctx.handle, n, b, tab, b)
+    with pytest.raises(_lib.EmcError):
+        _lib.call("emc_align_pick", ctx.handle, 0, 1, b, b, b, b, b, b, None, None)
+    # a good call: one particle, one orientation (0, 0, 0) from the identity start
+    score = DeviceArray.from_numpy(np.array([0.5], np.float32))
+    shift = DeviceArray.from_numpy(np.array([1.0, -2.0, 0.5], np.float32))
+    start = DeviceArray.from_numpy(np.eye(3).ravel())
+    apts = DeviceArray.from_numpy(np.zeros(3))
+    R0 = DeviceArray((9,), np.float64)
+    s0 = DeviceArray((3,), np.float64)
+    _lib.call("emc_align_pick", ctx.handle, 1, 1, score.vp, shift.vp, start.vp, apts.vp, R0.vp, s0.vp, None, None)
+    assert np.max(np.abs(R0.get() - np.eye(3).ravel())) < 1e-12
+    assert np.array_equal(s0.get(), [1.0, -2.0, 0.5])
