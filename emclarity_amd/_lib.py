@@ -50,6 +50,8 @@ SIGNATURES = {
     "emc_sf3d": (_st, [_vp, C.c_int, C.c_int, _i3, C.c_int, _f, _f, _f, _f, _f, _f, _f, _f, _f, _f,
                        _vp]),
     "emc_weight_mask": (_st, [_vp, _i3, C.c_double, _d, C.c_int, _d, _d, _d, _vp]),
+    "emc_weight_mask_batched": (_st, [_vp, _i3, C.c_double, C.c_int, _d, C.POINTER(C.c_int32), _d, _d, _d,
+                                      _vp]),
     "emc_ctf2d": (_st, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
                         C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, _vp]),
     "emc_fft_plan_create": (_st, [_vp, _i3, C.c_int, C.POINTER(_vp)]),

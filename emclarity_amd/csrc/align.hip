@@ -220,6 +220,7 @@ emc_status emc_align_grid_padded(emc_ctx* ctx, int w, int n, const int pad_low[3
     }
     EMC_REQUIRE(n_orient > 0, "n_orient must be > 0");
     EMC_REQUIRE(mode >= 1 && mode <= 3, "mode must be 1 (translation), 2 (score) or 3 (both)");
+    EMC_REQUIRE(!(mode & 1) || d_peak_mask, "the translation stage (mode 1 or 3) needs a peak mask");
     EMC_REQUIRE(nsym >= 1 && nsym <= kMaxSym, "nsym must be 1..%d", kMaxSym);
     EMC_REQUIRE(nsym == 1 || S_colmajor, "symmetry matrices required for nsym > 1");
     const int dims[3] = {n, n, n};

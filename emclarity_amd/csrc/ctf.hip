@@ -124,6 +124,29 @@ __global__ void shift_min_kernel(float* __restrict__ d, int64_t n, const double*
     d[i] = d[i] - sub;
 }
 
+// Per-tilt constants of an SF3D call (mexSF3D.cu:125-245): Fourier voxel
+// size from ctf_dims after the half-grid reduction, each tilt with its own
+// pixel size (mexSF3D.cu:234-245); the slab rotation of -tilt (:131-137).
+std::vector<CtfTilt> sf3d_tilts(int half, const int dims[3], int n, const std::vector<double>& pixel,
+                                const std::vector<double>& wl, const std::vector<double>& cs,
+                                const std::vector<double>& df1, const std::vector<double>& df2,
+                                const std::vector<double>& astig, const std::vector<double>& amp,
+                                const std::vector<double>& tilt, const std::vector<double>& exposure,
+                                const std::vector<double>& occ) {
+  const int cx = half ? dims[0] / 2 + 1 : dims[0], ny = dims[1];
+  std::vector<CtfTilt> h(n);
+  for (int i = 0; i < n; ++i) {
+    const double fvx = 1.0 / (pixel[i] * (double)cx), fvy = 1.0 / (pixel[i] * (double)ny);
+    h[i] = make_tilt(pixel[i], wl[i], cs[i], df1[i], df2[i], astig[i], amp[i], fvx, fvy);
+    h[i].occupancy = occ[i];
+    h[i].exposure = exposure[i];
+    const double a = -tilt[i] * M_PI / 180.0;
+    h[i].sin_t = (float)std::sin(a);
+    h[i].cos_t = (float)std::cos(a);
+  }
+  return h;
+}
+
 void sf3d_impl(emc_ctx* ctx, int half, int sq, const int dims[3], int n,
                const std::vector<double>& pixel, const std::vector<double>& wl,
                const std::vector<double>& cs, const std::vector<double>& df1,
@@ -134,18 +157,7 @@ void sf3d_impl(emc_ctx* ctx, int half, int sq, const int dims[3], int n,
   EMC_REQUIRE(nx > 0 && ny > 0 && nz > 0, "bad dims");
   EMC_REQUIRE(n > 0, "n_tilts must be > 0");
   const int cx = half ? nx / 2 + 1 : nx;
-  std::vector<CtfTilt> h(n);
-  for (int i = 0; i < n; ++i) {
-    // Fourier voxel size from ctf_dims after the half-grid reduction; tilt
-    // 0 and the others use their own pixel size (mexSF3D.cu:234-245)
-    const double fvx = 1.0 / (pixel[i] * (double)cx), fvy = 1.0 / (pixel[i] * (double)ny);
-    h[i] = make_tilt(pixel[i], wl[i], cs[i], df1[i], df2[i], astig[i], amp[i], fvx, fvy);
-    h[i].occupancy = occ[i];
-    h[i].exposure = exposure[i];
-    const double a = -tilt[i] * M_PI / 180.0;
-    h[i].sin_t = (float)std::sin(a);
-    h[i].cos_t = (float)std::cos(a);
-  }
+  const std::vector<CtfTilt> h = sf3d_tilts(half, dims, n, pixel, wl, cs, df1, df2, astig, amp, tilt, exposure, occ);
   CtfTilt* d_t = (CtfTilt*)scratch(ctx, "ctf_tilts", sizeof(CtfTilt) * n);
   float* d_ctf = (float*)scratch(ctx, "ctf_stack", sizeof(float) * (size_t)n * cx * ny);
   EMC_HIP(hipMemcpyAsync(d_t, h.data(), sizeof(CtfTilt) * n, hipMemcpyHostToDevice, ctx->stream));
@@ -157,6 +169,125 @@ void sf3d_impl(emc_ctx* ctx, int half, int sq, const int dims[3], int n,
   EMC_HIP(hipGetLastError());
   // the tilt table lives in pageable memory; finish before it goes away
   EMC_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+// The SF3D arguments BH_weightMaskMex marshals from one particle's TLT
+// (masking/BH_weightMaskMex.m:12-51): rows sorted by column 1 (stable),
+// every value through single(), the particle's defocus lowered by its
+// height along the beam after the tilt, occupancy = dose fraction x the
+// elastic fraction exp(-75 / (400 cos(tilt))) relative to its maximum.
+std::vector<CtfTilt> weight_mask_tilts(const int dims[3], double sampling, const double* tlt, int n,
+                                       const double xyz[3], const double recon_size[3],
+                                       const double recon_shift[3]) {
+  std::vector<int> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return tlt[22 * a] < tlt[22 * b]; });
+  auto col = [&](int row, int c) { return tlt[22 * order[row] + c]; };
+  auto sgl = [](double x) { return (double)(float)x; };
+  double origin[3], prj[3];
+  for (int k = 0; k < 3; ++k) {
+    origin[k] = std::ceil((recon_size[k] + 1.0) / 2.0);
+    prj[k] = xyz[k] - origin[k] + recon_shift[k];  // :17-20
+  }
+  std::vector<double> pixel(n), wl(n), cs(n), dfa(n), dfb(n), ast(n), amp(n), tilt(n), ex(n), occ(n);
+  double dose_mean = 0.0;
+  for (int i = 0; i < n; ++i) dose_mean += sgl(col(i, 13));
+  dose_mean /= n;
+  std::vector<double> elastic(n);
+  double emax = 0.0;
+  for (int i = 0; i < n; ++i) {
+    elastic[i] = std::exp(-75.0 / (std::cos(sgl(col(i, 3)) * M_PI / 180.0) * 400.0));  // :43
+    emax = std::max(emax, elastic[i]);
+  }
+  for (int i = 0; i < n; ++i) {
+    tilt[i] = sgl(col(i, 3));
+    pixel[i] = sgl(sgl(col(i, 15)) * sampling * 1e10);  // :16
+    cs[i] = sgl(sgl(col(i, 16)) * 1e3);                 // :22
+    wl[i] = sgl(sgl(col(i, 17)) * 1e10);                // :23
+    amp[i] = sgl(col(i, 18));                           // :24
+    double defocus = sgl(-sgl(col(i, 14)) * 1e10);      // :25
+    const double ddf = sgl(sgl(col(i, 11)) * 1e10);     // :26
+    ast[i] = sgl(sgl(col(i, 12)) * 180.0 / M_PI);       // :27
+    ex[i] = sgl(col(i, 10));                            // :30
+    // particle height along the beam after the tilt (:33-37):
+    // BH_defineMatrix(tilt,'TILT','forwardVector') row 3 . prjVector
+    const double a = tilt[i] * M_PI / 180.0;
+    const double z = -std::sin(a) * prj[0] + std::cos(a) * prj[2];
+    defocus = defocus - z * pixel[i];
+    dfa[i] = sgl(defocus + ddf);  // :48-49
+    dfb[i] = sgl(defocus - ddf);
+    occ[i] = sgl(elastic[i] / emax * (sgl(col(i, 13)) / dose_mean));  // :42-44
+  }
+  return sf3d_tilts(0, dims, n, pixel, wl, cs, dfa, dfb, ast, amp, tilt, ex, occ);
+}
+
+// SF3D of many particles of one box size, the tilts of particle p being
+// entries [off[p] - base, off[p+1] - base) of the chunk's tilt table and CTF
+// stack; blockIdx.z = z + nz * p. The same slab sum as sf3d_kernel; each block
+// also writes the minimum of its voxels (part_min[p][block]).
+__global__ void __launch_bounds__(256) sf3d_many_kernel(float* __restrict__ out, const float* __restrict__ ctfs,
+                                                        const CtfTilt* __restrict__ tilts,
+                                                        const int* __restrict__ off, int base, int nx, int ny,
+                                                        int nz, int cx, float* __restrict__ part_min) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int z = blockIdx.z % nz, p = blockIdx.z / nz;
+  const float ssx = (nx % 2 == 0) ? 0.5f : 0.0f;
+  const float ssz = (nz % 2 == 0) ? 0.5f : 0.0f;
+  const float u = (float)x - (float)nx / 2.0f;   // float division, as mexSF3D.cu:47
+  const float w = (float)z - (float)(nz / 2);
+  const float xscale = (float)cx / (float)nx;
+  const float xoff = 0.5f * (float)cx - 0.5f;
+  const float kPi = 3.14159265358979323f;
+  const int t0 = off[p] - base, t1 = off[p + 1] - base;
+  float acc = 0.0f;
+  const bool inside = x < nx && y < ny;
+  if (inside) {
+    for (int t = t0; t < t1; ++t) {
+      const float s = tilts[t].sin_t, c = tilts[t].cos_t;
+      const float tw = (-u * s + ssz) + w * c;
+      if (tw < -4.0f || tw > 4.0f) continue;
+      const float zw = (0.5f + 0.5f * cosf(tw * (kPi / 4.0f))) * 0.25f;
+      const float xt = (u * c + w * s + ssx) * xscale + xoff;  // texel coordinate
+      const float xf = floorf(xt);
+      const float f = xt - xf;
+      const int x0 = (int)xf;
+      const float* row = ctfs + ((int64_t)t * ny + y) * cx;
+      const float v0 = (unsigned)x0 < (unsigned)cx ? row[x0] : 0.0f;
+      const float v1 = (unsigned)(x0 + 1) < (unsigned)cx ? row[x0 + 1] : 0.0f;
+      acc += zw * (v0 + f * (v1 - v0));
+    }
+    out[(((int64_t)p * nz + z) * ny + y) * nx + x] = acc;
+  }
+  // block minimum: one wave per threadIdx.y row (blockDim = 64 x 4)
+  __shared__ float lds[4];
+  const float m = wave_min(inside ? acc : __builtin_inff());
+  if (threadIdx.x == 0) lds[threadIdx.y] = m;
+  __syncthreads();
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    const float b = fminf(fminf(lds[0], lds[1]), fminf(lds[2], lds[3]));
+    const int per = gridDim.x * gridDim.y * nz;
+    part_min[(int64_t)p * per + ((int64_t)z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = b;
+  }
+}
+
+// Per-particle minimum of the block minima (blockIdx.x = particle).
+__global__ void __launch_bounds__(kRedThreads) fold_min_many_kernel(const float* __restrict__ part, int nb,
+                                                                    float* __restrict__ out) {
+  const float* q = part + (int64_t)blockIdx.x * nb;
+  float m = __builtin_inff();
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) m = fminf(m, q[b]);
+  m = block_min(m);
+  if (threadIdx.x == 0) out[blockIdx.x] = m;
+}
+
+// SF3D - (min - floor) per particle (BH_weightMaskMex.m:55); blockIdx.y = particle.
+__global__ void shift_min_many_kernel(float* __restrict__ d, int64_t nv, const float* __restrict__ mins,
+                                      float floor_value) {
+  float* v = d + (int64_t)blockIdx.y * nv;
+  const float sub = mins[blockIdx.y] - floor_value;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += (int64_t)gridDim.x * blockDim.x)
+    v[i] = v[i] - sub;
 }
 
 }  // namespace emc
@@ -189,56 +320,98 @@ emc_status emc_weight_mask(emc_ctx* ctx, const int dims[3], double sampling,
     use_device(ctx);
     EMC_REQUIRE(dims && tlt && xyz && recon_size && recon_shift && d_out, "null pointer");
     EMC_REQUIRE(n > 0, "empty TLT");
-    // sortrows(TLT,1) (BH_weightMaskMex.m:12), stable
-    std::vector<int> order(n);
-    for (int i = 0; i < n; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int a, int b) { return tlt[22 * a] < tlt[22 * b]; });
-    auto col = [&](int row, int c) { return tlt[22 * order[row] + c]; };
-    // everything passes through single() in the reference (:12)
-    auto sgl = [](double x) { return (double)(float)x; };
-    double origin[3], prj[3];
-    for (int k = 0; k < 3; ++k) {
-      origin[k] = std::ceil((recon_size[k] + 1.0) / 2.0);
-      prj[k] = xyz[k] - origin[k] + recon_shift[k];  // :17-20
-    }
-    std::vector<double> pixel(n), wl(n), cs(n), dfa(n), dfb(n), ast(n), amp(n), tilt(n), ex(n),
-        occ(n);
-    double dose_mean = 0.0;
-    for (int i = 0; i < n; ++i) dose_mean += sgl(col(i, 13));
-    dose_mean /= n;
-    std::vector<double> elastic(n);
-    double emax = 0.0;
-    for (int i = 0; i < n; ++i) {
-      elastic[i] = std::exp(-75.0 / (std::cos(sgl(col(i, 3)) * M_PI / 180.0) * 400.0));  // :43
-      emax = std::max(emax, elastic[i]);
-    }
-    for (int i = 0; i < n; ++i) {
-      tilt[i] = sgl(col(i, 3));
-      pixel[i] = sgl(sgl(col(i, 15)) * sampling * 1e10);  // :16
-      cs[i] = sgl(sgl(col(i, 16)) * 1e3);                 // :22
-      wl[i] = sgl(sgl(col(i, 17)) * 1e10);                // :23
-      amp[i] = sgl(col(i, 18));                           // :24
-      double defocus = sgl(-sgl(col(i, 14)) * 1e10);      // :25
-      const double ddf = sgl(sgl(col(i, 11)) * 1e10);     // :26
-      ast[i] = sgl(sgl(col(i, 12)) * 180.0 / M_PI);       // :27
-      ex[i] = sgl(col(i, 10));                            // :30
-      // particle height along the beam after the tilt (:33-37):
-      // BH_defineMatrix(tilt,'TILT','forwardVector') row 3 . prjVector
-      const double a = tilt[i] * M_PI / 180.0;
-      const double z = -std::sin(a) * prj[0] + std::cos(a) * prj[2];
-      defocus = defocus - z * pixel[i];
-      dfa[i] = sgl(defocus + ddf);  // :48-49
-      dfb[i] = sgl(defocus - ddf);
-      occ[i] = sgl(elastic[i] / emax * (sgl(col(i, 13)) / dose_mean));  // :42-44
-    }
-    sf3d_impl(ctx, 0, 1, dims, n, pixel, wl, cs, dfa, dfb, ast, amp, tilt, ex, occ, d_out);
+    EMC_REQUIRE(dims[0] > 0 && dims[1] > 0 && dims[2] > 0, "bad dims");
+    const std::vector<CtfTilt> h = weight_mask_tilts(dims, sampling, tlt, n, xyz, recon_size, recon_shift);
+    const int cx = dims[0], ny = dims[1];
+    CtfTilt* d_t = (CtfTilt*)scratch(ctx, "ctf_tilts", sizeof(CtfTilt) * n);
+    float* d_ctf = (float*)scratch(ctx, "ctf_stack", sizeof(float) * (size_t)n * cx * ny);
+    EMC_HIP(hipMemcpyAsync(d_t, h.data(), sizeof(CtfTilt) * n, hipMemcpyHostToDevice, ctx->stream));
+    const dim3 b2(64, 4, 1);
+    hipLaunchKernelGGL(ctf2d_kernel, dim3(ceil_div(cx, 64), ceil_div(ny, 4), n), b2, 0, ctx->stream, d_ctf,
+                       (const CtfTilt*)d_t, cx, ny, cx / 2, ny / 2, 0, 1, 1, 1);
+    hipLaunchKernelGGL(sf3d_kernel, dim3(ceil_div(dims[0], 64), ceil_div(ny, 4), dims[2]), b2, 0, ctx->stream,
+                       d_out, (const float*)d_ctf, (const CtfTilt*)d_t, n, dims[0], ny, dims[2], cx);
+    EMC_HIP(hipGetLastError());
     const int64_t nv = (int64_t)dims[0] * dims[1] * dims[2];
     double* d_min = (double*)scratch(ctx, "wm_min", sizeof(double));
     reduce_min(ctx, d_out, nv, 1, d_min);
     hipLaunchKernelGGL(shift_min_kernel, dim3(256), dim3(256), 0, ctx->stream, d_out, nv,
                        (const double*)d_min, 1e-4f);  // :55
     EMC_HIP(hipGetLastError());
+    // the tilt table lives in pageable memory; finish before it goes away
+    EMC_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+emc_status emc_weight_mask_batched(emc_ctx* ctx, const int dims[3], double sampling, int n_part,
+                                   const double* tlt, const int32_t* n_tilts, const double* xyz,
+                                   const double recon_size[3], const double recon_shift[3], float* d_out) {
+  return guard([&] {
+    use_device(ctx);
+    EMC_REQUIRE(dims && recon_size && recon_shift, "null pointer");
+    EMC_REQUIRE(dims[0] > 0 && dims[1] > 0 && dims[2] > 0, "bad dims");
+    EMC_REQUIRE(n_part >= 0, "n_part must be >= 0");
+    if (n_part == 0) return;
+    EMC_REQUIRE(tlt && n_tilts && xyz && d_out, "null pointer");
+    const int nx = dims[0], ny = dims[1], nz = dims[2], cx = nx;
+    const int64_t nv = (int64_t)nx * ny * nz;
+    // every particle's tilt constants, and its first entry in the table
+    std::vector<CtfTilt> all;
+    std::vector<int> first(n_part + 1, 0);
+    const double* rows = tlt;
+    for (int p = 0; p < n_part; ++p) {
+      EMC_REQUIRE(n_tilts[p] > 0, "particle %d: empty TLT", p);
+      const std::vector<CtfTilt> h =
+          weight_mask_tilts(dims, sampling, rows, n_tilts[p], xyz + 3 * (int64_t)p, recon_size, recon_shift);
+      all.insert(all.end(), h.begin(), h.end());
+      first[p + 1] = first[p] + n_tilts[p];
+      rows += 22 * (int64_t)n_tilts[p];
+    }
+    // chunks of particles whose CTF stack stays within 256 MiB
+    const size_t tilt_bytes = sizeof(float) * (size_t)cx * ny;
+    const int max_tilts = (int)std::min<size_t>(65535, std::max<size_t>(1, (size_t(256) << 20) / tilt_bytes));
+    const int max_parts = std::max(1, 65535 / nz);  // grid z = nz x particles of a chunk
+    const dim3 b2(64, 4, 1);
+    const int gx = ceil_div(nx, 64), gy = ceil_div(ny, 4);
+    const int per = gx * gy * nz;  // blocks (minima) per particle
+    std::vector<int> cuts = {0};
+    int max_nt = 0;
+    while (cuts.back() < n_part) {
+      const int p0 = cuts.back();
+      int p1 = p0 + 1;
+      while (p1 < n_part && p1 - p0 < max_parts && first[p1 + 1] - first[p0] <= max_tilts) ++p1;
+      max_nt = std::max(max_nt, first[p1] - first[p0]);
+      cuts.push_back(p1);
+    }
+    EMC_REQUIRE(max_nt <= 65535, "a particle has more than 65535 tilts");
+    // tilt table and absolute first-tilt offsets through one pinned buffer
+    const size_t tab_bytes = sizeof(CtfTilt) * all.size();
+    const size_t table_bytes = tab_bytes + sizeof(int) * (n_part + 1);
+    char* h_tab = (char*)pinned(ctx, table_bytes);
+    EMC_HIP(hipStreamSynchronize(ctx->stream));  // the pinned buffer may be in use
+    std::memcpy(h_tab, all.data(), tab_bytes);
+    std::memcpy(h_tab + tab_bytes, first.data(), sizeof(int) * (n_part + 1));
+    char* d_tab = (char*)scratch(ctx, "wmb_tilts", table_bytes);
+    EMC_HIP(hipMemcpyAsync(d_tab, h_tab, table_bytes, hipMemcpyHostToDevice, ctx->stream));
+    const CtfTilt* d_t = (const CtfTilt*)d_tab;
+    const int* d_off = (const int*)(d_tab + tab_bytes);
+    float* d_ctf = (float*)scratch(ctx, "wmb_stack", tilt_bytes * (size_t)max_nt);
+    float* d_part = (float*)scratch(ctx, "wmb_min", sizeof(float) * ((size_t)per + 1) * n_part);
+    float* d_mins = d_part + (size_t)per * n_part;
+    for (size_t c = 0; c + 1 < cuts.size(); ++c) {
+      const int p0 = cuts[c], p1 = cuts[c + 1];
+      const int np = p1 - p0, nt = first[p1] - first[p0];
+      const CtfTilt* dt = d_t + first[p0];
+      hipLaunchKernelGGL(ctf2d_kernel, dim3(ceil_div(cx, 64), ceil_div(ny, 4), nt), b2, 0, ctx->stream, d_ctf, dt,
+                         cx, ny, cx / 2, ny / 2, 0, 1, 1, 1);
+      hipLaunchKernelGGL(sf3d_many_kernel, dim3(gx, gy, nz * np), b2, 0, ctx->stream, d_out + p0 * nv,
+                         (const float*)d_ctf, dt, d_off + p0, first[p0], nx, ny, nz, cx, d_part);
+      hipLaunchKernelGGL(fold_min_many_kernel, dim3(np), dim3(kRedThreads), 0, ctx->stream, (const float*)d_part,
+                         per, d_mins);
+      hipLaunchKernelGGL(shift_min_many_kernel, dim3(ceil_div(nv, 256 * 8), np), dim3(256), 0, ctx->stream,
+                         d_out + p0 * nv, nv, (const float*)d_mins, 1e-4f);  // BH_weightMaskMex.m:55
+      EMC_HIP(hipGetLastError());
+    }
   });
 }
 

@@ -96,6 +96,36 @@ def BH_weightMaskMex(SIZE, SAMPLING, TLT, xyzSubTomo, reconGeometry, ctx=None) -
     return out
 
 
+def BH_weightMaskMex_batched(SIZE, SAMPLING, TLTs, xyzSubTomos, reconGeometry, out=None,
+                             ctx=None) -> DeviceArray:
+    """BH_weightMaskMex for many particles of one box size in one call
+    (emc_weight_mask_batched): TLTs, a list of per-particle TLT tables
+    (nTilts_p x 22), xyzSubTomos (n, 3) the particle centres, one shared
+    reconGeometry. Returns the (n, nz, ny, nx) stack of SF3D - (min - 1e-4)
+    (into ``out`` when given)."""
+    ctx = ctx or (out.ctx if out is not None else default_context())
+    nx, ny, nz = (int(s) for s in SIZE)
+    tabs = [np.asarray(T, dtype=np.float64) for T in TLTs]
+    for T in tabs:
+        if T.ndim != 2 or T.shape[1] < 22 or T.shape[0] < 1:
+            raise ValueError("every TLT must have 22 columns and at least one row")
+    n = len(tabs)
+    xyz = np.ascontiguousarray(np.asarray(xyzSubTomos, dtype=np.float64).reshape(n, 3))
+    rows = np.ascontiguousarray(np.concatenate([T[:, :22] for T in tabs], axis=0)) if n else np.zeros((0, 22))
+    counts = np.ascontiguousarray([T.shape[0] for T in tabs], dtype=np.int32)
+    rg = np.asarray(reconGeometry, dtype=np.float64).reshape(2, 3)
+    shape = (n, nz, ny, nx)
+    if out is None:
+        out = DeviceArray(shape, np.float32, ctx)
+    elif out.shape != shape or out.dtype != np.float32:
+        raise ValueError(f"out must be float32 {shape}")
+    _lib.call("emc_weight_mask_batched", ctx.handle, _lib.i3((nx, ny, nz)), float(SAMPLING), n,
+              rows.ctypes.data_as(_lib.C.POINTER(_lib.C.c_double)),
+              counts.ctypes.data_as(_lib.C.POINTER(_lib.C.c_int32)),
+              xyz.ctypes.data_as(_lib.C.POINTER(_lib.C.c_double)), _lib.d_arr(rg[0]), _lib.d_arr(rg[1]), out.vp)
+    return out
+
+
 def mexCTF(doHalfGrid, doSqCTF, nX, nY, pixelSize, waveLength, CS, defocus1, defocus2, defocusAst,
            AmpContrast, centered=False, ctx=None) -> DeviceArray:
     """ctf = mexCTF(...) (mexFiles/mexCTF.cu:1-121): shape (nY, nx_out)."""

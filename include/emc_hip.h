@@ -174,6 +174,21 @@ emc_status emc_weight_mask(emc_ctx* ctx, const int dims[3], double sampling,
                            const double xyz_subtomo[3], const double recon_size[3],
                            const double recon_shift[3], float* d_out);
 
+/* BH_weightMaskMex for many particles of one box size (masking/BH_weightMaskMex.m:1-59 per
+ * particle, as alignment/BH_alignRaw3d_v2.m:1093-1094 calls it inside its particle loop):
+ * particle p has its own TLT, n_tilts[p] rows x 22 columns (row-major doubles, the
+ * particles' tables one after another in tlt_rowmajor), and its own centre
+ * xyz_subtomo[3p..3p+2]; the recon geometry is shared. d_out: n_part volumes of dims,
+ * one after another, each SF3D_p - (min(SF3D_p) - 1e-4). One launch writes the CTF
+ * images of every (particle, tilt), one the SF3D voxels of every particle (with each
+ * block's minimum), one folds the minima and one applies the shift (particles in chunks
+ * whose CTF stack stays within 256 MiB). Stream-ordered; returns once the tilt table is
+ * queued for the device. */
+emc_status emc_weight_mask_batched(emc_ctx* ctx, const int dims[3], double sampling, int n_part,
+                                   const double* tlt_rowmajor, const int32_t* n_tilts,
+                                   const double* xyz_subtomo, const double recon_size[3],
+                                   const double recon_shift[3], float* d_out);
+
 /* mexCTF(half, sq, nX, nY, pix, lambda, Cs, df1, df2, astig, amp [, centered])
  * (mexFiles/mexCTF.cu:1-121, kernel mexFiles/utils/ctf.cu:12-65). Output
  * (half ? nx/2+1 : nx) x ny FP32. The reference gateway accepts occupancy and

@@ -33,6 +33,21 @@ import math
 from dataclasses import dataclass, field
 
 import numpy as np
+import scipy.fft as _sfft
+
+# FP64 FFTs on every host core (pocketfft, as numpy.fft, multithreaded): the
+# chunk-sized transforms of the template search (480^3) and the 2n-padded
+# reference spectra stay within seconds.
+def _rfftn(x):
+    return _sfft.rfftn(x, workers=-1)
+
+
+def _irfftn(x, s):
+    return _sfft.irfftn(x, s=s, axes=(0, 1, 2), workers=-1)
+
+
+def _ifftn(x):
+    return _sfft.ifftn(x, workers=-1)
 
 # --------------------------------------------------------------------------
 # A3  BH_defineMatrix  (coordinates/BH_defineMatrix.m:51-126)
@@ -810,37 +825,49 @@ def matlab_std(x):
 
 
 def template_search_chunk(tomo_chunk, template, grid_angles, pad_bin, trim_valid,
-                          prev_peak=None, prev_angle=None, first_angle_id=1, init_from_first=True):
+                          prev_peak=None, prev_angle=None, first_angle_id=1, init_from_first=True,
+                          return_margin=False):
     """Per chunk, per orientation (:713-887): tempRot = interp3d(template,
     [phi,theta,psi-phi], [1,1,1], 'forward') (:744-747), zero-pad into the
     chunk (:751-754), ccf = trim(real(C2R(swapQ R2C(chunk) conj(R2C(tempPAD)))))
     (:759-761), / std (:764), strict-< max/argmax update (:851-854, :883-887).
 
-    Returns (peak, angle_id) over the valid area, angle ids 1-based."""
+    Returns (peak, angle_id) over the valid area, angle ids 1-based. With
+    return_margin, also the per-voxel margin between the best and the
+    second-best score of this call's orientations (and the previous peak,
+    when given): where it exceeds twice a score's error, the argmax id is
+    determined by the scores alone."""
     nz, ny, nx = tomo_chunk.shape
     size = (nx, ny, nz)
-    tomo_fou = swap_quadrants_half(size) * np.fft.rfftn(tomo_chunk)
+    tomo_fou = swap_quadrants_half(size) * _rfftn(np.asarray(tomo_chunk, dtype=np.float64))
     mag = None if prev_peak is None else prev_peak.copy()
     ang = None if prev_angle is None else prev_angle.copy()
+    second = None if prev_peak is None else np.full(prev_peak.shape, -np.inf)
     aid = first_angle_id
     for (phi, theta, psi) in grid_angles:
         trot = interp3d(template, [phi, theta, psi - phi], [1, 1, 1], "Bah", "forward")
         tpad = pad_zeros3d(trot, pad_bin[0], pad_bin[1])
-        temp_fou = np.conj(np.fft.rfftn(tpad))
-        ccf = np.fft.irfftn(tomo_fou * temp_fou, s=(nz, ny, nx), axes=(0, 1, 2)) * (nx * ny * nz)   # unscaled C2R
+        temp_fou = np.conj(_rfftn(tpad))
+        ccf = _irfftn(tomo_fou * temp_fou, (nz, ny, nx)) * (nx * ny * nz)   # unscaled C2R
+        del temp_fou, tpad
         ccf = pad_zeros3d(ccf, trim_valid[0], trim_valid[1])          # padVal rows: negative = trim
         ccf = ccf / matlab_std(ccf)
         if mag is None and init_from_first:
             mag = ccf.copy()
             ang = np.ones(ccf.shape)
+            second = np.full(ccf.shape, -np.inf)
         else:
             if mag is None:
                 mag = np.zeros(ccf.shape)
                 ang = np.zeros(ccf.shape)
+                second = np.zeros(ccf.shape)
             rep = mag < ccf
+            second = np.where(rep, mag, np.maximum(second, ccf))
             mag = np.where(rep, ccf, mag)
             ang = np.where(rep, aid, ang)
         aid += 1
+    if return_margin:
+        return mag, ang, mag - second
     return mag, ang
 
 
@@ -867,7 +894,7 @@ def batch_reference_spectrum(ref, table="c64"):
     pad = np.zeros((m, m, m))
     idx = (np.arange(n) - n // 2) % m
     pad[np.ix_(idx, idx, idx)] = ref
-    F = np.fft.rfftn(pad)
+    F = _rfftn(pad)
     if table == "f16":
         a = max(np.max(np.abs(F.real)), np.max(np.abs(F.imag)))
         e = int(np.frexp(a)[1]) if a > 0 else 0
@@ -962,7 +989,7 @@ def batch_score_map(F2, n, bp_: BatchParticle, R, peak_radius):
     c = f * bp_.w[bp_.sel]
     X = np.zeros((n, n, n), dtype=np.complex128)
     X[bp_.sel] = bp_.a[bp_.sel] * np.conj(c)
-    M = np.real(np.fft.ifftn(X)) * X.size                # sum_k Re[X e^{+2 pi i k.d/n}]
+    M = np.real(_ifftn(X)) * X.size                # sum_k Re[X e^{+2 pi i k.d/n}]
     D = float(np.sum(np.abs(c) ** 2))
     r = peak_radius
     best, bv = None, -np.inf

@@ -11,7 +11,7 @@ from emclarity_amd.fourier import FFTPlan, fourierTransformer, mexFFT
 from emclarity_amd.geometry import BH_defineMatrix
 from emclarity_amd.interpolator import interpolator, mexXform3d
 from emclarity_amd.masking import (BH_bandLimitCenterNormalize, BH_bandpass3d, BH_padZeros3d,
-                                   BH_weightMaskMex, mexCTF, mexSF3D)
+                                   BH_weightMaskMex, BH_weightMaskMex_batched, mexCTF, mexSF3D)
 from emclarity_amd.xcf import BH_multi_xcf_Rotational, BH_multi_xcf_Translational
 from oracle import emc_oracle as O
 from tests.conftest import load_tlt
@@ -93,6 +93,40 @@ def test_weight_mask_on_reference_tlt(ctx):
     ref = O.weight_mask_mex(size, 4, T, xyz, rg)
     assert rel(out, ref) < 1e-4
     assert abs(float(out.min()) - 1e-4) < 1e-6
+
+
+def test_weight_mask_batched_matches_single_and_oracle(ctx):
+    """Many particles, each with its own TLT (the reference's two fixtures and
+    synthetic tables of other tilt ranges and defoci) and centre: the batched
+    entry equals the single-particle entry bit for bit (same per-voxel
+    arithmetic; the minimum is order-free), and a few against the oracle.
+    110 particles of 128^3 with 31-47 tilts exceed one 256 MiB CTF stack, so
+    the particles run in two chunks."""
+    from emclarity_amd.synthetic import tilt_table
+    rng = np.random.default_rng(12)
+    fixtures = [load_tlt("riboNew05_ali1_ctf.tlt"), load_tlt("riboNew11_ali1_ctf.tlt")]
+    tlts, xyzs = [], []
+    for i in range(110):
+        if i % 3 == 2:
+            tlts.append(fixtures[i % 2])
+        else:
+            tlts.append(tilt_table(rng.uniform(45.0, 70.0), 3.0, rng.uniform(1.5, 4.0), 2.5))
+        xyzs.append([rng.uniform(100, 3700), rng.uniform(100, 3600), rng.uniform(20, 280)])
+    size, rg = (128, 128, 128), [[3838, 3710, 300], [0, 0, 0]]
+    stack = BH_weightMaskMex_batched(size, 4, tlts, xyzs, rg)
+    assert sum(len(t) for t in tlts) * 128 * 128 * 4 > (256 << 20)
+    out = stack.get()
+    for i in range(110):
+        assert np.array_equal(out[i], BH_weightMaskMex(size, 4, tlts[i], xyzs[i], rg).get()), i
+    for i in (0, 2, 57, 109):
+        ref = O.weight_mask_mex(size, 4, tlts[i], xyzs[i], rg)
+        assert rel(out[i], ref) < 1e-4, i
+    small = BH_weightMaskMex_batched((33, 30, 28), 4, tlts[:3], xyzs[:3], rg).get()   # odd and uneven dims
+    for i in range(3):
+        assert np.array_equal(small[i], BH_weightMaskMex((33, 30, 28), 4, tlts[i], xyzs[i], rg).get())
+    with pytest.raises(_lib.EmcError):
+        _lib.call("emc_weight_mask_batched", ctx.handle, _lib.i3(size), 4.0, 1, None, None, None,
+                  _lib.d_arr(rg[0]), _lib.d_arr(rg[1]), stack.vp)
 
 
 # ------------------------------------------------------ spectral (A8-A12)
@@ -264,6 +298,13 @@ def test_new_entry_points_reject_bad_arguments(ctx):
                   None, z, None, None, big.vp, z, big.vp, _lib.d_arr(np.eye(3).ravel()), _lib.d_arr([0, 0, 0]),
                   _lib.i3((1, 1, 1)), np.zeros(3).ctypes.data_as(dp), 1, 3, ccc.ctypes.data_as(dp),
                   est.ctypes.data_as(dp))
+    zv = (C.c_void_p * 1)(v.ptr)
+    for mode in (1, 3):                  # the translation stage without a peak mask
+        with pytest.raises(_lib.EmcError, match="peak mask"):
+            _lib.call("emc_align_grid_padded", ctx.handle, 8, 8, _lib.i3((0, 0, 0)), 1, None, 1, v.vp, v.vp, None,
+                      None, zv, None, None, None, zv, v.vp, _lib.d_arr(np.eye(3).ravel()), _lib.d_arr([0, 0, 0]),
+                      _lib.i3((1, 1, 1)), np.zeros(3).ctypes.data_as(dp), 1, mode, ccc.ctypes.data_as(dp),
+                      est.ctypes.data_as(dp))
     _lib.call("emc_ft_condition", ctx.handle, F.vp, _lib.i3((8, 8, 8)), 1, 0, None, None)
 
 

@@ -8,21 +8,20 @@ from emclarity_amd import synthetic as S
 from oracle import emc_oracle as O
 
 
-def _cpu():
+@pytest.fixture
+def cpu():
+    """The C restatement, loaded only when a test of this file runs (a GPU
+    session that deselects these tests never maps the library)."""
     try:
         from oracle import emc_cpu
         emc_cpu.load()
-        return emc_cpu
-    except Exception:
-        return None
-
-
-cpu = _cpu()
-pytestmark = pytest.mark.skipif(cpu is None, reason="oracle/lib/libemc_cpu.so not built")
+    except Exception as e:  # noqa: BLE001
+        pytest.skip(f"oracle/lib/libemc_cpu.so not built ({e})")
+    return emc_cpu
 
 
 @pytest.mark.parametrize("n", [24, 32])
-def test_c_restatement_matches_numpy_oracle(n):
+def test_c_restatement_matches_numpy_oracle(cpu, n):
     truth = (40.0, 30.0, 20.0)
     case = S.make_align_case(3, n=n, euler=truth, snr=2.0, n_blobs=16, blob_radius=0.22 * n)
     wedge = np.fft.fftshift(S.wedge_mask(n, 60.0)).astype(np.float32)
