@@ -6,8 +6,11 @@ batched-alignment and template-search workloads:
   peaks    greedy peak extraction on a 512 x 512 x 256 score map, 150 peaks
   tsprep   pre-processing of one 480^3 template-search chunk
   tsearch  one 480^3 chunk against a 48^3 template, 32 orientations
+  adjoint  one adjoint evaluation of 1,000 / 600 / 400 particles of 64 / 96 / 128^3
+  search   the 28-orientation coarse search of the same size classes
+  copy     a 1 GiB device-to-device copy (byte-counter calibration)
 
-    python -m emclarity_amd.bench_paths [--path all|extract|average|peaks|tsprep] [--reps N]
+    python -m emclarity_amd.bench_paths [--path all|extract|...|copy] [--reps N] [--table c64|f16]
 
 Prints one JSON line per path: wall time per call and, where the library
 times the dominant kernel with HIP events on its stream, that kernel's
@@ -168,19 +171,120 @@ def bench_tsearch(ctx, reps):
                             "GBps": round(b_ts / per / 1e9, 1)})
 
 
+# ------------------------------------------------ batched-alignment kernels
+# The alignment workload's size classes (box, particles). The kernels' work
+# and memory access pattern depend on the box size, the support radius and
+# the particles' rotations, not on the voxel values: noise particles with
+# random start orientations stand in for the rendered ones.
+CLASSES = ((64, 1000), (96, 600), (128, 400))
+
+
+def _noise_class(ctx, n, count, table, seed):
+    from .batched import BatchAligner
+    from .workload import make_size_class, random_rotation
+    rng = np.random.default_rng(seed)
+    sc, _ = make_size_class(n)
+    al = BatchAligner(n, sc.ref, sc.bandpass, peak_radius=6, table=table, ctx=ctx)
+    data = DeviceArray.from_numpy(rng.standard_normal((count, n, n, n), dtype=np.float32), ctx=ctx)
+    nv = n ** 3
+    views = [data.view((n, n, n), np.float32, 4 * nv * i) for i in range(count)]
+    starts = np.stack([random_rotation(rng) for _ in range(count)])
+    al.set_particles(views, [None] * count, starts)
+    del views, data
+    return al, starts
+
+
+def table_entries_in_ball(n, radius):
+    """Entries of the centred reference table (x in [0, n+1], y, z in [-n, n])
+    whose 2x2x2 footprints trilinear samples at |j| <= radius read."""
+    r = radius + 2.0
+    x = np.arange(0, n + 2)[:, None, None]
+    y = np.arange(-n, n + 1)[None, :, None]
+    z = np.arange(-n, n + 1)[None, None, :]
+    return int(np.count_nonzero(x * x + y * y + z * z <= r * r))
+
+
+def eval_grad_bytes(n, count, support, table):
+    """Algorithmic bytes of one adjoint evaluation of ``count`` particles:
+    each particle's Q (8 B) and N (4 B) at every support frequency, the
+    support list once (8 B an entry), the table region the samples reach
+    once (32 B an entry complex64, 16 B fp16). Returns (bytes, samples)."""
+    from .batched import support_voxels
+    ns = support_voxels(n, support) - 1                       # DC excluded
+    entry = 32 if table == "c64" else 16
+    tab = table_entries_in_ball(n, 2.0 * support) * entry
+    return count * ns * 12 + ns * 8 + tab, count * ns
+
+
+def bench_adjoint(ctx, reps, table="c64"):
+    """One adjoint evaluation (score + 6-component gradient) of every
+    particle of a size class at its pose, full support: the refinement's
+    kernel (align_eval_grad_kernel), HIP events on the library stream."""
+    for k, (n, count) in enumerate(CLASSES):
+        al, starts = _noise_class(ctx, n, count, table, 60 + k)
+        s = np.zeros((count, 3))
+        wall = _timed(ctx, lambda: al._eval_local(starts, s), reps)
+        ms, cnt = _ktime(ctx, "align_eval_grad")
+        per = ms / max(cnt, 1)
+        b, samples = eval_grad_bytes(n, count, al.support, table)
+        _line("adjoint", wall, {"kernel": f"align_eval_grad_kernel<{table}>", "n": n, "particles": count,
+                                "support_radius": round(al.support, 3), "samples": samples,
+                                "kernel_ms": round(per, 4), "bytes": b, "GBps": round(b / (per * 1e-3) / 1e9, 1),
+                                "hbm_frac": round(b / (per * 1e-3) / 8e12, 4),
+                                "gsamples_per_s": round(samples / (per * 1e-3) / 1e9, 2)})
+        del al
+
+
+def bench_search(ctx, reps, table="c64", frac=0.6):
+    """The coarse search of a size class: 28 orientations of the 10-degree
+    local grid scored over 0.6 x the support radius (align_pass_kernel +
+    align_final_kernel)."""
+    from .batched import fused_samples_per_unit
+    from .geometry import eulerSearch
+    grid = eulerSearch("C1", 15, 10, 15, 10)
+    for k, (n, count) in enumerate(CLASSES):
+        al, _ = _noise_class(ctx, n, count, table, 70 + k)
+        wall = _timed(ctx, lambda: al.search_device(grid, support_radius=frac * al.support), reps)
+        ms, cnt = _ktime(ctx, "align_fused")
+        fms, fcnt = _ktime(ctx, "align_final")
+        units = count * len(grid)
+        samples = fused_samples_per_unit(n, frac * al.support) * units
+        _line("search", wall, {"kernel": f"align_pass_kernel<6, {table}>", "n": n, "particles": count,
+                               "orientations": len(grid), "launches_per_call": round(cnt / reps, 2),
+                               "pass_ms_per_call": round(ms / reps, 4), "final_ms_per_call": round(fms / reps, 4),
+                               "gsamples_per_s": round(samples / (ms / reps * 1e-3) / 1e9, 2)})
+        del al
+
+
+def bench_copy(ctx, reps, mib=1024):
+    """A device-to-device copy of known size: the calibration kernel of the
+    HBM byte counters (FETCH_SIZE, WRITE_SIZE)."""
+    src = DeviceArray.from_numpy(np.ones(mib << 18, np.float32), ctx=ctx)
+    dst = DeviceArray(src.shape, np.float32, ctx)
+    wall = _timed(ctx, lambda: _lib.call("emc_memcpy_d2d", ctx.handle, dst.vp, src.vp, C.c_size_t(src.nbytes)),
+                  reps)
+    _line("copy", wall, {"bytes_read": src.nbytes, "bytes_written": src.nbytes,
+                         "GBps": round(2 * src.nbytes / wall / 1e9, 1)})
+
+
 PATHS = {"extract": bench_extract, "average": bench_average, "peaks": bench_peaks, "tsprep": bench_tsprep,
-         "tsearch": bench_tsearch}
+         "tsearch": bench_tsearch, "adjoint": bench_adjoint, "search": bench_search, "copy": bench_copy}
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--path", default="all", choices=["all", *PATHS])
     ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--table", choices=["c64", "f16"], default="c64",
+                    help="reference table format of the adjoint and search paths")
     a = ap.parse_args(argv)
     ctx = default_context()
     for name in (PATHS if a.path == "all" else [a.path]):
         print(f"[bench_paths] {name}", file=sys.stderr, flush=True)
-        PATHS[name](ctx, a.reps)
+        if name in ("adjoint", "search"):
+            PATHS[name](ctx, a.reps, table=a.table)
+        else:
+            PATHS[name](ctx, a.reps)
 
 
 if __name__ == "__main__":

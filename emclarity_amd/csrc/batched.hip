@@ -1240,8 +1240,11 @@ emc_status emc_batch_prepare_particles(emc_ctx* ctx, int n, int n_part, const fl
                          0, ctx->stream, pp, sh, n);
       EMC_HIP(hipGetLastError());
       fft_r2c(cached_plan(ctx, n, n, n, B), sh, P);
-      hipLaunchKernelGGL(prep_particle_kernel, dim3(nb, B), dim3(kRedThreads), 0, ctx->stream,
-                         (const float2*)P, d_bandpass, pp, n, (float2*)d_Q + i0 * nh, d_N + i0 * nh, part);
+      {
+        KTimer kt(ctx, "prep_particle");
+        hipLaunchKernelGGL(prep_particle_kernel, dim3(nb, B), dim3(kRedThreads), 0, ctx->stream,
+                           (const float2*)P, d_bandpass, pp, n, (float2*)d_Q + i0 * nh, d_N + i0 * nh, part);
+      }
       hipLaunchKernelGGL(fold_partials_kernel<1>, dim3(B), dim3(kRedThreads), 0, ctx->stream,
                          (const double*)part, nb, d_A + i0);
       EMC_HIP(hipGetLastError());

@@ -22,6 +22,7 @@ struct CtfTilt {
   double cs_term, df_term, d1, d2, astig, amp_phase;
   double fvx, fvy;
   double occupancy, exposure;
+  double cos2a, sin2a;  // of 2 astig
   float sin_t, cos_t;  // of -tilt (mexSF3D.cu:131-137)
 };
 
@@ -34,6 +35,8 @@ static CtfTilt make_tilt(double pixel, double wl, double cs_mm, double df1, doub
   t.d1 = 0.5 * (df1 + df2);
   t.d2 = 0.5 * (df1 - df2);
   t.astig = astig_deg * M_PI / 180.0 - M_PI * std::nearbyint(astig_deg / 180.0);
+  t.cos2a = std::cos(2.0 * t.astig);
+  t.sin2a = std::sin(2.0 * t.astig);
   t.amp_phase = std::atan(amp / std::sqrt(1.0 - amp * amp));
   t.fvx = fvx;
   t.fvy = fvy;
@@ -43,18 +46,23 @@ static CtfTilt make_tilt(double pixel, double wl, double cs_mm, double df1, doub
 
 // sin(chi) with optional square and optional radial x exposure weight
 // (ctf.cu:45-64, :103-127). (x, y) are centred / wrapped frequency indices.
+// The astigmatic term cos(2(phi - astig)), phi = atan2(ky, kx), is formed
+// without the angle: cos 2phi = (kx^2 - ky^2) / r^2, sin 2phi = 2 kx ky / r^2
+// (phi = 0 at the origin, as atan2(0, 0)). The phase and its sine are FP64;
+// the exposure weight exp(-E / 2 / (0.245 k^-1.665 + 2.81)) is FP32.
 __device__ inline double ctf_value(const CtfTilt& p, int x, int y, int sq, int weighted) {
   const double ky = (double)y * p.fvy;
   const double kx = (double)x * p.fvx;
-  const double phi = atan2(ky, kx);
   const double r2 = kx * kx + ky * ky;
-  double v = sin(p.cs_term * r2 * r2 - p.df_term * r2 * (p.d1 + p.d2 * cos(2.0 * (phi - p.astig))) -
-                 p.amp_phase);
+  const double c2 = r2 > 0.0 ? (kx * kx - ky * ky) / r2 : 1.0;
+  const double s2 = r2 > 0.0 ? 2.0 * kx * ky / r2 : 0.0;
+  const double ca = c2 * p.cos2a + s2 * p.sin2a;
+  double v = sin(p.cs_term * r2 * r2 - p.df_term * r2 * (p.d1 + p.d2 * ca) - p.amp_phase);
   if (sq) v *= v;
   if (weighted) {
-    const double rw = p.occupancy * (x == 0 ? 0.2 : fabs((double)x));
-    const double pw = r2 > 0.0 ? pow(r2, -0.8325) : __builtin_inf();
-    v *= rw * exp((-0.5 * p.exposure) / (0.24499 * pw + 2.81410));
+    const float rw = (float)p.occupancy * (x == 0 ? 0.2f : fabsf((float)x));
+    const float pw = r2 > 0.0 ? powf((float)r2, -0.8325f) : __builtin_inff();
+    v *= (double)(rw * expf((-0.5f * (float)p.exposure) / (0.24499f * pw + 2.81410f)));
   }
   return v;
 }
@@ -83,23 +91,75 @@ __global__ void __launch_bounds__(256) ctf2d_kernel(float* __restrict__ out,
 // SF3D accumulation (mexSF3D.cu:37-77): for each voxel and tilt, the slab
 // coordinate tw = -u sin + size_shift_z + w cos; inside |tw| <= 4 add
 // 0.25 (0.5 + 0.5 cos(pi tw / 4)) x CTF_t(row y, linear in x).
-__global__ void __launch_bounds__(256) sf3d_kernel(float* __restrict__ out,
-                                                   const float* __restrict__ ctfs,
-                                                   const CtfTilt* __restrict__ tilts, int n_tilts,
-                                                   int nx, int ny, int nz, int cx) {
+//
+// A block (64 x, 4 waves of kSlabRows rows each, one z) first lists, in
+// table order, the tilts whose slab reaches any of its voxels: tw is linear
+// in u and independent of y, so a tilt is listed when [tw(u_first),
+// tw(u_last)] meets [-4, 4] (widened by 1e-3 against rounding). The slab
+// weight and the CTF x coordinate depend on (x, z, tilt) only, so a thread
+// owns kSlabRows voxels of one (x, z) column, forms them once per listed
+// tilt and adds the interpolated CTF row of each of its voxels (lanes run
+// along x: the CTF rows are read coalesced). Every voxel still runs the
+// exact slab test; tilts left out add nothing, and per voxel the tilts are
+// summed in table order, as the reference's per-voxel loop.
+constexpr int kSlabListMax = 1024;
+constexpr int kSlabRows = 16;
+
+// The block's tilt list in LDS (built by wave 0, in table order) for tilts
+// [0, nt); returns its length, or -1 when it would not fit (the caller then
+// loops over every tilt).
+__device__ inline int slab_list(const CtfTilt* __restrict__ tilts, int nt, int nx, int nz, int z, int* list) {
+  __shared__ int count;
+  const int lane = threadIdx.x + blockDim.x * threadIdx.y;
+  const int x0 = blockIdx.x * blockDim.x, x1 = min(nx, x0 + (int)blockDim.x) - 1;
+  const float ssz = (nz % 2 == 0) ? 0.5f : 0.0f;
+  const float ua = (float)x0 - (float)nx / 2.0f, ub = (float)x1 - (float)nx / 2.0f;
+  const float w = (float)z - (float)(nz / 2);
+  if (lane < kWave) {
+    int n = 0;
+    for (int base = 0; base < nt; base += kWave) {
+      const int t = base + lane;
+      bool hit = false;
+      if (t < nt) {
+        const float s = tilts[t].sin_t, c = tilts[t].cos_t;
+        const float ta = (-ua * s + ssz) + w * c, tb = (-ub * s + ssz) + w * c;
+        hit = fminf(ta, tb) <= 4.001f && fmaxf(ta, tb) >= -4.001f;
+      }
+      const unsigned long long m = __ballot(hit);
+      const int before = __popcll(m & ((1ull << lane) - 1ull));
+      if (hit && n + before < kSlabListMax) list[n + before] = t;
+      n += __popcll(m);
+    }
+    if (lane == 0) count = n <= kSlabListMax ? n : -1;
+  }
+  __syncthreads();
+  return count;
+}
+
+// blockDim (64, 4): x = blockIdx.x * 64 + threadIdx.x, rows from
+// (blockIdx.y * 4 + threadIdx.y) * kSlabRows, z given. out: the volume;
+// ctfs, tilts: its tilts [0, nt). Returns the minimum of the thread's voxels
+// (+inf when it has none).
+__device__ inline float sf3d_column(float* __restrict__ out, const float* __restrict__ ctfs,
+                                    const CtfTilt* __restrict__ tilts, int nt, int nx, int ny, int nz, int cx,
+                                    int z, const int* list, int nlist) {
+  const float kPi = 3.14159265358979323f;
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y * blockDim.y + threadIdx.y;
-  const int z = blockIdx.z;
-  if (x >= nx || y >= ny) return;
+  const int y0 = (blockIdx.y * blockDim.y + threadIdx.y) * kSlabRows;
+  if (x >= nx || y0 >= ny) return __builtin_inff();
   const float ssx = (nx % 2 == 0) ? 0.5f : 0.0f;
   const float ssz = (nz % 2 == 0) ? 0.5f : 0.0f;
-  const float u = (float)x - (float)nx / 2.0f;   // float division, as :47
+  const float u = (float)x - (float)nx / 2.0f;   // float division, as mexSF3D.cu:47
   const float w = (float)z - (float)(nz / 2);
   const float xscale = (float)cx / (float)nx;
   const float xoff = 0.5f * (float)cx - 0.5f;
-  const float kPi = 3.14159265358979323f;
-  float acc = 0.0f;
-  for (int t = 0; t < n_tilts; ++t) {
+  float acc[kSlabRows];
+#pragma unroll
+  for (int r = 0; r < kSlabRows; ++r) acc[r] = 0.0f;
+  const int rows = min(kSlabRows, ny - y0);  // uniform over the wave
+  const int n = nlist >= 0 ? nlist : nt;
+  for (int i = 0; i < n; ++i) {
+    const int t = nlist >= 0 ? list[i] : i;
     const float s = tilts[t].sin_t, c = tilts[t].cos_t;
     const float tw = (-u * s + ssz) + w * c;
     if (tw < -4.0f || tw > 4.0f) continue;
@@ -108,12 +168,36 @@ __global__ void __launch_bounds__(256) sf3d_kernel(float* __restrict__ out,
     const float xf = floorf(xt);
     const float f = xt - xf;
     const int x0 = (int)xf;
-    const float* row = ctfs + ((int64_t)t * ny + y) * cx;
-    const float v0 = (unsigned)x0 < (unsigned)cx ? row[x0] : 0.0f;
-    const float v1 = (unsigned)(x0 + 1) < (unsigned)cx ? row[x0 + 1] : 0.0f;
-    acc += zw * (v0 + f * (v1 - v0));
+    const bool in0 = (unsigned)x0 < (unsigned)cx, in1 = (unsigned)(x0 + 1) < (unsigned)cx;
+    const float* row = ctfs + ((int64_t)t * ny + y0) * cx;
+#pragma unroll
+    for (int r = 0; r < kSlabRows; ++r) {
+      if (r < rows) {
+        const float v0 = in0 ? row[(int64_t)r * cx + x0] : 0.0f;
+        const float v1 = in1 ? row[(int64_t)r * cx + x0 + 1] : 0.0f;
+        acc[r] += zw * (v0 + f * (v1 - v0));
+      }
+    }
   }
-  out[((int64_t)z * ny + y) * nx + x] = acc;
+  float m = __builtin_inff();
+  float* o = out + ((int64_t)z * ny + y0) * nx + x;
+#pragma unroll
+  for (int r = 0; r < kSlabRows; ++r)
+    if (r < rows) {
+      o[(int64_t)r * nx] = acc[r];
+      m = fminf(m, acc[r]);
+    }
+  return m;
+}
+
+__global__ void __launch_bounds__(256) sf3d_kernel(float* __restrict__ out,
+                                                   const float* __restrict__ ctfs,
+                                                   const CtfTilt* __restrict__ tilts, int n_tilts,
+                                                   int nx, int ny, int nz, int cx) {
+  __shared__ int list[kSlabListMax];
+  const int z = blockIdx.z;
+  const int nl = slab_list(tilts, n_tilts, nx, nz, z, list);
+  (void)sf3d_column(out, ctfs, tilts, n_tilts, nx, ny, nz, cx, z, list, nl);
 }
 
 __global__ void shift_min_kernel(float* __restrict__ d, int64_t n, const double* __restrict__ dmin,
@@ -164,7 +248,7 @@ void sf3d_impl(emc_ctx* ctx, int half, int sq, const int dims[3], int n,
   dim3 b2(64, 4, 1);
   hipLaunchKernelGGL(ctf2d_kernel, dim3(ceil_div(cx, 64), ceil_div(ny, 4), n), b2, 0, ctx->stream,
                      d_ctf, (const CtfTilt*)d_t, cx, ny, half ? 0 : cx / 2, ny / 2, half, sq, 1, 1);
-  hipLaunchKernelGGL(sf3d_kernel, dim3(ceil_div(nx, 64), ceil_div(ny, 4), nz), b2, 0, ctx->stream,
+  hipLaunchKernelGGL(sf3d_kernel, dim3(ceil_div(nx, 64), ceil_div(ny, 4 * kSlabRows), nz), b2, 0, ctx->stream,
                      d_out, (const float*)d_ctf, (const CtfTilt*)d_t, n, nx, ny, nz, cx);
   EMC_HIP(hipGetLastError());
   // the tilt table lives in pageable memory; finish before it goes away
@@ -223,45 +307,21 @@ std::vector<CtfTilt> weight_mask_tilts(const int dims[3], double sampling, const
 
 // SF3D of many particles of one box size, the tilts of particle p being
 // entries [off[p] - base, off[p+1] - base) of the chunk's tilt table and CTF
-// stack; blockIdx.z = z + nz * p. The same slab sum as sf3d_kernel; each block
-// also writes the minimum of its voxels (part_min[p][block]).
+// stack; blockIdx.z = z + nz * p. The same slab sum as sf3d_kernel; each
+// block also writes the minimum of its voxels (part_min[p][block]).
 __global__ void __launch_bounds__(256) sf3d_many_kernel(float* __restrict__ out, const float* __restrict__ ctfs,
                                                         const CtfTilt* __restrict__ tilts,
                                                         const int* __restrict__ off, int base, int nx, int ny,
                                                         int nz, int cx, float* __restrict__ part_min) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y * blockDim.y + threadIdx.y;
-  const int z = blockIdx.z % nz, p = blockIdx.z / nz;
-  const float ssx = (nx % 2 == 0) ? 0.5f : 0.0f;
-  const float ssz = (nz % 2 == 0) ? 0.5f : 0.0f;
-  const float u = (float)x - (float)nx / 2.0f;   // float division, as mexSF3D.cu:47
-  const float w = (float)z - (float)(nz / 2);
-  const float xscale = (float)cx / (float)nx;
-  const float xoff = 0.5f * (float)cx - 0.5f;
-  const float kPi = 3.14159265358979323f;
-  const int t0 = off[p] - base, t1 = off[p + 1] - base;
-  float acc = 0.0f;
-  const bool inside = x < nx && y < ny;
-  if (inside) {
-    for (int t = t0; t < t1; ++t) {
-      const float s = tilts[t].sin_t, c = tilts[t].cos_t;
-      const float tw = (-u * s + ssz) + w * c;
-      if (tw < -4.0f || tw > 4.0f) continue;
-      const float zw = (0.5f + 0.5f * cosf(tw * (kPi / 4.0f))) * 0.25f;
-      const float xt = (u * c + w * s + ssx) * xscale + xoff;  // texel coordinate
-      const float xf = floorf(xt);
-      const float f = xt - xf;
-      const int x0 = (int)xf;
-      const float* row = ctfs + ((int64_t)t * ny + y) * cx;
-      const float v0 = (unsigned)x0 < (unsigned)cx ? row[x0] : 0.0f;
-      const float v1 = (unsigned)(x0 + 1) < (unsigned)cx ? row[x0 + 1] : 0.0f;
-      acc += zw * (v0 + f * (v1 - v0));
-    }
-    out[(((int64_t)p * nz + z) * ny + y) * nx + x] = acc;
-  }
-  // block minimum: one wave per threadIdx.y row (blockDim = 64 x 4)
+  __shared__ int list[kSlabListMax];
   __shared__ float lds[4];
-  const float m = wave_min(inside ? acc : __builtin_inff());
+  const int z = blockIdx.z % nz, p = blockIdx.z / nz;
+  const int t0 = off[p] - base, t1 = off[p + 1] - base;
+  const int nl = slab_list(tilts + t0, t1 - t0, nx, nz, z, list);
+  float m = sf3d_column(out + (int64_t)p * nz * ny * nx, ctfs + (int64_t)t0 * ny * cx, tilts + t0, t1 - t0, nx,
+                        ny, nz, cx, z, list, nl);
+  // block minimum: one wave per threadIdx.y (blockDim = 64 x 4)
+  m = wave_min(m);
   if (threadIdx.x == 0) lds[threadIdx.y] = m;
   __syncthreads();
   if (threadIdx.x == 0 && threadIdx.y == 0) {
@@ -329,8 +389,8 @@ emc_status emc_weight_mask(emc_ctx* ctx, const int dims[3], double sampling,
     const dim3 b2(64, 4, 1);
     hipLaunchKernelGGL(ctf2d_kernel, dim3(ceil_div(cx, 64), ceil_div(ny, 4), n), b2, 0, ctx->stream, d_ctf,
                        (const CtfTilt*)d_t, cx, ny, cx / 2, ny / 2, 0, 1, 1, 1);
-    hipLaunchKernelGGL(sf3d_kernel, dim3(ceil_div(dims[0], 64), ceil_div(ny, 4), dims[2]), b2, 0, ctx->stream,
-                       d_out, (const float*)d_ctf, (const CtfTilt*)d_t, n, dims[0], ny, dims[2], cx);
+    hipLaunchKernelGGL(sf3d_kernel, dim3(ceil_div(dims[0], 64), ceil_div(ny, 4 * kSlabRows), dims[2]), b2, 0,
+                       ctx->stream, d_out, (const float*)d_ctf, (const CtfTilt*)d_t, n, dims[0], ny, dims[2], cx);
     EMC_HIP(hipGetLastError());
     const int64_t nv = (int64_t)dims[0] * dims[1] * dims[2];
     double* d_min = (double*)scratch(ctx, "wm_min", sizeof(double));
@@ -372,7 +432,7 @@ emc_status emc_weight_mask_batched(emc_ctx* ctx, const int dims[3], double sampl
     const int max_tilts = (int)std::min<size_t>(65535, std::max<size_t>(1, (size_t(256) << 20) / tilt_bytes));
     const int max_parts = std::max(1, 65535 / nz);  // grid z = nz x particles of a chunk
     const dim3 b2(64, 4, 1);
-    const int gx = ceil_div(nx, 64), gy = ceil_div(ny, 4);
+    const int gx = ceil_div(nx, 64), gy = ceil_div(ny, 4 * kSlabRows);
     const int per = gx * gy * nz;  // blocks (minima) per particle
     std::vector<int> cuts = {0};
     int max_nt = 0;
@@ -402,10 +462,16 @@ emc_status emc_weight_mask_batched(emc_ctx* ctx, const int dims[3], double sampl
       const int p0 = cuts[c], p1 = cuts[c + 1];
       const int np = p1 - p0, nt = first[p1] - first[p0];
       const CtfTilt* dt = d_t + first[p0];
-      hipLaunchKernelGGL(ctf2d_kernel, dim3(ceil_div(cx, 64), ceil_div(ny, 4), nt), b2, 0, ctx->stream, d_ctf, dt,
-                         cx, ny, cx / 2, ny / 2, 0, 1, 1, 1);
-      hipLaunchKernelGGL(sf3d_many_kernel, dim3(gx, gy, nz * np), b2, 0, ctx->stream, d_out + p0 * nv,
-                         (const float*)d_ctf, dt, d_off + p0, first[p0], nx, ny, nz, cx, d_part);
+      {
+        KTimer kt(ctx, "ctf2d_many");
+        hipLaunchKernelGGL(ctf2d_kernel, dim3(ceil_div(cx, 64), ceil_div(ny, 4), nt), b2, 0, ctx->stream, d_ctf,
+                           dt, cx, ny, cx / 2, ny / 2, 0, 1, 1, 1);
+      }
+      {
+        KTimer kt(ctx, "sf3d_many");
+        hipLaunchKernelGGL(sf3d_many_kernel, dim3(gx, gy, nz * np), b2, 0, ctx->stream, d_out + p0 * nv,
+                           (const float*)d_ctf, dt, d_off + p0, first[p0], nx, ny, nz, cx, d_part);
+      }
       hipLaunchKernelGGL(fold_min_many_kernel, dim3(np), dim3(kRedThreads), 0, ctx->stream, (const float*)d_part,
                          per, d_mins);
       hipLaunchKernelGGL(shift_min_many_kernel, dim3(ceil_div(nv, 256 * 8), np), dim3(256), 0, ctx->stream,
