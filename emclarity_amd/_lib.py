@@ -104,6 +104,8 @@ SIGNATURES = {
                                 C.c_double, _vp, _vp]),
     "emc_align_eval_grad": (_st, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, C.c_double,
                                   _vp]),
+    "emc_align_eval_grad_noalloc": (_st, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp,
+                                          C.c_double, _vp]),
 }
 
 _LIB = None

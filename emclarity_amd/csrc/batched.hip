@@ -670,8 +670,6 @@ __global__ void twiddle_kernel(int n, int RR, float2* __restrict__ tw) {
 // list is split over gridDim.x workgroups that write FP64 partial sums,
 // folded by adjoint_fold_kernel into [score, d/dphi, d/dtheta, d/dpsi
 // (per radian), d/dsx, d/dsy, d/dsz].
-constexpr int kGradTerms = 11;
-
 template <class Tab>
 __global__ void __launch_bounds__(kBatchThreads) align_eval_grad_kernel(
     const typename Tab::T* __restrict__ F2, int n, const float2* __restrict__ Q, const float* __restrict__ Nw,
@@ -764,18 +762,12 @@ __global__ void __launch_bounds__(kRedThreads) adjoint_fold_kernel(const double*
   }
 }
 
-// Compact support list of the half grid, cached per (n, radius) in the
-// context: x = index in the [kz][kx][ky] layout of Q/N, y = packed signed
-// (kx, ky, kz) + 128 in bytes 0..2 and the Hermitian multiplicity flag
-// (x plane not self-mirrored) in byte 3. DC excluded.
-const int2* support_list(emc_ctx* ctx, int n, double radius, int* count) {
-  char key[64];
-  snprintf(key, sizeof(key), "supp_%d_%.6f", n, radius);
-  auto it = ctx->scratch.find(key);
-  if (it != ctx->scratch.end() && it->second.ptr) {
-    *count = (int)(it->second.bytes / sizeof(int2));  // allocated with the exact size
-    return (const int2*)it->second.ptr;
-  }
+// Compact support list of the half grid for (n, radius): x = index in the
+// [kz][kx][ky] layout of Q/N, y = packed signed (kx, ky, kz) + 128 in bytes
+// 0..2 and the Hermitian multiplicity flag (x plane not self-mirrored) in
+// byte 3. DC excluded; padded to at least 32 entries with DC entries (Q = N
+// = 0 there, so they add nothing).
+std::vector<int2> support_host(int n, double radius) {
   const int cx = n / 2 + 1;
   const double r2 = radius * radius;
   std::vector<int2> v;
@@ -790,9 +782,20 @@ const int2* support_list(emc_ctx* ctx, int n, double radius, int* count) {
         e.y = (kx + 128) | ((ky + 128) << 8) | ((kz + 128) << 16) | (herm << 24);
         v.push_back(e);
       }
-  // pad to >= 32 entries so the scratch allocation is exactly the list size;
-  // padding points at DC, where Q = N = 0, so it adds nothing
   while (v.size() < 32) v.push_back(int2{0, 128 | (128 << 8) | (128 << 16)});
+  return v;
+}
+
+// The support list cached per (n, radius) in the context.
+const int2* support_list(emc_ctx* ctx, int n, double radius, int* count) {
+  char key[64];
+  snprintf(key, sizeof(key), "supp_%d_%.6f", n, radius);
+  auto it = ctx->scratch.find(key);
+  if (it != ctx->scratch.end() && it->second.ptr) {
+    *count = (int)(it->second.bytes / sizeof(int2));  // allocated with the exact size (>= 32 entries)
+    return (const int2*)it->second.ptr;
+  }
+  const std::vector<int2> v = support_host(n, radius);
   *count = (int)v.size();
   int2* d = (int2*)scratch(ctx, key, sizeof(int2) * v.size());
   EMC_HIP(hipMemcpyAsync(d, v.data(), sizeof(int2) * v.size(), hipMemcpyHostToDevice, ctx->stream));
@@ -934,24 +937,22 @@ const int* support_schedule(emc_ctx* ctx, int n, float kmax2, int* ngroups) {
   return d;
 }
 
+// Workgroups per particle of an adjoint evaluation: every particle's support
+// split over enough workgroups to fill the device ("grad_blocks" over the
+// whole batch, 4096 = ~16 per CU by default), at least 256 frequencies each.
+int eval_grad_blocks(emc_ctx* ctx, int n, int n_part, int ns) {
+  const int target = tune_value(ctx, "grad_blocks", n, 4096);
+  return std::max(1, std::min((ns + 255) / 256, (target + n_part - 1) / n_part));
+}
+
 // Score + gradient launch shared by emc_align_eval_grad and the refinement:
 // the support list and the partial-sum buffer are set up first (they may
 // allocate or copy), the launches alone can then be captured into a graph.
-struct EvalSetup {
-  const int2* supp;
-  int ns, g;
-  double* part;
-};
-
 EvalSetup eval_grad_setup(emc_ctx* ctx, int n, int n_part, double support_radius) {
   EvalSetup e;
   e.supp = support_list(ctx, n, support_radius, &e.ns);
   EMC_REQUIRE(e.ns > 0, "empty support");
-  // split every particle's support over enough workgroups to fill the
-  // device ("grad_blocks" over the whole batch, 4096 = ~16 per CU by
-  // default), at least 256 frequencies each
-  const int target = tune_value(ctx, "grad_blocks", n, 4096);
-  e.g = std::max(1, std::min((e.ns + 255) / 256, (target + n_part - 1) / n_part));
+  e.g = eval_grad_blocks(ctx, n, n_part, e.ns);
   e.part = (double*)scratch(ctx, "grad_part", sizeof(double) * kGradTerms * (size_t)e.g * n_part);
   return e;
 }

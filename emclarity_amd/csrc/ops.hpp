@@ -83,6 +83,31 @@ __global__ void __launch_bounds__(kWave) xcf_rot_shell_fold_kernel(const double*
                                                                   int nb, int abs_flag, double w,
                                                                   double* __restrict__ out);
 
+// ---------------------------------------------------------------- adjoint
+
+// Terms of an adjoint evaluation's partial sums per workgroup: num, d num /
+// d s (3), D, d num / d angle (3), d D / d angle (3).
+constexpr int kGradTerms = 11;
+
+// Support list of the half grid for (n, radius) (host memory, the order the
+// adjoint walks); see batched.hip.
+std::vector<int2> support_host(int n, double radius);
+
+// Device-side inputs of adjoint launches: the support list (ns entries), the
+// workgroups per particle (g) and the partial sums (n_part x g x kGradTerms).
+struct EvalSetup {
+  const int2* supp;
+  int ns, g;
+  double* part;
+};
+
+int eval_grad_blocks(emc_ctx* ctx, int n, int n_part, int ns);
+// align_eval_grad_kernel + adjoint_fold_kernel on the context's stream (no
+// allocation, copy or host wait); timed adds HIP events around the kernel
+void eval_grad_launch(emc_ctx* ctx, const EvalSetup& e, int n, int n_part, int table_format, const void* d_F2,
+                      const void* d_Q, const float* d_N, const double* d_A, const float* d_MdM, const float* d_shift,
+                      double* d_out, bool timed);
+
 }  // namespace emc
 
 struct emc_shell_plan {

@@ -434,6 +434,21 @@ emc_status emc_align_eval_grad(emc_ctx* ctx, int n, int n_part, int table_format
                                const float* d_N, const double* d_A, const float* d_MdM,
                                const float* d_shift, double support_radius, double* d_out);
 
+/* emc_align_eval_grad without allocating: no counterpart in the reference. For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Same arguments and
+ * results (bit for bit: the same kernels, support order and workgroup split);
+ * the support list and the partial sums live in a static 16 MiB device array
+ * per GPU (held until the process ends) instead of context scratch. Rejects a
+ * call whose support list plus n_part x workgroups x 88 bytes of partial sums
+ * exceed 16 MiB (EMC_ERR_INVALID). Calls on one GPU share the array: calls that
+ * overlap on one GPU (two threads, or two contexts' streams) race. A new
+ * (n, support_radius) copies its support list into the array and waits for
+ * the copy; the same pair as the previous call on that GPU reuses it. */
+emc_status emc_align_eval_grad_noalloc(emc_ctx* ctx, int n, int n_part, int table_format, const void* d_F2,
+                                       const void* d_Q, const float* d_N, const double* d_A, const float* d_MdM,
+                                       const float* d_shift, double support_radius, double* d_out);
+
 /* Gradient refinement of one pose per particle, replacing the dense local
  * refinement grid of alignment/BH_alignRaw3d_v2.m:1448-1644: iters steps of
  * batched quasi-Newton ascent of the CC over a right-multiplied rotation
