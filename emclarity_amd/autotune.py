@@ -1,6 +1,6 @@
 """Autotuner of the batched search's launch shapes.
 
-Four parameters shape the batched path's hot launches (emc_tune_set):
+Six parameters shape the batched path's hot launches (emc_tune_set):
   search_tile  kx extent of the (kx, ky) column patches of the search
                schedule (ky extent 256 / search_tile): the shape of the
                reference-table region the units of one pass read together
@@ -9,8 +9,11 @@ Four parameters shape the batched path's hot launches (emc_tune_set):
                each with its own y/x transform of the partial sums)
   search_passes  passes of that schedule per launch: they share the launch's
                last, partly filled round of workgroups
-  grad_blocks  workgroups an adjoint evaluation aims for: how finely each
-               particle's support is split (fill against fold work)
+  grad_kernel  adjoint evaluation order: brick-major (1) or support-list (0)
+  grad_brick   brick side of the brick-major adjoint (16 or 32)
+  grad_blocks  workgroups a support-list adjoint evaluation aims for: how
+               finely each particle's support is split (fill against fold
+               work)
 
 ``python -m emclarity_amd.autotune [--out PATH]`` times every candidate on
 the device with HIP events on the library's stream, for each box size of the
@@ -37,7 +40,9 @@ import numpy as np
 from . import _lib
 
 CANDIDATES = {"search_tile": [4, 8, 16, 32, 64], "search_zsplit": [1, 2, 3, 4], "search_passes": [1, 2, 3, 4],
+              "grad_kernel": [0, 1], "grad_brick": [16, 32],
               "grad_blocks": [2048, 4096, 8192, 16384, 32768, 65536]}
+GRAD = ("grad_kernel", "grad_brick", "grad_blocks")
 TABLE = Path(__file__).resolve().parent / "tuning" / "mi355x.json"
 _CACHE: dict | None = None
 
@@ -90,46 +95,53 @@ def _timed(ctx, fn, names, reps):
     return ms
 
 
-def tune(ctx, counts=((64, 1000), (96, 600), (128, 400)), tables=("c64", "f16"), reps=3, seed=7, pool=(50, 30, 20),
+def tune(ctx, counts=((64, 1000), (96, 600), (128, 400)), tables=("c64", "f16"), reps=3, seed=7,
          search_frac=0.6, log=sys.stderr):
     """Time every candidate; returns the table (params + all timings, ms).
-    Per box size, ``pool`` particles are rendered and the rest of ``count``
-    are copies of them with fresh noise and new starts (as the bench's
-    2,000-particle step)."""
+    Per box size ``count`` particles, each with its own random start
+    orientation (the launches' work and table access pattern depend on the
+    box size, the support and the rotations, not on the voxel values: noise
+    volumes stand in for the particles)."""
     from .batched import BatchAligner
+    from .device import DeviceArray
     from .geometry import eulerSearch
-    from .workload import copy_particle, make_particle, make_size_class
+    from .workload import make_size_class, random_rotation
     grid = eulerSearch("C1", 15, 10, 15, 10)
     out = {"device": "gfx950",
-           "workload": "batched alignment, 28 coarse orientations (10 deg), SNR 0.1, "
+           "workload": "batched alignment, 28 coarse orientations (10 deg) over 0.6 x the support radius, "
+                       "refinement evaluations over the full support, distinct random orientations, "
                        + ", ".join(f"{c} x {n}^3" for n, c in counts),
            "candidates": CANDIDATES, "params": {}, "timings_ms": {}}
     rng = np.random.default_rng(seed)
-    for (n, count), k in zip(counts, pool):
-        sc, blobs = make_size_class(n)
-        parts = [make_particle(rng, sc, blobs, keep_signal=True) for _ in range(min(k, count))]
-        parts += [copy_particle(rng, parts[i % len(parts)]) for i in range(count - len(parts))]
+    for n, count in counts:
+        sc, _ = make_size_class(n)
+        data = DeviceArray.from_numpy(rng.standard_normal((count, n, n, n), dtype=np.float32), ctx=ctx)
+        views = [data.view((n, n, n), np.float32, 4 * n ** 3 * i) for i in range(count)]
+        starts = np.stack([random_rotation(rng) for _ in range(count)])
         for table in tables:
             al = BatchAligner(n, sc.ref, sc.bandpass, table=table, ctx=ctx, tuning={})
-            al.set_particles([p.data for p in parts], [None] * count, [p.start for p in parts])
-            R = np.stack([p.start for p in parts])
-            s = np.zeros((count, 3))
+            al.set_particles(views, [None] * count, starts)
+            Rd = DeviceArray.from_numpy(np.ascontiguousarray(starts.reshape(count, 9)), ctx=ctx)
+            sd = DeviceArray.zeros((count, 3), np.float64, ctx)
+            _lib.call("emc_tune_set", ctx.handle, b"refine_graph", n, 0)   # eager: events per evaluation
             best, times = {}, {}
-            for name in ("search_tile", "search_zsplit", "search_passes", "grad_blocks"):
+            for name in ("search_tile", "search_zsplit", "search_passes", *GRAD):
                 times[name] = {}
                 for v in CANDIDATES[name]:
                     apply(ctx, n, {name: v})
-                    if name != "grad_blocks":
+                    if name not in GRAD:
                         ms = _timed(ctx, lambda: al.search(grid, search_frac * al.support),
                                     ("align_fused", "align_final"), reps)
-                    else:
-                        ms = _timed(ctx, lambda: al._eval_local(R, s), ("align_eval_grad",), reps)
+                    else:   # the refinement's evaluations (three per call)
+                        ms = _timed(ctx, lambda: al.refine_device(Rd, sd, iters=2), ("align_eval_grad",), reps)
                     times[name][str(v)] = round(ms, 4)
                     print(f"[autotune] n={n} {table} {name}={v}: {ms:.3f} ms", file=log, flush=True)
                 best[name] = int(min(CANDIDATES[name], key=lambda v: times[name][str(v)]))
                 apply(ctx, n, {name: best[name]})
             out["params"].setdefault(table, {})[str(n)] = best
             out["timings_ms"].setdefault(table, {})[str(n)] = times
+            del al
+        del views, data
     return out
 
 

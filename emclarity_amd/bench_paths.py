@@ -7,6 +7,7 @@ batched-alignment and template-search workloads:
   tsprep   pre-processing of one 480^3 template-search chunk
   tsearch  one 480^3 chunk against a 48^3 template, 32 orientations
   adjoint  one adjoint evaluation of 1,000 / 600 / 400 particles of 64 / 96 / 128^3
+  refine   the refinement's adjoint evaluations on the same size classes
   search   the 28-orientation coarse search of the same size classes
   copy     a 1 GiB device-to-device copy (byte-counter calibration)
 
@@ -235,6 +236,37 @@ def bench_adjoint(ctx, reps, table="c64"):
         del al
 
 
+def bench_refine(ctx, reps, table="c64", iters=8):
+    """The refinement's evaluations: emc_align_refine with eager launches
+    (HIP events per adjoint kernel) over the full support from each
+    particle's start pose; the kernel and brick side from the tuning table
+    unless REFINE_KERNEL / REFINE_BRICK say otherwise."""
+    import os
+    kern = os.environ.get("REFINE_KERNEL")
+    brick = os.environ.get("REFINE_BRICK")
+    for k, (n, count) in enumerate(CLASSES):
+        al, starts = _noise_class(ctx, n, count, table, 80 + k)
+        al._apply_tuning()
+        _lib.call("emc_tune_set", ctx.handle, b"refine_graph", n, 0)
+        if kern is not None:
+            _lib.call("emc_tune_set", ctx.handle, b"grad_kernel", n, int(kern))
+        if brick is not None:
+            _lib.call("emc_tune_set", ctx.handle, b"grad_brick", n, int(brick))
+        R = DeviceArray.from_numpy(np.ascontiguousarray(starts.reshape(count, 9)), ctx=ctx)
+        s = DeviceArray.zeros((count, 3), np.float64, ctx)
+        wall = _timed(ctx, lambda: al.refine_device(R, s, iters=iters), reps)
+        ms, cnt = _ktime(ctx, "align_eval_grad")
+        per = ms / max(cnt, 1)
+        b, samples = eval_grad_bytes(n, count, al.support, table)
+        v = C.c_int()
+        _lib.call("emc_tune_get", ctx.handle, b"grad_kernel", n, C.byref(v))
+        _line("refine", wall, {"kernel": f"adjoint<{table}, {'brick' if v.value else 'list'}>", "n": n,
+                               "particles": count, "evals_per_call": round(cnt / reps, 2), "kernel_ms": round(per, 4),
+                               "bytes": b, "GBps": round(b / (per * 1e-3) / 1e9, 1),
+                               "gsamples_per_s": round(samples / (per * 1e-3) / 1e9, 2)})
+        del al
+
+
 def bench_search(ctx, reps, table="c64", frac=0.6):
     """The coarse search of a size class: 28 orientations of the 10-degree
     local grid scored over 0.6 x the support radius (align_pass_kernel +
@@ -268,7 +300,8 @@ def bench_copy(ctx, reps, mib=1024):
 
 
 PATHS = {"extract": bench_extract, "average": bench_average, "peaks": bench_peaks, "tsprep": bench_tsprep,
-         "tsearch": bench_tsearch, "adjoint": bench_adjoint, "search": bench_search, "copy": bench_copy}
+         "tsearch": bench_tsearch, "adjoint": bench_adjoint, "refine": bench_refine, "search": bench_search,
+         "copy": bench_copy}
 
 
 def main(argv=None):
@@ -281,7 +314,7 @@ def main(argv=None):
     ctx = default_context()
     for name in (PATHS if a.path == "all" else [a.path]):
         print(f"[bench_paths] {name}", file=sys.stderr, flush=True)
-        if name in ("adjoint", "search"):
+        if name in ("adjoint", "refine", "search"):
             PATHS[name](ctx, a.reps, table=a.table)
         else:
             PATHS[name](ctx, a.reps)

@@ -741,6 +741,253 @@ __global__ void __launch_bounds__(kBatchThreads) align_eval_grad_kernel(
   }
 }
 
+// The same score and gradient, brick-major. The samples j = M k of every
+// particle cover the same ball of the table whatever its rotation, so the
+// table is cut into bricks of S^3 positions (the half ball jx >= 0, |j| <=
+// 2 r; brick list from brick_host) and one workgroup takes one (brick,
+// particle): it enumerates the integer k of the full support ball with M k
+// inside its brick (S b <= j < S (b + 1) per axis: the partition of the
+// samples) and sums their terms. The full ball with the jx >= 0 half
+// weighted 2 (1 on jx = 0) equals the half-grid sum with its Hermitian
+// multiplicities: every term is the real part of a Hermitian-symmetric
+// product, and k, -k have j, -j. Q and N of k with kx < 0 are conj(Q(-k)),
+// N(-k).
+// Workgroups are dealt to the XCDs round-robin; XCD x walks bricks x, x + 8,
+// x + 16, ..., each for every particle in turn, so the table region of one
+// brick (S^3 entries, 1.1 MB complex64 at S = 32) stays in that XCD's L2
+// while all particles read it.
+// Enumeration: the brick is the rotated cube k = M^T j / 4 (M = 2 R^T); its
+// axis-aligned bounding box in k is cut into (kz, kx) rows, each with the ky
+// interval of the cube and the ball (widened by 1e-3: the exact membership
+// test on the sample's own j decides), the row's phase e^{2 pi i (kx sx +
+// kz sz) / n} and an exclusive scan of the row lengths in LDS. Each wave
+// takes a contiguous quarter of the items, 64 consecutive items per step
+// (consecutive lanes along ky: Q and N read coalesced), each lane finding
+// its row by a short forward scan from the step's first row.
+// GEN: the derivative matrices are those of right-multiplied rotation
+// increments, dM_a = G_a^T M (the refinement's), so dj_a = G_a^T j is a
+// cross product with the axis and the three dM are not read.
+template <int S>
+struct BrickRows {
+  static constexpr int kMax = (S * 7 / 8 + 3) * (S * 7 / 8 + 3);  // rows of one pass
+};
+
+template <int S, class Tab, bool GEN>
+__global__ void __launch_bounds__(kBatchThreads) align_eval_brick_kernel(
+    const typename Tab::T* __restrict__ F2, int n, const float2* __restrict__ Q, const float* __restrict__ Nw,
+    const float* __restrict__ MdM, const float* __restrict__ shifts, const int4* __restrict__ bricks, int nb,
+    int n_part, int brick_slots, double r2, double* __restrict__ part) {
+  constexpr int kRows = BrickRows<S>::kMax;
+  constexpr int kNW = kBatchThreads / kWave;
+  __shared__ float Ms[36];
+  __shared__ int row_key[kRows];    // kz + 128 | (kx + 128) << 8 | (lo + 128) << 16
+  __shared__ int row_off[kRows + 1];
+  __shared__ float2 row_ph[kRows];  // e^{2 pi i (kx sx + kz sz) / n}
+  __shared__ float2 ph_y[128];      // e^{2 pi i ky sy / n}, ky + 64
+  __shared__ int wsum[kNW];
+  // XCD-aware (brick, particle): block L goes to XCD L % 8
+  const int L = blockIdx.x, xcd = L % 8, q = L / 8;
+  const int p = q % n_part, slot = q / n_part;
+  const int b = slot * 8 + xcd;
+  if (slot >= brick_slots || b >= nb) return;  // uniform per block
+  const int t = threadIdx.x, lane = t % kWave, wave = t / kWave;
+  if (t < (GEN ? 9 : 36)) Ms[t] = MdM[36 * (int64_t)p + t];
+  const float s0 = shifts[3 * p], s1 = shifts[3 * p + 1], s2 = shifts[3 * p + 2];
+  const float inv_n2 = 2.0f / (float)n, tp = 6.283185307179586f / (float)n;
+  if (t < 128) {
+    float sn, cs;
+    sincospif((float)(t - 64) * s1 * inv_n2, &sn, &cs);
+    ph_y[t] = make_float2(cs, sn);
+  }
+  __syncthreads();
+  const int4 bi = bricks[b];
+  const float j0x = (float)(bi.x * S), j0y = (float)(bi.y * S), j0z = (float)(bi.z * S);
+  const float j1x = j0x + (float)S, j1y = j0y + (float)S, j1z = j0z + (float)S;
+  const float m00 = Ms[0], m01 = Ms[1], m02 = Ms[2], m10 = Ms[3], m11 = Ms[4], m12 = Ms[5], m20 = Ms[6],
+              m21 = Ms[7], m22 = Ms[8];
+  // bounding box of the rotated brick in k: k_i = sum_a M_ai j_a / 4
+  const float hS = 0.5f * (float)S;
+  const float jcx = j0x + hS, jcy = j0y + hS, jcz = j0z + hS;
+  const float kcx = 0.25f * (m00 * jcx + m10 * jcy + m20 * jcz);
+  const float kcy = 0.25f * (m01 * jcx + m11 * jcy + m21 * jcz);
+  const float kcz = 0.25f * (m02 * jcx + m12 * jcy + m22 * jcz);
+  const float hkx = 0.25f * hS * (fabsf(m00) + fabsf(m10) + fabsf(m20));
+  const float hky = 0.25f * hS * (fabsf(m01) + fabsf(m11) + fabsf(m21));
+  const float hkz = 0.25f * hS * (fabsf(m02) + fabsf(m12) + fabsf(m22));
+  const int ir = (int)sqrt(r2);
+  const int ir2 = (int)floor(r2);  // k2 <= r2 for integer k2
+  const int kx0 = max((int)ceilf(kcx - hkx - 1e-3f), -ir), kx1 = min((int)floorf(kcx + hkx + 1e-3f), ir);
+  const int ky0 = max((int)ceilf(kcy - hky - 1e-3f), -ir), ky1 = min((int)floorf(kcy + hky + 1e-3f), ir);
+  const int kz0 = max((int)ceilf(kcz - hkz - 1e-3f), -ir), kz1 = min((int)floorf(kcz + hkz + 1e-3f), ir);
+  const int nxr = kx1 - kx0 + 1, nzr = kz1 - kz0 + 1;
+  const int nrows = (nxr > 0 && nzr > 0 && ky1 >= ky0) ? nxr * nzr : 0;
+  // ky interval of a row from the three slabs a0 <= c + m ky < a0 + S
+  const bool fx = fabsf(m01) > 1e-6f, fy = fabsf(m11) > 1e-6f, fz = fabsf(m21) > 1e-6f;
+  const float ix = fx ? 1.0f / m01 : 0.0f, iy = fy ? 1.0f / m11 : 0.0f, iz = fz ? 1.0f / m21 : 0.0f;
+  const int cx = n / 2 + 1;
+  const int64_t nh = (int64_t)cx * n * n;
+  const float2* Qp = Q + (int64_t)p * nh;
+  const float* Np = Nw + (int64_t)p * nh;
+  float v[kGradTerms];
+#pragma unroll
+  for (int i = 0; i < kGradTerms; ++i) v[i] = 0.0f;
+  for (int r0 = 0; r0 < nrows; r0 += kRows) {  // uniform
+    const int rn = min(kRows, nrows - r0);
+    constexpr int kPer = (kRows + kBatchThreads - 1) / kBatchThreads;
+    int len[kPer];
+    int local = 0;
+#pragma unroll
+    for (int c = 0; c < kPer; ++c) {
+      const int r = t * kPer + c;
+      len[c] = 0;
+      if (r >= rn) continue;
+      const int rr_ = r0 + r;
+      const int kz = kz0 + rr_ / nxr, kx = kx0 + rr_ % nxr;
+      const int rem = ir2 - (kx * kx + kz * kz);
+      int l = 0, ilo = 0;
+      if (rem >= 0) {
+        const float sr = sqrtf((float)rem);
+        float lo = fmaxf((float)ky0, -sr - 1e-3f), hi = fminf((float)ky1, sr + 1e-3f);
+        const float fkx = (float)kx, fkz = (float)kz;
+        auto clip = [&](bool f, float inv, float c0, float a0, float a1) {
+          if (f) {
+            float ya = (a0 - c0) * inv, yb = (a1 - c0) * inv;
+            if (ya > yb) { const float tmp = ya; ya = yb; yb = tmp; }
+            lo = fmaxf(lo, ya - 1e-3f);
+            hi = fminf(hi, yb + 1e-3f);
+          } else if (c0 < a0 - 1e-3f || c0 > a1 + 1e-3f) {
+            hi = lo - 1.0f;
+          }
+        };
+        clip(fx, ix, m00 * fkx + m02 * fkz, j0x, j1x);
+        clip(fy, iy, m10 * fkx + m12 * fkz, j0y, j1y);
+        clip(fz, iz, m20 * fkx + m22 * fkz, j0z, j1z);
+        ilo = (int)ceilf(lo);
+        const int ihi = (int)floorf(hi);
+        l = ihi >= ilo ? ihi - ilo + 1 : 0;
+        if (l > 0) {
+          float sn, cs;
+          sincospif((fkx * s0 + fkz * s2) * inv_n2, &sn, &cs);
+          row_ph[r] = make_float2(cs, sn);
+        }
+      }
+      row_key[r] = (kz + 128) | ((kx + 128) << 8) | ((ilo + 128) << 16);
+      len[c] = l;
+      local += l;
+    }
+    // exclusive scan of the row lengths (rows t*kPer .. t*kPer + kPer - 1 per thread)
+    int incl = local;
+    for (int o = 1; o < kWave; o <<= 1) {
+      const int y = __shfl_up(incl, o, kWave);
+      if (lane >= o) incl += y;
+    }
+    if (lane == kWave - 1) wsum[wave] = incl;
+    __syncthreads();
+    int base = 0;
+    for (int w = 0; w < wave; ++w) base += wsum[w];
+    int run = base + incl - local;
+#pragma unroll
+    for (int c = 0; c < kPer; ++c) {
+      const int r = t * kPer + c;
+      if (r < rn) row_off[r] = run;
+      run += len[c];
+    }
+    int total = 0;
+    for (int w = 0; w < kNW; ++w) total += wsum[w];
+    if (t == 0) row_off[rn] = total;
+    __syncthreads();
+    // each wave a contiguous quarter of the items; its first row by bisection
+    const int per = (total + kNW - 1) / kNW;
+    const int ibeg = min(total, wave * per), iend = min(total, ibeg + per);
+    int rw = 0;
+    if (ibeg < iend) {
+      int a = 0, bnd = rn;
+      while (bnd - a > 1) {
+        const int mid = (a + bnd) >> 1;
+        if (row_off[mid] <= ibeg) a = mid; else bnd = mid;
+      }
+      rw = a;
+    }
+    for (int i0 = ibeg; i0 < iend; i0 += kWave) {  // uniform over the wave
+      const int i = i0 + lane;
+      int r = rw;
+      const int ic = min(i, iend - 1);
+      while (row_off[r + 1] <= ic) ++r;
+      rw = __shfl(r, kWave - 1, kWave);
+      if (i >= iend) continue;
+      const int key = row_key[r];
+      const int kz = (key & 255) - 128, kx = ((key >> 8) & 255) - 128;
+      const int ky = ((key >> 16) & 255) - 128 + (i - row_off[r]);
+      const int k2 = kx * kx + ky * ky + kz * kz;
+      if (k2 > ir2 || k2 == 0) continue;
+      const float fk0 = (float)kx, fk1 = (float)ky, fk2 = (float)kz;
+      const float jx = m00 * fk0 + m01 * fk1 + m02 * fk2;
+      const float jy = m10 * fk0 + m11 * fk1 + m12 * fk2;
+      const float jz = m20 * fk0 + m21 * fk1 + m22 * fk2;
+      if (!(jx >= j0x && jx < j1x && jy >= j0y && jy < j1y && jz >= j0z && jz < j1z)) continue;
+      const float m = jx > 0.0f ? 2.0f : 1.0f;
+      // Q, N at k (the stored half grid holds kx >= 0; conj(Q(-k)) otherwise);
+      // |k| <= r < n/2 keeps the index inside the particle's arrays
+      const bool neg = kx < 0;
+      const int ax = neg ? -kx : kx, ay = neg ? -ky : ky, az = neg ? -kz : kz;
+      const int64_t idx = ((int64_t)(az < 0 ? az + n : az) * cx + ax) * n + (ay < 0 ? ay + n : ay);
+      float2 qv = Qp[idx];
+      if (neg) qv.y = -qv.y;
+      const float w = Np[idx];
+      const Tri tr = f2_sample<true, Tab>(F2, n, jx, jy, jz);
+      const float2 pr = row_ph[r], py = ph_y[ky + 64];
+      const float cs = pr.x * py.x - pr.y * py.y, sn = pr.x * py.y + pr.y * py.x;
+      const float xr = qv.x * tr.f.x + qv.y * tr.f.y;
+      const float xi = qv.y * tr.f.x - qv.x * tr.f.y;
+      const float yr = xr * cs - xi * sn, yi = xr * sn + xi * cs;
+      v[0] += m * yr;
+      const float myi = m * yi * tp;
+      v[1] -= myi * fk0;
+      v[2] -= myi * fk1;
+      v[3] -= myi * fk2;
+      v[4] += m * (tr.f.x * tr.f.x + tr.f.y * tr.f.y) * w;
+#pragma unroll
+      for (int aa = 0; aa < 3; ++aa) {
+        float dfr, dfi;
+        if (GEN) {
+          // dj_x = (0, jz, -jy), dj_y = (-jz, 0, jx), dj_z = (jy, -jx, 0)
+          if (aa == 0) {
+            dfr = tr.g[1].x * jz - tr.g[2].x * jy;
+            dfi = tr.g[1].y * jz - tr.g[2].y * jy;
+          } else if (aa == 1) {
+            dfr = tr.g[2].x * jx - tr.g[0].x * jz;
+            dfi = tr.g[2].y * jx - tr.g[0].y * jz;
+          } else {
+            dfr = tr.g[0].x * jy - tr.g[1].x * jx;
+            dfi = tr.g[0].y * jy - tr.g[1].y * jx;
+          }
+        } else {
+          const float* dM = Ms + 9 * (aa + 1);
+          const float dj0 = dM[0] * fk0 + dM[1] * fk1 + dM[2] * fk2;
+          const float dj1 = dM[3] * fk0 + dM[4] * fk1 + dM[5] * fk2;
+          const float dj2 = dM[6] * fk0 + dM[7] * fk1 + dM[8] * fk2;
+          dfr = tr.g[0].x * dj0 + tr.g[1].x * dj1 + tr.g[2].x * dj2;
+          dfi = tr.g[0].y * dj0 + tr.g[1].y * dj1 + tr.g[2].y * dj2;
+        }
+        const float zr = qv.x * dfr + qv.y * dfi;
+        const float zi = qv.y * dfr - qv.x * dfi;
+        v[5 + aa] += m * (zr * cs - zi * sn);
+        v[8 + aa] += m * 2.0f * (tr.f.x * dfr + tr.f.y * dfi) * w;
+      }
+    }
+    __syncthreads();  // rows of the next pass reuse the LDS arrays
+  }
+  double d[kGradTerms];
+#pragma unroll
+  for (int i = 0; i < kGradTerms; ++i) d[i] = (double)v[i];
+  block_sum<kGradTerms>(d);
+  if (t == 0) {
+    double* o = part + ((int64_t)p * nb + b) * kGradTerms;
+#pragma unroll
+    for (int i = 0; i < kGradTerms; ++i) o[i] = d[i];
+  }
+}
+
 __global__ void __launch_bounds__(kRedThreads) adjoint_fold_kernel(const double* __restrict__ part, int g,
                                                                    const double* __restrict__ A,
                                                                    double* __restrict__ out) {
@@ -945,29 +1192,115 @@ int eval_grad_blocks(emc_ctx* ctx, int n, int n_part, int ns) {
   return std::max(1, std::min((ns + 255) / 256, (target + n_part - 1) / n_part));
 }
 
+// Bricks of side S (table index units) covering the half ball jx >= 0,
+// |j| <= 2 radius the samples j = M k of the support reach: int4 {bx, by,
+// bz, 0} brick indices, z-major.
+std::vector<int4> brick_host(int n, double radius, int S) {
+  (void)n;
+  const double R = 2.0 * radius + 1e-3;
+  const int nbr = (int)std::ceil(R / S);
+  auto gap = [&](int b) {  // distance from 0 to the interval [b S, b S + S]
+    const double lo = (double)b * S, hi = lo + S;
+    return lo > 0.0 ? lo : (hi < 0.0 ? -hi : 0.0);
+  };
+  std::vector<int4> v;
+  for (int bz = -nbr; bz < nbr; ++bz)
+    for (int by = -nbr; by < nbr; ++by)
+      for (int bx = 0; bx < nbr; ++bx) {
+        const double dx = gap(bx), dy = gap(by), dz = gap(bz);
+        if (dx * dx + dy * dy + dz * dz <= R * R) v.push_back(int4{bx, by, bz, 0});
+      }
+  return v;
+}
+
+// The brick list cached per (radius, S) in the context.
+static const int4* brick_list(emc_ctx* ctx, int n, double radius, int S, int* count) {
+  char key[64];
+  snprintf(key, sizeof(key), "bricks_%d_%.6f_%d", n, radius, S);
+  auto it = ctx->scratch.find(key);
+  auto mt = ctx->meta.find(key);
+  if (it != ctx->scratch.end() && it->second.ptr && mt != ctx->meta.end()) {
+    *count = mt->second;
+    return (const int4*)it->second.ptr;
+  }
+  const std::vector<int4> v = brick_host(n, radius, S);
+  EMC_REQUIRE(!v.empty(), "empty brick list");
+  int4* d = (int4*)scratch(ctx, key, sizeof(int4) * v.size());
+  EMC_HIP(hipMemcpyAsync(d, v.data(), sizeof(int4) * v.size(), hipMemcpyHostToDevice, ctx->stream));
+  EMC_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->meta[key] = (int)v.size();
+  *count = (int)v.size();
+  return d;
+}
+
+// Which adjoint kernel ("grad_kernel": 1 brick-major, 0 support-list order)
+// and the brick side ("grad_brick": 16 or 32).
+void eval_grad_kind(emc_ctx* ctx, int n, int* kind, int* S) {
+  *kind = tune_value(ctx, "grad_kernel", n, 1) ? 1 : 0;
+  *S = tune_value(ctx, "grad_brick", n, 32) == 16 ? 16 : 32;
+}
+
 // Score + gradient launch shared by emc_align_eval_grad and the refinement:
-// the support list and the partial-sum buffer are set up first (they may
-// allocate or copy), the launches alone can then be captured into a graph.
+// the support (or brick) list and the partial-sum buffer are set up first
+// (they may allocate or copy), the launches alone can then be captured into
+// a graph.
 EvalSetup eval_grad_setup(emc_ctx* ctx, int n, int n_part, double support_radius) {
-  EvalSetup e;
-  e.supp = support_list(ctx, n, support_radius, &e.ns);
-  EMC_REQUIRE(e.ns > 0, "empty support");
-  e.g = eval_grad_blocks(ctx, n, n_part, e.ns);
+  EvalSetup e{};
+  eval_grad_kind(ctx, n, &e.kind, &e.S);
+  e.r2 = support_radius * support_radius;
+  if (e.kind == 1) {
+    e.list = brick_list(ctx, n, support_radius, e.S, &e.ns);
+    e.g = e.ns;
+  } else {
+    e.list = support_list(ctx, n, support_radius, &e.ns);
+    EMC_REQUIRE(e.ns > 0, "empty support");
+    e.g = eval_grad_blocks(ctx, n, n_part, e.ns);
+  }
   e.part = (double*)scratch(ctx, "grad_part", sizeof(double) * kGradTerms * (size_t)e.g * n_part);
   return e;
 }
 
+template <class Tab, bool GEN>
+static void launch_brick(emc_ctx* ctx, const EvalSetup& e, int n, int n_part, const void* d_F2, const void* d_Q,
+                         const float* d_N, const float* d_MdM, const float* d_shift) {
+  using T = typename Tab::T;
+  const int slots = ceil_div(e.ns, 8);
+  const dim3 grid((unsigned)(8 * slots * n_part));
+  if (e.S == 16)
+    hipLaunchKernelGGL((align_eval_brick_kernel<16, Tab, GEN>), grid, dim3(kBatchThreads), 0, ctx->stream,
+                       (const T*)d_F2, n, (const float2*)d_Q, d_N, d_MdM, d_shift, (const int4*)e.list, e.ns, n_part,
+                       slots, e.r2, e.part);
+  else
+    hipLaunchKernelGGL((align_eval_brick_kernel<32, Tab, GEN>), grid, dim3(kBatchThreads), 0, ctx->stream,
+                       (const T*)d_F2, n, (const float2*)d_Q, d_N, d_MdM, d_shift, (const int4*)e.list, e.ns, n_part,
+                       slots, e.r2, e.part);
+}
+
+template <class Tab>
+static void launch_eval_kernel(emc_ctx* ctx, const EvalSetup& e, int n, int n_part, const void* d_F2,
+                               const void* d_Q, const float* d_N, const float* d_MdM, const float* d_shift,
+                               bool gen) {
+  using T = typename Tab::T;
+  if (e.kind == 1) {
+    if (gen)
+      launch_brick<Tab, true>(ctx, e, n, n_part, d_F2, d_Q, d_N, d_MdM, d_shift);
+    else
+      launch_brick<Tab, false>(ctx, e, n, n_part, d_F2, d_Q, d_N, d_MdM, d_shift);
+  } else {
+    hipLaunchKernelGGL(align_eval_grad_kernel<Tab>, dim3(n_part, e.g), dim3(kBatchThreads), 0, ctx->stream,
+                       (const T*)d_F2, n, (const float2*)d_Q, d_N, d_MdM, d_shift, (const int2*)e.list, e.ns, e.part);
+  }
+}
+
 void eval_grad_launch(emc_ctx* ctx, const EvalSetup& e, int n, int n_part, int table_format, const void* d_F2,
                       const void* d_Q, const float* d_N, const double* d_A, const float* d_MdM, const float* d_shift,
-                      double* d_out, bool timed) {
+                      double* d_out, bool timed, bool gen) {
   {
     KTimer kt(timed ? ctx : nullptr, "align_eval_grad");
     if (table_format == EMC_TABLE_C64)
-      hipLaunchKernelGGL(align_eval_grad_kernel<TabC64>, dim3(n_part, e.g), dim3(kBatchThreads), 0, ctx->stream,
-                         (const float2*)d_F2, n, (const float2*)d_Q, d_N, d_MdM, d_shift, e.supp, e.ns, e.part);
+      launch_eval_kernel<TabC64>(ctx, e, n, n_part, d_F2, d_Q, d_N, d_MdM, d_shift, gen);
     else
-      hipLaunchKernelGGL(align_eval_grad_kernel<TabF16>, dim3(n_part, e.g), dim3(kBatchThreads), 0, ctx->stream,
-                         (const half2_t*)d_F2, n, (const float2*)d_Q, d_N, d_MdM, d_shift, e.supp, e.ns, e.part);
+      launch_eval_kernel<TabF16>(ctx, e, n, n_part, d_F2, d_Q, d_N, d_MdM, d_shift, gen);
   }
   hipLaunchKernelGGL(adjoint_fold_kernel, dim3(n_part), dim3(kRedThreads), 0, ctx->stream, (const double*)e.part,
                      e.g, d_A, d_out);
@@ -978,7 +1311,7 @@ void eval_grad_impl(emc_ctx* ctx, int n, int n_part, int table_format, const voi
                     const float* d_N, const double* d_A, const float* d_MdM, const float* d_shift,
                     double support_radius, double* d_out) {
   const EvalSetup e = eval_grad_setup(ctx, n, n_part, support_radius);
-  eval_grad_launch(ctx, e, n, n_part, table_format, d_F2, d_Q, d_N, d_A, d_MdM, d_shift, d_out, true);
+  eval_grad_launch(ctx, e, n, n_part, table_format, d_F2, d_Q, d_N, d_A, d_MdM, d_shift, d_out, true, false);
 }
 
 // ------------------------------------------------------------ refinement
@@ -1350,10 +1683,10 @@ emc_status emc_align_refine(emc_ctx* ctx, int n, int n_part, int table_format, c
     auto body = [&](bool timed) {
       hipLaunchKernelGGL(refine_begin_kernel, grid, block, 0, ctx->stream, st, (const double*)d_R,
                          (const double*)d_s, n_part);
-      eval_grad_launch(ctx, e, n, n_part, table_format, d_F2, d_Q, d_N, d_A, st.MdM, st.sf, st.out, timed);
+      eval_grad_launch(ctx, e, n, n_part, table_format, d_F2, d_Q, d_N, d_A, st.MdM, st.sf, st.out, timed, true);
       hipLaunchKernelGGL(refine_first_kernel, grid, block, 0, ctx->stream, st, n_part, step_angle, step_shift);
       for (int it = 0; it < iters; ++it) {
-        eval_grad_launch(ctx, e, n, n_part, table_format, d_F2, d_Q, d_N, d_A, st.MdM, st.sf, st.out, timed);
+        eval_grad_launch(ctx, e, n, n_part, table_format, d_F2, d_Q, d_N, d_A, st.MdM, st.sf, st.out, timed, true);
         hipLaunchKernelGGL(refine_step_kernel, grid, block, 0, ctx->stream, st, n_part);
       }
       hipLaunchKernelGGL(refine_end_kernel, grid, block, 0, ctx->stream, st, n_part, d_R, d_s, d_cc);
@@ -1368,15 +1701,16 @@ emc_status emc_align_refine(emc_ctx* ctx, int n, int n_part, int table_format, c
       body(true);
       return;
     }
-    const int ki[6] = {n, n_part, table_format, iters, e.ns, e.g};
-    const double kd[2] = {step_angle, step_shift};
-    const void* kp[12] = {d_F2, d_Q, d_N, d_A, d_R, d_s, d_cc, base, e.supp, e.part, nullptr, nullptr};
+    const int ki[8] = {n, n_part, table_format, iters, e.ns, e.g, e.kind, e.S};
+    const double kd[3] = {step_angle, step_shift, support_radius};
+    const void* kp[12] = {d_F2, d_Q, d_N, d_A, d_R, d_s, d_cc, base, e.list, e.part, nullptr, nullptr};
     std::vector<unsigned char> kb(sizeof(ki) + sizeof(kd) + sizeof(kp));
     std::memcpy(kb.data(), ki, sizeof(ki));
     std::memcpy(kb.data() + sizeof(ki), kd, sizeof(kd));
     std::memcpy(kb.data() + sizeof(ki) + sizeof(kd), kp, sizeof(kp));
     // one graph per support list (a coarse-to-fine refinement alternates two)
-    auto& gr = ctx->graphs["refine:" + std::to_string(e.ns)];
+    auto& gr = ctx->graphs["refine:" + std::to_string(e.kind) + ":" + std::to_string(e.ns) + ":" +
+                           std::to_string(support_radius)];
     if (!gr.exec || gr.key != kb) {
       hipGraph_t graph = nullptr;
       EMC_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));

@@ -1,15 +1,15 @@
 // The allocation-free variant of emc_align_eval_grad, for callers that cannot
 // allocate device memory (code under HIP graph capture, tight serving loops,
 // harnesses that forbid hipMalloc). It runs the same kernels
-// (align_eval_grad_kernel + adjoint_fold_kernel, batched.hip) with the same
-// support list order and workgroup split, so its results equal
-// emc_align_eval_grad's bit for bit; the support list and the per-workgroup
-// partial sums live in `scratch`, a file-scope device array of kScratchBytes
+// (the adjoint kernel the context selects + adjoint_fold_kernel, batched.hip)
+// with the same list order and workgroup split, so its results equal
+// emc_align_eval_grad's bit for bit; the brick (or support) list and the
+// per-workgroup partial sums live in `scratch`, a file-scope device array of kScratchBytes
 // that every GPU the library runs on holds until the process ends.
 //
 // Every call on one GPU shares that GPU's array and leaves its contents
-// behind (the support list of the last (n, radius) is kept and reused by the
-// next call with the same pair; the partial sums are written in full before
+// behind (the list of the last (n, radius, kernel) is kept and reused by the
+// next call with the same values; the partial sums are written in full before
 // they are read, so nothing needs clearing). Calls that overlap on one GPU
 // (two host threads, or two contexts' streams) race on the array.
 #include <cstring>
@@ -26,10 +26,11 @@ static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 // Per device: the array's device address (hipGetSymbolAddress with the
 // device current; the host-side name `scratch` is not a device address) and
-// the (n, radius) whose support list the array holds.
+// the list the array holds: (n, radius, kernel kind, brick side) and its
+// entry count.
 struct DeviceState {
   void* ptr = nullptr;
-  int n = 0;
+  int n = 0, kind = -1, S = 0;
   double radius = -1.0;
   int ns = 0;
 };
@@ -70,33 +71,47 @@ emc_status emc_align_eval_grad_noalloc(emc_ctx* ctx, int n, int n_part, int tabl
     std::lock_guard<std::mutex> lock(g_mutex);
     DeviceState& st = device_state(ctx->device);
     unsigned char* base = (unsigned char*)st.ptr;
-    const bool same = st.n == n && st.radius == support_radius;
-    std::vector<int2> list;
+    emc::EvalSetup e{};
+    emc::eval_grad_kind(ctx, n, &e.kind, &e.S);
+    e.r2 = support_radius * support_radius;
+    const bool same = st.n == n && st.radius == support_radius && st.kind == e.kind && st.S == e.S;
+    // the list in host memory only: bricks (kind 1) or support frequencies
+    std::vector<int4> bricks;
+    std::vector<int2> supp;
+    size_t entry = e.kind == 1 ? sizeof(int4) : sizeof(int2);
     int ns = st.ns;
     if (!same) {
-      list = emc::support_host(n, support_radius);  // host memory only
-      ns = (int)list.size();
+      if (e.kind == 1) {
+        bricks = emc::brick_host(n, support_radius, e.S);
+        ns = (int)bricks.size();
+      } else {
+        supp = emc::support_host(n, support_radius);
+        ns = (int)supp.size();
+      }
     }
-    emc::EvalSetup e;
     e.ns = ns;
-    e.g = emc::eval_grad_blocks(ctx, n, n_part, ns);
-    const size_t list_bytes = round256(sizeof(int2) * (size_t)ns);
+    e.g = e.kind == 1 ? ns : emc::eval_grad_blocks(ctx, n, n_part, ns);
+    const size_t list_bytes = round256(entry * (size_t)ns);
     const size_t part_bytes = sizeof(double) * emc::kGradTerms * (size_t)e.g * (size_t)n_part;
     EMC_REQUIRE(list_bytes + part_bytes <= kScratchBytes,
-                "support list (%zu bytes) and partial sums (%zu bytes) exceed the %zu-byte scratch of the "
+                "list (%zu bytes) and partial sums (%zu bytes) exceed the %zu-byte scratch of the "
                 "allocation-free entry; use emc_align_eval_grad",
                 list_bytes, part_bytes, kScratchBytes);
-    e.supp = (const int2*)base;
+    e.list = base;
     e.part = (double*)(base + list_bytes);
     if (!same) {
       st.n = 0;  // the array holds no valid list until the copy is done
-      EMC_HIP(hipMemcpyAsync(base, list.data(), sizeof(int2) * (size_t)ns, hipMemcpyHostToDevice, ctx->stream));
+      const void* src = e.kind == 1 ? (const void*)bricks.data() : (const void*)supp.data();
+      EMC_HIP(hipMemcpyAsync(base, src, entry * (size_t)ns, hipMemcpyHostToDevice, ctx->stream));
       EMC_HIP(hipStreamSynchronize(ctx->stream));  // the host list goes away at return
       st.n = n;
       st.radius = support_radius;
+      st.kind = e.kind;
+      st.S = e.S;
       st.ns = ns;
     }
-    emc::eval_grad_launch(ctx, e, n, n_part, table_format, d_F2, d_Q, d_N, d_A, d_MdM, d_shift, d_out, false);
+    emc::eval_grad_launch(ctx, e, n, n_part, table_format, d_F2, d_Q, d_N, d_A, d_MdM, d_shift, d_out, false,
+                          false);
   });
 }
 

@@ -93,20 +93,31 @@ constexpr int kGradTerms = 11;
 // adjoint walks); see batched.hip.
 std::vector<int2> support_host(int n, double radius);
 
-// Device-side inputs of adjoint launches: the support list (ns entries), the
-// workgroups per particle (g) and the partial sums (n_part x g x kGradTerms).
+// Bricks of side S covering the half ball of table positions the support of
+// radius `radius` reaches (host memory; brick-major adjoint, batched.hip).
+std::vector<int4> brick_host(int n, double radius, int S);
+
+// Device-side inputs of adjoint launches. kind 1 (brick-major): list = the
+// brick list (ns bricks of side S), g = ns; kind 0: list = the support list
+// (ns entries), g workgroups per particle. part: n_part x g x kGradTerms
+// partial sums. r2: the support radius squared.
 struct EvalSetup {
-  const int2* supp;
+  const void* list;
   int ns, g;
   double* part;
+  int kind, S;
+  double r2;
 };
 
 int eval_grad_blocks(emc_ctx* ctx, int n, int n_part, int ns);
-// align_eval_grad_kernel + adjoint_fold_kernel on the context's stream (no
-// allocation, copy or host wait); timed adds HIP events around the kernel
+void eval_grad_kind(emc_ctx* ctx, int n, int* kind, int* S);
+// the adjoint kernel + adjoint_fold_kernel on the context's stream (no
+// allocation, copy or host wait); timed adds HIP events around the kernel;
+// gen: d_MdM holds the derivatives of right-multiplied rotation increments
+// (dM_a = G_a^T M, the refinement's), which the brick kernel forms from M
 void eval_grad_launch(emc_ctx* ctx, const EvalSetup& e, int n, int n_part, int table_format, const void* d_F2,
                       const void* d_Q, const float* d_N, const double* d_A, const float* d_MdM, const float* d_shift,
-                      double* d_out, bool timed);
+                      double* d_out, bool timed, bool gen);
 
 }  // namespace emc
 

@@ -93,6 +93,13 @@ emc_status emc_timing_read(emc_ctx* ctx, const char* name, double* ms_total, int
  *                  the zero-padded volume (set with box 0)
  *   "refine_graph" 1 (default): emc_align_refine replays its iteration
  *                  chain as a captured HIP graph; 0: eager launches
+ *   "grad_kernel"  adjoint evaluation order: 1 (default) brick-major (the
+ *                  table in bricks, one workgroup per (brick, particle),
+ *                  the bricks dealt to the XCDs so each XCD's L2 holds the
+ *                  brick all particles read), 0 support-list order
+ *                  (workgroups over chunks of each particle's support)
+ *   "grad_brick"   brick side of the brick-major adjoint, 16 or 32
+ *                  (default 32)
  * box = 0 sets the value for every box size, else for that box size only
  * (which wins). emc_tune_get returns the value in effect for `box`. */
 emc_status emc_tune_set(emc_ctx* ctx, const char* name, int box, int value);

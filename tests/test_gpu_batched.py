@@ -209,9 +209,13 @@ def _host_refine(al, R0, s0, iters=12, step_angle=0.02, step_shift=0.3):
 
 
 def test_device_refinement_matches_host_restatement(ctx):
-    """emc_align_refine against the host BFGS driving the same adjoint
-    kernel: poses agree to 1e-6 except where an accept/reject decision falls
-    on a rounding-level tie (at most one particle of eight)."""
+    """emc_align_refine against the host BFGS driving the C-ABI adjoint:
+    poses agree to 1e-6 and final CCs to 1e-7 except where an accept/reject
+    decision falls on a rounding-level tie (at most one particle of eight).
+    The device refinement forms the derivatives of its rotation increments
+    from M (dM_a = G_a^T M, cross products in the brick-major kernel); the
+    host passes the same derivatives as matrices, so the gradients agree to
+    FP32 rounding, not bit for bit."""
     n = 48
     rng = np.random.default_rng(17)
     from emclarity_amd.workload import make_particle, make_size_class
@@ -232,7 +236,7 @@ def test_device_refinement_matches_host_restatement(ctx):
     assert sum(same) >= 7, (same, cd, ch)
     for i in range(8):
         if same[i]:
-            assert abs(cd[i] - ch[i]) < 1e-9
+            assert abs(cd[i] - ch[i]) < 1e-7 * abs(ch[i])
     f0, _ = al._eval_local(R0, s0)
     assert np.all(cd >= f0)   # only ascending steps are taken (deterministic adjoint)
 
