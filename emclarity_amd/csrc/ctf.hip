@@ -92,29 +92,31 @@ __global__ void __launch_bounds__(256) ctf2d_kernel(float* __restrict__ out,
 // coordinate tw = -u sin + size_shift_z + w cos; inside |tw| <= 4 add
 // 0.25 (0.5 + 0.5 cos(pi tw / 4)) x CTF_t(row y, linear in x).
 //
-// A block (64 x, 4 waves of kSlabRows rows each, one z) first lists, in
-// table order, the tilts whose slab reaches any of its voxels: tw is linear
-// in u and independent of y, so a tilt is listed when [tw(u_first),
-// tw(u_last)] meets [-4, 4] (widened by 1e-3 against rounding). The slab
-// weight and the CTF x coordinate depend on (x, z, tilt) only, so a thread
-// owns kSlabRows voxels of one (x, z) column, forms them once per listed
-// tilt and adds the interpolated CTF row of each of its voxels (lanes run
-// along x: the CTF rows are read coalesced). Every voxel still runs the
-// exact slab test; tilts left out add nothing, and per voxel the tilts are
-// summed in table order, as the reference's per-voxel loop.
+// tw, the slab weight and the CTF x coordinate depend on (x, z, tilt) only.
+// A wave covers an 8 x 8 patch of (x, z) (lane = x + 8 z within the patch)
+// and each lane kSlabRows consecutive y of its (x, z) column; the block's
+// four waves take consecutive row segments. The block first lists, in
+// table order, the tilts whose slab reaches its patch (tw is linear in (u,
+// w), so the patch corners bound it; widened by 1e-3 against rounding):
+// slabs 8 voxels thick through the origin cross a compact patch at a few
+// angles only. Every voxel still runs the exact slab test; tilts left out
+// add nothing, and per voxel the tilts are summed in table order, as the
+// reference's per-voxel loop. A lane reads its CTF row at its own x
+// coordinate: the 64 lanes read inside one row of n floats per y.
 constexpr int kSlabListMax = 1024;
 constexpr int kSlabRows = 16;
+constexpr int kPatch = 8;
 
 // The block's tilt list in LDS (built by wave 0, in table order) for tilts
-// [0, nt); returns its length, or -1 when it would not fit (the caller then
-// loops over every tilt).
-__device__ inline int slab_list(const CtfTilt* __restrict__ tilts, int nt, int nx, int nz, int z, int* list) {
+// [0, nt) and the patch x in [x0, x1], z in [z0, z1]; returns its length,
+// or -1 when it would not fit (the caller then loops over every tilt).
+__device__ inline int slab_list(const CtfTilt* __restrict__ tilts, int nt, int nx, int nz, int x0, int x1, int z0,
+                                int z1, int* list) {
   __shared__ int count;
   const int lane = threadIdx.x + blockDim.x * threadIdx.y;
-  const int x0 = blockIdx.x * blockDim.x, x1 = min(nx, x0 + (int)blockDim.x) - 1;
   const float ssz = (nz % 2 == 0) ? 0.5f : 0.0f;
   const float ua = (float)x0 - (float)nx / 2.0f, ub = (float)x1 - (float)nx / 2.0f;
-  const float w = (float)z - (float)(nz / 2);
+  const float wa = (float)z0 - (float)(nz / 2), wb = (float)z1 - (float)(nz / 2);
   if (lane < kWave) {
     int n = 0;
     for (int base = 0; base < nt; base += kWave) {
@@ -122,8 +124,10 @@ __device__ inline int slab_list(const CtfTilt* __restrict__ tilts, int nt, int n
       bool hit = false;
       if (t < nt) {
         const float s = tilts[t].sin_t, c = tilts[t].cos_t;
-        const float ta = (-ua * s + ssz) + w * c, tb = (-ub * s + ssz) + w * c;
-        hit = fminf(ta, tb) <= 4.001f && fmaxf(ta, tb) >= -4.001f;
+        const float t0 = (-ua * s + ssz) + wa * c, t1 = (-ub * s + ssz) + wa * c;
+        const float t2 = (-ua * s + ssz) + wb * c, t3 = (-ub * s + ssz) + wb * c;
+        const float lo = fminf(fminf(t0, t1), fminf(t2, t3)), hi = fmaxf(fmaxf(t0, t1), fmaxf(t2, t3));
+        hit = lo <= 4.001f && hi >= -4.001f;
       }
       const unsigned long long m = __ballot(hit);
       const int before = __popcll(m & ((1ull << lane) - 1ull));
@@ -136,17 +140,14 @@ __device__ inline int slab_list(const CtfTilt* __restrict__ tilts, int nt, int n
   return count;
 }
 
-// blockDim (64, 4): x = blockIdx.x * 64 + threadIdx.x, rows from
-// (blockIdx.y * 4 + threadIdx.y) * kSlabRows, z given. out: the volume;
-// ctfs, tilts: its tilts [0, nt). Returns the minimum of the thread's voxels
-// (+inf when it has none).
+// One lane's column: voxels (x, y0 .. y0 + kSlabRows - 1, z) of the volume
+// `out`; ctfs, tilts: its tilts [0, nt). Returns the minimum of the lane's
+// voxels (+inf when it has none).
 __device__ inline float sf3d_column(float* __restrict__ out, const float* __restrict__ ctfs,
                                     const CtfTilt* __restrict__ tilts, int nt, int nx, int ny, int nz, int cx,
-                                    int z, const int* list, int nlist) {
+                                    int x, int y0, int z, const int* list, int nlist) {
   const float kPi = 3.14159265358979323f;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y0 = (blockIdx.y * blockDim.y + threadIdx.y) * kSlabRows;
-  if (x >= nx || y0 >= ny) return __builtin_inff();
+  if (x >= nx || z >= nz || y0 >= ny) return __builtin_inff();
   const float ssx = (nx % 2 == 0) ? 0.5f : 0.0f;
   const float ssz = (nz % 2 == 0) ? 0.5f : 0.0f;
   const float u = (float)x - (float)nx / 2.0f;   // float division, as mexSF3D.cu:47
@@ -190,14 +191,35 @@ __device__ inline float sf3d_column(float* __restrict__ out, const float* __rest
   return m;
 }
 
+// blockDim (64, 4); blockIdx.x = patch (px + ceil(nx / 8) * pz), blockIdx.y =
+// segment of 4 x kSlabRows rows, blockIdx.z = particle (sf3d_many_kernel).
+struct SlabPatch {
+  int x, z, x0, x1, z0, z1, y0;
+};
+
+__device__ inline SlabPatch slab_patch(int nx, int ny, int nz) {
+  SlabPatch q;
+  const int npx = (nx + kPatch - 1) / kPatch;
+  const int px = blockIdx.x % npx, pz = blockIdx.x / npx;
+  q.x0 = px * kPatch;
+  q.z0 = pz * kPatch;
+  q.x1 = min(nx, q.x0 + kPatch) - 1;
+  q.z1 = min(nz, q.z0 + kPatch) - 1;
+  q.x = q.x0 + (int)threadIdx.x % kPatch;
+  q.z = q.z0 + (int)threadIdx.x / kPatch;
+  q.y0 = (blockIdx.y * blockDim.y + threadIdx.y) * kSlabRows;
+  (void)ny;
+  return q;
+}
+
 __global__ void __launch_bounds__(256) sf3d_kernel(float* __restrict__ out,
                                                    const float* __restrict__ ctfs,
                                                    const CtfTilt* __restrict__ tilts, int n_tilts,
                                                    int nx, int ny, int nz, int cx) {
   __shared__ int list[kSlabListMax];
-  const int z = blockIdx.z;
-  const int nl = slab_list(tilts, n_tilts, nx, nz, z, list);
-  (void)sf3d_column(out, ctfs, tilts, n_tilts, nx, ny, nz, cx, z, list, nl);
+  const SlabPatch q = slab_patch(nx, ny, nz);
+  const int nl = slab_list(tilts, n_tilts, nx, nz, q.x0, q.x1, q.z0, q.z1, list);
+  (void)sf3d_column(out, ctfs, tilts, n_tilts, nx, ny, nz, cx, q.x, q.y0, q.z, list, nl);
 }
 
 __global__ void shift_min_kernel(float* __restrict__ d, int64_t n, const double* __restrict__ dmin,
@@ -248,8 +270,8 @@ void sf3d_impl(emc_ctx* ctx, int half, int sq, const int dims[3], int n,
   dim3 b2(64, 4, 1);
   hipLaunchKernelGGL(ctf2d_kernel, dim3(ceil_div(cx, 64), ceil_div(ny, 4), n), b2, 0, ctx->stream,
                      d_ctf, (const CtfTilt*)d_t, cx, ny, half ? 0 : cx / 2, ny / 2, half, sq, 1, 1);
-  hipLaunchKernelGGL(sf3d_kernel, dim3(ceil_div(nx, 64), ceil_div(ny, 4 * kSlabRows), nz), b2, 0, ctx->stream,
-                     d_out, (const float*)d_ctf, (const CtfTilt*)d_t, n, nx, ny, nz, cx);
+  hipLaunchKernelGGL(sf3d_kernel, dim3(ceil_div(nx, kPatch) * ceil_div(nz, kPatch), ceil_div(ny, 4 * kSlabRows), 1),
+                     b2, 0, ctx->stream, d_out, (const float*)d_ctf, (const CtfTilt*)d_t, n, nx, ny, nz, cx);
   EMC_HIP(hipGetLastError());
   // the tilt table lives in pageable memory; finish before it goes away
   EMC_HIP(hipStreamSynchronize(ctx->stream));
@@ -307,27 +329,29 @@ std::vector<CtfTilt> weight_mask_tilts(const int dims[3], double sampling, const
 
 // SF3D of many particles of one box size, the tilts of particle p being
 // entries [off[p] - base, off[p+1] - base) of the chunk's tilt table and CTF
-// stack; blockIdx.z = z + nz * p. The same slab sum as sf3d_kernel; each
-// block also writes the minimum of its voxels (part_min[p][block]).
+// stack; blockIdx.z = particle of the chunk. The same slab sum as
+// sf3d_kernel; each block also writes the minimum of its voxels
+// (part_min[p][block]).
 __global__ void __launch_bounds__(256) sf3d_many_kernel(float* __restrict__ out, const float* __restrict__ ctfs,
                                                         const CtfTilt* __restrict__ tilts,
                                                         const int* __restrict__ off, int base, int nx, int ny,
                                                         int nz, int cx, float* __restrict__ part_min) {
   __shared__ int list[kSlabListMax];
   __shared__ float lds[4];
-  const int z = blockIdx.z % nz, p = blockIdx.z / nz;
+  const int p = blockIdx.z;
   const int t0 = off[p] - base, t1 = off[p + 1] - base;
-  const int nl = slab_list(tilts + t0, t1 - t0, nx, nz, z, list);
-  float m = sf3d_column(out + (int64_t)p * nz * ny * nx, ctfs + (int64_t)t0 * ny * cx, tilts + t0, t1 - t0, nx,
-                        ny, nz, cx, z, list, nl);
+  const SlabPatch q = slab_patch(nx, ny, nz);
+  const int nl = slab_list(tilts + t0, t1 - t0, nx, nz, q.x0, q.x1, q.z0, q.z1, list);
+  float m = sf3d_column(out + (int64_t)p * nz * ny * nx, ctfs + (int64_t)t0 * ny * cx, tilts + t0, t1 - t0, nx, ny,
+                        nz, cx, q.x, q.y0, q.z, list, nl);
   // block minimum: one wave per threadIdx.y (blockDim = 64 x 4)
   m = wave_min(m);
   if (threadIdx.x == 0) lds[threadIdx.y] = m;
   __syncthreads();
   if (threadIdx.x == 0 && threadIdx.y == 0) {
     const float b = fminf(fminf(lds[0], lds[1]), fminf(lds[2], lds[3]));
-    const int per = gridDim.x * gridDim.y * nz;
-    part_min[(int64_t)p * per + ((int64_t)z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = b;
+    const int per = gridDim.x * gridDim.y;
+    part_min[(int64_t)p * per + (int64_t)blockIdx.y * gridDim.x + blockIdx.x] = b;
   }
 }
 
@@ -389,8 +413,9 @@ emc_status emc_weight_mask(emc_ctx* ctx, const int dims[3], double sampling,
     const dim3 b2(64, 4, 1);
     hipLaunchKernelGGL(ctf2d_kernel, dim3(ceil_div(cx, 64), ceil_div(ny, 4), n), b2, 0, ctx->stream, d_ctf,
                        (const CtfTilt*)d_t, cx, ny, cx / 2, ny / 2, 0, 1, 1, 1);
-    hipLaunchKernelGGL(sf3d_kernel, dim3(ceil_div(dims[0], 64), ceil_div(ny, 4 * kSlabRows), dims[2]), b2, 0,
-                       ctx->stream, d_out, (const float*)d_ctf, (const CtfTilt*)d_t, n, dims[0], ny, dims[2], cx);
+    hipLaunchKernelGGL(sf3d_kernel,
+                       dim3(ceil_div(dims[0], kPatch) * ceil_div(dims[2], kPatch), ceil_div(ny, 4 * kSlabRows), 1), b2,
+                       0, ctx->stream, d_out, (const float*)d_ctf, (const CtfTilt*)d_t, n, dims[0], ny, dims[2], cx);
     EMC_HIP(hipGetLastError());
     const int64_t nv = (int64_t)dims[0] * dims[1] * dims[2];
     double* d_min = (double*)scratch(ctx, "wm_min", sizeof(double));
@@ -430,10 +455,10 @@ emc_status emc_weight_mask_batched(emc_ctx* ctx, const int dims[3], double sampl
     // chunks of particles whose CTF stack stays within 256 MiB
     const size_t tilt_bytes = sizeof(float) * (size_t)cx * ny;
     const int max_tilts = (int)std::min<size_t>(65535, std::max<size_t>(1, (size_t(256) << 20) / tilt_bytes));
-    const int max_parts = std::max(1, 65535 / nz);  // grid z = nz x particles of a chunk
+    const int max_parts = 65535;  // grid z = particles of a chunk
     const dim3 b2(64, 4, 1);
-    const int gx = ceil_div(nx, 64), gy = ceil_div(ny, 4 * kSlabRows);
-    const int per = gx * gy * nz;  // blocks (minima) per particle
+    const int gx = ceil_div(nx, kPatch) * ceil_div(nz, kPatch), gy = ceil_div(ny, 4 * kSlabRows);
+    const int per = gx * gy;  // blocks (minima) per particle
     std::vector<int> cuts = {0};
     int max_nt = 0;
     while (cuts.back() < n_part) {
@@ -469,7 +494,7 @@ emc_status emc_weight_mask_batched(emc_ctx* ctx, const int dims[3], double sampl
       }
       {
         KTimer kt(ctx, "sf3d_many");
-        hipLaunchKernelGGL(sf3d_many_kernel, dim3(gx, gy, nz * np), b2, 0, ctx->stream, d_out + p0 * nv,
+        hipLaunchKernelGGL(sf3d_many_kernel, dim3(gx, gy, np), b2, 0, ctx->stream, d_out + p0 * nv,
                            (const float*)d_ctf, dt, d_off + p0, first[p0], nx, ny, nz, cx, d_part);
       }
       hipLaunchKernelGGL(fold_min_many_kernel, dim3(np), dim3(kRedThreads), 0, ctx->stream, (const float*)d_part,
