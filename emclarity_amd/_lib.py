@@ -80,6 +80,7 @@ SIGNATURES = {
     "emc_align_refine": (_st, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int,
                                C.c_double, C.c_double, C.c_double, _vp]),
     "emc_mask3d_sphere": (_st, [_vp, _i3, _d, _vp]),
+    "emc_mask_shape": (_st, [_vp, C.c_int, _i3, _d, _d, C.c_double, _vp]),
     "emc_tsearch_peaks": (_st, [_vp, _vp, _vp, _i3, _vp, C.c_int, _d, C.c_int, C.c_int, C.c_double, C.c_double,
                                 C.c_int, C.c_int, _d, C.POINTER(C.c_int)]),
     "emc_moving_average": (_st, [_vp, _vp, _i3, C.c_int, _vp]),

@@ -178,3 +178,37 @@ def BH_movingAverage_2(img, featureSize, ctx=None) -> DeviceArray:
     out = DeviceArray(v.shape, np.float32, v.ctx)
     _lib.call("emc_moving_average", v.ctx.handle, v.vp, _lib.i3(v.dims), int(featureSize), out.vp)
     return out
+
+
+_MASK_SHAPES = {"sphere": 0, "cylinder": 1, "rectangle": 2}
+
+
+def EMC_maskShape(SHAPE, SIZE, RADIUS, METHOD="gpu", OPTION=None, ctx=None) -> DeviceArray:
+    """MASK = EMC_maskShape(SHAPE, SIZE, RADIUS, METHOD, OPTION)
+    (masking/EMC_maskShape.m) for 3-D SIZE [x, y, z]: SHAPE 'sphere',
+    'cylinder' or 'rectangle'; OPTION a dict or a cell-like list of (field,
+    value) pairs with 'shift' ([x, y, z], default 0) and 'kernel' (True = the
+    default 0.04, False / 0 = binary, or a fraction in (0, 1]); origin 1 and
+    single precision, the reference's defaults. METHOD is accepted for the
+    signature: the mask is built on the device. Returns (nz, ny, nx)."""
+    from .device import default_context
+    opt = dict(OPTION or {}) if not isinstance(OPTION, (list, tuple)) else {k: v for k, v in OPTION}
+    unknown = set(opt) - {"shift", "kernel", "origin", "precision"}
+    if unknown:
+        raise ValueError(f"EMC_maskShape: unknown option(s) {sorted(unknown)}")
+    if opt.get("origin", 1) != 1 or str(opt.get("precision", "single")).lower() != "single":
+        raise NotImplementedError("EMC_maskShape: origin 1 and single precision (the reference defaults)")
+    shape = str(SHAPE).lower()
+    if shape not in _MASK_SHAPES:
+        raise ValueError("SHAPE should be 'sphere', 'cylinder' or 'rectangle'")
+    size = [int(v) for v in SIZE]
+    if len(size) != 3:
+        raise NotImplementedError("EMC_maskShape: 3-D SIZE only")
+    kernel = opt.get("kernel", 0.04)
+    kernel = 0.04 if kernel is True else (0.0 if kernel is False else float(kernel))
+    shift = np.asarray(opt.get("shift", (0.0, 0.0, 0.0)), dtype=np.float64).reshape(3)
+    c = ctx or default_context()
+    out = DeviceArray((size[2], size[1], size[0]), np.float32, c)
+    _lib.call("emc_mask_shape", c.handle, _MASK_SHAPES[shape], _lib.i3(size),
+              _lib.d_arr(np.asarray(RADIUS, np.float64).reshape(3)), _lib.d_arr(shift), kernel, out.vp)
+    return out

@@ -544,6 +544,17 @@ emc_status emc_tsearch_preprocess_chunk(emc_ctx* ctx, const float* d_in, const i
  * dimension >= 14. Stream-ordered. */
 emc_status emc_mask3d_sphere(emc_ctx* ctx, const int dims[3], const double radius[3], float* d_out);
 
+/* EMC_maskShape(SHAPE, SIZE, RADIUS, METHOD, {'shift', shift; 'kernel', kernel}) for a 3-D SIZE,
+ * origin 1, single precision (masking/EMC_maskShape.m:93-215): the peak, volume and
+ * centre-of-mass masks of alignment/BH_alignRaw3d_v2.m:415, :482, :493, :504. shape 0
+ * 'sphere', 1 'cylinder', 2 'rectangle'; dims, radius (each > 1) and shift (x, y, z);
+ * kernel 0 = the binary shape, else the fraction of min(SIZE) the Gaussian roll-off spans
+ * (the reference's default 0.04; at least 9 taps): the shape enlarged by half the kernel,
+ * the outer ceil(K/2) layers zeroed, a separable 'same' convolution, normalised by its
+ * maximum, values <= 0.001 set to 0. d_out: nx*ny*nz floats. Stream-ordered. */
+emc_status emc_mask_shape(emc_ctx* ctx, int shape, const int dims[3], const double radius[3],
+                          const double shift[3], double kernel, float* d_out);
+
 /* Greedy peak extraction of alignment/BH_templateSearch3d_2.m:1100-1275 for
  * nPeaks = 1, on the padded score map d_mag (modified: erased in place) and
  * angle-id map d_ang (1-based ids into angle_list rows (phi, theta, psi-phi),

@@ -1218,6 +1218,78 @@ def mask3d_sphere(size, radius):
     return pad_zeros3d(w, [0, 0, 0], [0, 0, 0], taper=True)                   # :462-466
 
 
+def mask_shape_kernel_taps(size, kernel):
+    """The default blurring kernel of EMC_maskShape (masking/EMC_maskShape.m:96-107):
+    K = round((min(SIZE) kernel - 1) / 2) 2 + 1, at least 9; sigma keeps the
+    Gaussian at 0.5 at ceil(K/2)/2 (EMC_gaussianKernel.m, origin 0, single
+    precision), normalised to sum 1. Returns float32 taps."""
+    K = int(matlab_round((min(size) * kernel - 1) / 2)) * 2 + 1
+    K = max(K, 9)
+    middle = math.ceil(K / 2) / 2
+    sigma = np.float32(math.sqrt(-1 * middle ** 2 / (2 * math.log(0.5))))
+    v = (np.arange(K, dtype=np.float32) - np.float32((K - 1) / 2))
+    k = np.exp(np.float32(-0.5) * (v / sigma) ** 2).astype(np.float32)
+    return (k / np.float32(k.sum(dtype=np.float32))).astype(np.float32)
+
+
+def mask_shape(shape, size, radius, shift=(0.0, 0.0, 0.0), kernel=0.04):
+    """EMC_maskShape(SHAPE, SIZE, RADIUS, METHOD, {'shift', shift; 'kernel',
+    kernel}) for a 3-D SIZE (x, y, z), origin 1 and single precision
+    (masking/EMC_maskShape.m:93-208): 'sphere' (:126-146), 'cylinder' (:148-185,
+    the ellipse of x, y broadcast over 2 R_z + 1 slices about z = shift_z),
+    'rectangle' (:187-198, |v| <= R). kernel False or 0: the binary shape;
+    True: 0.04; a fraction: the Gaussian of mask_shape_kernel_taps, the shape's
+    radius enlarged by ceil(K/2) (sphere, cylinder x/y) or floor(K/2)
+    (rectangle, cylinder z), the outer ceil(K/2) layers of every face zeroed
+    (EMC_resize 'force_taper' with a zero taper, :212), a separable 'same'
+    convolution (EMC_convn), normalised by its maximum, values <= 0.001 set
+    to 0 (:213-215). The coordinates are v = i - floor(n/2) - shift
+    (EMC_coordVectors, origin 1). Returns (nz, ny, nx) float64."""
+    f32 = np.float32
+    nx, ny, nz = (int(v) for v in size)
+    R = np.array(radius, dtype=np.float64).reshape(3)
+    sh = np.array(shift, dtype=np.float64).reshape(3)
+    if kernel is True:
+        kernel = 0.04
+    taps = mask_shape_kernel_taps(size, kernel) if kernel else None
+    K = 0 if taps is None else len(taps)
+    vx = (np.arange(nx) - nx // 2).astype(f32) - f32(sh[0])
+    vy = (np.arange(ny) - ny // 2).astype(f32) - f32(sh[1])
+    vz = (np.arange(nz) - nz // 2).astype(f32) - f32(sh[2])
+    if shape == "sphere":
+        R = R + math.ceil(K / 2)
+        tx, ty, tz = ((vx / f32(R[0])) ** 2, (vy / f32(R[1])) ** 2, (vz / f32(R[2])) ** 2)
+        m = ((tx[None, None, :] + ty[None, :, None]) + tz[:, None, None]) <= f32(1.0)
+    elif shape == "cylinder":
+        rxy = R[:2] + math.ceil(K / 2)
+        rz = int(matlab_round(R[2])) + K // 2
+        vx = (np.arange(nx) - nx // 2).astype(f32) - f32(sh[0])
+        vy = (np.arange(ny) - ny // 2).astype(f32) - f32(sh[1])
+        ell = ((vx / f32(rxy[0])) ** 2)[None, :] + ((vy / f32(rxy[1])) ** 2)[:, None] <= f32(1.0)
+        iz = np.arange(nz) - nz // 2 - int(matlab_round(sh[2]))
+        m = ell[None, :, :] & (np.abs(iz) <= rz)[:, None, None]
+    elif shape == "rectangle":
+        r = matlab_round(R).astype(int) + K // 2
+        si = matlab_round(sh).astype(int)
+        ix, iy, iz = (np.arange(n) - n // 2 - si[a] for a, n in enumerate((nx, ny, nz)))
+        m = ((np.abs(ix) <= r[0])[None, None, :] & (np.abs(iy) <= r[1])[None, :, None]
+             & (np.abs(iz) <= r[2])[:, None, None])
+    else:
+        raise ValueError(shape)
+    out = m.astype(np.float64)
+    if taps is not None:
+        e = math.ceil(K / 2)
+        out[:e], out[-e:] = 0.0, 0.0
+        out[:, :e], out[:, -e:] = 0.0, 0.0
+        out[:, :, :e], out[:, :, -e:] = 0.0, 0.0
+        k = taps.astype(np.float64)
+        for axis in (2, 1, 0):
+            out = _conv_same_axis(out, k, axis)
+        out = out / out.max()
+        out[out <= 0.001] = 0.0
+    return out
+
+
 def tsearch_peaks(mag, ang, angle_list, removal_mask, rm_dim, mask_cutoff, high_thr, max_peaks, max_tries,
                   convention="Bah"):
     """The greedy loop of BH_templateSearch3d_2.m:1130-1272 for nPeaks = 1 on

@@ -11,7 +11,7 @@ from emclarity_amd.fourier import FFTPlan, fourierTransformer, mexFFT
 from emclarity_amd.geometry import BH_defineMatrix
 from emclarity_amd.interpolator import interpolator, mexXform3d
 from emclarity_amd.masking import (BH_bandLimitCenterNormalize, BH_bandpass3d, BH_padZeros3d,
-                                   BH_weightMaskMex, BH_weightMaskMex_batched, mexCTF, mexSF3D)
+                                   BH_weightMaskMex, BH_weightMaskMex_batched, EMC_maskShape, mexCTF, mexSF3D)
 from emclarity_amd.xcf import BH_multi_xcf_Rotational, BH_multi_xcf_Translational
 from oracle import emc_oracle as O
 from tests.conftest import load_tlt
@@ -127,6 +127,39 @@ def test_weight_mask_batched_matches_single_and_oracle(ctx):
     with pytest.raises(_lib.EmcError):
         _lib.call("emc_weight_mask_batched", ctx.handle, _lib.i3(size), 4.0, 1, None, None, None,
                   _lib.d_arr(rg[0]), _lib.d_arr(rg[1]), stack.vp)
+
+
+@pytest.mark.parametrize("shape,size,radius,shift,kernel", [
+    ("sphere", (64, 64, 64), (14, 14, 14), (0, 0, 0), False),          # peakMask, BH_alignRaw3d_v2.m:493
+    ("sphere", (48, 40, 36), (12.5, 9.0, 7.5), (1.5, -2.0, 0.25), False),
+    ("sphere", (64, 64, 64), (20, 20, 20), (0, 0, 0), True),           # volMask, :504
+    ("sphere", (63, 50, 44), (14, 12, 9), (2.0, -1.0, 0.5), 0.1),
+    ("cylinder", (48, 48, 40), (12, 10, 8), (1.0, 0.0, -2.0), True),
+    ("rectangle", (40, 36, 32), (8, 6.4, 5), (2, -1, 0), True),
+    ("rectangle", (40, 36, 32), (8, 6, 5), (0, 0, 0), False),
+])
+def test_emc_mask_shape_matches_oracle(ctx, shape, size, radius, shift, kernel):
+    """EMC_maskShape against the oracle: binary shapes exactly (single-
+    precision boundary test on both sides), blurred ones to 1e-5 except
+    where the oracle sits within 1e-5 of the 0.001 cut-off."""
+    out = EMC_maskShape(shape, size, radius, "gpu", {"shift": list(shift), "kernel": kernel}).get()
+    ref = O.mask_shape(shape, size, radius, shift, kernel)
+    assert out.shape == ref.shape
+    if not kernel:
+        assert np.array_equal(out, ref.astype(np.float32))
+    else:
+        near_cut = np.abs(ref - 0.001) < 1e-5
+        assert np.max(np.abs(out - ref)[~near_cut]) < 1e-5
+        assert float(out.max()) == 1.0
+
+
+def test_emc_mask_shape_rejects_bad_arguments(ctx):
+    with pytest.raises(_lib.EmcError):
+        EMC_maskShape("sphere", (64, 64, 64), (1.0, 5, 5), "gpu", {})          # RADIUS must exceed 1
+    with pytest.raises(_lib.EmcError):
+        EMC_maskShape("sphere", (8, 64, 64), (5, 5, 5), "gpu", {"kernel": True})  # below the kernel's minimum SIZE
+    with pytest.raises(ValueError):
+        EMC_maskShape("sphere", (64, 64, 64), (5, 5, 5), "gpu", {"taper": 1})
 
 
 # ------------------------------------------------------ spectral (A8-A12)

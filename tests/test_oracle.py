@@ -770,3 +770,46 @@ def test_median_filter3_brute_force(k, rng):
     spike[3, 4, 4] = 100.0
     assert not O.median_filter3(spike, k).any()
     assert np.array_equal(O.median_filter3(np.full((7, 8, 9), 2.5), k), np.full((7, 8, 9), 2.5))
+
+
+# ------------------------------------------------------------ EMC_maskShape
+
+def test_mask_shape_binary_sphere_and_shift():
+    """The binary sphere (kernel false) against a brute-force FP64 count away
+    from ties, and an integer shift is an exact translation."""
+    m = O.mask_shape("sphere", (40, 40, 40), (9.3, 9.3, 9.3), (0, 0, 0), False)
+    z, y, x = np.meshgrid(*(np.arange(40) - 20,) * 3, indexing="ij")
+    assert np.array_equal(m, ((x * x + y * y + z * z) / 9.3 ** 2 <= 1).astype(float))
+    s = O.mask_shape("sphere", (40, 40, 40), (9.3, 9.3, 9.3), (3, -2, 1), False)
+    assert np.array_equal(s, np.roll(m, (1, -2, 3), axis=(0, 1, 2)))
+
+
+def test_mask_shape_kernel_taps():
+    """The default kernel: 9 taps for boxes under ~237 voxels, symmetric,
+    sum 1, and the Gaussian is 0.5 at ceil(K/2)/2 from the centre."""
+    k = O.mask_shape_kernel_taps((64, 64, 64), 0.04)
+    assert len(k) == 9 and abs(float(k.sum()) - 1.0) < 1e-6 and np.allclose(k, k[::-1])
+    assert len(O.mask_shape_kernel_taps((300, 300, 300), 0.04)) == 13   # round((12 - 1) / 2) 2 + 1
+    middle = 2.5
+    sigma = math.sqrt(-middle ** 2 / (2 * math.log(0.5)))
+    assert abs(math.exp(-0.5 * (middle / sigma) ** 2) - 0.5) < 1e-12
+
+
+def test_mask_shape_blurred_rectangle_is_separable():
+    """A blurred rectangle is the outer product of its three blurred 1-D
+    profiles (normalised, before the cut-off); the blurred sphere is 1 at its
+    centre, exactly 0 beyond the enlarged radius plus the kernel's reach."""
+    r = O.mask_shape("rectangle", (40, 36, 32), (8, 6, 5), (0, 0, 0), True)
+    k = O.mask_shape_kernel_taps((40, 36, 32), 0.04).astype(np.float64)
+    prof = []
+    for n, rad in ((40, 8), (36, 6), (32, 5)):
+        v = (np.abs(np.arange(n) - n // 2) <= rad + 4).astype(float)
+        v[:5], v[-5:] = 0, 0
+        v = np.convolve(v, k, mode="same")
+        prof.append(v / v.max())
+    outer = prof[2][:, None, None] * prof[1][None, :, None] * prof[0][None, None, :]
+    keep = outer > 0.0011
+    assert np.max(np.abs(r[keep] - outer[keep])) < 1e-12 and np.all(r[outer < 0.0009] == 0)
+    s = O.mask_shape("sphere", (64, 64, 64), (20, 20, 20), (0, 0, 0), True)
+    z, y, x = np.meshgrid(*(np.arange(64) - 32,) * 3, indexing="ij")
+    assert s[32, 32, 32] == 1.0 and np.all(s[np.sqrt(x * x + y * y + z * z) > 25 + 4 * math.sqrt(3) + 1e-9] == 0)
