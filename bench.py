@@ -160,13 +160,8 @@ class BatchedStep:
 
 
 def accuracy(parts, Rs, shifts):
-    from emclarity_amd.workload import angle_error_deg
-    errs = np.array([(angle_error_deg(R, p.truth_R), float(np.max(np.abs(s - p.shift))))
-                     for p, R, s in zip(parts, Rs, shifts)])
-    ok = int(np.sum((errs[:, 0] <= 1.0) & (errs[:, 1] <= 0.5)))
-    return {"within_1deg_0p5px": ok, "of": len(parts), "fraction": round(ok / len(parts), 4),
-            "median_angle_err_deg": round(float(np.median(errs[:, 0])), 4),
-            "median_shift_err_px": round(float(np.median(errs[:, 1])), 4)}
+    from emclarity_amd.workload import accuracy as acc
+    return acc(parts, Rs, shifts)
 
 
 def step_accuracy(particles, out):

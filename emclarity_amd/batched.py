@@ -341,3 +341,61 @@ def align_batch(aligner: BatchAligner, grid_pts, refine_iters=12):
     adjoint refinement. Returns (R (P,3,3), shift (P,3), cc (P,), grid best
     index (P,))."""
     return collect(align_batch_async(aligner, grid_pts, refine_iters))
+
+
+# ---------------------------------------------------------------- checks
+
+
+def euler_eval(aligner: BatchAligner, local_pts, shifts):
+    """CC and gradient of every particle at pose start_i R(phi, theta, psi)
+    (local Euler angles in degrees, BH_defineMatrix 'inv' of [phi, theta,
+    psi - phi]) and shift s_i: (cc (P,), grad (P, 6) = d/dphi, d/dtheta,
+    d/dpsi per radian, then d/dsx, d/dsy, d/dsz per pixel)."""
+    P = aligner.n_part
+    pts = np.asarray(local_pts, dtype=np.float64).reshape(P, 3)
+    R = np.empty((P, 3, 3))
+    dR = np.empty((P, 3, 3, 3))
+    for i in range(P):
+        R[i], d = euler_derivatives(aligner.starts[i], *pts[i])
+        dR[i] = np.stack(d)
+    return aligner.eval_grad(R, dR, np.asarray(shifts, dtype=np.float64).reshape(P, 3))
+
+
+def finite_difference_check(aligner: BatchAligner, local_pts, shifts, h=1e-3, hs=1e-2):
+    """Central differences of the device CC in each of the six parameters
+    (h radians for the Euler angles, hs pixels for the shifts), every
+    particle at once: 12 batched evaluations. Returns (grad (P, 6) from the
+    adjoint, fd (P, 6))."""
+    P = aligner.n_part
+    pts = np.asarray(local_pts, dtype=np.float64).reshape(P, 3)
+    s = np.asarray(shifts, dtype=np.float64).reshape(P, 3)
+    _, g = euler_eval(aligner, pts, s)
+    fd = np.empty((P, 6))
+    for i in range(3):
+        e = np.zeros(3)
+        e[i] = math.degrees(h)
+        cp, _ = euler_eval(aligner, pts + e, s)
+        cm, _ = euler_eval(aligner, pts - e, s)
+        fd[:, i] = (cp - cm) / (2 * h)
+    for i in range(3):
+        e = np.zeros(3)
+        e[i] = hs
+        cp, _ = euler_eval(aligner, pts, s + e)
+        cm, _ = euler_eval(aligner, pts, s - e)
+        fd[:, 3 + i] = (cp - cm) / (2 * hs)
+    return g, fd
+
+
+def judge(aligner: BatchAligner, R_grid, s_grid, R, s, cc):
+    """The plain grid as fallback and judge of the gradient refinement: the
+    CC of each particle's grid pick is evaluated over the full support (the
+    refinement's own score) and a particle whose refined CC is lower keeps
+    its grid pose. Returns (R, s, cc, fell_back (P,) bool)."""
+    P = aligner.n_part
+    Rg = np.asarray(R_grid, dtype=np.float64).reshape(P, 3, 3)
+    sg = np.asarray(s_grid, dtype=np.float64).reshape(P, 3)
+    cg, _ = aligner._eval_local(Rg, sg)
+    back = cg > np.asarray(cc)
+    R = np.where(back[:, None, None], Rg, np.asarray(R).reshape(P, 3, 3))
+    s = np.where(back[:, None], sg, np.asarray(s).reshape(P, 3))
+    return R, s, np.where(back, cg, cc), back

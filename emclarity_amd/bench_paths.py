@@ -10,6 +10,20 @@ batched-alignment and template-search workloads:
   refine   the refinement's adjoint evaluations on the same size classes
   search   the 28-orientation coarse search of the same size classes
   copy     a 1 GiB device-to-device copy (byte-counter calibration)
+  plain    the drop-in per-particle grid search (emc_align_grid) of one
+           particle of 64 / 96 / 128^3 over the 1,045-orientation local grid,
+           orientations in blocks; the rotation kernel's rate, and the whole
+           per-particle alignment (grid + refinement + final pass)
+  pipeline template search of a 1024 x 1024 x 512 tomogram holding 600 copies
+           of a 64^3 template (SNR 0.03, random orientations) over the
+           7-degree grid as one whole-tomogram chunk, greedy extraction of
+           the top 1,000 peaks, 96^3 windows, batched adjoint refinement;
+           recovery and pose errors against the inserted copies, and the
+           plain per-particle grid search timed on a sample of the windows
+  gradient 256 particles of 128^3 (+-60 deg wedge, own defocus, SNR 0.1):
+           the adjoint against central differences, and gradient refinement
+           from the 10-degree coarse peak (judged by the grid) against the
+           dense 1-degree local grid (23,281 orientations), accuracy and time
 
     python -m emclarity_amd.bench_paths [--path all|extract|...|copy] [--reps N] [--table c64|f16]
 
@@ -288,6 +302,45 @@ def bench_search(ctx, reps, table="c64", frac=0.6):
         del al
 
 
+def bench_plain(ctx, reps):
+    """emc_align_grid over the local grid (both stages, 5 rotated volumes of
+    n^3 per orientation: reference, reference weight, peak mask, particle
+    and wedge), then align_particle as the bench's plain path runs it. The
+    rotation kernel's bytes: one read of the input and one write of the
+    output per rotated voxel (8 B; the particle's window-mask read is not
+    counted)."""
+    from . import synthetic as S
+    from .align_raw import ParticleAligner, align_particle
+    from .masking import BH_weightMaskMex
+    from .workload import local_grid, make_particle, make_size_class
+    grid = local_grid()
+    v = _lib.C.c_int()
+    for n in (64, 96, 128):
+        rng = np.random.default_rng(500 + n)
+        sc, blobs = make_size_class(n)
+        p = make_particle(rng, sc, blobs)
+        w = BH_weightMaskMex((n, n, n), 1, p.tlt, [n // 2 + 1] * 3, [[n, n, n], [0, 0, 0]], ctx=ctx).get()
+        al = ParticleAligner(p.data, sc.ref, w, sc.bandpass, sc.peak_mask, vol_mask=sc.vol_mask, angles=p.start,
+                             ctx=ctx)
+        sv = np.zeros(3)
+        wall = _timed(ctx, lambda: al.grid(grid, sv, mode=3), reps)
+        ms, cnt = _ktime(ctx, "xform3d")
+        _lib.call("emc_tune_get", ctx.handle, b"align_grid_batch", n, _lib.C.byref(v))
+        bytes_ = len(grid) * 5 * 8 * n ** 3
+        per = ms / reps
+        t0 = time.perf_counter()
+        align_particle(al, grid, phi_inc_of=None, theta_inc=3.0, psi_inc=3.0, refine=True)
+        ctx.synchronize()
+        full = time.perf_counter() - t0
+        _line("plain", wall, {"n": n, "orientations": len(grid), "orientation_block": v.value,
+                              "orientations_per_s": round(len(grid) / wall, 1),
+                              "xform3d_ms_per_call": round(per, 3), "xform3d_launches_per_call": round(cnt / reps, 1),
+                              "xform3d_bytes": bytes_, "xform3d_GBps": round(bytes_ / (per * 1e-3) / 1e9, 1),
+                              "xform3d_hbm_frac": round(bytes_ / (per * 1e-3) / 8e12, 4),
+                              "align_particle_s": round(full, 3)})
+        del al
+
+
 def bench_copy(ctx, reps, mib=1024):
     """A device-to-device copy of known size: the calibration kernel of the
     HBM byte counters (FETCH_SIZE, WRITE_SIZE)."""
@@ -299,9 +352,156 @@ def bench_copy(ctx, reps, mib=1024):
                          "GBps": round(2 * src.nbytes / wall / 1e9, 1)})
 
 
+def bench_gradient(ctx, reps, table="c64", count=256, n=128):
+    """The differentiable-alignment workload: one aligner over ``count``
+    particles of n^3, each with its own +-60 deg wedge mask and defocus.
+
+    * gradient check: the adjoint gradient at every particle's coarse-grid
+      pick against central differences of the device CC (h = 1e-3 rad,
+      1e-2 px), as the largest |grad - fd| over the particle's largest |fd|;
+    * refinement: coarse 10-degree local grid over 0.6 x the support, pick,
+      8 + 8 quasi-Newton iterations (0.6 x, then the full support), and the
+      grid as judge (a particle whose refined CC is below its grid pick's
+      keeps the grid pose);
+    * dense grid: the 1-degree local grid ([15, 1, 15, 1]) scored over the
+      full support, pick.
+    Each timed from the prepared particles to the poses on the host."""
+    from .batched import BatchAligner, collect, finite_difference_check, judge, refine_stage, search_stage
+    from .geometry import eulerSearch
+    from .masking import BH_weightMaskMex
+    from .workload import accuracy, gradient_batch
+    t0 = time.perf_counter()
+    sc, parts = gradient_batch(count, n)
+    gen_s = time.perf_counter() - t0
+    wedges = [BH_weightMaskMex((n, n, n), 1, p.tlt, [n // 2 + 1] * 3, [[n, n, n], [0, 0, 0]], ctx=ctx)
+              for p in parts]
+    al = BatchAligner(n, sc.ref, sc.bandpass, peak_radius=6, table=table, ctx=ctx)
+    al.set_particles([p.data for p in parts], wedges, [p.start for p in parts])
+    del wedges
+    coarse = eulerSearch("C1", 15, 10, 15, 10)
+    dense = eulerSearch("C1", 15, 1, 15, 1)
+
+    # the coarse picks (also the judge's poses and the gradient-check poses)
+    h0 = search_stage(al, coarse, support_frac=0.6)
+    _, sg, _, best = collect(h0)
+    g, fd = finite_difference_check(al, coarse[best], sg)
+    scale = np.maximum(np.max(np.abs(fd[:, :3]), axis=1), 1e-30)
+    scale_s = np.maximum(np.max(np.abs(fd[:, 3:]), axis=1), 1e-30)
+    err_a = np.max(np.abs(g[:, :3] - fd[:, :3]), axis=1) / scale
+    err_s = np.max(np.abs(g[:, 3:] - fd[:, 3:]), axis=1) / scale_s
+
+    def refined_judged():
+        h = search_stage(al, coarse, support_frac=0.6)
+        R0 = h["R"].copy()
+        s0 = h["s"].copy()
+        refine_stage(h, 16, coarse=(0.6, 8))
+        R, s, cc, _ = collect(h)
+        return judge(al, R0.get(), s0.get(), R, s, cc)
+
+    def dense_grid():
+        return collect(search_stage(al, dense, support_frac=1.0))
+
+    res = {}
+    for name, fn in (("refine", refined_judged), ("dense", dense_grid)):
+        out = fn()                                          # warm-up
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            out = fn()
+        ctx.synchronize()
+        res[name] = ((time.perf_counter() - t0) / reps, out)
+    (tr, (R, s, cc, back)), (td, (Rd, sd, ccd, _)) = res["refine"], res["dense"]
+    acc_r = accuracy(parts, R, s)
+    acc_d = accuracy(parts, Rd, sd)
+    _line("gradient", tr, {
+        "n": n, "particles": count, "table": table, "generation_s": round(gen_s, 1),
+        "gradient_check": {"h_rad": 1e-3, "h_px": 1e-2,
+                           "angle_rel_err_median": round(float(np.median(err_a)), 5),
+                           "angle_rel_err_max": round(float(np.max(err_a)), 5),
+                           "shift_rel_err_median": round(float(np.median(err_s)), 5),
+                           "shift_rel_err_max": round(float(np.max(err_s)), 5)},
+        "refinement": {"orientations": len(coarse), "iterations": 16, "ms": round(tr * 1e3, 2),
+                       "alignments_per_s": round(count / tr, 1), "judge_fallbacks": int(np.sum(back)),
+                       **acc_r},
+        "dense_grid": {"orientations": len(dense), "ms": round(td * 1e3, 2),
+                       "alignments_per_s": round(count / td, 1), **acc_d},
+        "speedup_vs_dense": round(td / tr, 2)})
+    del al
+
+
+def bench_pipeline(ctx, reps, table="c64", tomo=(1024, 1024, 512), edge=64, count=600, snr=0.03, step=7.0,
+                   n_peaks=1000, window=96, max_orient=0, plain_sample=8, pixel=10.0, pruned=-1):
+    """Template search -> peaks -> windows -> batched refinement (one pass;
+    ``reps`` is unused: the search alone runs minutes). ``max_orient`` > 0
+    searches only the first that many orientations of the grid (a timing
+    sample; recovery is then not meaningful). ``pruned`` 0 / 1 forces the
+    template spectrum's form (padded-volume R2C / pruned DFT); -1 keeps the
+    library's choice."""
+    from . import synthetic as S
+    from .align_raw import ParticleAligner, align_particle
+    from .geometry import eulerSearch
+    from .masking import BH_bandpass3d
+    from .pipeline import refine_peaks, score_against_truth, search_peaks, synthetic_tomogram, whole_chunk
+    from .workload import local_grid, local_matrix, render_fast
+    rng = np.random.default_rng(404)
+    blobs = S.make_blobs(rng, 30, radius=0.3 * edge, sigma=(2.0, 4.5))
+    t0 = time.perf_counter()
+    vol, inserted = synthetic_tomogram(405, tomo, blobs, edge, count, snr)
+    gen_s = time.perf_counter() - t0
+    template = render_fast(blobs, edge).astype(np.float32)
+    grid = eulerSearch("C1", 180, step, 180, step)
+    n_grid = len(grid)
+    if max_orient:
+        grid = grid[:max_orient]
+    print(f"[bench_paths] pipeline: tomogram {tomo}, {len(inserted)} copies, {len(grid)} of {n_grid} orientations, "
+          f"chunk {whole_chunk(tomo, (edge,) * 3)[0]}", file=sys.stderr, flush=True)
+    if pruned >= 0:
+        _lib.call("emc_tune_set", ctx.handle, b"tsearch_pruned", 0, int(pruned))
+    peaks, ts, tp = search_peaks(vol, template, grid, n_peaks, 22, ctx=ctx)
+    print(f"[bench_paths] pipeline: search {ts:.1f} s, {len(peaks)} peaks in {tp:.2f} s", file=sys.stderr, flush=True)
+    reference = render_fast(blobs, window).astype(np.float32)
+    bp = BH_bandpass3d((window,) * 3, 1e-3, 400.0, 2.5 * pixel, "GPU", pixel)
+    R, centres, cc, valid, back, tr = refine_peaks(vol, peaks, reference, bp, window=window, table=table, ctx=ctx)
+    starts = np.stack([local_matrix(p[3], p[4], p[5] + p[3]) for p in peaks])
+    refined = score_against_truth(inserted, R, centres)
+    searched = score_against_truth(inserted, starts, peaks[:, :3])
+    # the plain per-particle path on a sample of the windows: coarse local
+    # grid (1,045 orientations) + the reference's refinement grid
+    from .subtomo import ResidentTomogram, extract_subtomos
+    res = ResidentTomogram.from_numpy(vol, ctx)
+    k = min(plain_sample, len(peaks))
+    wins, _, _ = extract_subtomos(res, np.rint(peaks[:k, :3]), (window,) * 3, 0.4 * window, taper=True)
+    wh = wins.get()
+    del wins, res
+    pm, vm = S.sphere_mask(window, window // 8 + 4), S.sphere_mask(window, 0.45 * window)
+    bph = bp.get()
+    ones = np.ones((window,) * 3, np.float32)                 # no missing wedge in this tomogram
+    lg = local_grid()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for i in range(k):
+        al = ParticleAligner(wh[i], reference, ones, bph, pm, vol_mask=vm, angles=starts[i])
+        align_particle(al, lg, phi_inc_of=None, theta_inc=3.0, psi_inc=3.0, refine=True)
+    ctx.synchronize()
+    plain_per = (time.perf_counter() - t0) / max(k, 1)
+    _line("pipeline", ts + tp + tr, {
+        "tomogram": list(tomo), "template": edge, "copies": len(inserted), "snr": snr, "generation_s": round(gen_s, 1),
+        "search": {"orientations": len(grid), "grid_orientations": n_grid, "step_deg": step, "pruned": pruned,
+                   "chunk": whole_chunk(tomo, (edge,) * 3)[0], "s": round(ts, 2),
+                   "ms_per_orientation": round(ts / len(grid) * 1e3, 3)},
+        "peaks": {"count": int(len(peaks)), "s": round(tp, 3)},
+        "refinement": {"window": window, "particles": int(len(peaks)), "valid_windows": int(valid.sum()),
+                       "s": round(tr, 3), "alignments_per_s": round(len(peaks) / tr, 1),
+                       "judge_fallbacks": int(back.sum())},
+        "against_truth_refined": refined, "against_truth_search_only": searched,
+        "plain_grid": {"sample": k, "orientations": len(lg), "s_per_particle": round(plain_per, 3),
+                       "alignments_per_s": round(1.0 / plain_per, 3)},
+        "refined_vs_plain_throughput": round((len(peaks) / tr) * plain_per, 1)})
+
+
 PATHS = {"extract": bench_extract, "average": bench_average, "peaks": bench_peaks, "tsprep": bench_tsprep,
          "tsearch": bench_tsearch, "adjoint": bench_adjoint, "refine": bench_refine, "search": bench_search,
-         "copy": bench_copy}
+         "copy": bench_copy, "plain": bench_plain, "gradient": bench_gradient, "pipeline": bench_pipeline}
 
 
 def main(argv=None):
@@ -310,12 +510,19 @@ def main(argv=None):
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--table", choices=["c64", "f16"], default="c64",
                     help="reference table format of the adjoint and search paths")
+    ap.add_argument("--max-orient", type=int, default=0,
+                    help="pipeline: search only the first N orientations of the grid (timing sample)")
+    ap.add_argument("--step", type=float, default=7.0, help="pipeline: angular step of the search grid (deg)")
+    ap.add_argument("--pruned", type=int, default=-1, choices=[-1, 0, 1],
+                    help="pipeline: template spectrum by pruned DFT (1) or padded R2C (0); -1 = library default")
     a = ap.parse_args(argv)
     ctx = default_context()
-    for name in (PATHS if a.path == "all" else [a.path]):
+    for name in ([p for p in PATHS if p not in ("gradient", "pipeline")] if a.path == "all" else [a.path]):
         print(f"[bench_paths] {name}", file=sys.stderr, flush=True)
-        if name in ("adjoint", "refine", "search"):
+        if name in ("adjoint", "refine", "search", "gradient"):
             PATHS[name](ctx, a.reps, table=a.table)
+        elif name == "pipeline":
+            PATHS[name](ctx, a.reps, table=a.table, max_orient=a.max_orient, step=a.step, pruned=a.pruned)
         else:
             PATHS[name](ctx, a.reps)
 

@@ -163,15 +163,17 @@ class Degrader:
         return (sig + rng.standard_normal(sig.shape, dtype=np.float32) * np.float32(sd)).astype(np.float32)
 
 
-def make_particle_fast(rng, sc: SizeClass, blobs, degrader: Degrader, pixel_A=2.5, snr=0.1):
+def make_particle_fast(rng, sc: SizeClass, blobs, degrader: Degrader, pixel_A=2.5, snr=0.1, max_tilt=None):
     """make_particle with render_fast and the half-grid degradation; the same
-    distributions (start, local truth, shift, max tilt, defocus)."""
+    distributions (start, local truth, shift, max tilt, defocus). max_tilt:
+    a fixed wedge (degrees) instead of U[50, 70]."""
     n = sc.n
     A = random_rotation(rng)
     phi, theta, psi = _local_truth(rng)
     Rt = A @ local_matrix(phi, theta, psi)
     shift = rng.uniform(-3.0, 3.0, 3)
-    max_tilt = rng.uniform(50.0, 70.0)
+    drawn = rng.uniform(50.0, 70.0)
+    max_tilt = drawn if max_tilt is None else float(max_tilt)
     df = rng.uniform(1.5, 4.0)
     data = degrader(render_fast(blobs, n, Rt, shift), rng, max_tilt, df, snr)
     T = S.tilt_table(max_tilt, 3.0, df, pixel_A)
@@ -234,6 +236,35 @@ def angle_error_deg(Ra, Rb):
     return math.degrees(math.acos(max(-1.0, min(1.0, c))))
 
 
+def accuracy(parts, Rs, shifts):
+    """Alignment accuracy against the ground truth: how many particles land
+    within 1 degree (orientation) and 0.5 px (largest shift component), and
+    the median errors."""
+    errs = np.array([(angle_error_deg(R, p.truth_R), float(np.max(np.abs(np.asarray(s) - p.shift))))
+                     for p, R, s in zip(parts, Rs, shifts)])
+    ok = int(np.sum((errs[:, 0] <= 1.0) & (errs[:, 1] <= 0.5)))
+    return {"within_1deg_0p5px": ok, "of": len(parts), "fraction": round(ok / len(parts), 4),
+            "median_angle_err_deg": round(float(np.median(errs[:, 0])), 4),
+            "median_shift_err_px": round(float(np.median(errs[:, 1])), 4)}
+
+
 def local_grid():
     """The 1,045-orientation local search ([15,3,15,3], eulerSearch)."""
     return eulerSearch("C1", 15, 3, 15, 3)
+
+
+def gradient_batch(count=256, n=128, seed=31, workers=8, max_tilt=60.0):
+    """The differentiable-alignment workload: ``count`` particles of n^3 at
+    SNR 0.1, a +-``max_tilt`` wedge about Y, defocus U[1.5, 4.0] um each,
+    shifts U[-3, 3] px, the truth off-grid inside the +-15 deg local search
+    around each start. Seeded per particle. Returns (size class, particles)."""
+    import concurrent.futures as cf
+    sc, blobs = make_size_class(n)
+    deg = Degrader(n)
+
+    def one(i):
+        return make_particle_fast(np.random.default_rng([seed, n, i]), sc, blobs, deg, max_tilt=max_tilt)
+
+    with cf.ThreadPoolExecutor(max_workers=max(1, workers)) as ex:
+        parts = list(ex.map(one, range(count)))
+    return sc, parts
