@@ -31,12 +31,17 @@ void define_matrix(const double ang_deg[3], int convention, int direction, doubl
 // X(k) = P1(k) conj(F(k)) ws(k) / N over the half grid; F(0) treated as 0
 // (BH_bandLimitCenterNormalize.m:51). ws = (wa(k)wb(k) + wa(-k)wb(-k))/2
 // with wa = ifftshift(rotated ref weight), wb = ifftshift(SF3D wedge).
+// blockIdx.y: orientation of a block (F, wa and X sF / sWa / sX elements
+// apart; P1 and wb shared).
 __global__ void __launch_bounds__(256) xcorr_half_kernel(const float2* __restrict__ P1,
                                                          const float2* __restrict__ F,
                                                          const float* __restrict__ wa_c,
                                                          const float* __restrict__ wb_c,
                                                          float2* __restrict__ X, int nx, int ny, int nz,
-                                                         float inv_n) {
+                                                         float inv_n, int64_t sF, int64_t sWa, int64_t sX) {
+  F += blockIdx.y * sF;
+  if (wa_c) wa_c += blockIdx.y * sWa;
+  X += blockIdx.y * sX;
   const int cx = nx / 2 + 1;
   const int64_t nh = (int64_t)cx * ny * nz;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nh;
@@ -61,11 +66,15 @@ __global__ void __launch_bounds__(256) xcorr_half_kernel(const float2* __restric
 // Rotational CC sums (BH_multi_xcf_Rotational.m:73-78) over the half grid:
 //   P = F2 * bp (DC = 0), R = ref_ft1, Wp = ifftshift(rotated wedge),
 //   Wr = ref weight (FFT order, null = 1). Writes per-block partials of
-//   (num, sum|R Wp|^2, sum|P Wr|^2).
+//   (num, sum|R Wp|^2, sum|P Wr|^2). blockIdx.y: orientation of a block
+//   (F2 and Wp sF2 / sWp elements apart, partials part[3 b gridDim.x ..]).
 __global__ void __launch_bounds__(kRedThreads) rot_sums_half_kernel(
     const float2* __restrict__ F2, const float* __restrict__ bp, const float2* __restrict__ R,
     const float* __restrict__ wp_c, const float* __restrict__ wr, int nx, int ny, int nz,
-    double* __restrict__ part) {
+    double* __restrict__ part, int64_t sF2, int64_t sWp) {
+  F2 += blockIdx.y * sF2;
+  if (wp_c) wp_c += blockIdx.y * sWp;
+  part += (int64_t)3 * blockIdx.y * gridDim.x;
   const int cx = nx / 2 + 1;
   const int64_t nh = (int64_t)cx * ny * nz;
   // NaN first element -> weight 1 (:34-40); the first element of
@@ -154,8 +163,12 @@ __global__ void __launch_bounds__(kRedThreads) rot_shell_sums_half_kernel(
     for (int k = 0; k < 3; ++k) part[((int64_t)s * gridDim.x + blockIdx.x) * 3 + k] = v[k];
 }
 
+// block b folds the partials of orientation b into cc[b * cc_stride]
 __global__ void __launch_bounds__(kRedThreads) rot_cc_fold_kernel(const double* __restrict__ part,
-                                                                  int nb, double* __restrict__ cc) {
+                                                                  int nb, double* __restrict__ cc,
+                                                                  int cc_stride) {
+  part += (int64_t)3 * blockIdx.x * nb;
+  cc += (int64_t)blockIdx.x * cc_stride;
   double v[3] = {0.0, 0.0, 0.0};
   for (int b = threadIdx.x; b < nb; b += blockDim.x)
     for (int k = 0; k < 3; ++k) v[k] += part[3 * b + k];
@@ -163,8 +176,14 @@ __global__ void __launch_bounds__(kRedThreads) rot_cc_fold_kernel(const double* 
   if (threadIdx.x == 0) cc[0] = v[0] / sqrt(v[1] * v[2]);
 }
 
-__global__ void copy_peak_kernel(const PeakResult* __restrict__ r, double* __restrict__ est) {
-  if (threadIdx.x < 3) est[threadIdx.x] = r->peak[threadIdx.x];
+// est[3 b + k] = r[b].peak[k] for b < count
+__global__ void copy_peak_kernel(const PeakResult* __restrict__ r, double* __restrict__ est, int count) {
+  for (int i = threadIdx.x; i < 3 * count; i += blockDim.x) est[i] = r[i / 3].peak[i % 3];
+}
+
+// the rotation kernel's per-entry FP32 shifts from FP64 peak estimates
+__global__ void shifts_to_float_kernel(const double* __restrict__ est, float* __restrict__ out, int n3) {
+  for (int i = threadIdx.x; i < n3; i += blockDim.x) out[i] = (float)est[i];
 }
 
 struct AlignBuffers {
@@ -254,7 +273,10 @@ emc_status emc_align_grid_padded(emc_ctx* ctx, int w, int n, const int pad_low[3
     double* shell_part =
         max_shells ? (double*)scratch(ctx, "align_shell_part", sizeof(double) * 3 * kShellBlocks * (size_t)max_shells)
                    : nullptr;
-    double* h_stage = (double*)pinned(ctx, sizeof(double) * (3 + (size_t)n_ref) * n_orient + 64);
+    // host staging: est0, the CC results and (batched orientations) the
+    // orientation matrices after them
+    double* h_stage = (double*)pinned(ctx, sizeof(double) * (3 + (size_t)n_ref) * n_orient + 64 +
+                                               sizeof(float) * 18 * (size_t)n_orient);
     std::memcpy(h_stage, est0, sizeof(double) * 3);
     EMC_HIP(hipMemcpyAsync(B.est0, h_stage, sizeof(double) * 3, hipMemcpyHostToDevice, ctx->stream));
     EMC_HIP(hipStreamSynchronize(ctx->stream));
@@ -282,10 +304,11 @@ emc_status emc_align_grid_padded(emc_ctx* ctx, int w, int n, const int pad_low[3
       blcn_half(ctx, B.P1, dims, 1, d_bandpass, d_scale, false);
     }
     const int g1 = 1024;
-    for (int o = 0; o < n_orient; ++o) {
+    // RotMat of orientation o: column-major FP64 R and FP32 rows
+    auto rot_rows = [&](int o, float rows[9], double R[9]) {
       const double* a = angles_pts + 3 * o;
       const double ang[3] = {a[0], a[1], a[2] - a[0]};  // [phi, theta, psi - phi] (:1175)
-      double Rl[9], R[9];
+      double Rl[9];
       define_matrix(ang, 0, 1, Rl);  // 'Bah', 'inv'
       // RotMat = reshape(angles,3,3) * RotMat (:1176), column-major
       for (int c = 0; c < 3; ++c)
@@ -294,8 +317,109 @@ emc_status emc_align_grid_padded(emc_ctx* ctx, int w, int n, const int pad_low[3
           for (int k = 0; k < 3; ++k) s += angles_colmajor[k * 3 + r] * Rl[c * 3 + k];
           R[c * 3 + r] = s;
         }
-      float rows[9], rowsT[9];
       colmajor_to_rows(R, rows);
+    };
+    // Blocks of orientations in one launch chain per kernel (rotations with
+    // per-entry matrices, batched FFTs, per-entry correlation, peak search
+    // and CC sums), for one symmetry mate, no padding and unshelled scores;
+    // the loop below handles the rest one orientation at a time.
+    const int bmax = tune_value(ctx, "align_grid_batch", n, 16);
+    const bool batched = bmax > 1 && n_orient > 1 && nsym == 1 && !padded && max_shells == 0;
+    if (batched) {
+      const int64_t per = (int64_t)sizeof(float) * nv * 6 + (int64_t)sizeof(float2) * nh * 3;
+      int nbm = (int)std::max<int64_t>(1, std::min<int64_t>(bmax, ((int64_t)1 << 30) / per));
+      nbm = std::min(nbm, n_orient);
+      float* b_rref = (float*)scratch(ctx, "alb_rref", sizeof(float) * nv * nbm);
+      float* b_rwdg = (float*)scratch(ctx, "alb_rwdg", sizeof(float) * nv * nbm);
+      float* b_rmask = (float*)scratch(ctx, "alb_rmask", sizeof(float) * nv * nbm);
+      float* b_map = (float*)scratch(ctx, "alb_map", sizeof(float) * nv * nbm);
+      float* b_rpart = (float*)scratch(ctx, "alb_rpart", sizeof(float) * nv * nbm);
+      float* b_rwedge = (float*)scratch(ctx, "alb_rwedge", sizeof(float) * nv * nbm);
+      float2* b_fref = (float2*)scratch(ctx, "alb_fref", sizeof(float2) * nh * nbm);
+      float2* b_X = (float2*)scratch(ctx, "alb_X", sizeof(float2) * nh * nbm);
+      float2* b_F2 = (float2*)scratch(ctx, "alb_F2", sizeof(float2) * nh * nbm);
+      double* b_part = (double*)scratch(ctx, "alb_part", sizeof(double) * 3 * kRedBlocks * nbm);
+      float* b_shift = (float*)scratch(ctx, "alb_shift", sizeof(float) * 3 * nbm);
+      // stage-1 rows (RotMat') and stage-2 rows (RotMat) of every orientation
+      float* d_m = (float*)scratch(ctx, "alb_mats", sizeof(float) * 18 * (size_t)n_orient);
+      float* h_m = (float*)((char*)h_stage + sizeof(double) * (3 + (size_t)n_ref) * n_orient + 64);
+      for (int o = 0; o < n_orient; ++o) {
+        float rows[9];
+        double R[9];
+        rot_rows(o, rows, R);
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c) h_m[9 * o + r * 3 + c] = rows[c * 3 + r];
+        float* m2 = h_m + 9 * ((size_t)n_orient + o);
+        if (S_colmajor) {  // the one symmetry mate: R * S_1
+          double P[9];
+          for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < 3; ++r) {
+              double acc = 0.0;
+              for (int k = 0; k < 3; ++k) acc += R[k * 3 + r] * S_colmajor[c * 3 + k];
+              P[c * 3 + r] = acc;
+            }
+          colmajor_to_rows(P, m2);
+        } else {
+          std::memcpy(m2, rows, sizeof(rows));
+        }
+      }
+      EMC_HIP(hipMemcpyAsync(d_m, h_m, sizeof(float) * 18 * (size_t)n_orient, hipMemcpyHostToDevice,
+                             ctx->stream));
+      const float* d_mT = d_m;
+      const float* d_mR = d_m + 9 * (size_t)n_orient;
+      for (int o0 = 0; o0 < n_orient; o0 += nbm) {
+        const int nb = std::min(nbm, n_orient - o0);
+        emc_fft_plan* bplan = cached_plan(ctx, n, n, n, nb);
+        const double* d_shift2 = B.est0;
+        const float* d_shift2f = nullptr;
+        if (stage1) {
+          launch_xform_batched(ctx, d_ref, b_rref, d_mT + 9 * (size_t)o0, nullptr, nb,
+                               xform_args(wdims, nullptr, 1, est0f, 1));  // :1287-1290
+          const XformArgs a0 = xform_args(dims, nullptr, 1, zero3, 1);
+          const float* wa = nullptr;
+          if (d_ref_wgt_rot) {
+            launch_xform_batched(ctx, d_ref_wgt_rot, b_rwdg, d_mT + 9 * (size_t)o0, nullptr, nb, a0);  // :1294-1297
+            wa = b_rwdg;
+          }
+          launch_xform_batched(ctx, d_peak_mask, b_rmask, d_mT + 9 * (size_t)o0, nullptr, nb, a0);  // :1300-1303
+          fft_r2c(bplan, b_rref, b_fref);  // :1311-1314
+          hipLaunchKernelGGL(xcorr_half_kernel, dim3(g1, nb), dim3(256), 0, ctx->stream, (const float2*)B.P1,
+                             (const float2*)b_fref, wa, d_wedge, b_X, n, n, n, (float)(1.0 / (double)nv), nh, nv,
+                             nh);
+          EMC_HIP(hipGetLastError());
+          fft_c2r(bplan, b_X, b_map);
+          peak_search_batched(ctx, b_map, 1, dims, b_rmask, peak_com, B.res + o0, B.est0, nb, nv, nv);
+          hipLaunchKernelGGL(copy_peak_kernel, dim3(1), dim3(256), 0, ctx->stream,
+                             (const PeakResult*)(B.res + o0), B.est + 3 * o0, nb);
+          hipLaunchKernelGGL(shifts_to_float_kernel, dim3(1), dim3(256), 0, ctx->stream,
+                             (const double*)(B.est + 3 * o0), b_shift, 3 * nb);
+          EMC_HIP(hipGetLastError());
+          d_shift2f = b_shift;
+        }
+        if (stage2) {
+          XformArgs a2 = xform_args(wdims, nullptr, 1, zero3, 0);
+          a2.d_mul = d_vol_mask;
+          if (!d_shift2f) a2.d_shift = d_shift2;  // stage 2 alone: estPeak = est0 for every entry
+          launch_xform_batched(ctx, d_particle, b_rpart, d_mR + 9 * (size_t)o0, d_shift2f, nb, a2);  // :1213-1216
+          launch_xform_batched(ctx, d_wedge, b_rwedge, d_mR + 9 * (size_t)o0, nullptr, nb,
+                               xform_args(dims, nullptr, 1, zero3, 0));  // :1251-1254
+          fft_r2c(bplan, b_rpart, b_F2);  // :1375-1377
+          for (int r = 0; r < n_ref; ++r) {
+            const float* wr = d_ref_wgts ? d_ref_wgts[r] : nullptr;
+            hipLaunchKernelGGL(rot_sums_half_kernel, dim3(kRedBlocks, nb), dim3(kRedThreads), 0, ctx->stream,
+                               (const float2*)b_F2, d_bandpasses[r], (const float2*)d_ref_ft1s[r],
+                               (const float*)b_rwedge, wr, n, n, n, b_part, nh, nv);
+            hipLaunchKernelGGL(rot_cc_fold_kernel, dim3(nb), dim3(kRedThreads), 0, ctx->stream,
+                               (const double*)b_part, kRedBlocks, B.ccc + (int64_t)o0 * n_ref + r, n_ref);
+          }
+          EMC_HIP(hipGetLastError());
+        }
+      }
+    }
+    for (int o = 0; o < (batched ? 0 : n_orient); ++o) {
+      float rows[9], rowsT[9];
+      double R[9];
+      rot_rows(o, rows, R);
       // stage 2 rotates with the symmetry mates R * S_i and averages them
       // (the particle and wedge interp3d calls pass `symmetry`, :1213-1254)
       float srows[9 * kMaxSym];
@@ -330,13 +454,13 @@ emc_status emc_align_grid_padded(emc_ctx* ctx, int w, int n, const int pad_low[3
         fft_r2c(plan, to_calc(B.rref), B.fref);        // :1311-1314
         hipLaunchKernelGGL(xcorr_half_kernel, dim3(g1), dim3(256), 0, ctx->stream, (const float2*)B.P1,
                            (const float2*)B.fref, wa, d_wedge, B.X, n, n, n,
-                           (float)(1.0 / (double)nv));
+                           (float)(1.0 / (double)nv), (int64_t)0, (int64_t)0, (int64_t)0);
         EMC_HIP(hipGetLastError());
         fft_c2r(plan, B.X, B.map);
         // estPeak = peakCoord + estPeakCoord (:1341-1344)
         peak_search(ctx, B.map, 1, dims, B.rmask, peak_com, B.res + o, B.est0);
         hipLaunchKernelGGL(copy_peak_kernel, dim3(1), dim3(64), 0, ctx->stream,
-                           (const PeakResult*)(B.res + o), B.est + 3 * o);
+                           (const PeakResult*)(B.res + o), B.est + 3 * o, 1);
         d_shift2 = B.est + 3 * o;
       }
       if (stage2) {
@@ -363,9 +487,9 @@ emc_status emc_align_grid_padded(emc_ctx* ctx, int w, int n, const int pad_low[3
           } else {
             hipLaunchKernelGGL(rot_sums_half_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, ctx->stream,
                                (const float2*)B.F2, d_bandpasses[r], (const float2*)d_ref_ft1s[r],
-                               (const float*)B.rwedge, wr, n, n, n, B.part);
+                               (const float*)B.rwedge, wr, n, n, n, B.part, (int64_t)0, (int64_t)0);
             hipLaunchKernelGGL(rot_cc_fold_kernel, dim3(1), dim3(kRedThreads), 0, ctx->stream,
-                               (const double*)B.part, kRedBlocks, cc);
+                               (const double*)B.part, kRedBlocks, cc, 0);
           }
         }
         EMC_HIP(hipGetLastError());

@@ -265,7 +265,8 @@ constexpr TuneRange kTune[] = {{"search_tile", 2, 128, 16},
                                {"search_zsplit", 1, 4, 1},
                                {"tsearch_pruned", 0, 1, 1},
                                {"grad_kernel", 0, 1, 1},
-                               {"grad_brick", 16, 32, 32}};
+                               {"grad_brick", 16, 32, 32},
+                               {"align_grid_batch", 1, 64, 16}};
 
 const TuneRange* find_tune(const char* name) {
   for (const auto& t : kTune)

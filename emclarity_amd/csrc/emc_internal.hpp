@@ -151,6 +151,10 @@ void reduce_sum_sumsq(emc_ctx* ctx, const float* d, int64_t n, double* d_out2);
 // minimum of a real array read with a stride (stride 2 reads real parts of
 // complex data), out[0] = min as double.
 void reduce_min(emc_ctx* ctx, const float* d, int64_t n, int stride, double* d_out1);
+// minima of `batch` volumes of n values (stride between values, bstride floats
+// between volumes) into d_out[0 .. batch)
+void reduce_min_batched(emc_ctx* ctx, const float* d, int64_t n, int stride, int64_t bstride, int batch,
+                        double* d_out);
 
 // ---------------------------------------------------------- device helpers
 

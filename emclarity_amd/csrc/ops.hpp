@@ -75,6 +75,11 @@ struct PeakResult {
 // d_add: optional device shift (3 doubles) added to the returned peak.
 void peak_search(emc_ctx* ctx, const float* map, int stride, const int dims[3], const float* mask,
                  const int com[3], PeakResult* d_res, const double* d_add);
+// The same for `batch` maps map_bs floats apart, each with its own mask
+// (mask_bs floats apart; 0 = one mask for all), results d_res[0 .. batch).
+void peak_search_batched(emc_ctx* ctx, const float* map, int stride, const int dims[3], const float* mask,
+                         const int com[3], PeakResult* d_res, const double* d_add, int batch, int64_t map_bs,
+                         int64_t mask_bs);
 
 // Shells of the shell-weighted CCC (emc_shell_plan_create): concatenated
 // 1-based full-grid voxel lists, shell s at d_idx[d_offsets[s] .. d_offsets[s+1]).

@@ -15,10 +15,17 @@
 
 namespace emc {
 
-// argmax over the masked, fftshifted map -> per-block keys
+// argmax over the masked, fftshifted map -> per-block keys. blockIdx.y: map
+// of a batch (map_bs / mask_bs floats apart, its own minimum d_min[b] and
+// partials part[b * gridDim.x ..])
 __global__ void __launch_bounds__(kRedThreads) peak_argmax_partial_kernel(
     const float* __restrict__ map, int stride, int nx, int ny, int nz, const float* __restrict__ mask,
-    const double* __restrict__ d_min, unsigned long long* __restrict__ part) {
+    const double* __restrict__ d_min, unsigned long long* __restrict__ part, int64_t map_bs, int64_t mask_bs) {
+  const int bb = blockIdx.y;
+  map += bb * map_bs;
+  if (mask) mask += bb * mask_bs;
+  d_min += bb;
+  part += (int64_t)bb * gridDim.x;
   const int64_t n = (int64_t)nx * ny * nz;
   const float mn = (float)d_min[0];
   unsigned long long best = 0ull;
@@ -41,11 +48,19 @@ __global__ void __launch_bounds__(kRedThreads) peak_argmax_partial_kernel(
 __device__ inline double matlab_round(double x) { return x >= 0 ? floor(x + 0.5) : ceil(x - 0.5); }
 
 // Fold the keys, then the two COM passes (:47-91) and the origin shift
-// (:97), all in one block.
+// (:97), all in one block; block b takes map b of a batch (offsets as in
+// peak_argmax_partial_kernel, result res[b]).
 __global__ void __launch_bounds__(kRedThreads) peak_com_kernel(
     const float* __restrict__ map, int stride, int nx, int ny, int nz, const float* __restrict__ mask,
     const double* __restrict__ d_min, const unsigned long long* __restrict__ part, int nparts, int cx,
-    int cy, int cz, PeakResult* __restrict__ res, const double* __restrict__ d_add) {
+    int cy, int cz, PeakResult* __restrict__ res, const double* __restrict__ d_add, int64_t map_bs,
+    int64_t mask_bs) {
+  const int bb = blockIdx.x;
+  map += bb * map_bs;
+  if (mask) mask += bb * mask_bs;
+  d_min += bb;
+  part += (int64_t)bb * nparts;
+  res += bb;
   unsigned long long best = 0ull;
   for (int b = threadIdx.x; b < nparts; b += blockDim.x) best = part[b] > best ? part[b] : best;
   best = block_max_u64(best);
@@ -121,17 +136,25 @@ __global__ void __launch_bounds__(kRedThreads) peak_com_kernel(
 
 void peak_search(emc_ctx* ctx, const float* map, int stride, const int dims[3], const float* mask,
                  const int com[3], PeakResult* d_res, const double* d_add) {
+  peak_search_batched(ctx, map, stride, dims, mask, com, d_res, d_add, 1, 0, 0);
+}
+
+void peak_search_batched(emc_ctx* ctx, const float* map, int stride, const int dims[3], const float* mask,
+                         const int com[3], PeakResult* d_res, const double* d_add, int batch, int64_t map_bs,
+                         int64_t mask_bs) {
   const int64_t n = (int64_t)dims[0] * dims[1] * dims[2];
   EMC_REQUIRE(n < ((int64_t)1 << 32), "map too large for 32-bit peak index");
-  double* d_min = (double*)scratch(ctx, "peak_min", sizeof(double));
+  EMC_REQUIRE(batch >= 1 && batch <= 65535, "peak search batch %d out of range", batch);
+  double* d_min = (double*)scratch(ctx, "peak_min", sizeof(double) * (size_t)batch);
   unsigned long long* part =
-      (unsigned long long*)scratch(ctx, "peak_keys", sizeof(unsigned long long) * kRedBlocks);
-  reduce_min(ctx, map, n, stride, d_min);  // :33 min over the whole map
-  hipLaunchKernelGGL(peak_argmax_partial_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, ctx->stream,
-                     map, stride, dims[0], dims[1], dims[2], mask, (const double*)d_min, part);
-  hipLaunchKernelGGL(peak_com_kernel, dim3(1), dim3(kRedThreads), 0, ctx->stream, map, stride, dims[0],
+      (unsigned long long*)scratch(ctx, "peak_keys", sizeof(unsigned long long) * kRedBlocks * (size_t)batch);
+  reduce_min_batched(ctx, map, n, stride, map_bs, batch, d_min);  // :33 min over the whole map
+  hipLaunchKernelGGL(peak_argmax_partial_kernel, dim3(kRedBlocks, batch), dim3(kRedThreads), 0, ctx->stream,
+                     map, stride, dims[0], dims[1], dims[2], mask, (const double*)d_min, part, map_bs, mask_bs);
+  hipLaunchKernelGGL(peak_com_kernel, dim3(batch), dim3(kRedThreads), 0, ctx->stream, map, stride, dims[0],
                      dims[1], dims[2], mask, (const double*)d_min, (const unsigned long long*)part,
-                     kRedBlocks, com ? com[0] : 0, com ? com[1] : 0, com ? com[2] : 0, d_res, d_add);
+                     kRedBlocks, com ? com[0] : 0, com ? com[1] : 0, com ? com[2] : 0, d_res, d_add, map_bs,
+                     mask_bs);
   EMC_HIP(hipGetLastError());
 }
 

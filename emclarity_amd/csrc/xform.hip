@@ -224,6 +224,7 @@ void launch_xform(emc_ctx* ctx, const float* in, float* out, const XformArgs& a)
 
 void launch_xform_batched(emc_ctx* ctx, const float* in, float* out, const float* d_mats,
                           const float* d_shifts, int batch, const XformArgs& a) {
+  KTimer kt(ctx, "xform3d");
   launch_tiles(ctx, in, out, a, d_mats, d_shifts, batch);
 }
 

@@ -100,6 +100,11 @@ emc_status emc_timing_read(emc_ctx* ctx, const char* name, double* ms_total, int
  *                  (workgroups over chunks of each particle's support)
  *   "grad_brick"   brick side of the brick-major adjoint, 16 or 32
  *                  (default 32)
+ *   "align_grid_batch" orientations per launch chain of emc_align_grid /
+ *                  emc_align_grid_padded, 1..64 (default 16; capped so the
+ *                  block's buffers stay within 1 GiB). Applies to one
+ *                  symmetry mate, no window padding and unshelled scores;
+ *                  otherwise, or with 1, one orientation at a time
  * box = 0 sets the value for every box size, else for that box size only
  * (which wins). emc_tune_get returns the value in effect for `box`. */
 emc_status emc_tune_set(emc_ctx* ctx, const char* name, int box, int value);

@@ -189,3 +189,32 @@ def test_align_particle_recovers_ground_truth(ctx):
     ang = np.degrees(np.arccos(np.clip((np.trace(R.T @ case.rotmat) - 1) / 2, -1, 1)))
     assert ang <= 1.0, (r.row, ang)
     assert np.max(np.abs(r.row[7:10] - case.shift)) <= 0.5, (r.row, case.shift)
+
+
+@pytest.mark.parametrize("n,mode", [(32, 3), (48, 3), (32, 1), (32, 2)])
+def test_align_grid_orientation_blocks_match_single_orientations(ctx, n, mode):
+    """Orientations in blocks (one launch chain per kernel for up to 16
+    orientations: per-entry rotation matrices, batched FFTs, per-entry peak
+    search and CC sums) against the one-orientation-at-a-time loop
+    (align_grid_batch = 1), over 37 orientations (blocks of 16, 16 and 5),
+    with two references. Batched and single FFT plans may round
+    differently: CC to 1e-5 relative, shifts to 1e-4 px."""
+    from emclarity_amd import _lib
+    truth = (40.0, 30.0, 20.0)
+    case, wedge, bp, pm, vm, rwr = _inputs(n, 13, truth, snr=1.0)
+    decoy = S.make_align_case(31, n=n, euler=truth, snr=1.0, n_blobs=16, blob_radius=0.22 * n).ref
+    pts = eulerSearch("C1", 30, 10, 30, 10)[:37]
+    assert len(pts) == 37
+    sv = np.array([0.25, -0.5, 0.0])
+    al = ParticleAligner(case.particle, [case.ref, decoy], wedge, [bp, bp], pm, vol_mask=vm, ref_wgt_rot=rwr,
+                         shift_val=sv)
+    out = {}
+    for b in (1, 16):
+        _lib.call("emc_tune_set", ctx.handle, b"align_grid_batch", 0, b)
+        out[b] = al.grid_refs(pts, sv, mode=mode, refs=[0, 1])
+    _lib.call("emc_tune_set", ctx.handle, b"align_grid_batch", 0, 16)
+    (c1, e1), (cb, eb) = out[1], out[16]
+    if mode & 2:
+        assert np.max(np.abs(cb - c1) / np.abs(c1)) < 1e-5
+    if mode & 1:
+        assert np.max(np.abs(eb - e1)) < 1e-4
