@@ -84,7 +84,6 @@ def synthetic_tomogram(seed, tomo_size, blobs, template_edge, count, snr):
     amp = math.sqrt(snr) / float(np.std(t0))
     h = template_edge // 2
     placed = []
-    occupied = np.zeros((nz // template_edge + 2, ny // template_edge + 2, nx // template_edge + 2), bool)
     tries = 0
     while len(placed) < count and tries < 200 * count:
         tries += 1
@@ -98,32 +97,40 @@ def synthetic_tomogram(seed, tomo_size, blobs, template_edge, count, snr):
         tomo[z - h:z - h + template_edge, y - h:y - h + template_edge, x - h:x - h + template_edge] -= \
             (amp * copy).astype(np.float32)
         placed.append(Inserted(c + frac, R))
-    del occupied
     return tomo, placed
 
 
-def search_peaks(tomo, template, grid, n_peaks, lattice_radius, ctx=None, chunking=None):
+def search_peaks(tomo, template, grid, n_peaks, lattice_radius, ctx=None, chunking=None, progress=None):
     """Template search and greedy peak extraction. Returns (peaks (n, 7):
     1-based x, y, z, phi, theta, psi - phi, score; seconds of the search;
-    seconds of the extraction)."""
+    seconds of the extraction). ``progress`` as for ``template_search``."""
     from .template_search import extract_peaks, template_search
     ctx = ctx or default_context()
     tsize = (tomo.shape[2], tomo.shape[1], tomo.shape[0])
     tdims = (template.shape[2], template.shape[1], template.shape[0])
     sc, va, it, pre, post = chunking or whole_chunk(tsize, tdims)
     t0 = time.perf_counter()
-    mag, ang = template_search(tomo, template, grid, sc, va, it, pre, post, ctx=ctx)
+    mag, ang = template_search(tomo, template, grid, sc, va, it, pre, post, ctx=ctx, progress=progress)
     ts = time.perf_counter() - t0
     t0 = time.perf_counter()
     r = [int(lattice_radius)] * 3
-    peaks = extract_peaks(mag, ang, grid, r, r, n_peaks / 2.0, 0.0, ctx=ctx)
+    # the peak list's angle rows are (phi, theta, psi - phi), as the
+    # reference's ANGLE_LIST (BH_templateSearch3d_2.m:740)
+    g = np.asarray(grid, dtype=np.float64).reshape(-1, 3)
+    angle_list = np.column_stack([g[:, 0], g[:, 1], g[:, 2] - g[:, 0]])
+    peaks = extract_peaks(mag, ang, angle_list, r, r, n_peaks / 2.0, 0.0, ctx=ctx)
     tp = time.perf_counter() - t0
     return peaks[:n_peaks], ts, tp
 
 
-def refine_peaks(tomo, peaks, reference, bandpass, window=96, iters=16, coarse=(0.6, 8), table="c64", ctx=None):
+def refine_peaks(tomo, peaks, reference, bandpass, window=96, iters=16, coarse=(0.6, 8), table="c64", ctx=None,
+                 mask_radius=None):
     """Cut a window around every peak and refine its pose with the adjoint,
-    the search pose kept where its CC is higher. Returns (R (n, 3, 3),
+    the search pose kept where its CC is higher. ``reference`` must have the
+    particles' contrast (for ``synthetic_tomogram``, whose copies are dark,
+    the negated template). ``mask_radius``: a soft sphere (EMC_maskShape with
+    its default roll-off) applied to every window first, so that neighbouring
+    particles inside the window stay out of the score. Returns (R (n, 3, 3),
     centres (n, 3) 1-based sub-voxel, cc (n,), valid (n,) bool, fell back
     (n,) bool, seconds)."""
     from .batched import BatchAligner, judge
@@ -138,7 +145,11 @@ def refine_peaks(tomo, peaks, reference, bandpass, window=96, iters=16, coarse=(
     views = [wins.view((window,) * 3, np.float32, nb * i) for i in range(P)]
     starts = np.stack([local_matrix(p[3], p[4], p[5] + p[3]) for p in peaks])
     al = BatchAligner(window, reference, bandpass, peak_radius=6, table=table, ctx=ctx)
-    al.set_particles(views, [None] * P, starts)
+    mask = None
+    if mask_radius:
+        from .masking import EMC_maskShape
+        mask = EMC_maskShape("sphere", (window,) * 3, [float(mask_radius)] * 3, "gpu", {}, ctx=ctx)
+    al.set_particles(views, [None] * P, starts, mask=mask)
     R0 = DeviceArray.from_numpy(np.ascontiguousarray(starts.reshape(P, 9)), ctx=ctx)
     s0 = DeviceArray.zeros((P, 3), np.float64, ctx)
     R = R0.copy()
