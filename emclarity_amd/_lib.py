@@ -39,6 +39,7 @@ SIGNATURES = {
     "emc_malloc": (_st, [_vp, C.c_size_t, C.POINTER(_vp)]),
     "emc_free": (_st, [_vp, _vp]),
     "emc_memcpy_h2d": (_st, [_vp, _vp, _vp, C.c_size_t]),
+    "emc_memcpy_h2d_async": (_st, [_vp, _vp, _vp, C.c_size_t]),
     "emc_memcpy_d2h": (_st, [_vp, _vp, _vp, C.c_size_t]),
     "emc_memcpy_d2d": (_st, [_vp, _vp, _vp, C.c_size_t]),
     "emc_memset_zero": (_st, [_vp, _vp, C.c_size_t]),
@@ -101,6 +102,8 @@ SIGNATURES = {
     "emc_batch_prepare_particle": (_st, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "emc_batch_prepare_particles": (_st, [_vp, C.c_int, C.c_int, C.POINTER(C.c_void_p), _vp,
                                           C.POINTER(C.c_void_p), _vp, _vp, _vp]),
+    "emc_batch_prepare_particles_masked": (_st, [_vp, C.c_int, C.c_int, C.POINTER(C.c_void_p), _vp, _vp,
+                                                 C.POINTER(C.c_void_p), _vp, _vp, _vp]),
     "emc_align_batched": (_st, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _d, C.c_int, C.c_int,
                                 C.c_double, _vp, _vp]),
     "emc_align_eval_grad": (_st, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, C.c_double,

@@ -1,6 +1,6 @@
 """Autotuner of the batched search's launch shapes.
 
-Six parameters shape the batched path's hot launches (emc_tune_set):
+Seven parameters shape the batched path's hot launches (emc_tune_set):
   search_tile  kx extent of the (kx, ky) column patches of the search
                schedule (ky extent 256 / search_tile): the shape of the
                reference-table region the units of one pass read together
@@ -11,6 +11,8 @@ Six parameters shape the batched path's hot launches (emc_tune_set):
                last, partly filled round of workgroups
   grad_kernel  adjoint evaluation order: brick-major (1) or support-list (0)
   grad_brick   brick side of the brick-major adjoint (16 or 32)
+  grad_waves   waves per SIMD the 32-brick adjoint's registers are
+               allocated for (4 or 5)
   grad_blocks  workgroups a support-list adjoint evaluation aims for: how
                finely each particle's support is split (fill against fold
                work)
@@ -40,9 +42,9 @@ import numpy as np
 from . import _lib
 
 CANDIDATES = {"search_tile": [4, 8, 16, 32, 64], "search_zsplit": [1, 2, 3, 4], "search_passes": [1, 2, 3, 4],
-              "grad_kernel": [0, 1], "grad_brick": [16, 32],
+              "grad_kernel": [0, 1], "grad_brick": [16, 32], "grad_waves": [4, 5],
               "grad_blocks": [2048, 4096, 8192, 16384, 32768, 65536]}
-GRAD = ("grad_kernel", "grad_brick", "grad_blocks")
+GRAD = ("grad_kernel", "grad_brick", "grad_waves", "grad_blocks")
 TABLE = Path(__file__).resolve().parent / "tuning" / "mi355x.json"
 _CACHE: dict | None = None
 

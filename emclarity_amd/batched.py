@@ -113,9 +113,11 @@ class BatchAligner:
     def nh(self):
         return (self.n // 2 + 1) * self.n * self.n
 
-    def set_particles(self, particles, wedges, starts):
+    def set_particles(self, particles, wedges, starts, mask=None):
         """particles / wedges: lists of (n,n,n) arrays (wedge None = 1);
-        starts: list of 3x3 start matrices."""
+        starts: list of 3x3 start matrices; mask: an (n,n,n) centred
+        real-space mask applied to every particle first (the reference's
+        volMask; None = none)."""
         P = len(particles)
         self.n_part = P
         self.Q = DeviceArray((P, self.nh), np.complex64, self.ctx)
@@ -128,14 +130,21 @@ class BatchAligner:
         self._d_out = DeviceArray((P, 7), np.float64, self.ctx)
         self.starts = np.stack([np.asarray(s, dtype=np.float64).reshape(3, 3) for s in starts])
         self.d_starts = DeviceArray.from_numpy(
-            np.ascontiguousarray(self.starts.transpose(0, 2, 1).reshape(P, 9)), ctx=self.ctx)
+            np.ascontiguousarray(self.starts.transpose(0, 2, 1).reshape(P, 9)), ctx=self.ctx, wait=False)
         # device copies of host inputs stay referenced until the call returns
         pd = [as_device(p, np.float32, self.ctx) for p in particles]
         wd = [None if w is None else as_device(w, np.float32, self.ctx) for w in wedges]
         hp = (C.c_void_p * P)(*[a.ptr for a in pd])
         hw = (C.c_void_p * P)(*[None if a is None else a.ptr for a in wd])
-        _lib.call("emc_batch_prepare_particles", self.ctx.handle, self.n, P, hp, self.bandpass.vp, hw,
-                  self.Q.vp, self.N.vp, self.A.vp)
+        if mask is None:
+            _lib.call("emc_batch_prepare_particles", self.ctx.handle, self.n, P, hp, self.bandpass.vp, hw,
+                      self.Q.vp, self.N.vp, self.A.vp)
+        else:
+            md = as_device(mask, np.float32, self.ctx)
+            if md.shape != (self.n,) * 3:
+                raise ValueError(f"mask must be ({self.n},)*3")
+            _lib.call("emc_batch_prepare_particles_masked", self.ctx.handle, self.n, P, hp, md.vp,
+                      self.bandpass.vp, hw, self.Q.vp, self.N.vp, self.A.vp)
 
     # ------------------------------------------------------------ search
     def _apply_tuning(self):
@@ -297,7 +306,7 @@ def search_stage(aligner: BatchAligner, grid_pts, after=None, support_frac=1.0):
     over that fraction of the support radius (a coarse grid only has to find
     the refinement's basin; the refinement uses the full support)."""
     pts = np.ascontiguousarray(np.asarray(grid_pts, dtype=np.float64).reshape(-1, 3))
-    d_pts = DeviceArray.from_numpy(pts, ctx=aligner.ctx)
+    d_pts = DeviceArray.from_numpy(pts, ctx=aligner.ctx, wait=False)
     if after is not None:
         aligner.ctx.wait_for(after)
     score, shift = aligner.search_device(pts, support_radius=support_frac * aligner.support)

@@ -17,7 +17,8 @@ batched-alignment and template-search workloads:
   pipeline template search of a 1024 x 1024 x 512 tomogram holding 600 copies
            of a 64^3 template (SNR 0.03, random orientations) over the
            7-degree grid as one whole-tomogram chunk, greedy extraction of
-           the top 1,000 peaks, 96^3 windows, batched adjoint refinement;
+           the top 1,000 peaks, 96^3 windows under a soft sphere of radius
+           26, batched adjoint refinement;
            recovery and pose errors against the inserted copies, and the
            plain per-particle grid search timed on a sample of the windows
   gradient 256 particles of 128^3 (+-60 deg wedge, own defocus, SNR 0.1):
@@ -177,13 +178,19 @@ def bench_tsearch(ctx, reps):
                   _lib.i3((pl, pl, pl)), _lib.i3((vlo, vlo, vlo)), _lib.i3((vd, vd, vd)),
                   pts.ctypes.data_as(C.POINTER(C.c_double)), n_orient, 1, 1, peak.vp, ang.vp)
 
-    wall = _timed(ctx, run, reps)
     hc = (s // 2 + 1) * s * s
     b_ts = 8 * t ** 3 + 12 * s ** 3 + 40 * hc + 24 * vd ** 3      # per (chunk, orientation)
-    per = wall / n_orient
-    _line("tsearch", wall, {"chunk": s, "template": t, "orientations": n_orient,
-                            "ms_per_orientation": round(per * 1e3, 4), "bytes_per_orientation": b_ts,
-                            "GBps": round(b_ts / per / 1e9, 1)})
+    v = C.c_int()
+    _lib.call("emc_tune_get", ctx.handle, b"tsearch_pruned", 0, C.byref(v))
+    default = v.value
+    for mode in (2, 1, 0):   # matrix-core z stage, vector z stage, padded-volume R2C
+        _lib.call("emc_tune_set", ctx.handle, b"tsearch_pruned", 0, mode)
+        wall = _timed(ctx, run, reps)
+        per = wall / n_orient
+        _line("tsearch", wall, {"chunk": s, "template": t, "orientations": n_orient, "tsearch_pruned": mode,
+                                "ms_per_orientation": round(per * 1e3, 4), "bytes_per_orientation": b_ts,
+                                "GBps": round(b_ts / per / 1e9, 1)})
+    _lib.call("emc_tune_set", ctx.handle, b"tsearch_pruned", 0, default)
 
 
 # ------------------------------------------------ batched-alignment kernels
@@ -434,9 +441,9 @@ def bench_pipeline(ctx, reps, table="c64", tomo=(1024, 1024, 512), edge=64, coun
     """Template search -> peaks -> windows -> batched refinement (one pass;
     ``reps`` is unused: the search alone runs minutes). ``max_orient`` > 0
     searches only the first that many orientations of the grid (a timing
-    sample; recovery is then not meaningful). ``pruned`` 0 / 1 forces the
-    template spectrum's form (padded-volume R2C / pruned DFT); -1 keeps the
-    library's choice."""
+    sample; recovery is then not meaningful). ``pruned`` 0 / 1 / 2 forces the
+    template spectrum's form (padded-volume R2C / pruned DFT / pruned DFT
+    with the z stage on the matrix cores); -1 keeps the library's choice."""
     from . import synthetic as S
     from .align_raw import ParticleAligner, align_particle
     from .geometry import eulerSearch
@@ -457,11 +464,18 @@ def bench_pipeline(ctx, reps, table="c64", tomo=(1024, 1024, 512), edge=64, coun
           f"chunk {whole_chunk(tomo, (edge,) * 3)[0]}", file=sys.stderr, flush=True)
     if pruned >= 0:
         _lib.call("emc_tune_set", ctx.handle, b"tsearch_pruned", 0, int(pruned))
-    peaks, ts, tp = search_peaks(vol, template, grid, n_peaks, 22, ctx=ctx)
+    t_start = time.perf_counter()
+
+    def progress(done, total):
+        print(f"[bench_paths] pipeline: theta block {done}/{total} searched, "
+              f"{time.perf_counter() - t_start:.1f} s", file=sys.stderr, flush=True)
+
+    peaks, ts, tp = search_peaks(vol, template, grid, n_peaks, 22, ctx=ctx, progress=progress)
     print(f"[bench_paths] pipeline: search {ts:.1f} s, {len(peaks)} peaks in {tp:.2f} s", file=sys.stderr, flush=True)
-    reference = render_fast(blobs, window).astype(np.float32)
+    reference = (-render_fast(blobs, window)).astype(np.float32)   # the copies' contrast (dark)
     bp = BH_bandpass3d((window,) * 3, 1e-3, 400.0, 2.5 * pixel, "GPU", pixel)
-    R, centres, cc, valid, back, tr = refine_peaks(vol, peaks, reference, bp, window=window, table=table, ctx=ctx)
+    R, centres, cc, valid, back, tr = refine_peaks(vol, peaks, reference, bp, window=window, table=table, ctx=ctx,
+                                                   mask_radius=26)
     starts = np.stack([local_matrix(p[3], p[4], p[5] + p[3]) for p in peaks])
     refined = score_against_truth(inserted, R, centres)
     searched = score_against_truth(inserted, starts, peaks[:, :3])
@@ -513,8 +527,9 @@ def main(argv=None):
     ap.add_argument("--max-orient", type=int, default=0,
                     help="pipeline: search only the first N orientations of the grid (timing sample)")
     ap.add_argument("--step", type=float, default=7.0, help="pipeline: angular step of the search grid (deg)")
-    ap.add_argument("--pruned", type=int, default=-1, choices=[-1, 0, 1],
-                    help="pipeline: template spectrum by pruned DFT (1) or padded R2C (0); -1 = library default")
+    ap.add_argument("--pruned", type=int, default=-1, choices=[-1, 0, 1, 2],
+                    help="pipeline: template spectrum by pruned DFT (1; 2 with the z stage on the matrix cores) "
+                         "or padded R2C (0); -1 = library default")
     a = ap.parse_args(argv)
     ctx = default_context()
     for name in ([p for p in PATHS if p not in ("gradient", "pipeline")] if a.path == "all" else [a.path]):

@@ -78,14 +78,19 @@ struct PrepPtrs {
 };
 
 // ifftshift of each chunk volume (centre n/2 -> origin) into a contiguous
-// batch; blockIdx.z = z + n * particle.
-__global__ void ifftshift_many_kernel(PrepPtrs pp, float* __restrict__ out, int n) {
+// batch, times the centred real-space mask when one is given (the volMask
+// the reference multiplies the rotated particle by, BH_alignRaw3d_v2.m:1375;
+// a mask symmetric about the centre commutes with the rotation);
+// blockIdx.z = z + n * particle.
+__global__ void ifftshift_many_kernel(PrepPtrs pp, const float* __restrict__ mask, float* __restrict__ out, int n) {
   const int X = blockIdx.x * blockDim.x + threadIdx.x;
   const int Y = blockIdx.y * blockDim.y + threadIdx.y;
   const int Z = blockIdx.z % n, b = blockIdx.z / n;
   if (X >= n || Y >= n) return;
   const int x = shifted_index(X, n), y = shifted_index(Y, n), z = shifted_index(Z, n);
-  out[(((int64_t)b * n + Z) * n + Y) * n + X] = pp.part[b][((int64_t)z * n + y) * n + x];
+  const int64_t c = ((int64_t)z * n + y) * n + x;
+  const float v = pp.part[b][c];
+  out[(((int64_t)b * n + Z) * n + Y) * n + X] = mask ? v * mask[c] : v;
 }
 
 // Q, N in the fused kernel's layout [kz][kx][ky] (ky fastest, coalesced for
@@ -772,8 +777,12 @@ struct BrickRows {
   static constexpr int kMax = (S * 7 / 8 + 3) * (S * 7 / 8 + 3);  // rows of one pass
 };
 
-template <int S, class Tab, bool GEN>
-__global__ void __launch_bounds__(kBatchThreads) align_eval_brick_kernel(
+// WPE: the waves per SIMD the register allocation aims for (4: up to 128
+// VGPRs; 5: at most 102, the brick kernel fits 96 without spills); the
+// kernel waits on dependent gathers, so more resident waves hide more of it.
+template <int S, class Tab, bool GEN, int WPE>
+__global__ void __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
+align_eval_brick_kernel(
     const typename Tab::T* __restrict__ F2, int n, const float2* __restrict__ Q, const float* __restrict__ Nw,
     const float* __restrict__ MdM, const float* __restrict__ shifts, const int4* __restrict__ bricks, int nb,
     int n_part, int brick_slots, double r2, double* __restrict__ part) {
@@ -1235,9 +1244,11 @@ static const int4* brick_list(emc_ctx* ctx, int n, double radius, int S, int* co
 
 // Which adjoint kernel ("grad_kernel": 1 brick-major, 0 support-list order)
 // and the brick side ("grad_brick": 16 or 32).
-void eval_grad_kind(emc_ctx* ctx, int n, int* kind, int* S) {
+void eval_grad_kind(emc_ctx* ctx, int n, int* kind, int* S, int* waves) {
   *kind = tune_value(ctx, "grad_kernel", n, 1) ? 1 : 0;
   *S = tune_value(ctx, "grad_brick", n, 32) == 16 ? 16 : 32;
+  // 5 waves per SIMD only for the 32-brick kernel (the 16-brick one keeps 4)
+  *waves = (*S == 32 && tune_value(ctx, "grad_waves", n, 4) >= 5) ? 5 : 4;
 }
 
 // Score + gradient launch shared by emc_align_eval_grad and the refinement:
@@ -1246,7 +1257,7 @@ void eval_grad_kind(emc_ctx* ctx, int n, int* kind, int* S) {
 // a graph.
 EvalSetup eval_grad_setup(emc_ctx* ctx, int n, int n_part, double support_radius) {
   EvalSetup e{};
-  eval_grad_kind(ctx, n, &e.kind, &e.S);
+  eval_grad_kind(ctx, n, &e.kind, &e.S, &e.waves);
   e.r2 = support_radius * support_radius;
   if (e.kind == 1) {
     e.list = brick_list(ctx, n, support_radius, e.S, &e.ns);
@@ -1267,11 +1278,15 @@ static void launch_brick(emc_ctx* ctx, const EvalSetup& e, int n, int n_part, co
   const int slots = ceil_div(e.ns, 8);
   const dim3 grid((unsigned)(8 * slots * n_part));
   if (e.S == 16)
-    hipLaunchKernelGGL((align_eval_brick_kernel<16, Tab, GEN>), grid, dim3(kBatchThreads), 0, ctx->stream,
+    hipLaunchKernelGGL((align_eval_brick_kernel<16, Tab, GEN, 4>), grid, dim3(kBatchThreads), 0, ctx->stream,
+                       (const T*)d_F2, n, (const float2*)d_Q, d_N, d_MdM, d_shift, (const int4*)e.list, e.ns, n_part,
+                       slots, e.r2, e.part);
+  else if (e.waves == 5)
+    hipLaunchKernelGGL((align_eval_brick_kernel<32, Tab, GEN, 5>), grid, dim3(kBatchThreads), 0, ctx->stream,
                        (const T*)d_F2, n, (const float2*)d_Q, d_N, d_MdM, d_shift, (const int4*)e.list, e.ns, n_part,
                        slots, e.r2, e.part);
   else
-    hipLaunchKernelGGL((align_eval_brick_kernel<32, Tab, GEN>), grid, dim3(kBatchThreads), 0, ctx->stream,
+    hipLaunchKernelGGL((align_eval_brick_kernel<32, Tab, GEN, 4>), grid, dim3(kBatchThreads), 0, ctx->stream,
                        (const T*)d_F2, n, (const float2*)d_Q, d_N, d_MdM, d_shift, (const int4*)e.list, e.ns, n_part,
                        slots, e.r2, e.part);
 }
@@ -1549,6 +1564,13 @@ emc_status emc_batch_reference_spectrum(emc_ctx* ctx, int n, const float* d_ref,
 emc_status emc_batch_prepare_particles(emc_ctx* ctx, int n, int n_part, const float* const* h_particles,
                                        const float* d_bandpass, const float* const* h_wedges, void* d_Q,
                                        float* d_N, double* d_A) {
+  return emc_batch_prepare_particles_masked(ctx, n, n_part, h_particles, nullptr, d_bandpass, h_wedges, d_Q, d_N,
+                                            d_A);
+}
+
+emc_status emc_batch_prepare_particles_masked(emc_ctx* ctx, int n, int n_part, const float* const* h_particles,
+                                              const float* d_mask, const float* d_bandpass,
+                                              const float* const* h_wedges, void* d_Q, float* d_N, double* d_A) {
   return guard([&] {
     use_device(ctx);
     EMC_REQUIRE(n >= 8 && n <= kMaxN, "box size must be 8..%d", kMaxN);
@@ -1571,7 +1593,7 @@ emc_status emc_batch_prepare_particles(emc_ctx* ctx, int n, int n_part, const fl
         pp.wedge[b] = h_wedges ? h_wedges[i0 + b] : nullptr;
       }
       hipLaunchKernelGGL(ifftshift_many_kernel, dim3(ceil_div(n, 64), ceil_div(n, 4), n * B), dim3(64, 4, 1),
-                         0, ctx->stream, pp, sh, n);
+                         0, ctx->stream, pp, d_mask, sh, n);
       EMC_HIP(hipGetLastError());
       fft_r2c(cached_plan(ctx, n, n, n, B), sh, P);
       {
@@ -1612,10 +1634,7 @@ emc_status emc_align_batched(emc_ctx* ctx, int n, int n_part, int table_format, 
     const int units = (int)units64;
     double* d_pts = (double*)scratch(ctx, "bat_pts", sizeof(double) * 3 * n_orient);
     float* d_M = (float*)scratch(ctx, "bat_M", sizeof(float) * 9 * (size_t)units);
-    double* h = (double*)pinned(ctx, sizeof(double) * 3 * n_orient);
-    EMC_HIP(hipStreamSynchronize(ctx->stream));  // the pinned buffer may be in use
-    std::memcpy(h, angles_pts, sizeof(double) * 3 * n_orient);
-    EMC_HIP(hipMemcpyAsync(d_pts, h, sizeof(double) * 3 * n_orient, hipMemcpyHostToDevice, ctx->stream));
+    upload_async(ctx, d_pts, angles_pts, sizeof(double) * 3 * n_orient);
     hipLaunchKernelGGL(compose_kernel, dim3(ceil_div(units, 256)), dim3(256), 0, ctx->stream, d_start_cm,
                        (const double*)d_pts, n_part, n_orient, d_M);
     EMC_HIP(hipGetLastError());
@@ -1701,7 +1720,7 @@ emc_status emc_align_refine(emc_ctx* ctx, int n, int n_part, int table_format, c
       body(true);
       return;
     }
-    const int ki[8] = {n, n_part, table_format, iters, e.ns, e.g, e.kind, e.S};
+    const int ki[9] = {n, n_part, table_format, iters, e.ns, e.g, e.kind, e.S, e.waves};
     const double kd[3] = {step_angle, step_shift, support_radius};
     const void* kp[12] = {d_F2, d_Q, d_N, d_A, d_R, d_s, d_cc, base, e.list, e.part, nullptr, nullptr};
     std::vector<unsigned char> kb(sizeof(ki) + sizeof(kd) + sizeof(kp));

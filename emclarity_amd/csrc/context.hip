@@ -56,6 +56,57 @@ void* pinned(emc_ctx* ctx, size_t bytes) {
   return ctx->h_pinned;
 }
 
+// ------------------------------------------------------------- staging
+
+constexpr size_t kRingBytes = size_t(32) << 20;
+
+void upload_async(emc_ctx* ctx, void* d_dst, const void* src, size_t bytes) {
+  if (bytes == 0) return;
+  const size_t need = (bytes + 255) & ~size_t(255);
+  if (need > kRingBytes / 4) {  // large: the copy waits for the stream
+    EMC_HIP(hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    EMC_HIP(hipStreamSynchronize(ctx->stream));
+    return;
+  }
+  if (!ctx->h_ring) {
+    EMC_HIP(hipHostMalloc((void**)&ctx->h_ring, kRingBytes, hipHostMallocDefault));
+    ctx->ring_bytes = kRingBytes;
+    ctx->ring_head = 0;
+  }
+  auto& busy = ctx->ring_busy;
+  // retire finished uploads, oldest first (they are kept in upload order)
+  size_t done = 0;
+  while (done < busy.size() && hipEventQuery(busy[done].ev) == hipSuccess) {
+    ctx->ring_events.push_back(busy[done].ev);
+    ++done;
+  }
+  busy.erase(busy.begin(), busy.begin() + (std::ptrdiff_t)done);
+  if (ctx->ring_head + need > ctx->ring_bytes) ctx->ring_head = 0;
+  const size_t beg = ctx->ring_head, end = beg + need;
+  // wait for (and retire) every earlier upload whose region this one reuses
+  for (size_t i = 0; i < busy.size();) {
+    if (busy[i].beg < end && beg < busy[i].end) {
+      EMC_HIP(hipEventSynchronize(busy[i].ev));
+      ctx->ring_events.push_back(busy[i].ev);
+      busy.erase(busy.begin() + (std::ptrdiff_t)i);
+    } else {
+      ++i;
+    }
+  }
+  std::memcpy(ctx->h_ring + beg, src, bytes);
+  EMC_HIP(hipMemcpyAsync(d_dst, ctx->h_ring + beg, bytes, hipMemcpyHostToDevice, ctx->stream));
+  hipEvent_t ev;
+  if (!ctx->ring_events.empty()) {
+    ev = ctx->ring_events.back();
+    ctx->ring_events.pop_back();
+  } else {
+    EMC_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  }
+  EMC_HIP(hipEventRecord(ev, ctx->stream));
+  busy.push_back({beg, end, ev});
+  ctx->ring_head = end;
+}
+
 // ------------------------------------------------------------- timing
 
 static hipEvent_t take_event(emc_ctx* c) {
@@ -263,10 +314,11 @@ constexpr TuneRange kTune[] = {{"search_tile", 2, 128, 16},
                                {"search_passes", 1, 4, 2},
                                {"refine_graph", 0, 1, 1},
                                {"search_zsplit", 1, 4, 1},
-                               {"tsearch_pruned", 0, 1, 1},
+                               {"tsearch_pruned", 0, 2, 1},
                                {"grad_kernel", 0, 1, 1},
                                {"grad_brick", 16, 32, 32},
-                               {"align_grid_batch", 1, 64, 16}};
+                               {"align_grid_batch", 1, 64, 16},
+                               {"grad_waves", 4, 5, 4}};
 
 const TuneRange* find_tune(const char* name) {
   for (const auto& t : kTune)
@@ -322,6 +374,9 @@ emc_status emc_ctx_destroy(emc_ctx* ctx) {
       (void)hipEventDestroy(p.b);
     }
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+    for (auto& r : ctx->ring_busy) (void)hipEventDestroy(r.ev);
+    for (auto e : ctx->ring_events) (void)hipEventDestroy(e);
+    if (ctx->h_ring) (void)hipHostFree(ctx->h_ring);
     for (auto& kv : ctx->graphs)
       if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
     for (auto& kv : ctx->scratch)
@@ -454,6 +509,14 @@ emc_status emc_memcpy_h2d(emc_ctx* ctx, void* d_dst, const void* src, size_t byt
     use_device(ctx);
     EMC_HIP(hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
     EMC_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+emc_status emc_memcpy_h2d_async(emc_ctx* ctx, void* d_dst, const void* src, size_t bytes) {
+  return guard([&] {
+    use_device(ctx);
+    EMC_REQUIRE(d_dst && (src || bytes == 0), "null pointer");
+    upload_async(ctx, d_dst, src, bytes);
   });
 }
 

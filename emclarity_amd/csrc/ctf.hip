@@ -469,15 +469,13 @@ emc_status emc_weight_mask_batched(emc_ctx* ctx, const int dims[3], double sampl
       cuts.push_back(p1);
     }
     EMC_REQUIRE(max_nt <= 65535, "a particle has more than 65535 tilts");
-    // tilt table and absolute first-tilt offsets through one pinned buffer
+    // tilt table and absolute first-tilt offsets, staged without waiting for
+    // the work already queued
     const size_t tab_bytes = sizeof(CtfTilt) * all.size();
     const size_t table_bytes = tab_bytes + sizeof(int) * (n_part + 1);
-    char* h_tab = (char*)pinned(ctx, table_bytes);
-    EMC_HIP(hipStreamSynchronize(ctx->stream));  // the pinned buffer may be in use
-    std::memcpy(h_tab, all.data(), tab_bytes);
-    std::memcpy(h_tab + tab_bytes, first.data(), sizeof(int) * (n_part + 1));
     char* d_tab = (char*)scratch(ctx, "wmb_tilts", table_bytes);
-    EMC_HIP(hipMemcpyAsync(d_tab, h_tab, table_bytes, hipMemcpyHostToDevice, ctx->stream));
+    upload_async(ctx, d_tab, all.data(), tab_bytes);
+    upload_async(ctx, d_tab + tab_bytes, first.data(), sizeof(int) * (n_part + 1));
     const CtfTilt* d_t = (const CtfTilt*)d_tab;
     const int* d_off = (const int*)(d_tab + tab_bytes);
     float* d_ctf = (float*)scratch(ctx, "wmb_stack", tilt_bytes * (size_t)max_nt);

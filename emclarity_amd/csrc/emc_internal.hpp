@@ -91,6 +91,17 @@ struct emc_ctx {
   size_t cached_bytes = 0;
   double* h_pinned = nullptr;  // small pinned host buffer for results
   size_t h_pinned_bytes = 0;
+  // pinned staging ring for host-to-device uploads that must not wait for
+  // the stream: each upload's region is held until the event recorded after
+  // its copy has passed
+  unsigned char* h_ring = nullptr;
+  size_t ring_bytes = 0, ring_head = 0;
+  struct StageRegion {
+    size_t beg, end;
+    hipEvent_t ev;
+  };
+  std::vector<StageRegion> ring_busy;
+  std::vector<hipEvent_t> ring_events;  // free events (timing disabled)
   // optional per-kernel timing with HIP events on the context stream
   bool timing = false;
   struct Pending {
@@ -150,6 +161,12 @@ void fft_c2c(emc_fft_plan* p, const void* in, void* out, int direction);
 void reduce_sum_sumsq(emc_ctx* ctx, const float* d, int64_t n, double* d_out2);
 // minimum of a real array read with a stride (stride 2 reads real parts of
 // complex data), out[0] = min as double.
+// Copy `bytes` from host memory to d_dst on the context's stream through the
+// pinned staging ring: returns once `src` may be reused, without waiting for
+// the work already queued (uploads above a quarter of the ring wait for the
+// stream instead).
+void upload_async(emc_ctx* ctx, void* d_dst, const void* src, size_t bytes);
+
 void reduce_min(emc_ctx* ctx, const float* d, int64_t n, int stride, double* d_out1);
 // minima of `batch` volumes of n values (stride between values, bstride floats
 // between volumes) into d_out[0 .. batch)

@@ -512,13 +512,9 @@ emc_status emc_extract_subtomos(emc_ctx* ctx, const void* d_tomo, int mode, cons
     }
     const size_t bytes = sizeof(Window) * (size_t)n;
     Window* d_wins = static_cast<Window*>(scratch(ctx, "extract_windows", bytes));
-    // the shared pinned staging may still feed an earlier copy: wait, fill,
-    // copy, and wait for the copy, so that the cut itself stays queued
-    EMC_HIP(hipStreamSynchronize(ctx->stream));
-    void* stage = pinned(ctx, bytes);
-    std::memcpy(stage, wins.data(), bytes);
-    EMC_HIP(hipMemcpyAsync(d_wins, stage, bytes, hipMemcpyHostToDevice, ctx->stream));
-    EMC_HIP(hipStreamSynchronize(ctx->stream));
+    // the window table through the staging ring: the cut stays queued
+    // behind the work already on the stream
+    upload_async(ctx, d_wins, wins.data(), bytes);
     const int64_t quads = (int64_t)((window[0] + 3) / 4) * window[1] * window[2];
     const int64_t tnx = tomo_dims[0], tny = tomo_dims[1];
     const int vec4 = (window[0] % 4 == 0) && ((reinterpret_cast<uintptr_t>(d_out) & 15) == 0);

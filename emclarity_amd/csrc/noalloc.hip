@@ -72,7 +72,7 @@ emc_status emc_align_eval_grad_noalloc(emc_ctx* ctx, int n, int n_part, int tabl
     DeviceState& st = device_state(ctx->device);
     unsigned char* base = (unsigned char*)st.ptr;
     emc::EvalSetup e{};
-    emc::eval_grad_kind(ctx, n, &e.kind, &e.S);
+    emc::eval_grad_kind(ctx, n, &e.kind, &e.S, &e.waves);
     e.r2 = support_radius * support_radius;
     const bool same = st.n == n && st.radius == support_radius && st.kind == e.kind && st.S == e.S;
     // the list in host memory only: bricks (kind 1) or support frequencies

@@ -110,12 +110,12 @@ struct EvalSetup {
   const void* list;
   int ns, g;
   double* part;
-  int kind, S;
+  int kind, S, waves;
   double r2;
 };
 
 int eval_grad_blocks(emc_ctx* ctx, int n, int n_part, int ns);
-void eval_grad_kind(emc_ctx* ctx, int n, int* kind, int* S);
+void eval_grad_kind(emc_ctx* ctx, int n, int* kind, int* S, int* waves);
 // the adjoint kernel + adjoint_fold_kernel on the context's stream (no
 // allocation, copy or host wait); timed adds HIP events around the kernel;
 // gen: d_MdM holds the derivatives of right-multiplied rotation increments

@@ -186,13 +186,80 @@ __global__ void __launch_bounds__(256) tdft_z_mul_kernel(const float2* __restric
   }
 }
 
+// The same z stage on the matrix cores. For one orientation b, a tile of 32
+// columns (ky, kx) and a tile of 32 kz, the product of the 32 x ZP twiddle
+// block e_z(kz, z) by the ZP x 32 block of partial DFTs X2[b][z][col] is a
+// complex matrix product: four FP32 MFMA products per pair of z
+// (Re = Er Xr - Ei Xi, Im = Er Xi + Ei Xr) with v_mfma_f32_32x32x2f32, then
+// each lane multiplies its 16 outputs by the tomogram spectrum and writes
+// them (per output register, lanes 0-31 and 32-63 each write 32 consecutive
+// columns of one kz row: 256-byte segments).
+// Operand layout of the 32x32x2 FP32 MFMA (one wave): A (32 x 2), lane l
+// holds A[l % 32][l / 32]; B (2 x 32), lane l holds B[l / 32][l % 32];
+// C (32 x 32), lane l holds C[8 (i / 4) + 4 (l / 32) + i % 4][l % 32] in
+// its register i = 0..15.
+// Block: 4 waves, one kz tile each (grid z covers kz in steps of 128),
+// grid x the column tiles, grid y the orientations of the block.
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+template <int ZP>
+__global__ void __launch_bounds__(256) tdft_z_mul_mfma_kernel(const float2* __restrict__ X2,
+                                                              const float2* __restrict__ tomo, int nz, int ny,
+                                                              int cx, const float2* __restrict__ ez,
+                                                              float2* __restrict__ out) {
+  const int b = blockIdx.y;
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  const int kt = blockIdx.z * 4 + wave;  // kz tile of this wave
+  if (kt * 32 >= nz) return;             // uniform per wave; no barrier below
+  const int64_t ncol = (int64_t)ny * cx;
+  const int64_t col = (int64_t)blockIdx.x * 32 + (lane % 32);
+  const bool col_ok = col < ncol;
+  const int kh = lane / 32;  // which z of each pair this lane supplies
+  // B operand: X2[b][2 s + kh][col] for s < ZP / 2
+  const float2* xb = X2 + (int64_t)b * ZP * ncol + (col_ok ? col : 0);
+  float2 xv[ZP / 2];
+#pragma unroll
+  for (int s2 = 0; s2 < ZP / 2; ++s2)
+    xv[s2] = col_ok ? xb[(int64_t)(2 * s2 + kh) * ncol] : make_float2(0.0f, 0.0f);
+  // A operand: e_z(kz, 2 s + kh) for the lane's row kz of the tile
+  const int kza = kt * 32 + (lane % 32);
+  const bool kz_ok = kza < nz;
+  const float2* ea = ez + (int64_t)(kz_ok ? kza : 0) * kTMax + kh;
+  floatx16 cr, ci;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    cr[i] = 0.0f;
+    ci[i] = 0.0f;
+  }
+#pragma unroll
+  for (int s2 = 0; s2 < ZP / 2; ++s2) {
+    const float2 e = kz_ok ? ea[2 * s2] : make_float2(0.0f, 0.0f);
+    const float2 x = xv[s2];
+    cr = __builtin_amdgcn_mfma_f32_32x32x2f32(e.x, x.x, cr, 0, 0, 0);
+    cr = __builtin_amdgcn_mfma_f32_32x32x2f32(-e.y, x.y, cr, 0, 0, 0);
+    ci = __builtin_amdgcn_mfma_f32_32x32x2f32(e.x, x.y, ci, 0, 0, 0);
+    ci = __builtin_amdgcn_mfma_f32_32x32x2f32(e.y, x.x, ci, 0, 0, 0);
+  }
+  if (!col_ok) return;
+  float2* ob = out + (int64_t)b * nz * ncol + col;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int kz = kt * 32 + 8 * (i / 4) + 4 * kh + (i % 4);
+    if (kz < nz) {
+      const float fr = cr[i], fi = ci[i];
+      const float2 tv = tomo[(int64_t)kz * ncol + col];
+      ob[(int64_t)kz * ncol] = make_float2(tv.x * fr + tv.y * fi, tv.y * fr - tv.x * fi);
+    }
+  }
+}
+
 int pruned_pad(int t) { return t <= 16 ? 16 : t <= 32 ? 32 : t <= 48 ? 48 : 64; }
 
 // the separable pruned template spectrum times the tomogram spectrum for a
 // block of nb rotated templates tb[nb][tz][ty][tx] (pad_low = p)
 void pruned_spectrum_mul(emc_ctx* ctx, const float* tb, const int td[3], const int cd[3], const float2* tomo,
                          const float2* ex, const float2* ey, const float2* ez, int nb, float2* X1, float2* X2,
-                         float2* out) {
+                         float2* out, bool mfma) {
   const int cx = cd[0] / 2 + 1, ny = cd[1], nz = cd[2];
   const int xp = pruned_pad(td[0]), yp = pruned_pad(td[1]), zp = pruned_pad(td[2]);
   const int kx_chunk = 32;
@@ -222,6 +289,17 @@ void pruned_spectrum_mul(emc_ctx* ctx, const float* tb, const int td[3], const i
     case 32: hipLaunchKernelGGL(tdft_y_kernel<32>, g2, dim3(256), 0, ctx->stream, (const float2*)X1, td[2], zp, ny, cx, ey, ky_chunk, X2); break;
     case 48: hipLaunchKernelGGL(tdft_y_kernel<48>, g2, dim3(256), 0, ctx->stream, (const float2*)X1, td[2], zp, ny, cx, ey, ky_chunk, X2); break;
     default: hipLaunchKernelGGL(tdft_y_kernel<64>, g2, dim3(256), 0, ctx->stream, (const float2*)X1, td[2], zp, ny, cx, ey, ky_chunk, X2); break;
+  }
+  if (mfma) {
+    const dim3 gm(ceil_div((int64_t)ny * cx, 32), nb, ceil_div(nz, 128));
+    switch (zp) {
+      case 16: hipLaunchKernelGGL(tdft_z_mul_mfma_kernel<16>, gm, dim3(256), 0, ctx->stream, (const float2*)X2, tomo, nz, ny, cx, ez, out); break;
+      case 32: hipLaunchKernelGGL(tdft_z_mul_mfma_kernel<32>, gm, dim3(256), 0, ctx->stream, (const float2*)X2, tomo, nz, ny, cx, ez, out); break;
+      case 48: hipLaunchKernelGGL(tdft_z_mul_mfma_kernel<48>, gm, dim3(256), 0, ctx->stream, (const float2*)X2, tomo, nz, ny, cx, ez, out); break;
+      default: hipLaunchKernelGGL(tdft_z_mul_mfma_kernel<64>, gm, dim3(256), 0, ctx->stream, (const float2*)X2, tomo, nz, ny, cx, ez, out); break;
+    }
+    EMC_HIP(hipGetLastError());
+    return;
   }
   const int kz_chunk = 128;
   const dim3 g3(ceil_div((int64_t)ny * cx, 256), nb, ceil_div(nz, kz_chunk));
@@ -358,12 +436,13 @@ emc_status emc_template_search_chunk(emc_ctx* ctx, const float* d_chunk, const i
     const int64_t nh = (int64_t)(cd[0] / 2 + 1) * cd[1] * cd[2];
     const int64_t nvalid = (int64_t)vd[0] * vd[1] * vd[2];
     EMC_REQUIRE(nvalid > 1, "valid area must hold more than one voxel");
-    // orientation block: up to 16 orientations, bounded by 16 GiB of batch
-    // buffers (a 480^3 chunk takes ~0.93 GB per orientation: the C2R map, the
-    // product spectrum and the template's partial DFT); one pass of the
-    // max/argmax update then serves the whole block
+    // orientation block: up to 16 orientations, bounded by 48 GiB of batch
+    // buffers (a sixth of MI355X's HBM; a 480^3 chunk takes ~0.93 GB per
+    // orientation: the C2R map, the product spectrum and the template's
+    // partial DFT; a whole 1152 x 1152 x 576 tomogram chunk ~6.5 GB); one pass
+    // of the max/argmax update then serves the whole block
     const int64_t per_orient = nc * 4 + nh * 8 + (int64_t)64 * cd[1] * (cd[0] / 2 + 1) * 8;
-    int blk = (int)std::max<int64_t>(1, std::min<int64_t>(16, ((int64_t)16 << 30) / per_orient));
+    int blk = (int)std::max<int64_t>(1, std::min<int64_t>(16, ((int64_t)48 << 30) / per_orient));
     blk = std::min(blk, n_orient);
     emc_fft_plan* plan1 = cached_plan(ctx, cd[0], cd[1], cd[2], 1);
     emc_fft_plan* planb = cached_plan(ctx, cd[0], cd[1], cd[2], blk);
@@ -385,8 +464,8 @@ emc_status emc_template_search_chunk(emc_ctx* ctx, const float* d_chunk, const i
 
     // pruned template spectrum (every template edge <= kTMax; "tsearch_pruned"
     // 0 selects the padded-volume R2C instead)
-    const bool pruned = tune_value(ctx, "tsearch_pruned", 0, 1) != 0 && td[0] <= kTMax && td[1] <= kTMax &&
-                        td[2] <= kTMax;
+    const int pmode = tune_value(ctx, "tsearch_pruned", 0, 1);
+    const bool pruned = pmode != 0 && td[0] <= kTMax && td[1] <= kTMax && td[2] <= kTMax;
     const int cx = cd[0] / 2 + 1;
     float* tblk = nullptr;
     float2 *X1 = nullptr, *X2 = nullptr, *ex = nullptr, *ey = nullptr, *ez = nullptr;
@@ -430,7 +509,7 @@ emc_status emc_template_search_chunk(emc_ctx* ctx, const float* d_chunk, const i
         // padded position (:751-756) times the tomogram spectrum
         XformArgs xa = xform_args(td, nullptr, 1, one3, 1);
         launch_xform_batched(ctx, d_template, tblk, mats, mats + 9 * blk, nb, xa);
-        pruned_spectrum_mul(ctx, tblk, td, cd, (const float2*)tomo, ex, ey, ez, nb, X1, X2, T);
+        pruned_spectrum_mul(ctx, tblk, td, cd, (const float2*)tomo, ex, ey, ez, nb, X1, X2, T, pmode == 2);
       } else {
         // rotate into the zero-padded chunk (:744-754)
         XformArgs xa = xform_args(td, nullptr, 1, one3, 1);

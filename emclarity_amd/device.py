@@ -84,10 +84,14 @@ class DeviceArray:
         return a
 
     @classmethod
-    def from_numpy(cls, arr, dtype=None, ctx=None):
+    def from_numpy(cls, arr, dtype=None, ctx=None, wait=True):
+        """A device copy of ``arr``. wait=False: staged through the context's
+        pinned ring (emc_memcpy_h2d_async), ordered on its stream without
+        waiting for the work already queued there; use the array on that
+        context's stream only."""
         arr = np.ascontiguousarray(arr, dtype=dtype or arr.dtype)
         a = cls(arr.shape, arr.dtype, ctx)
-        _lib.call("emc_memcpy_h2d", a.ctx.handle, C.c_void_p(a.ptr),
+        _lib.call("emc_memcpy_h2d" if wait else "emc_memcpy_h2d_async", a.ctx.handle, C.c_void_p(a.ptr),
                   arr.ctypes.data_as(C.c_void_p), C.c_size_t(a.nbytes))
         return a
 

@@ -79,7 +79,7 @@ def preprocess_chunks(padded, corners, size_chunk, valid_calc, bp_vals, pixel, s
 
 
 def template_search(tomogram, template, angles_pts, size_chunk, valid_area, n_iters, tomo_pre,
-                    tomo_post, ctx=None, preprocess=None):
+                    tomo_post, ctx=None, preprocess=None, progress=None):
     """Per-voxel max CC and angle id (1-based into angles_pts, 0 where no
     orientation scored above 0 in a chunk other than the first, as the
     reference initialises RESULTS with zeros). Returns (peak, angle) with the
@@ -87,7 +87,8 @@ def template_search(tomogram, template, angles_pts, size_chunk, valid_area, n_it
     order theta -> phi -> psi; blocks of one theta are passed per chunk call.
     ``preprocess``: None (the tomogram is already pre-processed) or a dict
     with valid_calc, bp_vals, pixel, stats_radius and should_be_ctf for
-    ``preprocess_chunks``."""
+    ``preprocess_chunks``. ``progress``: called as progress(done, total)
+    after each theta block (all chunks)."""
     tomogram = np.asarray(tomogram, dtype=np.float32)
     template = np.asarray(template, dtype=np.float32)
     nz, ny, nx = tomogram.shape
@@ -121,13 +122,16 @@ def template_search(tomogram, template, angles_pts, size_chunk, valid_area, n_it
     angs = [DeviceArray.zeros(vshape, np.int32, ctx) for _ in cuts]
     dp = _lib.C.POINTER(_lib.C.c_double)
     first = True
-    for (_, a0, a1) in thetas:
+    for ti, (_, a0, a1) in enumerate(thetas):
         for k, ch in enumerate(chunks):
             _lib.call("emc_template_search_chunk", ctx.handle, ch.vp, _lib.i3(sc), t_dev.vp,
                       _lib.i3(td), _lib.i3(pad_low), _lib.i3(v_low), _lib.i3(va),
                       pts[a0:a1].ctypes.data_as(dp), int(a1 - a0), int(a0 + 1), int(first and k == 0),
                       peaks[k].vp, angs[k].vp)
         first = False
+        if progress is not None:
+            ctx.synchronize()
+            progress(ti + 1, len(thetas))
     res_peak = np.zeros(padded.shape, np.float32)
     res_ang = np.zeros(padded.shape, np.int32)
     for (cx, cy, cz), p, a in zip(cuts, peaks, angs):

@@ -89,8 +89,10 @@ emc_status emc_timing_read(emc_ctx* ctx, const char* name, double* ms_total, int
  *   "tsearch_pruned" 1 (default): emc_template_search_chunk forms the
  *                  padded template's spectrum as direct DFTs over the
  *                  template block (every template edge <= 64) fused with
- *                  the product by the tomogram spectrum; 0: rocFFT R2C of
- *                  the zero-padded volume (set with box 0)
+ *                  the product by the tomogram spectrum; 2: the same with
+ *                  the z-direction DFT and the product as FP32 matrix-core
+ *                  (MFMA) products; 0: rocFFT R2C of the zero-padded volume
+ *                  (set with box 0)
  *   "refine_graph" 1 (default): emc_align_refine replays its iteration
  *                  chain as a captured HIP graph; 0: eager launches
  *   "grad_kernel"  adjoint evaluation order: 1 (default) brick-major (the
@@ -100,6 +102,8 @@ emc_status emc_timing_read(emc_ctx* ctx, const char* name, double* ms_total, int
  *                  (workgroups over chunks of each particle's support)
  *   "grad_brick"   brick side of the brick-major adjoint, 16 or 32
  *                  (default 32)
+ *   "grad_waves"   waves per SIMD the 32-brick adjoint kernel's registers
+ *                  are allocated for, 4 (default) or 5
  *   "align_grid_batch" orientations per launch chain of emc_align_grid /
  *                  emc_align_grid_padded, 1..64 (default 16; capped so the
  *                  block's buffers stay within 1 GiB). Applies to one
@@ -121,6 +125,12 @@ emc_status emc_tune_get(emc_ctx* ctx, const char* name, int box, int* value);
 emc_status emc_malloc(emc_ctx* ctx, size_t bytes, void** d_ptr);
 emc_status emc_free(emc_ctx* ctx, void* d_ptr);
 emc_status emc_memcpy_h2d(emc_ctx* ctx, void* d_dst, const void* src, size_t bytes);
+/* A host-to-device copy that does not wait for the work already queued on
+ * the context's stream: src is staged through a pinned ring buffer of the
+ * context and the call returns once src may be reused; the copy is
+ * stream-ordered. Copies above 8 MiB wait for the stream as emc_memcpy_h2d
+ * does. No reference counterpart (MATLAB's gpuArray uploads). */
+emc_status emc_memcpy_h2d_async(emc_ctx* ctx, void* d_dst, const void* src, size_t bytes);
 emc_status emc_memcpy_d2h(emc_ctx* ctx, void* dst, const void* d_src, size_t bytes);
 emc_status emc_memcpy_d2d(emc_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
 emc_status emc_memset_zero(emc_ctx* ctx, void* d_dst, size_t bytes);
@@ -423,6 +433,14 @@ emc_status emc_batch_prepare_particle(emc_ctx* ctx, int n, const float* d_partic
 emc_status emc_batch_prepare_particles(emc_ctx* ctx, int n, int n_part, const float* const* h_particles,
                                        const float* d_bandpass, const float* const* h_wedges, void* d_Q,
                                        float* d_N, double* d_A);
+/* The same with a centred real-space mask (n^3 FP32, NULL = none) applied to
+ * every particle before its transform: the volMask of the reference's score
+ * stage (alignment/BH_alignRaw3d_v2.m:1375, the rotated particle times
+ * volMask; a mask symmetric about the centre commutes with the rotation), for
+ * windows that hold neighbouring density. */
+emc_status emc_batch_prepare_particles_masked(emc_ctx* ctx, int n, int n_part, const float* const* h_particles,
+                                              const float* d_mask, const float* d_bandpass,
+                                              const float* const* h_wedges, void* d_Q, float* d_N, double* d_A);
 
 /* Score every (particle, orientation): particles p = 0..n_part-1 with their
  * Q/N/A stacked contiguously, start matrices d_start_cm (n_part x 9 doubles,
@@ -657,8 +675,9 @@ emc_status emc_mrc_write(const char* path, const float* h_data, const int dims[3
  * samplingRate + binShift, :1034). d_out: n windows of window[0] x
  * window[1] x window[2] floats. h_valid[i] = 1, or 0 when the window keeps
  * less than the mask needs ('noUse': that window is zero-filled);
- * h_shifts: n x 3 (shiftVAL = centre - floor(centre)). Returns once the
- * window table is on the device; the cut itself is stream-ordered. */
+ * h_shifts: n x 3 (shiftVAL = centre - floor(centre)). The window table is
+ * staged through the context's pinned ring without waiting for queued work;
+ * h_valid and h_shifts are filled on return, the cut is stream-ordered. */
 emc_status emc_extract_subtomos(emc_ctx* ctx, const void* d_tomo, int mode, const int tomo_dims[3], int n,
                                 const double* h_centres, const int window[3], const double mask_radius[3],
                                 int taper, float* d_out, int32_t* h_valid, double* h_shifts);

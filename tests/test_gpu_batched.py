@@ -173,6 +173,27 @@ def test_batched_preparation_matches_single(ctx):
         assert abs(Ab[i] - ref.A) <= 1e-5 * ref.A, (i, Ab[i], ref.A)
 
 
+def test_masked_preparation_equals_premultiplied_particles(ctx):
+    """emc_batch_prepare_particles_masked (19 particles: chunks 16 + 2 + 1)
+    equals the unmasked preparation of the particles multiplied by the mask
+    on the host, bit for bit (one FP32 product per voxel either way); a mask
+    of ones equals no mask."""
+    n, P = 16, 19
+    rng = np.random.default_rng(8)
+    bp = O.bandpass3d((n, n, n), 1e-3, 40.0, 8.0, 2.5).astype(np.float32)
+    vols = [rng.standard_normal((n, n, n)).astype(np.float32) for _ in range(P)]
+    mask = rng.uniform(0.0, 1.0, (n, n, n)).astype(np.float32)
+    out = []
+    for parts, m in ((vols, mask), ([v * mask for v in vols], None), (vols, np.ones((n, n, n), np.float32)),
+                     (vols, None)):
+        al = BatchAligner(n, np.zeros((n, n, n), np.float32), bp)
+        al.set_particles(parts, [None] * P, [np.eye(3)] * P, mask=m)
+        out.append((al.Q.get(), al.N.get(), al.A.get()))
+    for a, b in ((out[0], out[1]), (out[2], out[3])):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+
+
 def _host_refine(al, R0, s0, iters=12, step_angle=0.02, step_shift=0.3):
     """The refinement restated on the host (numpy FP64) around the device
     adjoint: the same batched BFGS emc_align_refine runs on the device."""

@@ -27,10 +27,11 @@ def _check(peak, ang, rpeak, rang):
     return True
 
 
-@pytest.mark.parametrize("pruned", [1, 0])
+@pytest.mark.parametrize("pruned", [2, 1, 0])
 def test_template_search_chunk_matches_oracle(ctx, pruned):
     """pruned 1: the template's spectrum as direct DFTs over its block at
     the padded position, fused with the product by the tomogram spectrum;
+    2: the same with the z-stage DFT and product on the matrix cores;
     0: the zero-padded volume through rocFFT's R2C."""
     _lib.call("emc_tune_set", ctx.handle, b"tsearch_pruned", 0, pruned)
     rng = np.random.default_rng(21)
@@ -62,9 +63,11 @@ def test_template_search_chunk_matches_oracle(ctx, pruned):
 
 
 def test_template_search_pruned_matches_padded_odd_sizes(ctx):
-    """The pruned spectrum against the padded-volume R2C on an odd-sized,
-    non-cubic chunk and template with an off-centre block (every DFT length
-    and offset different), over several orientation blocks."""
+    """The pruned spectrum (vector and matrix-core z stage) against the
+    padded-volume R2C on an odd-sized, non-cubic chunk and template with an
+    off-centre block (every DFT length and offset different; 29 kz and
+    37 x 34 / 2 columns leave partial matrix-core tiles), over several
+    orientation blocks."""
     rng = np.random.default_rng(5)
     t = rng.standard_normal((9, 7, 11)).astype(np.float32)        # (z, y, x): 11 x 7 x 9
     chunk = rng.standard_normal((29, 34, 37)).astype(np.float32)  # 37 x 34 x 29
@@ -74,7 +77,7 @@ def test_template_search_pruned_matches_padded_odd_sizes(ctx):
     d_t = DeviceArray.from_numpy(t)
     dp = _lib.C.POINTER(_lib.C.c_double)
     out = []
-    for pruned in (1, 0):
+    for pruned in (2, 1, 0):
         _lib.call("emc_tune_set", ctx.handle, b"tsearch_pruned", 0, pruned)
         peak = DeviceArray.zeros((vd[2], vd[1], vd[0]), np.float32)
         ang = DeviceArray.zeros((vd[2], vd[1], vd[0]), np.int32)
@@ -83,9 +86,10 @@ def test_template_search_pruned_matches_padded_odd_sizes(ctx):
                   peak.vp, ang.vp)
         out.append((peak.get(), ang.get()))
     _lib.call("emc_tune_set", ctx.handle, b"tsearch_pruned", 0, 1)
-    (p1, a1), (p0, a0) = out
-    assert np.max(np.abs(p1 - p0)) < 1e-4 * np.max(np.abs(p0))
-    assert np.mean(a1 == a0) > 0.999
+    (p2, a2), (p1, a1), (p0, a0) = out
+    for p, a in ((p1, a1), (p2, a2)):
+        assert np.max(np.abs(p - p0)) < 1e-4 * np.max(np.abs(p0))
+        assert np.mean(a == a0) > 0.999
 
 
 def test_template_search_driver_chunks(ctx):

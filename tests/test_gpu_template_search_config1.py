@@ -11,8 +11,8 @@ the 15-degree grid ([180, 15, 180, 15], 4,600 orientations).
   the same oracle result, and every inserted template found at its centre
   with its own orientation id (the map index is the particle centre).
 * Templates with edges above 32 and non-cubic (48^3, 40 x 33 x 64): the
-  pruned template spectrum against the oracle and against the padded
-  volume's R2C."""
+  pruned template spectrum (vector and matrix-core z stage) against the
+  oracle and against the padded volume's R2C."""
 import numpy as np
 import pytest
 
@@ -137,7 +137,7 @@ def test_large_templates_pruned_padded_and_oracle(ctx, tdims, cdims):
     d_t = DeviceArray.from_numpy(t)
     dp = _lib.C.POINTER(_lib.C.c_double)
     out = []
-    for pruned in (1, 0):
+    for pruned in (2, 1, 0):
         _lib.call("emc_tune_set", ctx.handle, b"tsearch_pruned", 0, pruned)
         peak = DeviceArray.zeros((vd[2], vd[1], vd[0]), np.float32)
         ang = DeviceArray.zeros((vd[2], vd[1], vd[0]), np.int32)
