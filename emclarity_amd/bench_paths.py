@@ -10,6 +10,9 @@ batched-alignment and template-search workloads:
   refine   the refinement's adjoint evaluations on the same size classes
   search   the 28-orientation coarse search of the same size classes
   copy     a 1 GiB device-to-device copy (byte-counter calibration)
+  wedges   the batched wedge build (emc_weight_mask_batched) of 1,000 / 600 /
+           400 particles of 64 / 96 / 128^3, each its own tilt range U[50, 70]
+           deg in 3-degree steps and defocus U[1.5, 4] um
   plain    the drop-in per-particle grid search (emc_align_grid) of one
            particle of 64 / 96 / 128^3 over the 1,045-orientation local grid,
            orientations in blocks; the rotation kernel's rate, and the whole
@@ -348,6 +351,29 @@ def bench_plain(ctx, reps):
         del al
 
 
+def bench_wedges(ctx, reps):
+    from . import synthetic as S
+    from .masking import BH_weightMaskMex_batched
+    rng = np.random.default_rng(5)
+    out = {}
+    for n, count in ((64, 1000), (96, 600), (128, 400)):
+        tlts = [S.tilt_table(rng.uniform(50.0, 70.0), 3.0, rng.uniform(1.5, 4.0), 2.5) for _ in range(count)]
+        recon = np.array([[1024.0, 1024.0, 512.0], [0.0, 0.0, 0.0]])
+        xyz = np.stack([rng.uniform(n, 1024 - n, count), rng.uniform(n, 1024 - n, count),
+                        rng.uniform(n, 512 - n, count)], 1)
+        dst = DeviceArray((count, n, n, n), np.float32, ctx)
+        wall = _timed(ctx, lambda: BH_weightMaskMex_batched((n, n, n), 1, tlts, xyz, recon, out=dst), reps)
+        sf, c_sf = _ktime(ctx, "sf3d_many")
+        ct, c_ct = _ktime(ctx, "ctf2d_many")
+        written = count * n ** 3 * 4
+        out[n] = {"particles": count, "tilts": int(sum(len(t) for t in tlts)),
+                  "ms_per_call": round(wall * 1e3, 3), "sf3d_ms": round(sf / reps, 3),
+                  "sf3d_launches": c_sf // reps, "ctf2d_ms": round(ct / reps, 3),
+                  "sf3d_written_GBps": round(written / (sf / reps * 1e-3) / 1e9, 1)}
+        del dst
+    _line("wedges", sum(v["ms_per_call"] for v in out.values()) * 1e-3, {"classes": out})
+
+
 def bench_copy(ctx, reps, mib=1024):
     """A device-to-device copy of known size: the calibration kernel of the
     HBM byte counters (FETCH_SIZE, WRITE_SIZE)."""
@@ -515,7 +541,8 @@ def bench_pipeline(ctx, reps, table="c64", tomo=(1024, 1024, 512), edge=64, coun
 
 PATHS = {"extract": bench_extract, "average": bench_average, "peaks": bench_peaks, "tsprep": bench_tsprep,
          "tsearch": bench_tsearch, "adjoint": bench_adjoint, "refine": bench_refine, "search": bench_search,
-         "copy": bench_copy, "plain": bench_plain, "gradient": bench_gradient, "pipeline": bench_pipeline}
+         "copy": bench_copy, "wedges": bench_wedges, "plain": bench_plain, "gradient": bench_gradient,
+         "pipeline": bench_pipeline}
 
 
 def main(argv=None):

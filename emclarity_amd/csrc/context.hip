@@ -314,7 +314,7 @@ constexpr TuneRange kTune[] = {{"search_tile", 2, 128, 16},
                                {"search_passes", 1, 4, 2},
                                {"refine_graph", 0, 1, 1},
                                {"search_zsplit", 1, 4, 1},
-                               {"tsearch_pruned", 0, 2, 1},
+                               {"tsearch_pruned", 0, 2, 2},
                                {"grad_kernel", 0, 1, 1},
                                {"grad_brick", 16, 32, 32},
                                {"align_grid_batch", 1, 64, 16},

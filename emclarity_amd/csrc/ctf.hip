@@ -6,7 +6,7 @@
 // The reference builds SF3D with 2 launches + a texture copy/bind per tilt
 // (mexSF3D.cu:238-277) and a per-thread serial z loop accumulating into
 // global memory. Here: one launch writes the 2D CTF image of every tilt
-// (phase evaluated in FP64, stored FP32), one voxel-parallel launch
+// (FP32 phase, as the reference's), one voxel-parallel launch
 // accumulates all tilts in registers and writes each output voxel once.
 #include <algorithm>
 #include <cmath>
@@ -24,7 +24,22 @@ struct CtfTilt {
   double occupancy, exposure;
   double cos2a, sin2a;  // of 2 astig
   float sin_t, cos_t;  // of -tilt (mexSF3D.cu:131-137)
+  // FP32 copies the device evaluates the phase with (the reference's own
+  // precision, ctf.cu:45-64)
+  float cs_f, dfd1_f, dfd2_f, amp_f, fvx_f, fvy_f, cos2a_f, sin2a_f;
 };
+
+// the FP32 copies of a tilt's constants (df_term folded into d1, d2)
+static void finish_tilt(CtfTilt& t) {
+  t.cs_f = (float)t.cs_term;
+  t.dfd1_f = (float)(t.df_term * t.d1);
+  t.dfd2_f = (float)(t.df_term * t.d2);
+  t.amp_f = (float)t.amp_phase;
+  t.fvx_f = (float)t.fvx;
+  t.fvy_f = (float)t.fvy;
+  t.cos2a_f = (float)t.cos2a;
+  t.sin2a_f = (float)t.sin2a;
+}
 
 static CtfTilt make_tilt(double pixel, double wl, double cs_mm, double df1, double df2,
                          double astig_deg, double amp, double fvx, double fvy) {
@@ -41,6 +56,7 @@ static CtfTilt make_tilt(double pixel, double wl, double cs_mm, double df1, doub
   t.fvx = fvx;
   t.fvy = fvy;
   (void)pixel;
+  finish_tilt(t);
   return t;
 }
 
@@ -48,21 +64,24 @@ static CtfTilt make_tilt(double pixel, double wl, double cs_mm, double df1, doub
 // (ctf.cu:45-64, :103-127). (x, y) are centred / wrapped frequency indices.
 // The astigmatic term cos(2(phi - astig)), phi = atan2(ky, kx), is formed
 // without the angle: cos 2phi = (kx^2 - ky^2) / r^2, sin 2phi = 2 kx ky / r^2
-// (phi = 0 at the origin, as atan2(0, 0)). The phase and its sine are FP64;
-// the exposure weight exp(-E / 2 / (0.245 k^-1.665 + 2.81)) is FP32.
-__device__ inline double ctf_value(const CtfTilt& p, int x, int y, int sq, int weighted) {
-  const double ky = (double)y * p.fvy;
-  const double kx = (double)x * p.fvx;
-  const double r2 = kx * kx + ky * ky;
-  const double c2 = r2 > 0.0 ? (kx * kx - ky * ky) / r2 : 1.0;
-  const double s2 = r2 > 0.0 ? 2.0 * kx * ky / r2 : 0.0;
-  const double ca = c2 * p.cos2a + s2 * p.sin2a;
-  double v = sin(p.cs_term * r2 * r2 - p.df_term * r2 * (p.d1 + p.d2 * ca) - p.amp_phase);
+// (phi = 0 at the origin, as atan2(0, 0)). All FP32, as the reference: the
+// phase reaches ~100 rad at the Nyquist frequency of a 2.5 A/px box with
+// 4 um defocus, where FP32's relative 6e-8 leaves ~1e-5 rad; sinf reduces
+// the argument exactly. The exposure weight is
+// exp(-E / 2 / (0.245 k^-1.665 + 2.81)).
+__device__ inline float ctf_value(const CtfTilt& p, int x, int y, int sq, int weighted) {
+  const float ky = (float)y * p.fvy_f;
+  const float kx = (float)x * p.fvx_f;
+  const float r2 = kx * kx + ky * ky;
+  const float c2 = r2 > 0.0f ? (kx * kx - ky * ky) / r2 : 1.0f;
+  const float s2 = r2 > 0.0f ? 2.0f * kx * ky / r2 : 0.0f;
+  const float ca = c2 * p.cos2a_f + s2 * p.sin2a_f;
+  float v = sinf(p.cs_f * r2 * r2 - r2 * (p.dfd1_f + p.dfd2_f * ca) - p.amp_f);
   if (sq) v *= v;
   if (weighted) {
     const float rw = (float)p.occupancy * (x == 0 ? 0.2f : fabsf((float)x));
-    const float pw = r2 > 0.0 ? powf((float)r2, -0.8325f) : __builtin_inff();
-    v *= (double)(rw * expf((-0.5f * (float)p.exposure) / (0.24499f * pw + 2.81410f)));
+    const float pw = r2 > 0.0f ? powf(r2, -0.8325f) : __builtin_inff();
+    v *= rw * expf((-0.5f * (float)p.exposure) / (0.24499f * pw + 2.81410f));
   }
   return v;
 }
@@ -85,7 +104,7 @@ __global__ void __launch_bounds__(256) ctf2d_kernel(float* __restrict__ out,
     if (yy > o_y) yy -= ny;
     if (!half && xx > o_x) xx -= cx;
   }
-  out[((int64_t)t * ny + y) * cx + x] = (float)ctf_value(tilts[t], xx, yy, sq, weighted);
+  out[((int64_t)t * ny + y) * cx + x] = ctf_value(tilts[t], xx, yy, sq, weighted);
 }
 
 // SF3D accumulation (mexSF3D.cu:37-77): for each voxel and tilt, the slab
@@ -170,14 +189,24 @@ __device__ inline float sf3d_column(float* __restrict__ out, const float* __rest
     const float f = xt - xf;
     const int x0 = (int)xf;
     const bool in0 = (unsigned)x0 < (unsigned)cx, in1 = (unsigned)(x0 + 1) < (unsigned)cx;
+    // Every load is unconditional from an address inside this tilt's CTF
+    // (x clamped to the row, rows past the volume re-read its last row), so
+    // the 2 x kSlabRows loads issue together; texels outside the row read
+    // as 0 through the selects, and sums of rows past the volume are never
+    // stored.
+    const int xa = min(max(x0, 0), cx - 1), xb = min(max(x0 + 1, 0), cx - 1);
     const float* row = ctfs + ((int64_t)t * ny + y0) * cx;
+    float v0[kSlabRows], v1[kSlabRows];
 #pragma unroll
     for (int r = 0; r < kSlabRows; ++r) {
-      if (r < rows) {
-        const float v0 = in0 ? row[(int64_t)r * cx + x0] : 0.0f;
-        const float v1 = in1 ? row[(int64_t)r * cx + x0 + 1] : 0.0f;
-        acc[r] += zw * (v0 + f * (v1 - v0));
-      }
+      const int rr = min(r, rows - 1);
+      v0[r] = row[rr * cx + xa];
+      v1[r] = row[rr * cx + xb];
+    }
+#pragma unroll
+    for (int r = 0; r < kSlabRows; ++r) {
+      const float a = in0 ? v0[r] : 0.0f, b = in1 ? v1[r] : 0.0f;
+      acc[r] += zw * (a + f * (b - a));
     }
   }
   float m = __builtin_inff();
@@ -191,23 +220,23 @@ __device__ inline float sf3d_column(float* __restrict__ out, const float* __rest
   return m;
 }
 
-// blockDim (64, 4); blockIdx.x = patch (px + ceil(nx / 8) * pz), blockIdx.y =
-// segment of 4 x kSlabRows rows, blockIdx.z = particle (sf3d_many_kernel).
+// blockDim (64, 4); bx = patch (px + ceil(nx / 8) * pz), by = segment of
+// 4 x kSlabRows rows.
 struct SlabPatch {
   int x, z, x0, x1, z0, z1, y0;
 };
 
-__device__ inline SlabPatch slab_patch(int nx, int ny, int nz) {
+__device__ inline SlabPatch slab_patch(int nx, int ny, int nz, int bx, int by) {
   SlabPatch q;
   const int npx = (nx + kPatch - 1) / kPatch;
-  const int px = blockIdx.x % npx, pz = blockIdx.x / npx;
+  const int px = bx % npx, pz = bx / npx;
   q.x0 = px * kPatch;
   q.z0 = pz * kPatch;
   q.x1 = min(nx, q.x0 + kPatch) - 1;
   q.z1 = min(nz, q.z0 + kPatch) - 1;
   q.x = q.x0 + (int)threadIdx.x % kPatch;
   q.z = q.z0 + (int)threadIdx.x / kPatch;
-  q.y0 = (blockIdx.y * blockDim.y + threadIdx.y) * kSlabRows;
+  q.y0 = (by * blockDim.y + threadIdx.y) * kSlabRows;
   (void)ny;
   return q;
 }
@@ -217,7 +246,7 @@ __global__ void __launch_bounds__(256) sf3d_kernel(float* __restrict__ out,
                                                    const CtfTilt* __restrict__ tilts, int n_tilts,
                                                    int nx, int ny, int nz, int cx) {
   __shared__ int list[kSlabListMax];
-  const SlabPatch q = slab_patch(nx, ny, nz);
+  const SlabPatch q = slab_patch(nx, ny, nz, blockIdx.x, blockIdx.y);
   const int nl = slab_list(tilts, n_tilts, nx, nz, q.x0, q.x1, q.z0, q.z1, list);
   (void)sf3d_column(out, ctfs, tilts, n_tilts, nx, ny, nz, cx, q.x, q.y0, q.z, list, nl);
 }
@@ -329,18 +358,26 @@ std::vector<CtfTilt> weight_mask_tilts(const int dims[3], double sampling, const
 
 // SF3D of many particles of one box size, the tilts of particle p being
 // entries [off[p] - base, off[p+1] - base) of the chunk's tilt table and CTF
-// stack; blockIdx.z = particle of the chunk. The same slab sum as
-// sf3d_kernel; each block also writes the minimum of its voxels
-// (part_min[p][block]).
+// stack. The same slab sum as sf3d_kernel; each block also writes the
+// minimum of its voxels (part_min[p][block]).
+// XCD-aware: a 1-D grid of 8 x ceil(np / 8) x (gx gy) blocks; block L runs
+// on XCD L % 8, which takes particles L % 8, L % 8 + 8, ... one after
+// another, so each particle's CTF stack (41 tilts of 128^2: 2.7 MB) is read
+// into one XCD's L2 instead of all eight.
 __global__ void __launch_bounds__(256) sf3d_many_kernel(float* __restrict__ out, const float* __restrict__ ctfs,
                                                         const CtfTilt* __restrict__ tilts,
                                                         const int* __restrict__ off, int base, int nx, int ny,
-                                                        int nz, int cx, float* __restrict__ part_min) {
+                                                        int nz, int cx, int gx, int gy, int np,
+                                                        float* __restrict__ part_min) {
   __shared__ int list[kSlabListMax];
   __shared__ float lds[4];
-  const int p = blockIdx.z;
+  const int per = gx * gy;
+  const int xcd = (int)(blockIdx.x % 8), qb = (int)(blockIdx.x / 8);
+  const int p = xcd + 8 * (qb / per), r = qb % per;
+  if (p >= np) return;  // uniform per block, before any barrier
+  const int bx = r % gx, by = r / gx;
   const int t0 = off[p] - base, t1 = off[p + 1] - base;
-  const SlabPatch q = slab_patch(nx, ny, nz);
+  const SlabPatch q = slab_patch(nx, ny, nz, bx, by);
   const int nl = slab_list(tilts + t0, t1 - t0, nx, nz, q.x0, q.x1, q.z0, q.z1, list);
   float m = sf3d_column(out + (int64_t)p * nz * ny * nx, ctfs + (int64_t)t0 * ny * cx, tilts + t0, t1 - t0, nx, ny,
                         nz, cx, q.x, q.y0, q.z, list, nl);
@@ -350,8 +387,7 @@ __global__ void __launch_bounds__(256) sf3d_many_kernel(float* __restrict__ out,
   __syncthreads();
   if (threadIdx.x == 0 && threadIdx.y == 0) {
     const float b = fminf(fminf(lds[0], lds[1]), fminf(lds[2], lds[3]));
-    const int per = gridDim.x * gridDim.y;
-    part_min[(int64_t)p * per + (int64_t)blockIdx.y * gridDim.x + blockIdx.x] = b;
+    part_min[(int64_t)p * per + (int64_t)by * gx + bx] = b;
   }
 }
 
@@ -492,8 +528,10 @@ emc_status emc_weight_mask_batched(emc_ctx* ctx, const int dims[3], double sampl
       }
       {
         KTimer kt(ctx, "sf3d_many");
-        hipLaunchKernelGGL(sf3d_many_kernel, dim3(gx, gy, np), b2, 0, ctx->stream, d_out + p0 * nv,
-                           (const float*)d_ctf, dt, d_off + p0, first[p0], nx, ny, nz, cx, d_part);
+        const int64_t nblk = (int64_t)8 * ceil_div(np, 8) * gx * gy;
+        EMC_REQUIRE(nblk < ((int64_t)1 << 31), "too many SF3D blocks in one launch");
+        hipLaunchKernelGGL(sf3d_many_kernel, dim3((unsigned)nblk), b2, 0, ctx->stream, d_out + p0 * nv,
+                           (const float*)d_ctf, dt, d_off + p0, first[p0], nx, ny, nz, cx, gx, gy, np, d_part);
       }
       hipLaunchKernelGGL(fold_min_many_kernel, dim3(np), dim3(kRedThreads), 0, ctx->stream, (const float*)d_part,
                          per, d_mins);

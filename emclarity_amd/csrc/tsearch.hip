@@ -464,7 +464,7 @@ emc_status emc_template_search_chunk(emc_ctx* ctx, const float* d_chunk, const i
 
     // pruned template spectrum (every template edge <= kTMax; "tsearch_pruned"
     // 0 selects the padded-volume R2C instead)
-    const int pmode = tune_value(ctx, "tsearch_pruned", 0, 1);
+    const int pmode = tune_value(ctx, "tsearch_pruned", 0, 2);
     const bool pruned = pmode != 0 && td[0] <= kTMax && td[1] <= kTMax && td[2] <= kTMax;
     const int cx = cd[0] / 2 + 1;
     float* tblk = nullptr;

@@ -86,13 +86,13 @@ emc_status emc_timing_read(emc_ctx* ctx, const char* name, double* ms_total, int
  *                  filled round of workgroups
  *   "search_zsplit" kz segments per support column, each in its own pass
  *                  of the search schedule, 1..4 (default 1)
- *   "tsearch_pruned" 1 (default): emc_template_search_chunk forms the
+ *   "tsearch_pruned" 2 (default): emc_template_search_chunk forms the
  *                  padded template's spectrum as direct DFTs over the
  *                  template block (every template edge <= 64) fused with
- *                  the product by the tomogram spectrum; 2: the same with
- *                  the z-direction DFT and the product as FP32 matrix-core
- *                  (MFMA) products; 0: rocFFT R2C of the zero-padded volume
- *                  (set with box 0)
+ *                  the product by the tomogram spectrum, the z-direction DFT
+ *                  and the product as FP32 matrix-core (MFMA) products; 1:
+ *                  the same with the z stage on the vector units; 0: rocFFT
+ *                  R2C of the zero-padded volume (set with box 0)
  *   "refine_graph" 1 (default): emc_align_refine replays its iteration
  *                  chain as a captured HIP graph; 0: eager launches
  *   "grad_kernel"  adjoint evaluation order: 1 (default) brick-major (the

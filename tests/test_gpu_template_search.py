@@ -57,7 +57,7 @@ def test_template_search_chunk_matches_oracle(ctx, pruned):
         _lib.call("emc_template_search_chunk", ctx.handle, d_chunk.vp, _lib.i3((32, 32, 32)), d_t.vp,
                   _lib.i3((8, 8, 8)), _lib.i3(pad[0]), _lib.i3(vlow), _lib.i3((20, 20, 20)),
                   pts[a0:a1].ctypes.data_as(dp), a1 - a0, a0 + 1, first, peak.vp, ang.vp)
-    _lib.call("emc_tune_set", ctx.handle, b"tsearch_pruned", 0, 1)
+    _lib.call("emc_tune_set", ctx.handle, b"tsearch_pruned", 0, 2)   # the library default
     rpeak, rang = O.template_search_chunk(chunk.astype(np.float64), t, grid, pad[0:2], valid[0:2])
     _check(peak.get(), ang.get(), rpeak, rang)
 
@@ -85,7 +85,7 @@ def test_template_search_pruned_matches_padded_odd_sizes(ctx):
                   _lib.i3(pad), _lib.i3(vlow), _lib.i3(vd), grid.ctypes.data_as(dp), len(grid), 1, 1,
                   peak.vp, ang.vp)
         out.append((peak.get(), ang.get()))
-    _lib.call("emc_tune_set", ctx.handle, b"tsearch_pruned", 0, 1)
+    _lib.call("emc_tune_set", ctx.handle, b"tsearch_pruned", 0, 2)   # the library default
     (p2, a2), (p1, a1), (p0, a0) = out
     for p, a in ((p1, a1), (p2, a2)):
         assert np.max(np.abs(p - p0)) < 1e-4 * np.max(np.abs(p0))

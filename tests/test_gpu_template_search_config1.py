@@ -145,7 +145,7 @@ def test_large_templates_pruned_padded_and_oracle(ctx, tdims, cdims):
                   _lib.i3(pad[0]), _lib.i3(vA[0]), _lib.i3(vd), pts.ctypes.data_as(dp), len(pts), 1, 1,
                   peak.vp, ang.vp)
         out.append((peak.get(), ang.get()))
-    _lib.call("emc_tune_set", ctx.handle, b"tsearch_pruned", 0, 1)
+    _lib.call("emc_tune_set", ctx.handle, b"tsearch_pruned", 0, 2)   # the library default
     rpeak, rang, margin = O.template_search_chunk(chunk.astype(np.float64), t.astype(np.float64), pts, pad[0:2],
                                                   trim[0:2], return_margin=True)
     for p, a in out:
