@@ -44,7 +44,6 @@ from . import _lib
 CANDIDATES = {"search_tile": [4, 8, 16, 32, 64], "search_zsplit": [1, 2, 3, 4], "search_passes": [1, 2, 3, 4],
               "grad_kernel": [0, 1], "grad_brick": [16, 32], "grad_waves": [4, 5],
               "grad_blocks": [2048, 4096, 8192, 16384, 32768, 65536]}
-GRAD = ("grad_kernel", "grad_brick", "grad_waves", "grad_blocks")
 TABLE = Path(__file__).resolve().parent / "tuning" / "mi355x.json"
 _CACHE: dict | None = None
 
@@ -127,19 +126,39 @@ def tune(ctx, counts=((64, 1000), (96, 600), (128, 400)), tables=("c64", "f16"),
             sd = DeviceArray.zeros((count, 3), np.float64, ctx)
             _lib.call("emc_tune_set", ctx.handle, b"refine_graph", n, 0)   # eager: events per evaluation
             best, times = {}, {}
-            for name in ("search_tile", "search_zsplit", "search_passes", *GRAD):
+
+            def sweep(name, timer):
                 times[name] = {}
                 for v in CANDIDATES[name]:
                     apply(ctx, n, {name: v})
-                    if name not in GRAD:
-                        ms = _timed(ctx, lambda: al.search(grid, search_frac * al.support),
-                                    ("align_fused", "align_final"), reps)
-                    else:   # the refinement's evaluations (three per call)
-                        ms = _timed(ctx, lambda: al.refine_device(Rd, sd, iters=2), ("align_eval_grad",), reps)
+                    ms = timer()
                     times[name][str(v)] = round(ms, 4)
                     print(f"[autotune] n={n} {table} {name}={v}: {ms:.3f} ms", file=log, flush=True)
                 best[name] = int(min(CANDIDATES[name], key=lambda v: times[name][str(v)]))
                 apply(ctx, n, {name: best[name]})
+                return times[name][str(best[name])]
+
+            def t_search():
+                return _timed(ctx, lambda: al.search(grid, search_frac * al.support),
+                              ("align_fused", "align_final"), reps)
+
+            def t_eval():   # the refinement's evaluations (three per call)
+                return _timed(ctx, lambda: al.refine_device(Rd, sd, iters=2), ("align_eval_grad",), reps)
+
+            for name in ("search_tile", "search_zsplit", "search_passes"):
+                sweep(name, t_search)
+            # each adjoint form with its own knobs: the list form's workgroup
+            # target, the brick form's side and waves per SIMD; then the faster
+            apply(ctx, n, {"grad_kernel": 0})
+            t_list = sweep("grad_blocks", t_eval)
+            apply(ctx, n, {"grad_kernel": 1, "grad_waves": 4})
+            sweep("grad_brick", t_eval)
+            t_brick = sweep("grad_waves", t_eval)
+            times["grad_kernel"] = {"0": t_list, "1": t_brick}
+            best["grad_kernel"] = 0 if t_list <= t_brick else 1
+            apply(ctx, n, {"grad_kernel": best["grad_kernel"]})
+            print(f"[autotune] n={n} {table} grad_kernel={best['grad_kernel']} (list {t_list:.3f} ms, "
+                  f"brick {t_brick:.3f} ms)", file=log, flush=True)
             out["params"].setdefault(table, {})[str(n)] = best
             out["timings_ms"].setdefault(table, {})[str(n)] = times
             del al

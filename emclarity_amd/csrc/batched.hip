@@ -1247,7 +1247,9 @@ static const int4* brick_list(emc_ctx* ctx, int n, double radius, int S, int* co
 void eval_grad_kind(emc_ctx* ctx, int n, int* kind, int* S, int* waves) {
   *kind = tune_value(ctx, "grad_kernel", n, 1) ? 1 : 0;
   *S = tune_value(ctx, "grad_brick", n, 32) == 16 ? 16 : 32;
-  // 5 waves per SIMD only for the 32-brick kernel (the 16-brick one keeps 4)
+  // 5 waves per SIMD only for the 32-brick kernel with the refinement's
+  // generator derivatives (the 16-brick one, and general derivative
+  // matrices, keep 4)
   *waves = (*S == 32 && tune_value(ctx, "grad_waves", n, 4) >= 5) ? 5 : 4;
 }
 
@@ -1281,10 +1283,10 @@ static void launch_brick(emc_ctx* ctx, const EvalSetup& e, int n, int n_part, co
     hipLaunchKernelGGL((align_eval_brick_kernel<16, Tab, GEN, 4>), grid, dim3(kBatchThreads), 0, ctx->stream,
                        (const T*)d_F2, n, (const float2*)d_Q, d_N, d_MdM, d_shift, (const int4*)e.list, e.ns, n_part,
                        slots, e.r2, e.part);
-  else if (e.waves == 5)
-    hipLaunchKernelGGL((align_eval_brick_kernel<32, Tab, GEN, 5>), grid, dim3(kBatchThreads), 0, ctx->stream,
-                       (const T*)d_F2, n, (const float2*)d_Q, d_N, d_MdM, d_shift, (const int4*)e.list, e.ns, n_part,
-                       slots, e.r2, e.part);
+  else if (GEN && e.waves == 5)  // general dM (GEN false) needs more than 102 VGPRs: it keeps 4 waves
+    hipLaunchKernelGGL((align_eval_brick_kernel<32, Tab, GEN, GEN ? 5 : 4>), grid, dim3(kBatchThreads), 0,
+                       ctx->stream, (const T*)d_F2, n, (const float2*)d_Q, d_N, d_MdM, d_shift, (const int4*)e.list,
+                       e.ns, n_part, slots, e.r2, e.part);
   else
     hipLaunchKernelGGL((align_eval_brick_kernel<32, Tab, GEN, 4>), grid, dim3(kBatchThreads), 0, ctx->stream,
                        (const T*)d_F2, n, (const float2*)d_Q, d_N, d_MdM, d_shift, (const int4*)e.list, e.ns, n_part,
