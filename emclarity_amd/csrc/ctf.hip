@@ -160,11 +160,11 @@ __device__ inline int slab_list(const CtfTilt* __restrict__ tilts, int nt, int n
 }
 
 // One lane's column: voxels (x, y0 .. y0 + kSlabRows - 1, z) of the volume
-// `out`; ctfs, tilts: its tilts [0, nt). Returns the minimum of the lane's
-// voxels (+inf when it has none).
+// `out`, each stored plus `add`; ctfs, tilts: its tilts [0, nt). Returns
+// the minimum of the lane's sums before `add` (+inf when it has none).
 __device__ inline float sf3d_column(float* __restrict__ out, const float* __restrict__ ctfs,
                                     const CtfTilt* __restrict__ tilts, int nt, int nx, int ny, int nz, int cx,
-                                    int x, int y0, int z, const int* list, int nlist) {
+                                    int x, int y0, int z, const int* list, int nlist, float add) {
   const float kPi = 3.14159265358979323f;
   if (x >= nx || z >= nz || y0 >= ny) return __builtin_inff();
   const float ssx = (nx % 2 == 0) ? 0.5f : 0.0f;
@@ -205,8 +205,9 @@ __device__ inline float sf3d_column(float* __restrict__ out, const float* __rest
     }
 #pragma unroll
     for (int r = 0; r < kSlabRows; ++r) {
+      // explicit FMAs: with a, b, zw >= 0 and f < 1 every term is >= 0
       const float a = in0 ? v0[r] : 0.0f, b = in1 ? v1[r] : 0.0f;
-      acc[r] += zw * (a + f * (b - a));
+      acc[r] = fmaf(zw, fmaf(f, b - a, a), acc[r]);
     }
   }
   float m = __builtin_inff();
@@ -214,7 +215,7 @@ __device__ inline float sf3d_column(float* __restrict__ out, const float* __rest
 #pragma unroll
   for (int r = 0; r < kSlabRows; ++r)
     if (r < rows) {
-      o[(int64_t)r * nx] = acc[r];
+      o[(int64_t)r * nx] = acc[r] + add;
       m = fminf(m, acc[r]);
     }
   return m;
@@ -248,7 +249,7 @@ __global__ void __launch_bounds__(256) sf3d_kernel(float* __restrict__ out,
   __shared__ int list[kSlabListMax];
   const SlabPatch q = slab_patch(nx, ny, nz, blockIdx.x, blockIdx.y);
   const int nl = slab_list(tilts, n_tilts, nx, nz, q.x0, q.x1, q.z0, q.z1, list);
-  (void)sf3d_column(out, ctfs, tilts, n_tilts, nx, ny, nz, cx, q.x, q.y0, q.z, list, nl);
+  (void)sf3d_column(out, ctfs, tilts, n_tilts, nx, ny, nz, cx, q.x, q.y0, q.z, list, nl, 0.0f);
 }
 
 __global__ void shift_min_kernel(float* __restrict__ d, int64_t n, const double* __restrict__ dmin,
@@ -367,7 +368,7 @@ std::vector<CtfTilt> weight_mask_tilts(const int dims[3], double sampling, const
 __global__ void __launch_bounds__(256) sf3d_many_kernel(float* __restrict__ out, const float* __restrict__ ctfs,
                                                         const CtfTilt* __restrict__ tilts,
                                                         const int* __restrict__ off, int base, int nx, int ny,
-                                                        int nz, int cx, int gx, int gy, int np,
+                                                        int nz, int cx, int gx, int gy, int np, float add,
                                                         float* __restrict__ part_min) {
   __shared__ int list[kSlabListMax];
   __shared__ float lds[4];
@@ -380,7 +381,7 @@ __global__ void __launch_bounds__(256) sf3d_many_kernel(float* __restrict__ out,
   const SlabPatch q = slab_patch(nx, ny, nz, bx, by);
   const int nl = slab_list(tilts + t0, t1 - t0, nx, nz, q.x0, q.x1, q.z0, q.z1, list);
   float m = sf3d_column(out + (int64_t)p * nz * ny * nx, ctfs + (int64_t)t0 * ny * cx, tilts + t0, t1 - t0, nx, ny,
-                        nz, cx, q.x, q.y0, q.z, list, nl);
+                        nz, cx, q.x, q.y0, q.z, list, nl, add);
   // block minimum: one wave per threadIdx.y (blockDim = 64 x 4)
   m = wave_min(m);
   if (threadIdx.x == 0) lds[threadIdx.y] = m;
@@ -408,6 +409,24 @@ __global__ void shift_min_many_kernel(float* __restrict__ d, int64_t nv, const f
   const float sub = mins[blockIdx.y] - floor_value;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += (int64_t)gridDim.x * blockDim.x)
     v[i] = v[i] - sub;
+}
+
+// True when a voxel of an (nx, *, nz) SF3D of these tilts provably gets no
+// term (sum exactly 0) and no term is negative: every tilt's occupancy is
+// >= 0 and the voxels of column (x = nx/2, z = 0), on the z axis at the
+// box edge, lie outside every tilt's slab (|tw| > 4 with a margin against
+// the device's FP32 rounding); true for any tilt range short of +-90 deg
+// minus the slab's reach at the box edge.
+static bool sf3d_has_empty_voxel(const CtfTilt* tilts, int nt, int nx, int nz) {
+  const double ssz = (nz % 2 == 0) ? 0.5 : 0.0;
+  const double u = (double)(nx / 2) - (double)nx / 2.0;
+  const double w = 0.0 - (double)(nz / 2);
+  for (int t = 0; t < nt; ++t) {
+    if (!(tilts[t].occupancy >= 0.0)) return false;
+    const double tw = (-u * tilts[t].sin_t + ssz) + w * tilts[t].cos_t;
+    if (std::fabs(tw) <= 4.01) return false;
+  }
+  return true;
 }
 
 }  // namespace emc
@@ -526,17 +545,28 @@ emc_status emc_weight_mask_batched(emc_ctx* ctx, const int dims[3], double sampl
         hipLaunchKernelGGL(ctf2d_kernel, dim3(ceil_div(cx, 64), ceil_div(ny, 4), nt), b2, 0, ctx->stream, d_ctf,
                            dt, cx, ny, cx / 2, ny / 2, 0, 1, 1, 1);
       }
+      // SF3D - (min - 1e-4) (BH_weightMaskMex.m:55). Every term is >= 0
+      // (squared CTF, slab weight >= 0, occupancy >= 0), so where a voxel
+      // provably gets no term its sum is exactly 0, the minimum is 0 and the
+      // shift is +1e-4 for the whole chunk, added at the kernel's store
+      // (v - (0 - 1e-4f) = v + 1e-4f exactly); otherwise a second pass
+      // subtracts each particle's minimum.
+      bool zero_min = true;
+      for (int p = p0; p < p1 && zero_min; ++p) zero_min = sf3d_has_empty_voxel(&all[first[p]], n_tilts[p], nx, nz);
       {
         KTimer kt(ctx, "sf3d_many");
         const int64_t nblk = (int64_t)8 * ceil_div(np, 8) * gx * gy;
         EMC_REQUIRE(nblk < ((int64_t)1 << 31), "too many SF3D blocks in one launch");
         hipLaunchKernelGGL(sf3d_many_kernel, dim3((unsigned)nblk), b2, 0, ctx->stream, d_out + p0 * nv,
-                           (const float*)d_ctf, dt, d_off + p0, first[p0], nx, ny, nz, cx, gx, gy, np, d_part);
+                           (const float*)d_ctf, dt, d_off + p0, first[p0], nx, ny, nz, cx, gx, gy, np,
+                           zero_min ? 1e-4f : 0.0f, d_part);
       }
-      hipLaunchKernelGGL(fold_min_many_kernel, dim3(np), dim3(kRedThreads), 0, ctx->stream, (const float*)d_part,
-                         per, d_mins);
-      hipLaunchKernelGGL(shift_min_many_kernel, dim3(ceil_div(nv, 256 * 8), np), dim3(256), 0, ctx->stream,
-                         d_out + p0 * nv, nv, (const float*)d_mins, 1e-4f);  // BH_weightMaskMex.m:55
+      if (!zero_min) {
+        hipLaunchKernelGGL(fold_min_many_kernel, dim3(np), dim3(kRedThreads), 0, ctx->stream, (const float*)d_part,
+                           per, d_mins);
+        hipLaunchKernelGGL(shift_min_many_kernel, dim3(ceil_div(nv, 256 * 8), np), dim3(256), 0, ctx->stream,
+                           d_out + p0 * nv, nv, (const float*)d_mins, 1e-4f);
+      }
       EMC_HIP(hipGetLastError());
     }
   });

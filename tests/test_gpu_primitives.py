@@ -142,6 +142,17 @@ def test_weight_mask_batched_matches_single_and_oracle(ctx):
     small = BH_weightMaskMex_batched((33, 30, 28), 4, tlts[:3], xyzs[:3], rg).get()   # odd and uneven dims
     for i in range(3):
         assert np.array_equal(small[i], BH_weightMaskMex((33, 30, 28), 4, tlts[i], xyzs[i], rg).get())
+    # The tables above leave the box's z-axis edge outside every slab, so
+    # each volume's minimum is provably 0 and the batched entry adds the
+    # 1e-4 floor at its store; a +-89 deg table reaches every voxel and
+    # sends its chunk through the separate minimum pass. Both bit for bit.
+    wide = [tilt_table(89.0, 3.0, 2.0, 2.5)] + tlts[:2]
+    mixed = BH_weightMaskMex_batched((64, 64, 64), 4, wide, xyzs[:3], rg).get()
+    for i in range(3):
+        one = BH_weightMaskMex((64, 64, 64), 4, wide[i], xyzs[i], rg).get()
+        assert np.array_equal(mixed[i], one), i
+        assert abs(float(one.min()) - 1e-4) < 1e-6
+    assert np.array_equal(BH_weightMaskMex_batched((64, 64, 64), 4, wide[1:], xyzs[1:3], rg).get(), mixed[1:])
     with pytest.raises(_lib.EmcError):
         _lib.call("emc_weight_mask_batched", ctx.handle, _lib.i3(size), 4.0, 1, None, None, None,
                   _lib.d_arr(rg[0]), _lib.d_arr(rg[1]), stack.vp)
