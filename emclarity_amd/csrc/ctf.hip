@@ -69,10 +69,28 @@ static CtfTilt make_tilt(double pixel, double wl, double cs_mm, double df1, doub
 // 4 um defocus, where FP32's relative 6e-8 leaves ~1e-5 rad; sinf reduces
 // the argument exactly. The exposure weight is
 // exp(-E / 2 / (0.245 k^-1.665 + 2.81)).
-__device__ inline float ctf_value(const CtfTilt& p, int x, int y, int sq, int weighted) {
+// r2 = kx^2 + ky^2 with every product and sum rounded on its own (no FMA
+// contraction, whatever the inlining context), so the exposure table and
+// the inline evaluation see the same r2
+__device__ inline float ctf_r2(float fvx, float fvy, int x, int y) {
+#pragma clang fp contract(off)
+  const float ky = (float)y * fvy;
+  const float kx = (float)x * fvx;
+  return kx * kx + ky * ky;
+}
+
+// r2^-0.8325 of the exposure weight (+inf at the origin). One out-of-line
+// function: the table kernel and the inline evaluation run the same code, so
+// inlining cannot fold the power's last steps differently in either.
+__device__ __attribute__((noinline)) float ctf_pw(float r2) {
+  return r2 > 0.0f ? powf(r2, -0.8325f) : __builtin_inff();
+}
+
+// pw_tab: ctf_pw of this tilt's (x, y), or null to evaluate it here
+__device__ inline float ctf_value(const CtfTilt& p, int x, int y, int sq, int weighted, const float* pw_tab) {
   const float ky = (float)y * p.fvy_f;
   const float kx = (float)x * p.fvx_f;
-  const float r2 = kx * kx + ky * ky;
+  const float r2 = ctf_r2(p.fvx_f, p.fvy_f, x, y);
   const float c2 = r2 > 0.0f ? (kx * kx - ky * ky) / r2 : 1.0f;
   const float s2 = r2 > 0.0f ? 2.0f * kx * ky / r2 : 0.0f;
   const float ca = c2 * p.cos2a_f + s2 * p.sin2a_f;
@@ -80,7 +98,7 @@ __device__ inline float ctf_value(const CtfTilt& p, int x, int y, int sq, int we
   if (sq) v *= v;
   if (weighted) {
     const float rw = (float)p.occupancy * (x == 0 ? 0.2f : fabsf((float)x));
-    const float pw = r2 > 0.0f ? powf(r2, -0.8325f) : __builtin_inff();
+    const float pw = pw_tab ? *pw_tab : ctf_pw(r2);
     v *= rw * expf((-0.5f * (float)p.exposure) / (0.24499f * pw + 2.81410f));
   }
   return v;
@@ -88,15 +106,10 @@ __device__ inline float ctf_value(const CtfTilt& p, int x, int y, int sq, int we
 
 // 2D CTF images for n tilts: out[t][y][x], x in [0,cx). centred: x -= o_x
 // (full grid only), y -= o_y; else FFT-order wrap (ctf.cu:31-40, :90-99).
-__global__ void __launch_bounds__(256) ctf2d_kernel(float* __restrict__ out,
-                                                    const CtfTilt* __restrict__ tilts, int cx,
-                                                    int ny, int o_x, int o_y, int half, int sq,
-                                                    int centered, int weighted) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y * blockDim.y + threadIdx.y;
-  const int t = blockIdx.z;
-  if (x >= cx || y >= ny) return;
-  int xx = x, yy = y;
+__device__ inline void ctf_index(int x, int y, int cx, int ny, int o_x, int o_y, int half, int centered, int& xx,
+                                 int& yy) {
+  xx = x;
+  yy = y;
   if (centered) {
     yy -= o_y;
     if (!half) xx -= o_x;
@@ -104,7 +117,37 @@ __global__ void __launch_bounds__(256) ctf2d_kernel(float* __restrict__ out,
     if (yy > o_y) yy -= ny;
     if (!half && xx > o_x) xx -= cx;
   }
-  out[((int64_t)t * ny + y) * cx + x] = ctf_value(tilts[t], xx, yy, sq, weighted);
+}
+
+// pw_tab ([y][x], of the Fourier voxel size (tfvx, tfvy)) serves the tilts
+// with that voxel size; others evaluate the power themselves.
+__global__ void __launch_bounds__(256) ctf2d_kernel(float* __restrict__ out,
+                                                    const CtfTilt* __restrict__ tilts, int cx,
+                                                    int ny, int o_x, int o_y, int half, int sq,
+                                                    int centered, int weighted, const float* __restrict__ pw_tab,
+                                                    float tfvx, float tfvy) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int t = blockIdx.z;
+  if (x >= cx || y >= ny) return;
+  int xx, yy;
+  ctf_index(x, y, cx, ny, o_x, o_y, half, centered, xx, yy);
+  const CtfTilt& p = tilts[t];
+  const bool tab = pw_tab && p.fvx_f == tfvx && p.fvy_f == tfvy;
+  out[((int64_t)t * ny + y) * cx + x] =
+      ctf_value(p, xx, yy, sq, weighted, tab ? pw_tab + (int64_t)y * cx + x : nullptr);
+}
+
+// The exposure weight's r2^-0.8325 over one CTF grid (the same indexing and
+// arithmetic as ctf2d_kernel) for the Fourier voxel size (fvx, fvy).
+__global__ void __launch_bounds__(256) ctf_pw_kernel(float* __restrict__ out, int cx, int ny, int o_x, int o_y,
+                                                     int half, int centered, float fvx, float fvy) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  if (x >= cx || y >= ny) return;
+  int xx, yy;
+  ctf_index(x, y, cx, ny, o_x, o_y, half, centered, xx, yy);
+  out[(int64_t)y * cx + x] = ctf_pw(ctf_r2(fvx, fvy, xx, yy));
 }
 
 // SF3D accumulation (mexSF3D.cu:37-77): for each voxel and tilt, the slab
@@ -299,7 +342,8 @@ void sf3d_impl(emc_ctx* ctx, int half, int sq, const int dims[3], int n,
   EMC_HIP(hipMemcpyAsync(d_t, h.data(), sizeof(CtfTilt) * n, hipMemcpyHostToDevice, ctx->stream));
   dim3 b2(64, 4, 1);
   hipLaunchKernelGGL(ctf2d_kernel, dim3(ceil_div(cx, 64), ceil_div(ny, 4), n), b2, 0, ctx->stream,
-                     d_ctf, (const CtfTilt*)d_t, cx, ny, half ? 0 : cx / 2, ny / 2, half, sq, 1, 1);
+                     d_ctf, (const CtfTilt*)d_t, cx, ny, half ? 0 : cx / 2, ny / 2, half, sq, 1, 1, nullptr, 0.0f,
+                     0.0f);
   hipLaunchKernelGGL(sf3d_kernel, dim3(ceil_div(nx, kPatch) * ceil_div(nz, kPatch), ceil_div(ny, 4 * kSlabRows), 1),
                      b2, 0, ctx->stream, d_out, (const float*)d_ctf, (const CtfTilt*)d_t, n, nx, ny, nz, cx);
   EMC_HIP(hipGetLastError());
@@ -467,7 +511,7 @@ emc_status emc_weight_mask(emc_ctx* ctx, const int dims[3], double sampling,
     EMC_HIP(hipMemcpyAsync(d_t, h.data(), sizeof(CtfTilt) * n, hipMemcpyHostToDevice, ctx->stream));
     const dim3 b2(64, 4, 1);
     hipLaunchKernelGGL(ctf2d_kernel, dim3(ceil_div(cx, 64), ceil_div(ny, 4), n), b2, 0, ctx->stream, d_ctf,
-                       (const CtfTilt*)d_t, cx, ny, cx / 2, ny / 2, 0, 1, 1, 1);
+                       (const CtfTilt*)d_t, cx, ny, cx / 2, ny / 2, 0, 1, 1, 1, nullptr, 0.0f, 0.0f);
     hipLaunchKernelGGL(sf3d_kernel,
                        dim3(ceil_div(dims[0], kPatch) * ceil_div(dims[2], kPatch), ceil_div(ny, 4 * kSlabRows), 1), b2,
                        0, ctx->stream, d_out, (const float*)d_ctf, (const CtfTilt*)d_t, n, dims[0], ny, dims[2], cx);
@@ -536,6 +580,11 @@ emc_status emc_weight_mask_batched(emc_ctx* ctx, const int dims[3], double sampl
     float* d_ctf = (float*)scratch(ctx, "wmb_stack", tilt_bytes * (size_t)max_nt);
     float* d_part = (float*)scratch(ctx, "wmb_min", sizeof(float) * ((size_t)per + 1) * n_part);
     float* d_mins = d_part + (size_t)per * n_part;
+    // the exposure weight's power of r2, once for the grid of the first
+    // tilt's Fourier voxel size (every tilt of that size reads it)
+    float* d_pw = (float*)scratch(ctx, "wmb_pw", sizeof(float) * (size_t)cx * ny);
+    hipLaunchKernelGGL(ctf_pw_kernel, dim3(ceil_div(cx, 64), ceil_div(ny, 4), 1), b2, 0, ctx->stream, d_pw, cx, ny,
+                       cx / 2, ny / 2, 0, 1, all[0].fvx_f, all[0].fvy_f);
     for (size_t c = 0; c + 1 < cuts.size(); ++c) {
       const int p0 = cuts[c], p1 = cuts[c + 1];
       const int np = p1 - p0, nt = first[p1] - first[p0];
@@ -543,7 +592,7 @@ emc_status emc_weight_mask_batched(emc_ctx* ctx, const int dims[3], double sampl
       {
         KTimer kt(ctx, "ctf2d_many");
         hipLaunchKernelGGL(ctf2d_kernel, dim3(ceil_div(cx, 64), ceil_div(ny, 4), nt), b2, 0, ctx->stream, d_ctf,
-                           dt, cx, ny, cx / 2, ny / 2, 0, 1, 1, 1);
+                           dt, cx, ny, cx / 2, ny / 2, 0, 1, 1, 1, (const float*)d_pw, all[0].fvx_f, all[0].fvy_f);
       }
       // SF3D - (min - 1e-4) (BH_weightMaskMex.m:55). Every term is >= 0
       // (squared CTF, slab weight >= 0, occupancy >= 0), so where a voxel
@@ -586,7 +635,7 @@ emc_status emc_ctf2d(emc_ctx* ctx, int half, int sq_ctf, int nx, int ny, float p
     EMC_HIP(hipMemcpyAsync(d_t, &t, sizeof(t), hipMemcpyHostToDevice, ctx->stream));
     hipLaunchKernelGGL(ctf2d_kernel, dim3(ceil_div(cx, 64), ceil_div(ny, 4), 1), dim3(64, 4, 1), 0,
                        ctx->stream, d_out, (const CtfTilt*)d_t, cx, ny, half ? 0 : nx / 2, ny / 2,
-                       half, sq_ctf, centered, 0);
+                       half, sq_ctf, centered, 0, nullptr, 0.0f, 0.0f);
     EMC_HIP(hipGetLastError());
     EMC_HIP(hipStreamSynchronize(ctx->stream));
   });
