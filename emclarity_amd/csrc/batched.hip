@@ -1493,23 +1493,25 @@ __global__ void refine_step_kernel(RefineState S, int P) {
       sy += step[i] * y[i];
     }
     if (sy > 1e-18) {
+      // H' = V H V^T + rho s s^T with V = I - rho s y^T; for symmetric H
+      // this is H - rho (s (Hy)^T + (Hy) s^T) + (rho^2 y^T H y + rho) s s^T
       const double rho = 1.0 / sy;
       double* H = S.H + 36 * p;
-      double V[36], VH[36];
+      double hy[6], yhy = 0.0;
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        double a = 0.0;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) a += H[6 * i + j] * y[j];
+        hy[i] = a;
+        yhy += y[i] * a;
+      }
+      const double c = rho * rho * yhy + rho;
+#pragma unroll
       for (int i = 0; i < 6; ++i)
-        for (int j = 0; j < 6; ++j) V[6 * i + j] = (i == j ? 1.0 : 0.0) - rho * step[i] * y[j];
-      for (int i = 0; i < 6; ++i)
-        for (int j = 0; j < 6; ++j) {
-          double a = 0.0;
-          for (int l = 0; l < 6; ++l) a += V[6 * i + l] * H[6 * l + j];
-          VH[6 * i + j] = a;
-        }
-      for (int i = 0; i < 6; ++i)
-        for (int j = 0; j < 6; ++j) {
-          double a = 0.0;
-          for (int l = 0; l < 6; ++l) a += VH[6 * i + l] * V[6 * j + l];
-          H[6 * i + j] = a + rho * step[i] * step[j];
-        }
+#pragma unroll
+        for (int j = 0; j < 6; ++j)
+          H[6 * i + j] = H[6 * i + j] - rho * (step[i] * hy[j] + hy[i] * step[j]) + c * step[i] * step[j];
     }
     for (int i = 0; i < 9; ++i) S.R[9 * p + i] = S.Rt[9 * p + i];
     for (int i = 0; i < 3; ++i) S.s[3 * p + i] = S.st[3 * p + i];
