@@ -234,32 +234,68 @@ __device__ __forceinline__ float edge_weight(int i, int n) {
   return w;
 }
 
+// Exact quotient u / d for u < 2^24 and 1 <= d <= 2^16 from m = ceil(2^40 / d):
+// u m / 2^40 = u / d + u (m - 2^40 / d) / 2^40 lies in [u / d, u / d + u / 2^40)
+// and u d < 2^40, so the floor is floor(u / d).
+__device__ __forceinline__ uint32_t div_magic(uint32_t u, uint64_t m) { return (uint32_t)(((uint64_t)u * m) >> 40); }
+
 // grid: x over quads of window voxels (4 consecutive x of one row), y over
 // particles. Stores are float4 (vec4) when the window's x size is a multiple
-// of 4 and the output is 16-byte aligned.
+// of 4 and the output is 16-byte aligned. The quad index splits into (x
+// quad, row, plane) by multiply-shift division (mq = ceil(2^40 / qx), my =
+// ceil(2^40 / wy); mq = 0: plain division). For 2-byte modes a quad's four
+// source values come from two aligned 8-byte loads (elements a0 .. a0 + 7,
+// a0 = the first element rounded down to a multiple of 4) when those lie
+// inside the tomogram's `total` elements, else one load per voxel.
 template <int MODE>
 __global__ void __launch_bounds__(256) extract_kernel(const void* __restrict__ tomo, int64_t tnx, int64_t tny,
-                                                      const Window* __restrict__ wins, int wx, int wy, int wz,
-                                                      int taper, int vec4, float* __restrict__ out) {
+                                                      int64_t total, const Window* __restrict__ wins, int wx, int wy,
+                                                      int wz, uint64_t mq, uint64_t my, int taper, int vec4,
+                                                      float* __restrict__ out) {
+  using T = typename ModeT<MODE>::T;
   const int p = blockIdx.y;
   const int qx = (wx + 3) >> 2;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= (int64_t)qx * wy * wz) return;
   int X0, Y, Z;
-  split3(q, qx, wy, X0, Y, Z);
-  X0 *= 4;
+  if (mq) {  // qx wy wz < 2^24, wy <= 2^16
+    const uint32_t r = div_magic((uint32_t)q, mq), zz = div_magic(r, my);
+    X0 = 4 * (int)((uint32_t)q - r * (uint32_t)qx);
+    Y = (int)(r - zz * (uint32_t)wy);
+    Z = (int)zz;
+  } else {
+    split3(q, qx, wy, X0, Y, Z);
+    X0 *= 4;
+  }
   const Window w = wins[p];
   const int y = Y - w.pad[1], z = Z - w.pad[2];
   const bool row_in = w.valid && (unsigned)y < (unsigned)w.ext[1] && (unsigned)z < (unsigned)w.ext[2];
   float v[4];
   const float wyz = row_in ? (taper ? edge_weight(y, w.ext[1]) * edge_weight(z, w.ext[2]) : 1.0f) : 0.0f;
   const int64_t row = ((int64_t)(w.src[2] + z) * tny + (w.src[1] + y)) * tnx + w.src[0];
+  float e[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  bool wide = false;
+  if constexpr (sizeof(T) == 2) {
+    const int64_t base = row + X0 - w.pad[0];  // source element of the quad's first voxel
+    const int64_t a0 = base & ~(int64_t)3;
+    if (row_in && a0 >= 0 && a0 + 8 <= total) {
+      const uint64_t* src = reinterpret_cast<const uint64_t*>(static_cast<const T*>(tomo) + a0);
+      const uint64_t lo = src[0], hi = src[1];
+      const int sh = 16 * (int)(base - a0);
+      const uint64_t bits = sh ? ((lo >> sh) | (hi << (64 - sh))) : lo;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) e[k] = (float)(T)(uint16_t)(bits >> (16 * k));
+      wide = true;
+    }
+  }
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int x = X0 + k - w.pad[0];
     v[k] = 0.0f;
-    if (row_in && (unsigned)x < (unsigned)w.ext[0])
-      v[k] = taper ? load_mode<MODE>(tomo, row + x) * (edge_weight(x, w.ext[0]) * wyz) : load_mode<MODE>(tomo, row + x);
+    if (row_in && (unsigned)x < (unsigned)w.ext[0]) {
+      const float a = wide ? e[k] : load_mode<MODE>(tomo, row + x);
+      v[k] = taper ? a * (edge_weight(x, w.ext[0]) * wyz) : a;
+    }
   }
   float* o = out + ((int64_t)p * wz + Z) * ((int64_t)wy * wx) + (int64_t)Y * wx + X0;
   if (vec4) {
@@ -516,8 +552,14 @@ emc_status emc_extract_subtomos(emc_ctx* ctx, const void* d_tomo, int mode, cons
     // behind the work already on the stream
     upload_async(ctx, d_wins, wins.data(), bytes);
     const int64_t quads = (int64_t)((window[0] + 3) / 4) * window[1] * window[2];
-    const int64_t tnx = tomo_dims[0], tny = tomo_dims[1];
+    const int64_t tnx = tomo_dims[0], tny = tomo_dims[1], total = tnx * tny * (int64_t)tomo_dims[2];
     const int vec4 = (window[0] % 4 == 0) && ((reinterpret_cast<uintptr_t>(d_out) & 15) == 0);
+    // the kernel's multiply-shift division needs quads < 2^24 and divisors
+    // <= 2^16; larger windows divide the plain way (mq = 0)
+    const bool magic = quads < ((int64_t)1 << 24) && window[1] <= (1 << 16);
+    const int qxw = (window[0] + 3) / 4;
+    const uint64_t mq = magic ? (((uint64_t)1 << 40) + (uint64_t)qxw - 1) / (uint64_t)qxw : 0;
+    const uint64_t my = magic ? (((uint64_t)1 << 40) + (uint64_t)window[1] - 1) / (uint64_t)window[1] : 0;
     KTimer t(ctx, "extract_subtomos");
     // grid.y carries the particles: at most 65535 per launch
     for (int p0 = 0; p0 < n; p0 += 65535) {
@@ -526,10 +568,10 @@ emc_status emc_extract_subtomos(emc_ctx* ctx, const void* d_tomo, int mode, cons
       const Window* w = d_wins + p0;
       float* o = d_out + (int64_t)p0 * wvox;
       switch (mode) {
-        case 0: hipLaunchKernelGGL(extract_kernel<0>, grid, dim3(256), 0, ctx->stream, d_tomo, tnx, tny, w, window[0], window[1], window[2], taper, vec4, o); break;
-        case 1: hipLaunchKernelGGL(extract_kernel<1>, grid, dim3(256), 0, ctx->stream, d_tomo, tnx, tny, w, window[0], window[1], window[2], taper, vec4, o); break;
-        case 2: hipLaunchKernelGGL(extract_kernel<2>, grid, dim3(256), 0, ctx->stream, d_tomo, tnx, tny, w, window[0], window[1], window[2], taper, vec4, o); break;
-        default: hipLaunchKernelGGL(extract_kernel<6>, grid, dim3(256), 0, ctx->stream, d_tomo, tnx, tny, w, window[0], window[1], window[2], taper, vec4, o); break;
+        case 0: hipLaunchKernelGGL(extract_kernel<0>, grid, dim3(256), 0, ctx->stream, d_tomo, tnx, tny, total, w, window[0], window[1], window[2], mq, my, taper, vec4, o); break;
+        case 1: hipLaunchKernelGGL(extract_kernel<1>, grid, dim3(256), 0, ctx->stream, d_tomo, tnx, tny, total, w, window[0], window[1], window[2], mq, my, taper, vec4, o); break;
+        case 2: hipLaunchKernelGGL(extract_kernel<2>, grid, dim3(256), 0, ctx->stream, d_tomo, tnx, tny, total, w, window[0], window[1], window[2], mq, my, taper, vec4, o); break;
+        default: hipLaunchKernelGGL(extract_kernel<6>, grid, dim3(256), 0, ctx->stream, d_tomo, tnx, tny, total, w, window[0], window[1], window[2], mq, my, taper, vec4, o); break;
       }
       EMC_HIP(hipGetLastError());
     }

@@ -37,12 +37,15 @@ def _centres(rng, dims, n):
     return c
 
 
-@pytest.mark.parametrize("mode,taper", [(1, True), (2, True), (2, False)])
+@pytest.mark.parametrize("mode,taper", [(1, True), (1, False), (2, True), (2, False), (6, True)])
 @pytest.mark.parametrize("window", [(32, 32, 32), (27, 30, 25)])
 def test_extract_matches_oracle(ctx, mode, taper, window):
+    """2-byte modes read each quad of voxels with two aligned 8-byte loads
+    (any alignment of the window's first voxel; one load per voxel where
+    those would leave the tomogram, at the padded edges)."""
     rng = np.random.default_rng(sum(window) + mode)
     shape = (70, 80, 90)
-    tomo = _data(mode, shape=shape, seed=mode).astype(np.int16 if mode == 1 else np.float32)
+    tomo = _data(mode, shape=shape, seed=mode).astype({1: np.int16, 2: np.float32, 6: np.uint16}[mode])
     t = ResidentTomogram.from_numpy(tomo)
     assert t.mode == mode
     mr = (4.0, 4.0, 4.0)
