@@ -70,8 +70,13 @@ __global__ void pad_origin_kernel(const float* __restrict__ ref, float* __restri
   out[((int64_t)Z * m + Y) * m + X] = v;
 }
 
-// Particles of one preparation chunk (kernel-argument pointer table).
-constexpr int kPrepChunk = 16;
+// Particles of one preparation chunk (kernel-argument pointer table): up to
+// 64 (1 KiB of kernel arguments); prep_chunk picks the chunk per box size.
+constexpr int kPrepChunk = 64;
+
+// Particles per preparation chunk: 16 at 128^3 and above, 32 up to 96^3, 64
+// up to 64^3, so a chunk's batched transform is ~32 MB of input or more.
+inline int prep_chunk(int n) { return n > 96 ? 16 : (n > 64 ? 32 : 64); }
 struct PrepPtrs {
   const float* part[kPrepChunk];
   const float* wedge[kPrepChunk];  // centred SF3D or null (= 1)
@@ -1592,13 +1597,15 @@ emc_status emc_batch_prepare_particles_masked(emc_ctx* ctx, int n, int n_part, c
     for (int i = 0; i < n_part; ++i) EMC_REQUIRE(h_particles[i], "null particle pointer %d", i);
     const int64_t nv = (int64_t)n * n * n, nh = (int64_t)(n / 2 + 1) * n * n;
     const bool even = n % 2 == 0;
-    float* sh = (float*)scratch(ctx, "bat_shift", sizeof(float) * nv * kPrepChunk);
-    float2* P = (float2*)scratch(ctx, "bat_P", sizeof(float2) * nh * kPrepChunk);
+    const int chunk = prep_chunk(n);
+    float* sh = (float*)scratch(ctx, "bat_shift", sizeof(float) * nv * chunk);
+    float2* P = (float2*)scratch(ctx, "bat_P", sizeof(float2) * nh * chunk);
     const int nb = 256;
-    double* part = (double*)scratch(ctx, "bat_part", sizeof(double) * nb * kPrepChunk);
-    // chunks of 16, 8, 4, 2, 1 particles: a handful of batched FFT plans per size
+    double* part = (double*)scratch(ctx, "bat_part", sizeof(double) * nb * chunk);
+    // chunks of `chunk`, then halves down to 1 particle: a handful of batched
+    // FFT plans per size
     for (int i0 = 0; i0 < n_part;) {
-      int B = kPrepChunk;
+      int B = chunk;
       while (B > n_part - i0) B /= 2;
       PrepPtrs pp{};
       for (int b = 0; b < B; ++b) {
