@@ -231,6 +231,42 @@ def test_preparation_of_stacked_particles_reads_them_in_place(ctx):
             assert abs(out[0][2][i] - ref.A) <= 1e-5 * ref.A
 
 
+def test_preparation_in_64_particle_chunks(ctx):
+    """70 stacked 64^3 particles with their own wedges prepare in chunks of
+    64 + 4 + 2 (the R2C reading the stack in place): Q, N and A of the first
+    and last particle of each chunk against the single-particle entry (Q to
+    1e-5 of its maximum, N exact, A to 1e-6), A against the FP64 oracle."""
+    import ctypes as C
+    from emclarity_amd import _lib
+    from emclarity_amd.device import DeviceArray
+    n, P = 64, 70
+    rng = np.random.default_rng(21)
+    bp = O.bandpass3d((n, n, n), 1e-3, 40.0, 8.0, 2.5)
+    vols = rng.standard_normal((P, n, n, n)).astype(np.float32)
+    w = [BH_weightMaskMex((n, n, n), 1, S.tilt_table(55.0 + (i % 3) * 5.0, 3.0, 2.0 + 0.1 * i, 2.5),
+                          [n // 2 + 1] * 3, [[n, n, n], [0, 0, 0]]).get() for i in range(4)]
+    wedges = [w[i % 4] for i in range(P)]
+    stack = DeviceArray.from_numpy(vols, ctx=ctx)
+    views = [stack.view((n, n, n), np.float32, 4 * n ** 3 * i) for i in range(P)]
+    al = BatchAligner(n, np.zeros((n, n, n), np.float32), bp.astype(np.float32))
+    al.set_particles(views, wedges, [np.eye(3)] * P)
+    Qb, Nb, Ab = al.Q.get(), al.N.get(), al.A.get()
+    d_bp = DeviceArray.from_numpy(bp.astype(np.float32))
+    for i in (0, 63, 64, 67, 68, 69):
+        Q1 = DeviceArray((al.nh,), np.complex64)
+        N1 = DeviceArray((al.nh,), np.float32)
+        A1 = DeviceArray((1,), np.float64)
+        wd = DeviceArray.from_numpy(wedges[i].astype(np.float32))
+        _lib.call("emc_batch_prepare_particle", ctx.handle, n, views[i].vp, d_bp.vp, wd.vp, Q1.vp, N1.vp, A1.vp)
+        q1, n1, a1 = Q1.get(), N1.get(), A1.get()[0]
+        assert np.max(np.abs(Qb[i] - q1)) <= 1e-5 * np.max(np.abs(q1)), i
+        assert np.array_equal(Nb[i], n1), i
+        assert abs(Ab[i] - a1) <= 1e-6 * a1, i
+        ref = O.batch_prepare(vols[i].astype(np.float64), bp, wedges[i], al.support)
+        assert abs(Ab[i] - ref.A) <= 1e-5 * ref.A, i
+    assert np.array_equal(stack.get(), vols)
+
+
 def _host_refine(al, R0, s0, iters=12, step_angle=0.02, step_shift=0.3):
     """The refinement restated on the host (numpy FP64) around the device
     adjoint: the same batched BFGS emc_align_refine runs on the device."""
