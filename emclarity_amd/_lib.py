@@ -84,6 +84,9 @@ SIGNATURES = {
     "emc_mask_shape": (_st, [_vp, C.c_int, _i3, _d, _d, C.c_double, _vp]),
     "emc_tsearch_peaks": (_st, [_vp, _vp, _vp, _i3, _vp, C.c_int, _d, C.c_int, C.c_int, C.c_double, C.c_double,
                                 C.c_int, C.c_int, _d, C.POINTER(C.c_int)]),
+    "emc_tsearch_peaks_multi": (_st, [_vp, _vp, _vp, _i3, _vp, C.c_int, _d, C.c_int, C.c_int, C.c_double,
+                                      C.c_double, C.c_int, C.c_int, C.c_int, C.c_double, _d, _d,
+                                      C.POINTER(C.c_int)]),
     "emc_moving_average": (_st, [_vp, _vp, _i3, C.c_int, _vp]),
     "emc_local_normalize": (_st, [_vp, _vp, _i3, C.c_int, _vp]),
     "emc_tsearch_preprocess_chunk": (_st, [_vp, _vp, _i3, _d, C.c_int, C.c_int, C.c_float, _vp, _vp, _d]),

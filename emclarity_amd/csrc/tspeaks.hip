@@ -14,6 +14,10 @@
 // argmax over the tile keys, one small kernel for the centre of mass, the
 // rotation of the removal mask and the erase, and a refresh of the tiles the
 // erase touched; one 32-byte result copy per step.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
 #include "ops.hpp"
 
 namespace emc {
@@ -59,21 +63,27 @@ struct PeakStep {
   double com[3];
 };
 
+// bd: half edge of the centre-of-mass box, 1 + round(log(nPeaks)) (:1156);
+// box_mag / box_ang (or null): that box's map values and angle ids, x
+// fastest, 0 outside the map, for the extra peaks.
 __global__ void peak_step_kernel(const unsigned long long* __restrict__ part, int nb, const float* __restrict__ mag,
-                                 const int32_t* __restrict__ ang, int nx, int ny, int nz,
-                                 PeakStep* __restrict__ out) {
+                                 const int32_t* __restrict__ ang, int nx, int ny, int nz, int bd,
+                                 PeakStep* __restrict__ out, float* __restrict__ box_mag,
+                                 int32_t* __restrict__ box_ang) {
+  __shared__ uint32_t s_i;
   unsigned long long k = 0ull;
   for (int i = threadIdx.x; i < nb; i += blockDim.x) k = part[i] > k ? part[i] : k;
   k = block_max_u64(k);
   if (threadIdx.x == 0) {
     uint32_t i = key_index(k);
     if ((int64_t)i >= (int64_t)nx * ny * nz) i = 0;  // keys always carry a voxel index; stay in bounds
+    s_i = i;
     const int x = (int)(i % (uint32_t)nx), y = (int)((i / (uint32_t)nx) % (uint32_t)ny),
               z = (int)(i / ((uint32_t)nx * (uint32_t)ny));
     double s = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
-    for (int dz = -1; dz <= 1; ++dz)
-      for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx) {
+    for (int dz = -bd; dz <= bd; ++dz)
+      for (int dy = -bd; dy <= bd; ++dy)
+        for (int dx = -bd; dx <= bd; ++dx) {
           const int xx = x + dx, yy = y + dy, zz = z + dz;
           const double v = (xx >= 0 && xx < nx && yy >= 0 && yy < ny && zz >= 0 && zz < nz)
                                ? (double)mag[((int64_t)zz * ny + yy) * nx + xx]
@@ -88,6 +98,20 @@ __global__ void peak_step_kernel(const unsigned long long* __restrict__ part, in
     out->com[0] = cx / s;
     out->com[1] = cy / s;
     out->com[2] = cz / s;
+  }
+  if (box_mag) {  // uniform
+    __syncthreads();
+    const uint32_t i = s_i;
+    const int x = (int)(i % (uint32_t)nx), y = (int)((i / (uint32_t)nx) % (uint32_t)ny),
+              z = (int)(i / ((uint32_t)nx * (uint32_t)ny));
+    const int e = 2 * bd + 1;
+    for (int v = threadIdx.x; v < e * e * e; v += blockDim.x) {
+      const int xx = x + v % e - bd, yy = y + (v / e) % e - bd, zz = z + v / (e * e) - bd;
+      const bool in = xx >= 0 && xx < nx && yy >= 0 && yy < ny && zz >= 0 && zz < nz;
+      const int64_t j = ((int64_t)zz * ny + yy) * nx + xx;
+      box_mag[v] = in ? mag[j] : 0.0f;
+      box_ang[v] = in ? ang[j] : 0;
+    }
   }
 }
 
@@ -209,88 +233,207 @@ emc_status emc_mask3d_sphere(emc_ctx* ctx, const int dims[3], const double radiu
   });
 }
 
+// The extra peaks of one taken peak (:1198-1240), on the host from the
+// centre-of-mass box: possible = box map values with the peak's angle id
+// zeroed; per extra peak, if any value is non-zero: cAng = first maximum,
+// iSNR = mean of the values of its id (then zeroed), and unless that id or
+// Ang(cAng) is <= 0 its angles are ANGLE_LIST(Ang(cAng)) -- Ang indexed
+// with the box's linear index, as the reference does -- else, and when every
+// value is zero, random: angles = peak position (times the sampling rate) +
+// nRandom^2 x the caller's normals, nRandom = 2, 3, .. 10, with iSNR = 0.75
+// x the mean score of the peaks before it (only where a value was non-zero).
+// Every extra peak has the position of the taken one.
+static void extra_peaks(double* row, int n_peaks, int box, const float* box_mag, const int32_t* box_ang,
+                        const int32_t* ang_head, int top_id, const double* angle_list, int n_angles,
+                        const double* normals) {
+  std::vector<float> pa(box_mag, box_mag + box);
+  for (int v = 0; v < box; ++v)
+    if (box_ang[v] == top_id) pa[v] = 0.0f;
+  double snr = 0.0;
+  int n_random = 2;
+  for (int ip = 2; ip <= n_peaks; ++ip) {
+    bool random = false;
+    double a3[3] = {0.0, 0.0, 0.0};
+    bool any = false;
+    for (int v = 0; v < box && !any; ++v) any = pa[v] != 0.0f;
+    if (any) {
+      int c = 0;  // first maximum over the whole box, zeros included
+      for (int v = 1; v < box; ++v)
+        if (pa[v] > pa[c]) c = v;
+      const int top = box_ang[c];
+      double sum = 0.0;
+      int cnt = 0;
+      for (int v = 0; v < box; ++v)
+        if (box_ang[v] == top) {
+          sum += pa[v];
+          ++cnt;
+          pa[v] = 0.0f;
+        }
+      snr = (double)(float)(sum / cnt);  // the mean of single values
+      const int head = ang_head[c];
+      if (top <= 0 || head <= 0) {
+        random = true;
+        double m = 0.0;
+        for (int q = 1; q < ip; ++q) m += row[10 * q - 1];
+        snr = m / (ip - 1) * 0.75;
+      } else {
+        EMC_REQUIRE(head <= n_angles, "angle id %d outside the angle list (%d)", head, n_angles);
+        for (int d = 0; d < 3; ++d) a3[d] = angle_list[3 * (head - 1) + d];
+      }
+    } else {
+      random = true;
+    }
+    if (random) {
+      const double* z = normals + 3 * (ip - 2);
+      for (int d = 0; d < 3; ++d) a3[d] = z[d] * (double)(n_random * n_random) + row[d];
+      if (n_random < 10) ++n_random;
+    }
+    double* sub = row + 10 * (ip - 1);
+    for (int d = 0; d < 3; ++d) {
+      sub[d] = row[d];
+      sub[3 + d] = a3[d];
+    }
+    sub[9] = snr;
+  }
+}
+
+// The loop for both entry points: peakMat rows of 10 n_peaks doubles.
+static void tsearch_peaks_impl(emc_ctx* ctx, float* d_mag, const int32_t* d_ang, const int dims[3],
+                               const float* d_removal_mask, int rm_dim, const double* angle_list, int n_angles,
+                               int convention, double mask_cutoff, double high_thr, int max_peaks, int max_tries,
+                               int n_peaks, double sampling_rate, const double* normals, double* h_rows,
+                               int* n_found) {
+  use_device(ctx);
+  EMC_REQUIRE(d_mag && d_ang && dims && d_removal_mask && angle_list && h_rows && n_found, "null pointer");
+  EMC_REQUIRE(dims[0] > 0 && dims[1] > 0 && dims[2] > 0, "bad dims");
+  EMC_REQUIRE((int64_t)dims[0] * dims[1] * dims[2] < ((int64_t)1 << 32), "map too large for 32-bit indices");
+  EMC_REQUIRE(rm_dim >= 0 && rm_dim < 512, "bad removal mask radius");
+  EMC_REQUIRE(n_angles > 0 && max_peaks >= 0 && max_tries >= 0, "bad counts");
+  EMC_REQUIRE(n_peaks >= 1 && n_peaks <= 1000, "n_peaks must be 1..1000");
+  EMC_REQUIRE(n_peaks == 1 || normals, "null normals");
+  const int nx = dims[0], ny = dims[1], nz = dims[2];
+  const int64_t nvox = (int64_t)nx * ny * nz;
+  const int bd = 1 + (int)std::lround(std::log((double)n_peaks));
+  const int be = 2 * bd + 1, box = be * be * be;
+  const int tx = ceil_div(nx, kPeakTile), ty = ceil_div(ny, kPeakTile), tz = ceil_div(nz, kPeakTile);
+  const int64_t nt = (int64_t)tx * ty * tz;
+  auto* keys = (unsigned long long*)scratch(ctx, "pk_keys", sizeof(unsigned long long) * nt);
+  const int nb = 256;
+  auto* part = (unsigned long long*)scratch(ctx, "pk_part", sizeof(unsigned long long) * nb);
+  auto* step = (PeakStep*)scratch(ctx, "pk_step", sizeof(PeakStep));
+  const int e = 2 * rm_dim + 1;
+  float* rot = (float*)scratch(ctx, "pk_rot", sizeof(float) * (size_t)e * e * e);
+  float* d_box = nullptr;
+  int32_t* d_box_ang = nullptr;
+  std::vector<int32_t> ang_head;
+  if (n_peaks > 1) {
+    EMC_REQUIRE(box <= nvox, "map smaller than the %d^3 centre-of-mass box", be);
+    d_box = (float*)scratch(ctx, "pk_box", sizeof(float) * box);
+    d_box_ang = (int32_t*)scratch(ctx, "pk_box_ang", sizeof(int32_t) * box);
+    ang_head.resize(box);  // Ang(1 .. box), read by the extra peaks
+    EMC_HIP(hipMemcpyAsync(ang_head.data(), d_ang, sizeof(int32_t) * box, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  const size_t hb = sizeof(PeakStep) + (n_peaks > 1 ? (sizeof(float) + sizeof(int32_t)) * box : 0);
+  auto* h = (PeakStep*)pinned(ctx, hb);
+  float* h_box = (float*)(h + 1);
+  int32_t* h_box_ang = (int32_t*)(h_box + box);
+  EMC_HIP(hipStreamSynchronize(ctx->stream));  // the pinned buffer may be in use
+  auto refresh = [&](int x0, int y0, int z0, int x1, int y1, int z1) {  // voxel box, inclusive
+    const int a0 = std::max(0, x0 / kPeakTile), b0 = std::max(0, y0 / kPeakTile), c0 = std::max(0, z0 / kPeakTile);
+    const int a1 = std::min(tx - 1, x1 / kPeakTile), b1 = std::min(ty - 1, y1 / kPeakTile),
+              c1 = std::min(tz - 1, z1 / kPeakTile);
+    if (a1 < a0 || b1 < b0 || c1 < c0) return;
+    const int tnx = a1 - a0 + 1, tny = b1 - b0 + 1, tnz = c1 - c0 + 1;
+    hipLaunchKernelGGL(tile_keys_kernel, dim3(tnx * tny * tnz), dim3(256), 0, ctx->stream, (const float*)d_mag,
+                       nx, ny, nz, tx, ty, a0, b0, c0, tnx, tny, keys);
+  };
+  auto next = [&]() {
+    hipLaunchKernelGGL(keys_max_kernel, dim3(nb), dim3(256), 0, ctx->stream, (const unsigned long long*)keys, nt,
+                       part);
+    hipLaunchKernelGGL(peak_step_kernel, dim3(1), dim3(256), 0, ctx->stream, (const unsigned long long*)part, nb,
+                       (const float*)d_mag, d_ang, nx, ny, nz, bd, step, d_box, d_box_ang);
+    EMC_HIP(hipGetLastError());
+    EMC_HIP(hipMemcpyAsync(h, step, sizeof(PeakStep), hipMemcpyDeviceToHost, ctx->stream));
+    if (d_box) {
+      EMC_HIP(hipMemcpyAsync(h_box, d_box, sizeof(float) * box, hipMemcpyDeviceToHost, ctx->stream));
+      EMC_HIP(hipMemcpyAsync(h_box_ang, d_box_ang, sizeof(int32_t) * box, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    EMC_HIP(hipStreamSynchronize(ctx->stream));
+    return *h;
+  };
+  refresh(0, 0, 0, nx - 1, ny - 1, nz - 1);
+  PeakStep s = next();
+  int n = 0, tries = 0;
+  const int64_t w = 10 * (int64_t)n_peaks;
+  while (n < max_peaks && tries < max_tries && (double)key_value(s.key) > high_thr) {
+    ++tries;
+    uint32_t i = key_index(s.key);
+    if ((int64_t)i >= nvox) i = 0;
+    const int x = (int)(i % (uint32_t)nx), y = (int)((i / (uint32_t)nx) % (uint32_t)ny),
+              z = (int)(i / ((uint32_t)nx * (uint32_t)ny));
+    if (s.ang > 0) {
+      EMC_REQUIRE(s.ang <= n_angles, "angle id %d outside the angle list (%d)", s.ang, n_angles);
+      const double* ang = angle_list + 3 * (s.ang - 1);
+      double* o = h_rows + w * n;
+      std::fill(o, o + w, 0.0);
+      // MATLAB 1-based centre plus centre of mass, minus the padding
+      o[0] = sampling_rate * (x + 1 + s.com[0] - rm_dim);
+      o[1] = sampling_rate * (y + 1 + s.com[1] - rm_dim);
+      o[2] = sampling_rate * (z + 1 + s.com[2] - rm_dim);
+      o[3] = ang[0];
+      o[4] = ang[1];
+      o[5] = ang[2];
+      o[9] = (double)key_value(s.key);
+      if (n_peaks > 1)
+        extra_peaks(o, n_peaks, box, h_box, h_box_ang, ang_head.data(), s.ang, angle_list, n_angles,
+                    normals + 3 * (n_peaks - 1) * (int64_t)n);
+      // removal mask rotated by the peak's angles ('forward', no shift)
+      double M[9];
+      define_matrix(ang, convention, 0, M);
+      float rows[9];
+      colmajor_to_rows(M, rows);
+      const int md[3] = {e, e, e};
+      const float t0[3] = {0.0f, 0.0f, 0.0f};
+      XformArgs xa = xform_args(md, rows, 1, t0, 1);
+      launch_xform(ctx, d_removal_mask, rot, xa);
+      hipLaunchKernelGGL(erase_kernel, dim3(ceil_div((int64_t)e * e * e, 256)), dim3(256), 0, ctx->stream, d_mag,
+                         nx, ny, nz, x, y, z, rm_dim, (const float*)rot, (float)mask_cutoff);
+      refresh(x - rm_dim, y - rm_dim, z - rm_dim, x + rm_dim, y + rm_dim, z + rm_dim);
+      ++n;
+    } else {
+      hipLaunchKernelGGL(zero_voxel_kernel, dim3(1), dim3(1), 0, ctx->stream, d_mag, (int64_t)i);
+      refresh(x, y, z, x, y, z);
+    }
+    s = next();
+  }
+  *n_found = n;
+}
+
 emc_status emc_tsearch_peaks(emc_ctx* ctx, float* d_mag, const int32_t* d_ang, const int dims[3],
                              const float* d_removal_mask, int rm_dim, const double* angle_list, int n_angles,
                              int convention, double mask_cutoff, double high_thr, int max_peaks, int max_tries,
                              double* h_peaks, int* n_found) {
   return guard([&] {
-    use_device(ctx);
-    EMC_REQUIRE(d_mag && d_ang && dims && d_removal_mask && angle_list && h_peaks && n_found, "null pointer");
-    EMC_REQUIRE(dims[0] > 0 && dims[1] > 0 && dims[2] > 0, "bad dims");
-    EMC_REQUIRE((int64_t)dims[0] * dims[1] * dims[2] < ((int64_t)1 << 32), "map too large for 32-bit indices");
-    EMC_REQUIRE(rm_dim >= 0 && rm_dim < 512, "bad removal mask radius");
-    EMC_REQUIRE(n_angles > 0 && max_peaks >= 0 && max_tries >= 0, "bad counts");
-    const int nx = dims[0], ny = dims[1], nz = dims[2];
-    const int tx = ceil_div(nx, kPeakTile), ty = ceil_div(ny, kPeakTile), tz = ceil_div(nz, kPeakTile);
-    const int64_t nt = (int64_t)tx * ty * tz;
-    auto* keys = (unsigned long long*)scratch(ctx, "pk_keys", sizeof(unsigned long long) * nt);
-    const int nb = 256;
-    auto* part = (unsigned long long*)scratch(ctx, "pk_part", sizeof(unsigned long long) * nb);
-    auto* step = (PeakStep*)scratch(ctx, "pk_step", sizeof(PeakStep));
-    const int e = 2 * rm_dim + 1;
-    float* rot = (float*)scratch(ctx, "pk_rot", sizeof(float) * (size_t)e * e * e);
-    auto* h = (PeakStep*)pinned(ctx, sizeof(PeakStep));
-    EMC_HIP(hipStreamSynchronize(ctx->stream));  // the pinned buffer may be in use
-    auto refresh = [&](int x0, int y0, int z0, int x1, int y1, int z1) {  // voxel box, inclusive
-      const int a0 = std::max(0, x0 / kPeakTile), b0 = std::max(0, y0 / kPeakTile), c0 = std::max(0, z0 / kPeakTile);
-      const int a1 = std::min(tx - 1, x1 / kPeakTile), b1 = std::min(ty - 1, y1 / kPeakTile),
-                c1 = std::min(tz - 1, z1 / kPeakTile);
-      if (a1 < a0 || b1 < b0 || c1 < c0) return;
-      const int tnx = a1 - a0 + 1, tny = b1 - b0 + 1, tnz = c1 - c0 + 1;
-      hipLaunchKernelGGL(tile_keys_kernel, dim3(tnx * tny * tnz), dim3(256), 0, ctx->stream, (const float*)d_mag,
-                         nx, ny, nz, tx, ty, a0, b0, c0, tnx, tny, keys);
-    };
-    auto next = [&]() {
-      hipLaunchKernelGGL(keys_max_kernel, dim3(nb), dim3(256), 0, ctx->stream, (const unsigned long long*)keys, nt,
-                         part);
-      hipLaunchKernelGGL(peak_step_kernel, dim3(1), dim3(256), 0, ctx->stream, (const unsigned long long*)part, nb,
-                         (const float*)d_mag, d_ang, nx, ny, nz, step);
-      EMC_HIP(hipGetLastError());
-      EMC_HIP(hipMemcpyAsync(h, step, sizeof(PeakStep), hipMemcpyDeviceToHost, ctx->stream));
-      EMC_HIP(hipStreamSynchronize(ctx->stream));
-      return *h;
-    };
-    refresh(0, 0, 0, nx - 1, ny - 1, nz - 1);
-    PeakStep s = next();
-    int n = 0, tries = 0;
-    while (n < max_peaks && tries < max_tries && (double)key_value(s.key) > high_thr) {
-      ++tries;
-      uint32_t i = key_index(s.key);
-      if ((int64_t)i >= (int64_t)nx * ny * nz) i = 0;
-      const int x = (int)(i % (uint32_t)nx), y = (int)((i / (uint32_t)nx) % (uint32_t)ny),
-                z = (int)(i / ((uint32_t)nx * (uint32_t)ny));
-      if (s.ang > 0) {
-        EMC_REQUIRE(s.ang <= n_angles, "angle id %d outside the angle list (%d)", s.ang, n_angles);
-        const double* ang = angle_list + 3 * (s.ang - 1);
-        double* o = h_peaks + 7 * (int64_t)n;
-        // MATLAB 1-based centre plus centre of mass, minus the padding
-        o[0] = x + 1 + s.com[0] - rm_dim;
-        o[1] = y + 1 + s.com[1] - rm_dim;
-        o[2] = z + 1 + s.com[2] - rm_dim;
-        o[3] = ang[0];
-        o[4] = ang[1];
-        o[5] = ang[2];
-        o[6] = (double)key_value(s.key);
-        // removal mask rotated by the peak's angles ('forward', no shift)
-        double M[9];
-        define_matrix(ang, convention, 0, M);
-        float rows[9];
-        colmajor_to_rows(M, rows);
-        const int md[3] = {e, e, e};
-        const float t0[3] = {0.0f, 0.0f, 0.0f};
-        XformArgs xa = xform_args(md, rows, 1, t0, 1);
-        launch_xform(ctx, d_removal_mask, rot, xa);
-        hipLaunchKernelGGL(erase_kernel, dim3(ceil_div((int64_t)e * e * e, 256)), dim3(256), 0, ctx->stream, d_mag,
-                           nx, ny, nz, x, y, z, rm_dim, (const float*)rot, (float)mask_cutoff);
-        refresh(x - rm_dim, y - rm_dim, z - rm_dim, x + rm_dim, y + rm_dim, z + rm_dim);
-        ++n;
-      } else {
-        hipLaunchKernelGGL(zero_voxel_kernel, dim3(1), dim3(1), 0, ctx->stream, d_mag, (int64_t)i);
-        refresh(x, y, z, x, y, z);
-      }
-      s = next();
+    EMC_REQUIRE(h_peaks && max_peaks >= 0, "null pointer");
+    std::vector<double> rows(10 * (size_t)std::max(max_peaks, 1));
+    tsearch_peaks_impl(ctx, d_mag, d_ang, dims, d_removal_mask, rm_dim, angle_list, n_angles, convention,
+                       mask_cutoff, high_thr, max_peaks, max_tries, 1, 1.0, nullptr, rows.data(), n_found);
+    for (int k = 0; k < *n_found; ++k) {
+      for (int d = 0; d < 6; ++d) h_peaks[7 * (int64_t)k + d] = rows[10 * (size_t)k + d];
+      h_peaks[7 * (int64_t)k + 6] = rows[10 * (size_t)k + 9];
     }
-    *n_found = n;
+  });
+}
+
+emc_status emc_tsearch_peaks_multi(emc_ctx* ctx, float* d_mag, const int32_t* d_ang, const int dims[3],
+                                   const float* d_removal_mask, int rm_dim, const double* angle_list, int n_angles,
+                                   int convention, double mask_cutoff, double high_thr, int max_peaks,
+                                   int max_tries, int n_peaks, double sampling_rate, const double* normals,
+                                   double* h_peak_mat, int* n_found) {
+  return guard([&] {
+    tsearch_peaks_impl(ctx, d_mag, d_ang, dims, d_removal_mask, rm_dim, angle_list, n_angles, convention,
+                       mask_cutoff, high_thr, max_peaks, max_tries, n_peaks, sampling_rate, normals, h_peak_mat,
+                       n_found);
   });
 }
 

@@ -155,7 +155,8 @@ def high_threshold(peak_threshold, tomo_size, n_angles_first):
 
 
 def extract_peaks(mag, ang, angle_list, lattice_radius, erase_radius, peak_threshold, high_thr,
-                  sampling_rate=1, erase_type="sphere", max_tries=10000, convention="Bah", n_peaks=1, ctx=None):
+                  sampling_rate=1, erase_type="sphere", max_tries=10000, convention="Bah", n_peaks=1, rng=None,
+                  ctx=None):
     """Greedy peak extraction of BH_templateSearch3d_2.m:1036-1275 on the
     cropped score map ``mag`` and angle-id map ``ang`` ((z, y, x), ids 1-based
     into ``angle_list`` rows (phi, theta, psi-phi), 0 = none).
@@ -165,12 +166,18 @@ def extract_peaks(mag, ang, angle_list, lattice_radius, erase_radius, peak_thres
     these copies); the erase mask is BH_mask3d(erase_type, 2 rmDim + 1,
     erase_radius) with its cut-off lowered by 1 % steps until it keeps the
     sphere's volume (:1102-1122); then emc_tsearch_peaks. Returns (n, 7):
-    x, y, z (times sampling_rate), phi, theta, psi - phi, score. Only
-    nPeaks = 1 (the reference's extra peaks draw random numbers)."""
+    x, y, z (times sampling_rate), phi, theta, psi - phi, score.
+    n_peaks > 1 (:1156-1240, emc_tsearch_peaks_multi): the centre of mass
+    over the wider box, and per taken peak n_peaks entries, (n, n_peaks, 7),
+    the extra ones with the taken position and their angles either from the
+    id map or random (position + nRandom^2 x standard normals drawn from
+    ``rng``, a numpy Generator or seed, where the reference calls randn);
+    column 6 of an extra peak is its iSNR."""
     from .geometry import _CONV
     from .masking import BH_mask3d, BH_padZeros3d
-    if n_peaks != 1:
-        raise NotImplementedError("nPeaks > 1 draws random numbers in the reference; only nPeaks = 1")
+    n_peaks = int(n_peaks)
+    if n_peaks < 1:
+        raise ValueError("n_peaks must be >= 1")
     szK = np.asarray(lattice_radius, dtype=np.int64).reshape(3)
     er = np.asarray(erase_radius, dtype=np.float64).reshape(3)
     rm = int(max(er.max(), szK.max())) + 7
@@ -193,8 +200,18 @@ def extract_peaks(mag, ang, angle_list, lattice_radius, erase_radius, peak_thres
             raise RuntimeError("Did not find an appropriate erase mask")
     al = np.ascontiguousarray(np.asarray(angle_list, dtype=np.float64).reshape(-1, 3))
     max_peaks = int(2 * peak_threshold)
-    out = np.zeros((max(max_peaks, 1), 7))
     nf = _lib.C.c_int()
+    pd = _lib.C.POINTER(_lib.C.c_double)
+    if n_peaks > 1:
+        normals = np.ascontiguousarray(np.random.default_rng(rng).standard_normal((max(max_peaks, 1), n_peaks - 1, 3)))
+        rows = np.zeros((max(max_peaks, 1), 10 * n_peaks))
+        _lib.call("emc_tsearch_peaks_multi", ctx.handle, m.vp, d_ang.vp, _lib.i3(m.dims), mask.vp, rm,
+                  al.ctypes.data_as(pd), al.shape[0], _CONV[convention], float(cutoff), float(high_thr), max_peaks,
+                  int(max_tries), n_peaks, float(sampling_rate), normals.ctypes.data_as(pd), rows.ctypes.data_as(pd),
+                  _lib.C.byref(nf))
+        r = rows[:nf.value].reshape(-1, n_peaks, 10)
+        return np.concatenate([r[:, :, 0:6], r[:, :, 9:10]], axis=2)
+    out = np.zeros((max(max_peaks, 1), 7))
     _lib.call("emc_tsearch_peaks", ctx.handle, m.vp, d_ang.vp, _lib.i3(m.dims), mask.vp, rm,
               al.ctypes.data_as(_lib.C.POINTER(_lib.C.c_double)), al.shape[0], _CONV[convention],
               float(cutoff), float(high_thr), max_peaks, int(max_tries),
@@ -218,22 +235,28 @@ def write_template_csv(csv_path, pos_path, peaks, sampling_rate, n_previous=0, c
     'score samplingRate 0 i+nPrevious 1 1 1 1 1 0 x y z phi theta psi r(9) 1'
     with r = BH_defineMatrix([phi, theta, psi], convention, 'inv') (times the
     in-plane symmetry matrix rem(n + S, S) + 1 when S > 1), column-major, and
-    the .pos line x y z / samplingRate. Formats as the reference's fprintf."""
+    the .pos line x y z / samplingRate. Formats as the reference's fprintf.
+    peaks (n, k, 7) (extract_peaks with n_peaks = k > 1): the k peaks of a
+    row on one line, each with the first peak's score (:1310-1325)."""
     from .geometry import BH_defineMatrix
-    peaks = np.asarray(peaks, dtype=np.float64).reshape(-1, 7)
+    peaks = np.asarray(peaks, dtype=np.float64)
+    peaks = peaks.reshape(-1, 1, 7) if peaks.ndim < 3 else peaks
     S = len(symmetry_matrices) if symmetry_matrices is not None else 1
     with open(csv_path, "w") as fc, open(pos_path, "w") as fp:
         n = 1
-        for i, pk in enumerate(peaks, start=1):
-            if not np.all(pk[0:3]):
+        for i, pks in enumerate(peaks, start=1):
+            if not np.all(pks[0, 0:3]):
                 continue
-            r = BH_defineMatrix(pk[3:6], convention, "inv")
-            if S > 1:
-                r = r @ np.asarray(symmetry_matrices[(n + S) % S])
-            rv = r.reshape(-1, order="F")
-            fields = ([f"{pk[6]:1.2f}", _matlab_d(sampling_rate), "0", _matlab_d(i + n_previous)]
-                      + ["1"] * 5 + ["0"] + [f"{v:f}" for v in pk[0:3]] + [_matlab_d(v) for v in pk[3:6]]
-                      + [f"{v:f}" for v in rv] + ["1"])
-            fc.write(" ".join(fields) + " \n")
-            fp.write("%f %f %f\n" % tuple(pk[0:3] / sampling_rate))
+            line = ""
+            for pk in pks:
+                r = BH_defineMatrix(pk[3:6], convention, "inv")
+                if S > 1:
+                    r = r @ np.asarray(symmetry_matrices[(n + S) % S])
+                rv = r.reshape(-1, order="F")
+                fields = ([f"{pks[0, 6]:1.2f}", _matlab_d(sampling_rate), "0", _matlab_d(i + n_previous)]
+                          + ["1"] * 5 + ["0"] + [f"{v:f}" for v in pk[0:3]] + [_matlab_d(v) for v in pk[3:6]]
+                          + [f"{v:f}" for v in rv] + ["1"])
+                line += " ".join(fields) + " "
+            fc.write(line + "\n")
+            fp.write("%f %f %f\n" % tuple(pks[0, 0:3] / sampling_rate))
             n += 1

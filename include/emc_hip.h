@@ -595,6 +595,27 @@ emc_status emc_tsearch_peaks(emc_ctx* ctx, float* d_mag, const int32_t* d_ang, c
                              int convention, double mask_cutoff, double high_thr, int max_peaks, int max_tries,
                              double* h_peaks, int* n_found);
 
+/* The same loop with nPeaks >= 1 peaks per taken maximum, as
+ * alignment/BH_templateSearch3d_2.m:1156-1240 and :1097 (replaces its
+ * peakMat): the centre of mass over the (2 bd + 1)^3 box, bd = 1 +
+ * round(log(nPeaks)); then per extra peak 2 .. nPeaks, from that box's map
+ * values with the taken id zeroed: if any is non-zero, cAng = the first
+ * maximum and iSNR = the mean of the values with its id (then zeroed); the
+ * angles are ANGLE_LIST(Ang(cAng)) when that id and Ang(cAng) (the map's
+ * id at the box's linear index cAng, as the reference indexes it) are > 0,
+ * else random: the taken peak's (x, y, z) times sampling_rate + nRandom^2 x
+ * normals (nRandom = 2, 3, .. 10), with iSNR = 0.75 x the mean of the earlier
+ * scores where a value was non-zero. h_peak_mat: max_peaks rows of
+ * 10 n_peaks doubles, peak k at columns 10 k: x, y, z (times sampling_rate),
+ * phi, theta, psi-phi, 0, 0, 0, score (iSNR for k > 0). normals: 3
+ * (n_peaks - 1) standard normals per row (the reference draws randn; null
+ * when n_peaks = 1). The map must hold at least the box's voxels. */
+emc_status emc_tsearch_peaks_multi(emc_ctx* ctx, float* d_mag, const int32_t* d_ang, const int dims[3],
+                                   const float* d_removal_mask, int rm_dim, const double* angle_list, int n_angles,
+                                   int convention, double mask_cutoff, double high_thr, int max_peaks,
+                                   int max_tries, int n_peaks, double sampling_rate, const double* normals,
+                                   double* h_peak_mat, int* n_found);
+
 /* d[i] *= s (the global-variance scaling of the chunk stack, :605-611). */
 emc_status emc_scale(emc_ctx* ctx, float* d, int64_t count, double s);
 

@@ -1325,6 +1325,82 @@ def tsearch_peaks(mag, ang, angle_list, removal_mask, rm_dim, mask_cutoff, high_
     return np.array(peaks).reshape(-1, 7)
 
 
+def tsearch_peaks_multi(mag, ang, angle_list, removal_mask, rm_dim, mask_cutoff, high_thr, max_peaks, max_tries,
+                        n_peaks, sampling_rate, normals, convention="Bah"):
+    """The loop of BH_templateSearch3d_2.m:1130-1272 with nPeaks extra
+    columns (peakMat of :1097, rows of 10 nPeaks). Centre of mass over the
+    box of half edge bDist = 1 + round(log(nPeaks)) (:1156-1180); the peaks
+    2..nPeaks of :1198-1240, with MATLAB's indexing kept: magBox and angBox
+    are linear over (x, y, z) (numpy's C order over (z, y, x)), Ang(cAng)
+    indexes the whole padded id map with the box's index; randn(1) is
+    normals[row][iPeak - 2][0..2]."""
+    mag = np.array(mag, dtype=np.float64, copy=True)
+    ang = np.asarray(ang)
+    ang_flat = ang.reshape(-1)
+    nz, ny, nx = mag.shape
+    flat = mag.reshape(-1)
+    b = 1 + int(np.floor(np.log(n_peaks) + 0.5))          # MATLAB round, log > 0 never ends on .5
+    d = np.arange(-b, b + 1)
+    rows, tries = [], 0
+    coord = int(np.argmax(flat))
+    while len(rows) < max_peaks and tries < max_tries and flat[coord] > high_thr:
+        tries += 1
+        z, r = divmod(coord, nx * ny)
+        y, x = divmod(r, nx)
+        if ang_flat[coord] > 0:
+            mag_box = mag[z - b:z + b + 1, y - b:y + b + 1, x - b:x + b + 1].astype(np.float32)
+            ang_box = ang[z - b:z + b + 1, y - b:y + b + 1, x - b:x + b + 1]
+            mb = mag_box.astype(np.float64)
+            s = mb.sum()
+            cen = [x + 1 + np.sum(mb * d[None, None, :]) / s - rm_dim,
+                   y + 1 + np.sum(mb * d[None, :, None]) / s - rm_dim,
+                   z + 1 + np.sum(mb * d[:, None, None]) / s - rm_dim]
+            row = np.zeros(10 * n_peaks)
+            top = int(ang_flat[coord])
+            row[0:3] = sampling_rate * np.asarray(cen)
+            row[3:6] = angle_list[top - 1]
+            row[9] = flat[coord]
+            snr = 0.0
+            if n_peaks > 1:
+                possible = mag_box.reshape(-1).copy()
+                abox = ang_box.reshape(-1)
+                possible[abox == top] = 0
+                n_random = 2
+                for ip in range(2, n_peaks + 1):
+                    use_random = False
+                    if np.any(possible != 0):
+                        c_ang = int(np.argmax(possible))
+                        top = int(abox[c_ang])
+                        snr = float(np.float32(np.mean(possible[abox == top], dtype=np.float64)))
+                        possible[abox == top] = 0
+                        if top <= 0 or ang_flat[c_ang] <= 0:
+                            use_random = True
+                            snr = np.mean(row[9:10 * (ip - 1):10]) * 0.75
+                        else:
+                            i_angles = np.asarray(angle_list[int(ang_flat[c_ang]) - 1], dtype=np.float64)
+                    else:
+                        use_random = True
+                    if use_random:
+                        i_angles = np.asarray(normals[len(rows)][ip - 2], dtype=np.float64) * n_random ** 2 + row[0:3]
+                        if n_random < 10:
+                            n_random += 1
+                    k = 10 * (ip - 1)
+                    row[k:k + 3] = row[0:3]
+                    row[k + 3:k + 6] = i_angles
+                    row[k + 9] = snr
+            rows.append(row)
+            R = define_matrix(np.asarray(angle_list[int(ang_flat[coord]) - 1], dtype=np.float64), convention,
+                              "forward")
+            rot = xform3d(removal_mask, R, np.zeros(3), True)
+            sl = (slice(z - rm_dim, z + rm_dim + 1), slice(y - rm_dim, y + rm_dim + 1),
+                  slice(x - rm_dim, x + rm_dim + 1))
+            mag[sl] = mag[sl] * (rot < mask_cutoff)
+        else:
+            flat[coord] = 0.0
+        coord = int(np.argmax(flat))
+    return np.array(rows).reshape(-1, 10 * n_peaks)
+
+
 # --------------------------------------------------------------------------
 # (f)2 MRC files (@MRCImage/private/readHeader.m:43-186, getVolume.m:40-298,
 # SAVE_IMG.m, private/writeHeader.m, private/setVolumeAndHeaderFromVolume.m)

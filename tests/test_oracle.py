@@ -668,6 +668,90 @@ def test_template_csv_row_format(tmp_path):
     assert (tmp_path / "a.pos").read_text() == "60.250000 32.000000 16.625000\n"
 
 
+def _one_peak_map(head_id=0):
+    """22^3 padded maps: a peak of 4 (id 5) at voxel (x, y, z) = (11, 10, 12)
+    (0-based), 1.0 at x + 2 (id 5) and 0.5 at y - 1 (id 7) - both inside the
+    5^3 box, only the latter inside the 3^3 one - every other voxel 0 with
+    id 5; the first 343 ids of the map (its linear head, which the extra peaks
+    read as Ang(cAng)) are head_id."""
+    mag = np.zeros((22, 22, 22))
+    ang = np.full((22, 22, 22), 5, np.int32)
+    ang.reshape(-1)[:343] = head_id
+    mag[12, 10, 11] = 4.0
+    mag[12, 10, 13] = 1.0
+    mag[12, 9, 11] = 0.5
+    ang[12, 9, 11] = 7
+    return mag, ang
+
+
+def _one_peak_args():
+    grid = np.stack([np.arange(9) * 10.0, np.full(9, 5.0), np.arange(9) * -2.0], 1)
+    return grid, O.mask3d_sphere((7, 7, 7), (2.0, 2.0, 2.0))
+
+
+def test_tsearch_peaks_multi_reduces_to_one_peak():
+    rng = np.random.default_rng(3)
+    mag, ang = _planted_map(rng)
+    mag = mag.astype(np.float32)             # a single-precision map, as the reference's
+    rm = 11
+    pad = lambda a: np.pad(a, rm)
+    grid = np.stack([np.arange(19) * 10.0, np.full(19, 5.0), np.zeros(19)], 1)
+    mask = O.mask3d_sphere((2 * rm + 1,) * 3, (4.0, 4.0, 4.0))
+    pk = O.tsearch_peaks(pad(mag), pad(ang), grid, mask, rm, 0.98, 1.0, 10, 1000)
+    pm = O.tsearch_peaks_multi(pad(mag), pad(ang), grid, mask, rm, 0.98, 1.0, 10, 1000, 1, 2.0, None)
+    assert pm.shape == (3, 10)
+    assert np.allclose(pm[:, 0:3], 2.0 * pk[:, 0:3], rtol=0, atol=1e-12)
+    assert np.array_equal(pm[:, 3:6], pk[:, 3:6]) and np.array_equal(pm[:, 9], pk[:, 6])
+    assert not np.any(pm[:, 6:9])
+
+
+def test_tsearch_peaks_multi_box_and_random_extras():
+    """bDist = 1 + round(log(nPeaks)): 2 for 4 peaks, so the x + 2 voxel
+    enters the centre of mass (x: 2 x 1 / 5.5, y: -0.5 / 5.5). The extra
+    peaks: the id-7 voxel is the only non-zero value left, so peak 2 takes
+    iSNR = 0.75 x 4 and (the map's head being id 0) a random angle; peaks 3
+    and 4 find nothing left and keep that iSNR; nRandom runs 2, 3, 4."""
+    mag, ang = _one_peak_map(0)
+    grid, mask = _one_peak_args()
+    normals = np.array([[[0.5, -1.0, 2.0], [1.5, 0.25, -0.5], [-2.0, 1.0, 0.125]]])
+    pm = O.tsearch_peaks_multi(mag, ang, grid, mask, 3, 0.5, 1.0, 1, 100, 4, 2.0, normals)
+    assert pm.shape == (1, 40)
+    pos = 2.0 * np.array([12 + 2.0 / 5.5 - 3, 11 - 0.5 / 5.5 - 3, 13 - 3])
+    assert np.allclose(pm[0, 0:3], pos, rtol=0, atol=1e-12)
+    assert np.array_equal(pm[0, 3:6], grid[4]) and pm[0, 9] == 4.0
+    for k, (nr, snr) in enumerate(zip((2, 3, 4), (3.0, 3.0, 3.0)), start=1):
+        sub = pm[0, 10 * k:10 * k + 10]
+        assert np.array_equal(sub[0:3], pm[0, 0:3])
+        assert np.allclose(sub[3:6], pos + nr ** 2 * normals[0, k - 1], rtol=0, atol=1e-12)
+        assert sub[9] == snr and not np.any(sub[6:9])
+    # one peak: the 3^3 box leaves the x + 2 voxel out
+    p1 = O.tsearch_peaks_multi(mag, ang, grid, mask, 3, 0.5, 1.0, 1, 100, 1, 1.0, None)
+    assert np.allclose(p1[0, 0:3], [12 - 3, 11 - 0.5 / 4.5 - 3, 13 - 3], rtol=0, atol=1e-12)
+
+
+def test_tsearch_peaks_multi_angles_from_the_id_map():
+    """With a positive id at the box's linear index in the map's head (id 6),
+    peak 2 takes ANGLE_LIST(6) and the mean of its class in the box (0.5, the
+    one id-7 voxel); peak 3 then finds nothing left and is the first random
+    one (nRandom = 2), keeping iSNR 0.5."""
+    mag, ang = _one_peak_map(6)
+    grid, mask = _one_peak_args()
+    normals = np.array([[[0.5, -1.0, 2.0], [1.5, 0.25, -0.5]]])
+    pm = O.tsearch_peaks_multi(mag, ang, grid, mask, 3, 0.5, 1.0, 1, 100, 3, 1.0, normals)
+    assert np.array_equal(pm[0, 13:16], grid[5]) and pm[0, 19] == 0.5
+    assert np.allclose(pm[0, 23:26], pm[0, 0:3] + 4 * normals[0, 1], rtol=0, atol=1e-12) and pm[0, 29] == 0.5
+
+
+def test_template_csv_extra_peaks_on_one_line(tmp_path):
+    from emclarity_amd.template_search import write_template_csv
+    pk = np.array([[[120.5, 64.0, 33.25, 10.0, 12.0, -30.0, 7.891], [120.5, 64.0, 33.25, 1.5, 2.0, 3.0, 5.9]]])
+    write_template_csv(tmp_path / "a.csv", tmp_path / "a.pos", pk, 2)
+    lines = (tmp_path / "a.csv").read_text().splitlines()
+    assert len(lines) == 1
+    row = lines[0].split()
+    assert len(row) == 52 and row[26] == "7.89" and row[39:42] == ["1.500000e+00", "2", "3"]
+
+
 # ------------------------------------------------------- point-group operators
 
 def _is_group(mats):
