@@ -49,3 +49,54 @@ def test_normalisation_weights_halves_and_skip():
     a, _, _, _, _ = O.average_particles(wins[:1], mats[:1], sh[:1], [1], [1.0], mask)
     _, e, _, _, _ = O.average_particles(wins[1:3], mats[1:3], sh[1:3], [2, 2], [1.0, 1.0], mask)
     assert np.allclose(odd, a, atol=1e-12) and np.allclose(eve, e, atol=1e-12)
+
+
+def _peak_row(scores, angles_deg, classes=None):
+    """26 columns per peak: score, class, z-rotations by the given angles."""
+    from emclarity_amd.geometry import BH_defineMatrix
+    n = len(scores)
+    row = np.zeros(26 * n)
+    for k, (s, a) in enumerate(zip(scores, angles_deg)):
+        row[26 * k] = s
+        row[26 * k + 10:26 * k + 13] = (10.0 + k, 20.0, 30.0)
+        row[26 * k + 16:26 * k + 25] = BH_defineMatrix([a, 0.0, 0.0], "Bah", "inv").reshape(-1, order="F")
+        row[26 * k + 25] = 1 if classes is None else classes[k]
+    return row
+
+
+def test_peak_weights_hand_worked():
+    """Scores 0.3, 0.9, 0.6 (sigma 1), rotations 40 deg apart: sorted 0.9,
+    0.6, 0.3; (s / 0.9)^2 = 1, 4/9, 1/9 over their sum 14/9 = 9/14, 4/14,
+    1/14, and 1/14 < 0.1 is dropped."""
+    from emclarity_amd.average import weight_ang_check_peaks
+    row = _peak_row([0.3, 0.9, 0.6], [0.0, 40.0, 80.0])
+    wgt, out = weight_ang_check_peaks(row, 3, 1.0)
+    assert np.allclose(wgt[:2], [9 / 14, 4 / 14], rtol=0, atol=1e-15) and wgt[2] == -9999
+    assert out[0::26].tolist() == [0.9, 0.6, 0.3] and out[10::26].tolist() == [11.0, 12.0, 10.0]
+    # score_sigma scales the scores before the 0.1 floor: 0.3 / 4 drops the third before weighting
+    wgt, _ = weight_ang_check_peaks(row, 3, 4.0)
+    assert np.allclose(wgt[:2], [9 / 13, 4 / 13], rtol=0, atol=1e-15) and wgt[2] == -9999
+    # track_stats: exponent sqrt(max score) = sqrt(0.9)
+    wgt, _ = weight_ang_check_peaks(row, 3, 1.0, track_stats=True)
+    c = np.sqrt(0.9)
+    v = np.array([1.0, (0.6 / 0.9) ** c, (0.3 / 0.9) ** c])
+    assert np.allclose(wgt, v / v.sum(), rtol=0, atol=1e-12)
+
+
+def test_peak_weights_duplicates_and_ignored():
+    """A peak within 5 deg of the one ranked just above it is dropped; one
+    within 5 deg only of a peak two ranks above is kept (the reference's
+    loop compares neighbours only). One included peak goes first with
+    weight 1, the rest follow in order."""
+    from emclarity_amd.average import weight_ang_check_peaks
+    row = _peak_row([0.9, 0.8, 0.7], [0.0, 3.0, 60.0])
+    wgt, out = weight_ang_check_peaks(row, 3, 1.0)
+    assert wgt[1] == -9999 and out[26 + 25] == -9999 and wgt[0] > 0 and wgt[2] > 0
+    assert np.isclose(wgt[0] + wgt[2], 1.0)
+    row = _peak_row([0.9, 0.8, 0.7], [0.0, 60.0, 3.0])
+    wgt, _ = weight_ang_check_peaks(row, 3, 1.0)
+    assert np.all(wgt > 0) and np.isclose(wgt.sum(), 1.0)
+    row = _peak_row([0.5, 0.9, 0.7], [0.0, 60.0, 3.0], classes=[-9999, 1, -9999])
+    wgt, out = weight_ang_check_peaks(row, 3, 1.0)
+    assert wgt.tolist() == [1.0, -9999.0, -9999.0]
+    assert np.array_equal(out[:26], row[26:52]) and np.array_equal(out[26:], np.r_[row[:26], row[52:]])
