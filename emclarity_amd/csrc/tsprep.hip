@@ -218,11 +218,58 @@ __global__ void __launch_bounds__(256) median3_kernel(const float* __restrict__ 
   }
 }
 
+// K = 3 by forgetful selection, in registers: the median of 27 is the
+// rank-13 element. Start with a working set of the first 15 (27/2 + 2)
+// window values; 13 times, move the set's minimum to its low end and its
+// maximum to its high end, drop the minimum, and put the next window value
+// in the maximum's place (no more values after 12 steps: the last step just
+// leaves the middle one): the forgetful selection of M. McGuire, "A fast,
+// small-radius GPU median filter" (ShaderX6, 2008). It returns one of the
+// inputs, the same value as a full sort's middle element. All indices are
+// compile-time, so the 27 values stay in registers.
+__device__ inline void cswap(float& a, float& b) {
+  const float lo = fminf(a, b), hi = fmaxf(a, b);
+  a = lo;
+  b = hi;
+}
+
+__global__ void __launch_bounds__(256) median27_kernel(const float* __restrict__ in, float* __restrict__ out, int nx,
+                                                       int ny, int nz) {
+  const int64_t nv = (int64_t)nx * ny * nz;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += (int64_t)gridDim.x * blockDim.x) {
+    int x, y, z;
+    split3(i, nx, ny, x, y, z);
+    float v[27];
+    const int xs[3] = {reflect_index(x - 1, nx), x, reflect_index(x + 1, nx)};
+#pragma unroll
+    for (int dz = 0; dz < 3; ++dz) {
+      const int zz = reflect_index(z + dz - 1, nz);
+#pragma unroll
+      for (int dy = 0; dy < 3; ++dy) {
+        const int64_t row = ((int64_t)zz * ny + reflect_index(y + dy - 1, ny)) * nx;
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) v[(dz * 3 + dy) * 3 + dx] = in[row + xs[dx]];
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < 13; ++it) {  // working set v[it .. 14]
+      cswap(v[it], v[14]);
+#pragma unroll
+      for (int j = it + 1; j < 14; ++j) {
+        cswap(v[it], v[j]);
+        cswap(v[j], v[14]);
+      }
+      if (it < 12) v[14] = v[15 + it];
+    }
+    out[i] = v[13];
+  }
+}
+
 void median3(emc_ctx* ctx, const float* in, float* out, const int dims[3], int k) {
   const int64_t nv = (int64_t)dims[0] * dims[1] * dims[2];
   const int grid = grid_for(nv);
   if (k == 3)
-    hipLaunchKernelGGL(median3_kernel<3>, dim3(grid), dim3(256), 0, ctx->stream, in, out, dims[0], dims[1], dims[2]);
+    hipLaunchKernelGGL(median27_kernel, dim3(grid), dim3(256), 0, ctx->stream, in, out, dims[0], dims[1], dims[2]);
   else if (k == 5)
     hipLaunchKernelGGL(median3_kernel<5>, dim3(grid), dim3(256), 0, ctx->stream, in, out, dims[0], dims[1], dims[2]);
   else
