@@ -162,11 +162,15 @@ def bench_tsprep(ctx, reps):
                                          bp.ctypes.data_as(C.POINTER(C.c_double)), 0, 8, -1.0, None, out.vp,
                                          C.byref(total)), reps)
     _line("tsprep", wall, {"chunk": s, "GB_in_out": round(2 * 4 * s ** 3 / 1e9, 3)})
-    # with the optional 3^3 median filter first (med_filt 3)
+    # with the optional 3^3 and 5^3 median filters first (med_filt 3, 5)
     wall = _timed(ctx, lambda: _lib.call("emc_tsearch_preprocess_chunk", ctx.handle, chunk.vp, _lib.i3((s, s, s)),
                                          bp.ctypes.data_as(C.POINTER(C.c_double)), 3, 8, -1.0, None, out.vp,
                                          C.byref(total)), reps)
     _line("tsprep_median3", wall, {"chunk": s, "GB_in_out": round(2 * 4 * s ** 3 / 1e9, 3)})
+    wall = _timed(ctx, lambda: _lib.call("emc_tsearch_preprocess_chunk", ctx.handle, chunk.vp, _lib.i3((s, s, s)),
+                                         bp.ctypes.data_as(C.POINTER(C.c_double)), 5, 8, -1.0, None, out.vp,
+                                         C.byref(total)), reps)
+    _line("tsprep_median5", wall, {"chunk": s, "GB_in_out": round(2 * 4 * s ** 3 / 1e9, 3)})
 
 
 def bench_tsearch(ctx, reps):

@@ -265,13 +265,48 @@ __global__ void __launch_bounds__(256) median27_kernel(const float* __restrict__
   }
 }
 
+// K = 5: the same selection over a working set of 125/2 + 2 = 64 registers;
+// the other 61 window values are read one per step, as they enter the set.
+__global__ void __launch_bounds__(256) median125_kernel(const float* __restrict__ in, float* __restrict__ out, int nx,
+                                                        int ny, int nz) {
+  constexpr int K = 5, M = K * K * K, R = M / 2 + 2;
+  const int64_t nv = (int64_t)nx * ny * nz;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += (int64_t)gridDim.x * blockDim.x) {
+    int x, y, z;
+    split3(i, nx, ny, x, y, z);
+    int xs[K], ys[K];
+    int64_t zs[K];
+#pragma unroll
+    for (int d = 0; d < K; ++d) {
+      xs[d] = reflect_index(x + d - K / 2, nx);
+      ys[d] = reflect_index(y + d - K / 2, ny);
+      zs[d] = (int64_t)reflect_index(z + d - K / 2, nz) * ny;
+    }
+    auto at = [&](int e) { return in[(zs[e / (K * K)] + ys[(e / K) % K]) * nx + xs[e % K]]; };
+    float v[R];
+#pragma unroll
+    for (int e = 0; e < R; ++e) v[e] = at(e);
+#pragma unroll
+    for (int it = 0; it < M / 2; ++it) {  // working set v[it .. R - 1]
+      cswap(v[it], v[R - 1]);
+#pragma unroll
+      for (int j = it + 1; j < R - 1; ++j) {
+        cswap(v[it], v[j]);
+        cswap(v[j], v[R - 1]);
+      }
+      if (R + it < M) v[R - 1] = at(R + it);
+    }
+    out[i] = v[M / 2];
+  }
+}
+
 void median3(emc_ctx* ctx, const float* in, float* out, const int dims[3], int k) {
   const int64_t nv = (int64_t)dims[0] * dims[1] * dims[2];
   const int grid = grid_for(nv);
   if (k == 3)
     hipLaunchKernelGGL(median27_kernel, dim3(grid), dim3(256), 0, ctx->stream, in, out, dims[0], dims[1], dims[2]);
   else if (k == 5)
-    hipLaunchKernelGGL(median3_kernel<5>, dim3(grid), dim3(256), 0, ctx->stream, in, out, dims[0], dims[1], dims[2]);
+    hipLaunchKernelGGL(median125_kernel, dim3(grid), dim3(256), 0, ctx->stream, in, out, dims[0], dims[1], dims[2]);
   else
     hipLaunchKernelGGL(median3_kernel<7>, dim3(grid), dim3(256), 0, ctx->stream, in, out, dims[0], dims[1], dims[2]);
   EMC_HIP(hipGetLastError());
