@@ -45,15 +45,16 @@ __global__ void __launch_bounds__(kAvgThreads) avg_rotate_kernel(
               tz = shifts ? shifts[3 * p + 2] : 0.0f;
   const float cx = (float)(nx / 2), cy = (float)(ny / 2), cz = (float)(nz / 2);
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  // the threads cover 8 x 8 x 4 output bricks (one wavefront = an 8 x 8
-  // patch of one z), so each wavefront's samples fall in a compact region
-  // of the input
+  // the threads cover 8 x 8 x 4 output bricks, one wavefront a 4 x 4 x 4
+  // cube of it, so each wavefront's samples fall in a compact region of
+  // the input
   static_assert(kAvgThreads == 256, "8 x 8 x 4 bricks");
   const int bx = (nx + 7) / 8, by = (ny + 7) / 8, bz = (nz + 3) / 4;
   const int nbrick = bx * by * bz;
+  const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
+  const int ox = (wv % 2) * 4 + lane % 4, oy = (wv / 2) * 4 + (lane / 4) % 4, oz = lane / 16;
   for (int br = blockIdx.x; br < nbrick; br += gridDim.x) {
-    const int x = (br % bx) * 8 + threadIdx.x % 8, y = ((br / bx) % by) * 8 + (threadIdx.x / 8) % 8,
-              z = (br / (bx * by)) * 4 + threadIdx.x / 64;
+    const int x = (br % bx) * 8 + ox, y = ((br / bx) % by) * 8 + oy, z = (br / (bx * by)) * 4 + oz;
     if (x >= nx || y >= ny || z >= nz) continue;
     const int64_t v = ((int64_t)z * ny + y) * nx + x;
     const float u = (float)x - cx, w = (float)y - cy, q = (float)z - cz;
