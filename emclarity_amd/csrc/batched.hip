@@ -1051,6 +1051,14 @@ std::vector<int2> support_host(int n, double radius) {
         e.y = (kx + 128) | ((ky + 128) << 8) | ((kz + 128) << 16) | (herm << 24);
         v.push_back(e);
       }
+  // 4 x 4 x 4 cubes of (kx, ky, kz), one wavefront each where the sphere
+  // fills them, so a wavefront's rotated samples fall in a compact region
+  // of the reference table
+  auto cube_key = [](const int2& e) {
+    const int kx = e.y & 255, ky = (e.y >> 8) & 255, kz = (e.y >> 16) & 255;  // offset by 128
+    return ((((kz >> 2) * 64 + (kx >> 2)) * 64 + (ky >> 2)) << 6) | ((kz & 3) << 4) | ((kx & 3) << 2) | (ky & 3);
+  };
+  std::stable_sort(v.begin(), v.end(), [&](const int2& a, const int2& b) { return cube_key(a) < cube_key(b); });
   while (v.size() < 32) v.push_back(int2{0, 128 | (128 << 8) | (128 << 16)});
   return v;
 }
