@@ -693,9 +693,8 @@ __global__ void __launch_bounds__(kBatchThreads) align_eval_grad_kernel(
     const typename Tab::T* __restrict__ F2, int n, const float2* __restrict__ Q, const float* __restrict__ Nw,
     const float* __restrict__ MdM, const float* __restrict__ shifts, const int2* __restrict__ supp,
     int n_supp, double* __restrict__ part) {
-  // blockIdx.x = particle, blockIdx.y = chunk of the support list (FFT
-  // order of the [kz][kx][ky] layout: lanes of a wave sample neighbouring
-  // frequencies)
+  // blockIdx.x = particle, blockIdx.y = chunk of the support list (4 x 4 x
+  // 4 frequency cubes: the lanes of a wave sample neighbouring frequencies)
   const int p = blockIdx.x, chunk = blockIdx.y, nchunk = gridDim.y;
   const int cx = n / 2 + 1;
   const int64_t nh = (int64_t)cx * n * n;
